@@ -1,0 +1,192 @@
+/*
+ * meshode.h — C ABI of libmeshode.so, the MI355X (gfx950) implementation of
+ * MeshODE's deformation hot path.
+ *
+ * Conventions
+ *   - Every function returns an int status: 0 on success, a negative MO_ERR_*
+ *     code on failure; mo_last_error() returns the message of the calling
+ *     thread's last failure.
+ *   - Bulk arrays are DEVICE pointers (HIP memory of the current device),
+ *     row-major and contiguous; sizes are element counts.  `stream` is a
+ *     hipStream_t (NULL = default stream).  Calls are stream-ordered; calls
+ *     that return host values (ids, scalars, stats) synchronize the stream.
+ *   - Vertex arrays are [n,3]; faces [n,3] int32 (0-based); edges [n,2] int32.
+ *   - No torch types cross this boundary.
+ *
+ * Each entry point cites the reference interface it replaces
+ * (paths relative to the hjwdzh/MeshODE checkout).
+ */
+#ifndef MESHODE_H_
+#define MESHODE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MO_OK 0
+#define MO_ERR_ARG (-1)
+#define MO_ERR_HIP (-2)
+#define MO_ERR_STATE (-3)
+#define MO_ERR_RANGE (-4)
+#define MO_ERR_NOMEM (-5)
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int mo_version(void);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* mo_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Templates: the per-target state of the pyDeform parameter store.
+ * ---------------------------------------------------------------------- */
+
+/* InitializeDeformTemplate(tensorV, tensorF, symmetry, grid_resolution)
+ * (src/interface/deform_params.h:30-34, deform_params.cc:16-40).
+ * Copies the target, normalizes it (Mesh::Normalize, mesh.cc:66-96) and
+ * builds the N^3 distance grid (Mesh::ConstructDistanceField,
+ * mesh.cc:106-152).  `symmetry` is accepted and ignored, as in the
+ * reference (it symmetrizes the still-empty mesh, deform_params.cc:28-29).
+ * Writes the new parameter id (ids count up from 0 like the reference's
+ * g_params index).  Synchronizes `stream`. */
+int mo_template_create_f32(const float* V, int64_t nv, const int32_t* F, int64_t nf, int32_t symmetry,
+                           int32_t grid_resolution, void* stream, int32_t* param_id);
+
+/* Same from FP64 vertices (the CLI path: Mesh::ReadOBJ stores doubles,
+ * rigid_deform.cc:49-52). */
+int mo_template_create_f64(const double* V, int64_t nv, const int32_t* F, int64_t nf, int32_t symmetry,
+                           int32_t grid_resolution, void* stream, int32_t* param_id);
+
+/* Releases a template's device memory (the reference never frees its
+ * g_params slots, deform_params.cc:7-14).  Ids are not reused. */
+int mo_template_destroy(int32_t param_id);
+
+/* Grid resolution and the normalization of a template:
+ * DeformParams::scale / trans (deform_params.h:15-16). */
+int mo_template_info(int32_t param_id, int32_t* grid_resolution, double* scale, double* trans3);
+
+/* Copies the template's grid to device buffers (any may be NULL):
+ * dist64 = UniformGrid voxels (uniformgrid.h:33) as [z][y][x] FP64,
+ * dist32 = the same rounded to float, nearest = nearest triangle index
+ * (igl::point_mesh_squared_distance's I, discarded by mesh.cc:137-150;
+ * lowest index among equal distances). */
+int mo_template_grid(int32_t param_id, double* dist64, float* dist32, int32_t* nearest, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Distance-field build primitive.
+ * ---------------------------------------------------------------------- */
+
+/* Mesh::ConstructDistanceField (mesh.cc:106-152) on already-normalized
+ * FP64 vertices: dist[(i*N+j)*N+k] = distance from (k/N, j/N, i/N) to the
+ * nearest triangle.  Outputs may be NULL. */
+int mo_grid_build(const double* Vn, int64_t nv, const int32_t* F, int64_t nf, int32_t grid_resolution,
+                  double* dist64, float* dist32, int32_t* nearest, void* stream);
+
+/* ------------------------------------------------------------------------
+ * pyDeform layer functions (src/interface/pydeform.cc:19-37).
+ * ---------------------------------------------------------------------- */
+
+/* NormalizeByTemplate / DenormalizeByTemplate (normalize.h:6-10,
+ * normalize.cc:5-45): in place on FP32 vertices. */
+int mo_normalize_by_template(float* V, int64_t nv, int32_t param_id, void* stream);
+int mo_denormalize_by_template(float* V, int64_t nv, int32_t param_id, void* stream);
+
+/* DistanceFieldLoss_forward (distance_layer.h:8-10, distance_layer.cc:8-37):
+ * out[i] = d(V_i)^2, out [nv]. */
+int mo_distance_loss_forward(const float* V, int64_t nv, int32_t param_id, float* out, void* stream);
+
+/* DistanceFieldLoss_backward (distance_layer.h:12-14, distance_layer.cc:39-80):
+ * out[i] = d(V_i) * grad d(V_i), out [nv,3]. */
+int mo_distance_loss_backward(const float* V, int64_t nv, int32_t param_id, float* out, void* stream);
+
+/* StoreRigidityInformation (rigid_layer.h:9-12, rigid_layer.cc:11-46). */
+int mo_rigidity_store(const float* V, int64_t nv, const int32_t* F, int64_t nf, int32_t param_id, void* stream);
+
+/* RigidEdgeLoss_forward (rigid_layer.h:14-17, rigid_layer.cc:48-89): out [3nf,3]. */
+int mo_rigid_loss_forward(const float* V, int64_t nv, const int32_t* F, int64_t nf, int32_t param_id, float* out,
+                          void* stream);
+
+/* RigidEdgeLoss_backward (rigid_layer.h:19-22, rigid_layer.cc:91-133): out [nv,3]. */
+int mo_rigid_loss_backward(const float* V, int64_t nv, const int32_t* F, int64_t nf, int32_t param_id, float* out,
+                           void* stream);
+
+/* StoreGraphInformation (graph_layer.h:9-12, graph_layer.cc:11-45). */
+int mo_graph_store(const float* V, int64_t nv, const int32_t* E, int64_t ne, int32_t param_id, void* stream);
+
+/* GraphEdgeLoss_forward (graph_layer.h:14-17, graph_layer.cc:47-91): out [ne,3]. */
+int mo_graph_loss_forward(const float* V, int64_t nv, const int32_t* E, int64_t ne, int32_t param_id, float* out,
+                          void* stream);
+
+/* GraphEdgeLoss_backward (graph_layer.h:19-22, graph_layer.cc:93-135): out [nv,3]. */
+int mo_graph_loss_backward(const float* V, int64_t nv, const int32_t* E, int64_t ne, int32_t param_id, float* out,
+                           void* stream);
+
+/* ------------------------------------------------------------------------
+ * Levenberg-Marquardt deformation (Deformer, src/lib/deformer.h:11-30).
+ * ---------------------------------------------------------------------- */
+
+typedef struct mo_lm_options {
+  int32_t max_num_iterations;            /* 100 (deformer.cc:56) */
+  int32_t max_consecutive_invalid_steps; /* 5 */
+  double function_tolerance;             /* 1e-6 */
+  double gradient_tolerance;             /* 1e-10 */
+  double parameter_tolerance;            /* 1e-8 */
+  double min_relative_decrease;          /* 1e-3 */
+  double initial_trust_region_radius;    /* 1e4 */
+  double max_trust_region_radius;        /* 1e16 */
+  double min_trust_region_radius;        /* 1e-32 */
+  double min_lm_diagonal;                /* 1e-6 */
+  double max_lm_diagonal;                /* 1e32 */
+  int32_t pcg_max_iterations;            /* inner PCG cap per LM step */
+  float pcg_relative_tolerance;          /* ||r|| <= tol ||r0|| ends PCG */
+  int32_t block_threads;                 /* threads per pair (multiple of 64, <= 1024) */
+} mo_lm_options;
+
+typedef struct mo_lm_stats {
+  double initial_cost;
+  double final_cost;     /* "Final cost" line, deformer.cc:90 */
+  double vertex_cost;    /* "Vertices cost" line, deformer.cc:81 */
+  double rigidity_cost;  /* "Rigidity cost" line, deformer.cc:87 */
+  int32_t iterations;
+  int32_t successful_steps;
+  int32_t unsuccessful_steps;
+  int32_t termination;   /* 0 convergence, 1 iteration cap, 2 failure */
+  int64_t pcg_iterations;
+} mo_lm_stats;
+
+/* Fills the defaults listed above. */
+void mo_lm_default_options(mo_lm_options* opt);
+
+/* Deformer::Deform (deformer.cc:18-92; kind 0, idx = faces [n,3]) or
+ * Deformer::DeformGraph (deformer.cc:370-442; kind 1, idx = edges [n,2]) for
+ * a batch of pairs in one persistent launch.  Pair p deforms V[p] ([nv[p],3]
+ * FP64, already in the template's normalized frame, updated in place)
+ * against the grid of template param_ids[p] with rigidity lambda[p].
+ * param_ids, V, nv, idx, nidx, kind, lambda and stats are HOST arrays of
+ * length npairs (V and idx hold device pointers).  Synchronizes `stream`. */
+int mo_deform_batch(int32_t npairs, const int32_t* param_ids, double* const* V, const int64_t* nv,
+                    const int32_t* const* idx, const int64_t* nidx, const int32_t* kind, const double* lambda,
+                    const mo_lm_options* opt, mo_lm_stats* stats, void* stream);
+
+/* The whole rigid_deform pipeline (rigid_deform.cc:49-61) for a batch of
+ * pairs, device-resident end to end: normalize target p
+ * (Mesh::Normalize), build its grid_resolution[p]^3 distance field, bring
+ * source p into the target frame (Mesh::ApplyTransform), solve
+ * Deformer::Deform with lambda[p], and write the deformed source in the
+ * target's original frame (Mesh::WriteOBJ's v * scale + pos, mesh.cc:46-64)
+ * to out[p] ([src_nv[p],3] FP64, may alias src[p]).  All per-pair
+ * arguments are HOST arrays of length npairs holding device pointers /
+ * sizes.  stats (host, optional) receives the per-pair solver summary.
+ * Synchronizes `stream`. */
+int mo_rigid_deform_batch(int32_t npairs, const double* const* src, const int64_t* src_nv,
+                          const int32_t* const* src_faces, const int64_t* src_nf, const double* const* tgt,
+                          const int64_t* tgt_nv, const int32_t* const* tgt_faces, const int64_t* tgt_nf,
+                          const int32_t* grid_resolution, const double* lambda, const mo_lm_options* opt,
+                          double* const* out, mo_lm_stats* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MESHODE_H_ */

@@ -1,0 +1,133 @@
+"""Loader for libmeshode.so (the HIP/gfx950 C ABI declared in include/meshode.h).
+
+There is no CPU fallback: if the library is missing or no GPU is visible,
+calls raise.  Build the library with ``__graft_entry__.build()`` or
+``make -C meshode_amd/csrc``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmeshode.so")
+
+# Exported symbols (kept in sync with include/meshode.h; tests check both).
+SYMBOLS = [
+    "mo_version", "mo_last_error",
+    "mo_template_create_f32", "mo_template_create_f64", "mo_template_destroy", "mo_template_info",
+    "mo_template_grid", "mo_grid_build",
+    "mo_normalize_by_template", "mo_denormalize_by_template",
+    "mo_distance_loss_forward", "mo_distance_loss_backward",
+    "mo_rigidity_store", "mo_rigid_loss_forward", "mo_rigid_loss_backward",
+    "mo_graph_store", "mo_graph_loss_forward", "mo_graph_loss_backward",
+    "mo_lm_default_options", "mo_deform_batch", "mo_rigid_deform_batch",
+]
+
+
+class MoError(RuntimeError):
+    """A libmeshode call returned a negative status."""
+
+    def __init__(self, func, code, msg):
+        super().__init__(f"{func} failed ({code}): {msg}")
+        self.code = code
+
+
+class LMOptions(C.Structure):
+    _fields_ = [
+        ("max_num_iterations", C.c_int32),
+        ("max_consecutive_invalid_steps", C.c_int32),
+        ("function_tolerance", C.c_double),
+        ("gradient_tolerance", C.c_double),
+        ("parameter_tolerance", C.c_double),
+        ("min_relative_decrease", C.c_double),
+        ("initial_trust_region_radius", C.c_double),
+        ("max_trust_region_radius", C.c_double),
+        ("min_trust_region_radius", C.c_double),
+        ("min_lm_diagonal", C.c_double),
+        ("max_lm_diagonal", C.c_double),
+        ("pcg_max_iterations", C.c_int32),
+        ("pcg_relative_tolerance", C.c_float),
+        ("block_threads", C.c_int32),
+    ]
+
+
+class LMStats(C.Structure):
+    _fields_ = [
+        ("initial_cost", C.c_double),
+        ("final_cost", C.c_double),
+        ("vertex_cost", C.c_double),
+        ("rigidity_cost", C.c_double),
+        ("iterations", C.c_int32),
+        ("successful_steps", C.c_int32),
+        ("unsuccessful_steps", C.c_int32),
+        ("termination", C.c_int32),
+        ("pcg_iterations", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+_lock = threading.Lock()
+
+_vp, _i32, _i64, _f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+_SIGS = {
+    "mo_version": (C.c_int, []),
+    "mo_last_error": (C.c_char_p, []),
+    "mo_template_create_f32": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _i32, _vp, C.POINTER(_i32)]),
+    "mo_template_create_f64": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _i32, _vp, C.POINTER(_i32)]),
+    "mo_template_destroy": (C.c_int, [_i32]),
+    "mo_template_info": (C.c_int, [_i32, C.POINTER(_i32), C.POINTER(_f64), _vp]),
+    "mo_template_grid": (C.c_int, [_i32, _vp, _vp, _vp, _vp]),
+    "mo_grid_build": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),
+    "mo_normalize_by_template": (C.c_int, [_vp, _i64, _i32, _vp]),
+    "mo_denormalize_by_template": (C.c_int, [_vp, _i64, _i32, _vp]),
+    "mo_distance_loss_forward": (C.c_int, [_vp, _i64, _i32, _vp, _vp]),
+    "mo_distance_loss_backward": (C.c_int, [_vp, _i64, _i32, _vp, _vp]),
+    "mo_rigidity_store": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _vp]),
+    "mo_rigid_loss_forward": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _vp, _vp]),
+    "mo_rigid_loss_backward": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _vp, _vp]),
+    "mo_graph_store": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _vp]),
+    "mo_graph_loss_forward": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _vp, _vp]),
+    "mo_graph_loss_backward": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _vp, _vp]),
+    "mo_lm_default_options": (None, [C.POINTER(LMOptions)]),
+    "mo_deform_batch": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(LMOptions), _vp, _vp]),
+    "mo_rigid_deform_batch": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                        C.POINTER(LMOptions), _vp, _vp, _vp]),
+}
+
+
+def load(path: str = LIB_PATH):
+    """Load libmeshode.so (no GPU needed to load; calls need one)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise FileNotFoundError(
+                    f"{path} not found: build it with `make -C meshode_amd/csrc` "
+                    "(or __graft_entry__.build()); there is no CPU fallback")
+            lib = C.CDLL(path)
+            for name, (res, args) in _SIGS.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def call(name, *args):
+    """Call an int-returning entry point and raise MoError on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        raise MoError(name, rc, lib.mo_last_error().decode(errors="replace"))
+    return rc
+
+
+def default_lm_options() -> LMOptions:
+    o = LMOptions()
+    load().mo_lm_default_options(C.byref(o))
+    return o

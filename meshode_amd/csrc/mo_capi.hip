@@ -1,0 +1,666 @@
+// C ABI of libmeshode.so (declared in include/meshode.h): the template
+// registry (the pyDeform parameter store, deform_params.cc:7-14), the
+// pyDeform layer entry points, and the batched LM deformation pipeline.
+#include "../../include/meshode.h"
+
+#include "mo_common.h"
+#include "mo_grid.h"
+#include "mo_layers.h"
+#include "mo_lm.h"
+#include "mo_topo.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#define MO_EXPORT extern "C" __attribute__((visibility("default")))
+
+namespace mo {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Device memory helpers.
+
+struct DevPtr {
+  void* p = nullptr;
+  DevPtr() = default;
+  DevPtr(const DevPtr&) = delete;
+  DevPtr& operator=(const DevPtr&) = delete;
+  ~DevPtr() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  }
+  int alloc(size_t bytes) {
+    reset();
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(MO_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return MO_OK;
+  }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+// Stream-ordered scratch arena: one hipMallocAsync carved into aligned slices.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+  hipStream_t stream = nullptr;
+  ~Arena() {
+    if (base) (void)hipFreeAsync(base, stream);
+  }
+  int reserve(size_t bytes, hipStream_t s) {
+    stream = s;
+    size = bytes + 256;
+    hipError_t e = hipMallocAsync((void**)&base, size, s);
+    if (e != hipSuccess) {
+      base = nullptr;
+      return fail(MO_ERR_NOMEM, "hipMallocAsync(%zu) failed: %s", size, hipGetErrorString(e));
+    }
+    return MO_OK;
+  }
+  template <class T> T* take(size_t count) {
+    size_t off = (used + 255) & ~(size_t)255;
+    used = off + count * sizeof(T);
+    return (T*)(base + off);
+  }
+};
+
+inline size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// ---------------------------------------------------------------------------
+// Template registry.
+
+struct Template {
+  int N = 0;
+  int device = 0;
+  double scale = 1.0;
+  double trans[3] = {0, 0, 0};
+  long long cells = 0;
+  DevPtr grid64, grid32, nearest;
+  // edge offsets shared by the Rigid / Graph layers (DeformParams::edge_offset,
+  // deform_params.h:18-20; the last Store call wins)
+  DevPtr edge_off;
+  long long edge_count = 0;
+  int edge_kind = -1;
+  // incidence lists for the backward gather, keyed by the index buffer
+  DevPtr inc_rp, inc;
+  const void* inc_key = nullptr;
+  long long inc_n = -1, inc_nv = -1;
+  int inc_kind = -1;
+};
+
+std::mutex g_mu;
+std::vector<std::unique_ptr<Template>> g_templates;
+
+Template* get_template(int id) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (id < 0 || id >= (int)g_templates.size() || !g_templates[id]) return nullptr;
+  return g_templates[id].get();
+}
+
+int register_template(std::unique_ptr<Template> t) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_templates.push_back(std::move(t));
+  return (int)g_templates.size() - 1;
+}
+
+hipStream_t as_stream(void* s) { return (hipStream_t)s; }
+
+__global__ void k_zero_u32(unsigned* p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+int zero_fill(void* p, size_t bytes, hipStream_t s) {
+  long long n = (long long)((bytes + 3) / 4);
+  if (n <= 0) return MO_OK;
+  hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (unsigned*)p, n);
+  MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+template <class T>
+int template_create(const T* V, int64_t nv, const int32_t* F, int64_t nf, int32_t grid_resolution, void* stream,
+                    int32_t* param_id) {
+  MO_CHECK(param_id, MO_ERR_ARG, "param_id output pointer is null");
+  MO_CHECK(V && F, MO_ERR_ARG, "template vertices/faces must be non-null device pointers");
+  MO_CHECK(nv > 0 && nf > 0, MO_ERR_ARG, "template needs vertices and faces (nv=%lld nf=%lld)", (long long)nv,
+           (long long)nf);
+  MO_CHECK(grid_resolution >= 2 && grid_resolution <= 1024, MO_ERR_ARG, "grid_resolution %d out of range [2, 1024]",
+           grid_resolution);
+  hipStream_t s = as_stream(stream);
+  auto t = std::make_unique<Template>();
+  MO_HIP(hipGetDevice(&t->device));
+  t->N = grid_resolution;
+  t->cells = (long long)grid_resolution * grid_resolution * grid_resolution;
+  int rc;
+  if ((rc = t->grid64.alloc(t->cells * sizeof(double)))) return rc;
+  if ((rc = t->grid32.alloc(t->cells * sizeof(float)))) return rc;
+  if ((rc = t->nearest.alloc(t->cells * sizeof(int)))) return rc;
+  size_t gws;
+  grid_workspace_bytes(nf, &gws);
+  Arena ar;
+  if ((rc = ar.reserve(al(nv * 3 * sizeof(double)) + al(8 * sizeof(double)) + al(gws) + 256, s))) return rc;
+  double* Vn = ar.take<double>(nv * 3);
+  double* sp = ar.take<double>(8);
+  int* err = (int*)(sp + 4);
+  void* ws = ar.take<char>(gws);
+  if ((rc = zero_fill(err, sizeof(int), s))) return rc;
+  if ((rc = normalize_launch<T>(V, nv, sp, Vn, s))) return rc;
+  if ((rc = grid_build_launch(Vn, nv, F, nf, t->N, t->grid64.as<double>(), t->grid32.as<float>(),
+                              t->nearest.as<int>(), ws, s, err)))
+    return rc;
+  double host[5];
+  MO_HIP(hipMemcpyAsync(host, sp, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+  MO_HIP(hipStreamSynchronize(s));
+  int herr;
+  std::memcpy(&herr, &host[4], sizeof(int));
+  MO_CHECK(herr == 0, MO_ERR_RANGE, "template face index out of range [0, %lld)", (long long)nv);
+  t->scale = host[0];
+  t->trans[0] = host[1];
+  t->trans[1] = host[2];
+  t->trans[2] = host[3];
+  *param_id = register_template(std::move(t));
+  return MO_OK;
+}
+
+// Incidence lists for the backward gather: vertex v's entries are
+// (h << 1) | role in increasing h, role 0 (tail, subtract) before role 1
+// (head, add) — the order in which the reference's sequential loop updates
+// that vertex.  Built on the host from the index buffer.
+int ensure_incidence(Template* t, const int32_t* idx, long long nidx, long long nv, int kind, hipStream_t s) {
+  if (t->inc_key == (const void*)idx && t->inc_n == nidx && t->inc_nv == nv && t->inc_kind == kind) return MO_OK;
+  const int w = kind == 0 ? 3 : 2;
+  std::vector<int32_t> h_idx((size_t)nidx * w);
+  if (nidx > 0) {
+    MO_HIP(hipMemcpyAsync(h_idx.data(), idx, h_idx.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MO_HIP(hipStreamSynchronize(s));
+  }
+  const long long nh = kind == 0 ? 3 * nidx : nidx;
+  std::vector<long long> rp((size_t)nv + 1, 0);
+  auto ends = [&](long long h, int* a, int* b) {
+    if (kind == 0) {
+      long long f = h / 3;
+      int j = (int)(h - f * 3);
+      *a = h_idx[(size_t)f * 3 + j];
+      *b = h_idx[(size_t)f * 3 + (j + 1) % 3];
+    } else {
+      *a = h_idx[(size_t)h * 2];
+      *b = h_idx[(size_t)h * 2 + 1];
+    }
+  };
+  for (long long h = 0; h < nh; ++h) {
+    int a, b;
+    ends(h, &a, &b);
+    MO_CHECK(a >= 0 && b >= 0 && a < nv && b < nv, MO_ERR_RANGE,
+             "edge %lld references vertex out of range [0, %lld)", h, nv);
+    rp[a + 1]++;
+    rp[b + 1]++;
+  }
+  for (long long v = 0; v < nv; ++v) rp[v + 1] += rp[v];
+  std::vector<long long> inc((size_t)rp[nv]);
+  std::vector<long long> cur(rp.begin(), rp.end() - 1);
+  for (long long h = 0; h < nh; ++h) {
+    int a, b;
+    ends(h, &a, &b);
+    inc[cur[a]++] = (h << 1) | 0;
+    inc[cur[b]++] = (h << 1) | 1;
+  }
+  int rc;
+  if ((rc = t->inc_rp.alloc(rp.size() * sizeof(long long)))) return rc;
+  if ((rc = t->inc.alloc(std::max<size_t>(inc.size(), 1) * sizeof(long long)))) return rc;
+  MO_HIP(hipMemcpyAsync(t->inc_rp.p, rp.data(), rp.size() * sizeof(long long), hipMemcpyHostToDevice, s));
+  if (!inc.empty())
+    MO_HIP(hipMemcpyAsync(t->inc.p, inc.data(), inc.size() * sizeof(long long), hipMemcpyHostToDevice, s));
+  MO_HIP(hipStreamSynchronize(s));
+  t->inc_key = idx;
+  t->inc_n = nidx;
+  t->inc_nv = nv;
+  t->inc_kind = kind;
+  return MO_OK;
+}
+
+int check_err_flag(int* err_dev, hipStream_t s, const char* what) {
+  int h = 0;
+  MO_HIP(hipMemcpyAsync(&h, err_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+  MO_HIP(hipStreamSynchronize(s));
+  MO_CHECK(h == 0, MO_ERR_RANGE, "%s: index out of range (flags 0x%x)", what, h);
+  return MO_OK;
+}
+
+int edge_store(const float* V, int64_t nv, const int32_t* idx, int64_t n, int32_t pid, int kind, void* stream) {
+  Template* t = get_template(pid);
+  MO_CHECK(t, MO_ERR_ARG, "unknown param_id %d", pid);
+  MO_CHECK(V && idx && nv >= 0 && n >= 0, MO_ERR_ARG, "invalid vertex/edge buffers");
+  hipStream_t s = as_stream(stream);
+  const long long nh = kind == 0 ? 3 * (long long)n : (long long)n;
+  int rc;
+  if ((rc = t->edge_off.alloc(std::max<long long>(nh, 1) * 3 * sizeof(float)))) return rc;
+  Arena ar;
+  if ((rc = ar.reserve(256, s))) return rc;
+  int* err = ar.take<int>(1);
+  if ((rc = zero_fill(err, sizeof(int), s))) return rc;
+  if ((rc = edge_store_launch(V, nv, idx, kind, nh, t->edge_off.as<float>(), err, s))) return rc;
+  if ((rc = check_err_flag(err, s, kind == 0 ? "StoreRigidityInformation" : "StoreGraphInformation"))) return rc;
+  t->edge_count = nh;
+  t->edge_kind = kind;
+  return ensure_incidence(t, idx, n, nv, kind, s);
+}
+
+int edge_forward(const float* V, int64_t nv, const int32_t* idx, int64_t n, int32_t pid, float* out, int kind,
+                 void* stream) {
+  Template* t = get_template(pid);
+  MO_CHECK(t, MO_ERR_ARG, "unknown param_id %d", pid);
+  MO_CHECK(V && idx && out, MO_ERR_ARG, "invalid buffers");
+  const long long nh = kind == 0 ? 3 * (long long)n : (long long)n;
+  MO_CHECK(t->edge_count >= 0 && t->edge_off.p && nh <= t->edge_count, MO_ERR_STATE,
+           "edge loss called with %lld edges but %lld offsets are stored (call the Store function first)", nh,
+           t->edge_count);
+  hipStream_t s = as_stream(stream);
+  Arena ar;
+  int rc;
+  if ((rc = ar.reserve(256, s))) return rc;
+  int* err = ar.take<int>(1);
+  if ((rc = zero_fill(err, sizeof(int), s))) return rc;
+  if ((rc = edge_fwd_launch(V, nv, idx, kind, nh, t->edge_off.as<float>(), out, err, s))) return rc;
+  return check_err_flag(err, s, "edge loss forward");
+}
+
+int edge_backward(const float* V, int64_t nv, const int32_t* idx, int64_t n, int32_t pid, float* out, int kind,
+                  void* stream) {
+  Template* t = get_template(pid);
+  MO_CHECK(t, MO_ERR_ARG, "unknown param_id %d", pid);
+  MO_CHECK(V && idx && out, MO_ERR_ARG, "invalid buffers");
+  const long long nh = kind == 0 ? 3 * (long long)n : (long long)n;
+  MO_CHECK(t->edge_off.p && nh <= t->edge_count, MO_ERR_STATE,
+           "edge loss backward called with %lld edges but %lld offsets are stored", nh, t->edge_count);
+  hipStream_t s = as_stream(stream);
+  int rc;
+  if ((rc = ensure_incidence(t, idx, n, nv, kind, s))) return rc;
+  return edge_bwd_launch(V, nv, idx, kind, t->edge_off.as<float>(), t->inc_rp.as<long long>(),
+                         t->inc.as<long long>(), out, s);
+}
+
+// ---------------------------------------------------------------------------
+// Batched LM pipeline.
+
+__global__ void k_make_x0(const double* __restrict__ V, int nv, const float* __restrict__ deg, float4* __restrict__ x0,
+                          float4* __restrict__ u) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  x0[i] = make_float4((float)V[i * 3], (float)V[i * 3 + 1], (float)V[i * 3 + 2], deg[i]);
+  u[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// out = x0_64 + u (normalized frame), or (x0_64 + u) * scale + pos when sp
+// is given (Mesh::WriteOBJ, mesh.cc:46-64).
+__global__ void k_finish(const double* __restrict__ X, int nv, const float4* __restrict__ u,
+                         const double* __restrict__ sp, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const float4 d = u[i];
+  double p[3] = {X[i * 3] + (double)d.x, X[i * 3 + 1] + (double)d.y, X[i * 3 + 2] + (double)d.z};
+  for (int k = 0; k < 3; ++k) out[i * 3 + k] = sp ? p[k] * sp[0] + sp[1 + k] : p[k];
+}
+
+struct CorePair {
+  const float* grid;
+  int N;
+  const double* x0;    // normalized FP64 source [nv,3]
+  long long nv;
+  const int32_t* idx;
+  long long nidx;
+  int kind;
+  double lambda;
+  const double* sp;    // device (scale, pos) for denormalized output, or null
+  double* out;
+};
+
+LMOptionsDev to_dev_options(const mo_lm_options& o) {
+  LMOptionsDev d;
+  d.max_num_iterations = o.max_num_iterations;
+  d.max_consecutive_invalid_steps = o.max_consecutive_invalid_steps;
+  d.function_tolerance = o.function_tolerance;
+  d.gradient_tolerance = o.gradient_tolerance;
+  d.parameter_tolerance = o.parameter_tolerance;
+  d.min_relative_decrease = o.min_relative_decrease;
+  d.initial_trust_region_radius = o.initial_trust_region_radius;
+  d.max_trust_region_radius = o.max_trust_region_radius;
+  d.min_trust_region_radius = o.min_trust_region_radius;
+  d.min_lm_diagonal = o.min_lm_diagonal;
+  d.max_lm_diagonal = o.max_lm_diagonal;
+  d.pcg_max_iterations = o.pcg_max_iterations;
+  d.pcg_relative_tolerance = o.pcg_relative_tolerance;
+  return d;
+}
+
+int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_stats* stats, hipStream_t s) {
+  const int np = (int)pairs.size();
+  if (np == 0) return MO_OK;
+  MO_CHECK(opt.block_threads >= 64 && opt.block_threads <= 1024 && opt.block_threads % 64 == 0, MO_ERR_ARG,
+           "block_threads must be a multiple of 64 in [64, 1024]");
+  long long nvtot = 0, nedges_tot = 0, max_edges = 0;
+  std::vector<long long> vbase(np);
+  for (int p = 0; p < np; ++p) {
+    const CorePair& c = pairs[p];
+    MO_CHECK(c.nv >= 0 && c.nv < (1 << 24), MO_ERR_ARG, "pair %d: vertex count %lld outside [0, 2^24)", p, c.nv);
+    MO_CHECK(c.kind == 0 || c.kind == 1, MO_ERR_ARG, "pair %d: kind must be 0 (faces) or 1 (edges)", p);
+    vbase[p] = nvtot;
+    nvtot += c.nv;
+    long long ne = c.kind == 0 ? 3 * c.nidx : c.nidx;
+    nedges_tot += ne;
+    max_edges = std::max(max_edges, ne);
+  }
+  const long long nvalloc = std::max<long long>(nvtot, 1);
+  size_t bytes = al(nvalloc * sizeof(float4)) * 2 + al(nvalloc * sizeof(int2)) + al(nvalloc * sizeof(float)) +
+                 al((size_t)kLMWorkSlots * nvalloc * sizeof(float4)) +
+                 al(std::max<long long>(2 * nedges_tot, 1) * sizeof(unsigned)) + al(topo_workspace_bytes(nvalloc)) +
+                 al(np * sizeof(LMPairDesc)) + al(np * sizeof(TopoPairDesc)) + al(np * sizeof(LMResultDev)) + 256;
+  Arena ar;
+  int rc;
+  if ((rc = ar.reserve(bytes, s))) return rc;
+  float4* x0 = ar.take<float4>(nvalloc);
+  float4* u = ar.take<float4>(nvalloc);
+  int2* adj = ar.take<int2>(nvalloc);
+  float* deg = ar.take<float>(nvalloc);
+  float4* work = ar.take<float4>((size_t)kLMWorkSlots * nvalloc);
+  unsigned* raw = ar.take<unsigned>(std::max<long long>(2 * nedges_tot, 1));
+  void* tws = ar.take<char>(topo_workspace_bytes(nvalloc));
+  LMPairDesc* d_pairs = ar.take<LMPairDesc>(np);
+  TopoPairDesc* d_topo = ar.take<TopoPairDesc>(np);
+  LMResultDev* d_res = ar.take<LMResultDev>(np);
+  int* err = ar.take<int>(1);
+
+  // largest pairs first: the hardware dispatches workgroups in order, so
+  // this is a longest-processing-time-first schedule
+  std::vector<int> order(np);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pairs[a].nv > pairs[b].nv; });
+  std::vector<LMPairDesc> h_pairs(np);
+  std::vector<TopoPairDesc> h_topo(np);
+  for (int q = 0; q < np; ++q) {
+    const int p = order[q];
+    const CorePair& c = pairs[p];
+    h_pairs[q] = LMPairDesc{c.grid, c.N, (int)c.nv, vbase[p], 0, (float)c.lambda, 0};
+    h_topo[p] = TopoPairDesc{c.idx, c.kind == 0 ? 3 * c.nidx : c.nidx, vbase[p], (int)c.nv, c.kind};
+  }
+  MO_HIP(hipMemcpyAsync(d_pairs, h_pairs.data(), np * sizeof(LMPairDesc), hipMemcpyHostToDevice, s));
+  MO_HIP(hipMemcpyAsync(d_topo, h_topo.data(), np * sizeof(TopoPairDesc), hipMemcpyHostToDevice, s));
+  if ((rc = zero_fill(err, sizeof(int), s))) return rc;
+  if ((rc = topo_build_launch(d_topo, np, max_edges, nvtot, 2 * nedges_tot, raw, adj, deg, tws, err, s))) return rc;
+  for (int p = 0; p < np; ++p) {
+    const CorePair& c = pairs[p];
+    if (c.nv == 0) continue;
+    hipLaunchKernelGGL(k_make_x0, dim3((unsigned)((c.nv + 255) / 256)), dim3(256), 0, s, c.x0, (int)c.nv,
+                       deg + vbase[p], x0 + vbase[p], u + vbase[p]);
+  }
+  MO_HIP(hipGetLastError());
+  LMArrays A{x0, u, adj, raw, work};
+  if ((rc = lm_solve_launch(d_pairs, np, A, to_dev_options(opt), d_res, nvalloc, opt.block_threads, s))) return rc;
+  for (int p = 0; p < np; ++p) {
+    const CorePair& c = pairs[p];
+    if (c.nv == 0) continue;
+    hipLaunchKernelGGL(k_finish, dim3((unsigned)((c.nv + 255) / 256)), dim3(256), 0, s, c.x0, (int)c.nv,
+                       u + vbase[p], c.sp, c.out);
+  }
+  MO_HIP(hipGetLastError());
+  std::vector<LMResultDev> h_res(np);
+  int herr = 0;
+  MO_HIP(hipMemcpyAsync(h_res.data(), d_res, np * sizeof(LMResultDev), hipMemcpyDeviceToHost, s));
+  MO_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  MO_HIP(hipStreamSynchronize(s));
+  MO_CHECK(herr == 0, MO_ERR_RANGE, "source topology: index out of range or too many vertices (flags 0x%x)", herr);
+  if (stats) {
+    for (int q = 0; q < np; ++q) {
+      const LMResultDev& r = h_res[q];
+      mo_lm_stats& o = stats[order[q]];
+      o.initial_cost = r.initial_cost;
+      o.final_cost = r.final_cost;
+      o.vertex_cost = r.vertex_cost;
+      o.rigidity_cost = r.rigidity_cost;
+      o.iterations = r.iterations;
+      o.successful_steps = r.successful_steps;
+      o.unsuccessful_steps = r.unsuccessful_steps;
+      o.termination = r.termination;
+      o.pcg_iterations = r.pcg_iterations;
+    }
+  }
+  return MO_OK;
+}
+
+}  // namespace
+}  // namespace mo
+
+using namespace mo;
+
+// ===========================================================================
+// Exported entry points.
+
+MO_EXPORT int mo_version(void) { return 100; }
+
+MO_EXPORT const char* mo_last_error(void) { return g_last_error.c_str(); }
+
+MO_EXPORT int mo_template_create_f32(const float* V, int64_t nv, const int32_t* F, int64_t nf, int32_t symmetry,
+                                     int32_t grid_resolution, void* stream, int32_t* param_id) {
+  (void)symmetry;   // no-op in the reference (deform_params.cc:28-29)
+  return template_create<float>(V, nv, F, nf, grid_resolution, stream, param_id);
+}
+
+MO_EXPORT int mo_template_create_f64(const double* V, int64_t nv, const int32_t* F, int64_t nf, int32_t symmetry,
+                                     int32_t grid_resolution, void* stream, int32_t* param_id) {
+  (void)symmetry;
+  return template_create<double>(V, nv, F, nf, grid_resolution, stream, param_id);
+}
+
+MO_EXPORT int mo_template_destroy(int32_t param_id) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  MO_CHECK(param_id >= 0 && param_id < (int)g_templates.size() && g_templates[param_id], MO_ERR_ARG,
+           "unknown param_id %d", param_id);
+  g_templates[param_id].reset();
+  return MO_OK;
+}
+
+MO_EXPORT int mo_template_info(int32_t param_id, int32_t* grid_resolution, double* scale, double* trans3) {
+  Template* t = get_template(param_id);
+  MO_CHECK(t, MO_ERR_ARG, "unknown param_id %d", param_id);
+  if (grid_resolution) *grid_resolution = t->N;
+  if (scale) *scale = t->scale;
+  if (trans3) for (int k = 0; k < 3; ++k) trans3[k] = t->trans[k];
+  return MO_OK;
+}
+
+MO_EXPORT int mo_template_grid(int32_t param_id, double* dist64, float* dist32, int32_t* nearest, void* stream) {
+  Template* t = get_template(param_id);
+  MO_CHECK(t, MO_ERR_ARG, "unknown param_id %d", param_id);
+  hipStream_t s = as_stream(stream);
+  if (dist64) MO_HIP(hipMemcpyAsync(dist64, t->grid64.p, t->cells * sizeof(double), hipMemcpyDeviceToDevice, s));
+  if (dist32) MO_HIP(hipMemcpyAsync(dist32, t->grid32.p, t->cells * sizeof(float), hipMemcpyDeviceToDevice, s));
+  if (nearest) MO_HIP(hipMemcpyAsync(nearest, t->nearest.p, t->cells * sizeof(int), hipMemcpyDeviceToDevice, s));
+  return MO_OK;
+}
+
+MO_EXPORT int mo_grid_build(const double* Vn, int64_t nv, const int32_t* F, int64_t nf, int32_t grid_resolution,
+                            double* dist64, float* dist32, int32_t* nearest, void* stream) {
+  MO_CHECK(Vn && F && nv > 0 && nf > 0, MO_ERR_ARG, "grid build needs vertices and faces");
+  hipStream_t s = as_stream(stream);
+  size_t gws;
+  grid_workspace_bytes(nf, &gws);
+  Arena ar;
+  int rc;
+  if ((rc = ar.reserve(al(gws) + 256, s))) return rc;
+  void* ws = ar.take<char>(gws);
+  int* err = ar.take<int>(1);
+  if ((rc = zero_fill(err, sizeof(int), s))) return rc;
+  if ((rc = grid_build_launch(Vn, nv, F, nf, grid_resolution, dist64, dist32, nearest, ws, s, err))) return rc;
+  return check_err_flag(err, s, "mo_grid_build");
+}
+
+MO_EXPORT int mo_normalize_by_template(float* V, int64_t nv, int32_t param_id, void* stream) {
+  Template* t = get_template(param_id);
+  MO_CHECK(t, MO_ERR_ARG, "unknown param_id %d", param_id);
+  MO_CHECK(V || nv == 0, MO_ERR_ARG, "null vertex buffer");
+  return normalize_by_template_launch(V, nv, t->scale, t->trans, false, as_stream(stream));
+}
+
+MO_EXPORT int mo_denormalize_by_template(float* V, int64_t nv, int32_t param_id, void* stream) {
+  Template* t = get_template(param_id);
+  MO_CHECK(t, MO_ERR_ARG, "unknown param_id %d", param_id);
+  MO_CHECK(V || nv == 0, MO_ERR_ARG, "null vertex buffer");
+  return normalize_by_template_launch(V, nv, t->scale, t->trans, true, as_stream(stream));
+}
+
+MO_EXPORT int mo_distance_loss_forward(const float* V, int64_t nv, int32_t param_id, float* out, void* stream) {
+  Template* t = get_template(param_id);
+  MO_CHECK(t, MO_ERR_ARG, "unknown param_id %d", param_id);
+  MO_CHECK((V && out) || nv == 0, MO_ERR_ARG, "null buffer");
+  return distance_loss_launch(t->grid32.as<float>(), t->N, V, nv, out, false, as_stream(stream));
+}
+
+MO_EXPORT int mo_distance_loss_backward(const float* V, int64_t nv, int32_t param_id, float* out, void* stream) {
+  Template* t = get_template(param_id);
+  MO_CHECK(t, MO_ERR_ARG, "unknown param_id %d", param_id);
+  MO_CHECK((V && out) || nv == 0, MO_ERR_ARG, "null buffer");
+  return distance_loss_launch(t->grid32.as<float>(), t->N, V, nv, out, true, as_stream(stream));
+}
+
+MO_EXPORT int mo_rigidity_store(const float* V, int64_t nv, const int32_t* F, int64_t nf, int32_t param_id,
+                                void* stream) {
+  return edge_store(V, nv, F, nf, param_id, 0, stream);
+}
+
+MO_EXPORT int mo_rigid_loss_forward(const float* V, int64_t nv, const int32_t* F, int64_t nf, int32_t param_id,
+                                    float* out, void* stream) {
+  return edge_forward(V, nv, F, nf, param_id, out, 0, stream);
+}
+
+MO_EXPORT int mo_rigid_loss_backward(const float* V, int64_t nv, const int32_t* F, int64_t nf, int32_t param_id,
+                                     float* out, void* stream) {
+  return edge_backward(V, nv, F, nf, param_id, out, 0, stream);
+}
+
+MO_EXPORT int mo_graph_store(const float* V, int64_t nv, const int32_t* E, int64_t ne, int32_t param_id,
+                             void* stream) {
+  return edge_store(V, nv, E, ne, param_id, 1, stream);
+}
+
+MO_EXPORT int mo_graph_loss_forward(const float* V, int64_t nv, const int32_t* E, int64_t ne, int32_t param_id,
+                                    float* out, void* stream) {
+  return edge_forward(V, nv, E, ne, param_id, out, 1, stream);
+}
+
+MO_EXPORT int mo_graph_loss_backward(const float* V, int64_t nv, const int32_t* E, int64_t ne, int32_t param_id,
+                                     float* out, void* stream) {
+  return edge_backward(V, nv, E, ne, param_id, out, 1, stream);
+}
+
+MO_EXPORT void mo_lm_default_options(mo_lm_options* o) {
+  if (!o) return;
+  o->max_num_iterations = 100;
+  o->max_consecutive_invalid_steps = 5;
+  o->function_tolerance = 1e-6;
+  o->gradient_tolerance = 1e-10;
+  o->parameter_tolerance = 1e-8;
+  o->min_relative_decrease = 1e-3;
+  o->initial_trust_region_radius = 1e4;
+  o->max_trust_region_radius = 1e16;
+  o->min_trust_region_radius = 1e-32;
+  o->min_lm_diagonal = 1e-6;
+  o->max_lm_diagonal = 1e32;
+  o->pcg_max_iterations = 1000;
+  o->pcg_relative_tolerance = 1e-5f;
+  o->block_threads = 256;
+}
+
+MO_EXPORT int mo_deform_batch(int32_t npairs, const int32_t* param_ids, double* const* V, const int64_t* nv,
+                              const int32_t* const* idx, const int64_t* nidx, const int32_t* kind,
+                              const double* lambda, const mo_lm_options* opt, mo_lm_stats* stats, void* stream) {
+  MO_CHECK(npairs >= 0, MO_ERR_ARG, "npairs < 0");
+  if (npairs == 0) return MO_OK;
+  MO_CHECK(param_ids && V && nv && idx && nidx && kind && lambda, MO_ERR_ARG, "null per-pair array");
+  mo_lm_options o;
+  if (opt) o = *opt; else mo_lm_default_options(&o);
+  std::vector<CorePair> pairs(npairs);
+  for (int p = 0; p < npairs; ++p) {
+    Template* t = get_template(param_ids[p]);
+    MO_CHECK(t, MO_ERR_ARG, "pair %d: unknown param_id %d", p, param_ids[p]);
+    MO_CHECK(V[p] || nv[p] == 0, MO_ERR_ARG, "pair %d: null vertex buffer", p);
+    MO_CHECK(idx[p] || nidx[p] == 0, MO_ERR_ARG, "pair %d: null index buffer", p);
+    pairs[p] = CorePair{t->grid32.as<float>(), t->N, V[p], nv[p], idx[p], nidx[p], kind[p], lambda[p], nullptr, V[p]};
+  }
+  return lm_core(pairs, o, stats, as_stream(stream));
+}
+
+MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, const int64_t* src_nv,
+                                    const int32_t* const* src_faces, const int64_t* src_nf, const double* const* tgt,
+                                    const int64_t* tgt_nv, const int32_t* const* tgt_faces, const int64_t* tgt_nf,
+                                    const int32_t* grid_resolution, const double* lambda, const mo_lm_options* opt,
+                                    double* const* out, mo_lm_stats* stats, void* stream) {
+  MO_CHECK(npairs >= 0, MO_ERR_ARG, "npairs < 0");
+  if (npairs == 0) return MO_OK;
+  MO_CHECK(src && src_nv && src_faces && src_nf && tgt && tgt_nv && tgt_faces && tgt_nf && grid_resolution &&
+               lambda && out,
+           MO_ERR_ARG, "null per-pair array");
+  mo_lm_options o;
+  if (opt) o = *opt; else mo_lm_default_options(&o);
+  hipStream_t s = as_stream(stream);
+  // per-pair scratch: normalized target, (scale, pos), normalized source, FP32 grid
+  size_t bytes = 256;
+  size_t gmax = 0;
+  for (int p = 0; p < npairs; ++p) {
+    MO_CHECK(tgt_nv[p] > 0 && tgt_nf[p] > 0, MO_ERR_ARG, "pair %d: empty target", p);
+    MO_CHECK(grid_resolution[p] >= 2 && grid_resolution[p] <= 1024, MO_ERR_ARG, "pair %d: bad grid resolution", p);
+    long long N = grid_resolution[p];
+    bytes += al(tgt_nv[p] * 3 * sizeof(double)) + al(8 * sizeof(double)) + al(src_nv[p] * 3 * sizeof(double)) +
+             al(N * N * N * sizeof(float));
+    size_t g;
+    grid_workspace_bytes(tgt_nf[p], &g);
+    gmax = std::max(gmax, g);
+  }
+  bytes += al(gmax) + 256;
+  Arena ar;
+  int rc;
+  if ((rc = ar.reserve(bytes, s))) return rc;
+  int* err = ar.take<int>(1);
+  void* gws = ar.take<char>(gmax);
+  if ((rc = zero_fill(err, sizeof(int), s))) return rc;
+  std::vector<CorePair> pairs(npairs);
+  for (int p = 0; p < npairs; ++p) {
+    const long long N = grid_resolution[p];
+    double* tn = ar.take<double>(tgt_nv[p] * 3);
+    double* sp = ar.take<double>(8);
+    double* sn = ar.take<double>(std::max<long long>(src_nv[p], 1) * 3);
+    float* g32 = ar.take<float>(N * N * N);
+    if ((rc = normalize_launch<double>(tgt[p], tgt_nv[p], sp, tn, s))) return rc;
+    if ((rc = grid_build_launch(tn, tgt_nv[p], tgt_faces[p], tgt_nf[p], (int)N, nullptr, g32, nullptr, gws, s, err)))
+      return rc;
+    if ((rc = apply_transform_launch<double>(src[p], src_nv[p], sp, sn, s))) return rc;
+    pairs[p] = CorePair{g32, (int)N, sn, src_nv[p], src_faces[p], src_nf[p], 0, lambda[p], sp, out[p]};
+  }
+  if ((rc = lm_core(pairs, o, stats, s))) return rc;
+  return check_err_flag(err, s, "target faces");
+}

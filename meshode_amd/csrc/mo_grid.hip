@@ -1,0 +1,388 @@
+// Distance-field build: Mesh::ConstructDistanceField (mesh.cc:106-152) on
+// gfx950, plus the template normalization (Mesh::Normalize, mesh.cc:66-96)
+// and Mesh::ApplyTransform (mesh.cc:98-105) it depends on.
+//
+// Grid point (i, j, k) sits at (k/N, j/N, i/N) (mesh.cc:110-116) and its
+// value is sqrt of the squared distance to the nearest target triangle
+// (mesh.cc:140-150, igl::point_mesh_squared_distance).  The squared distance
+// is computed in FP64 with a canonical operation sequence (face region when
+// the projection falls inside, else the three edges with endpoints ordered
+// by vertex index) so that triangles sharing an edge or vertex produce the
+// identical value and ties resolve to the lowest triangle index.  This file
+// is compiled with -ffp-contract=off: every FP64 operation is one IEEE
+// rounding, so results are bit-reproducible on the host.
+//
+// Kernel structure (HBM traffic is tiny; the build is FP64-VALU bound):
+//   1. k_face_prep   : per face, FP64 face record + AABB + Morton bin of the
+//                      centroid (16^3 bins).
+//   2. k_bin_sort    : one workgroup counting-sorts faces by bin (LDS
+//                      histogram + scan) so 64-face tiles are spatially tight.
+//   3. k_tile_bounds : AABB per tile of 64 sorted faces.
+//   4. k_grid_build  : one wavefront per 4x4x4 brick of grid points.  The
+//                      wave first evaluates the tile closest to its brick,
+//                      then walks all tiles, culling every tile whose AABB is
+//                      farther from the brick AABB than the wave's current
+//                      worst best distance (ballot over 64 tiles at a time).
+//                      Surviving tiles are staged into LDS (one face record
+//                      per lane) and every lane tests its grid point against
+//                      all 64 faces.  Ties keep the lowest face index, so
+//                      the visiting order never changes the result.
+#include "mo_common.h"
+#include "mo_grid.h"
+
+#include <cmath>
+
+namespace mo {
+
+namespace {
+
+constexpr int kTile = 64;
+constexpr int kBinBits = 4;                  // 16 bins per axis
+constexpr int kBins = 1 << (3 * kBinBits);   // 4096
+
+__device__ inline unsigned spread3(unsigned v) {
+  // spread the low 4 bits of v to every third bit
+  unsigned r = 0;
+  for (int b = 0; b < kBinBits; ++b) r |= ((v >> b) & 1u) << (3 * b);
+  return r;
+}
+
+__device__ inline double dmin(double a, double b) { return b < a ? b : a; }
+__device__ inline double dmax(double a, double b) { return b > a ? b : a; }
+
+__global__ void k_face_prep(const double* __restrict__ V, long long nv, const int* __restrict__ F,
+                            long long nf, GFace* __restrict__ faces, unsigned* __restrict__ bin,
+                            int* __restrict__ err) {
+  long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  GFace g;
+  int idx[3];
+  double p[3][3];
+  for (int j = 0; j < 3; ++j) {
+    int vi = F[f * 3 + j];
+    if (vi < 0 || vi >= nv) { atomicOr(err, 1); vi = 0; }
+    idx[j] = vi;
+    p[j][0] = V[(long long)vi * 3 + 0];
+    p[j][1] = V[(long long)vi * 3 + 1];
+    p[j][2] = V[(long long)vi * 3 + 2];
+  }
+  g.ax = p[0][0]; g.ay = p[0][1]; g.az = p[0][2];
+  g.bx = p[1][0]; g.by = p[1][1]; g.bz = p[1][2];
+  g.cx = p[2][0]; g.cy = p[2][1]; g.cz = p[2][2];
+  double e0x = g.bx - g.ax, e0y = g.by - g.ay, e0z = g.bz - g.az;
+  double e1x = g.cx - g.ax, e1y = g.cy - g.ay, e1z = g.cz - g.az;
+  g.nx = e0y * e1z - e0z * e1y;
+  g.ny = e0z * e1x - e0x * e1z;
+  g.nz = e0x * e1y - e0y * e1x;
+  double nn = g.nx * g.nx + g.ny * g.ny + g.nz * g.nz;
+  g.inv_nn = nn > 0.0 ? 1.0 / nn : 0.0;
+  g.d00 = e0x * e0x + e0y * e0y + e0z * e0z;
+  g.d01 = e0x * e1x + e0y * e1y + e0z * e1z;
+  g.d11 = e1x * e1x + e1y * e1y + e1z * e1z;
+  g.denom = g.d00 * g.d11 - g.d01 * g.d01;
+  int flip = 0;
+  double ll[3], il[3];
+  for (int j = 0; j < 3; ++j) {
+    int i0 = j, i1 = (j + 1) % 3;
+    if (idx[i1] < idx[i0]) { flip |= 1 << j; int t = i0; i0 = i1; i1 = t; }
+    double dx = p[i1][0] - p[i0][0], dy = p[i1][1] - p[i0][1], dz = p[i1][2] - p[i0][2];
+    ll[j] = dx * dx + dy * dy + dz * dz;
+    il[j] = ll[j] > 0.0 ? 1.0 / ll[j] : 0.0;
+  }
+  g.ll0 = ll[0]; g.ll1 = ll[1]; g.ll2 = ll[2];
+  g.il0 = il[0]; g.il1 = il[1]; g.il2 = il[2];
+  g.flip = flip;
+  g.face = (int)f;
+  faces[f] = g;
+  // centroid bin (Morton order of a 16^3 lattice over [0,1]^3, clamped)
+  unsigned c[3];
+  for (int a = 0; a < 3; ++a) {
+    double m = (p[0][a] + p[1][a] + p[2][a]) * (1.0 / 3.0);
+    int b = (int)floor(m * (1 << kBinBits));
+    b = b < 0 ? 0 : (b > (1 << kBinBits) - 1 ? (1 << kBinBits) - 1 : b);
+    c[a] = (unsigned)b;
+  }
+  bin[f] = spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
+}
+
+// Single-workgroup counting sort of face records by bin.
+__global__ void __launch_bounds__(1024) k_bin_sort(const GFace* __restrict__ in, const unsigned* __restrict__ bin,
+                                                   long long nf, GFace* __restrict__ out) {
+  __shared__ unsigned hist[kBins];
+  for (int b = threadIdx.x; b < kBins; b += blockDim.x) hist[b] = 0;
+  __syncthreads();
+  for (long long f = threadIdx.x; f < nf; f += blockDim.x) atomicAdd(&hist[bin[f]], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned s = 0;
+    for (int b = 0; b < kBins; ++b) { unsigned c = hist[b]; hist[b] = s; s += c; }
+  }
+  __syncthreads();
+  for (long long f = threadIdx.x; f < nf; f += blockDim.x) {
+    unsigned pos = atomicAdd(&hist[bin[f]], 1u);
+    out[pos] = in[f];
+  }
+}
+
+__global__ void k_tile_bounds(const GFace* __restrict__ faces, long long nf, double* __restrict__ tb) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long ntiles = (nf + kTile - 1) / kTile;
+  if (t >= ntiles) return;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  long long e = (t + 1) * kTile < nf ? (t + 1) * kTile : nf;
+  for (long long f = t * kTile; f < e; ++f) {
+    const GFace& g = faces[f];
+    double xs[3] = {g.ax, g.bx, g.cx}, ys[3] = {g.ay, g.by, g.cy}, zs[3] = {g.az, g.bz, g.cz};
+    for (int j = 0; j < 3; ++j) {
+      lo[0] = dmin(lo[0], xs[j]); hi[0] = dmax(hi[0], xs[j]);
+      lo[1] = dmin(lo[1], ys[j]); hi[1] = dmax(hi[1], ys[j]);
+      lo[2] = dmin(lo[2], zs[j]); hi[2] = dmax(hi[2], zs[j]);
+    }
+  }
+  for (int a = 0; a < 3; ++a) { tb[t * 6 + a] = lo[a]; tb[t * 6 + 3 + a] = hi[a]; }
+}
+
+// Canonical edge squared distance (endpoints ordered by vertex index).
+__device__ inline double edge_d2(double qx, double qy, double qz, double ux, double uy, double uz,
+                                 double wx, double wy, double wz, double ll, double il) {
+  double dx = wx - ux, dy = wy - uy, dz = wz - uz;
+  double px = qx - ux, py = qy - uy, pz = qz - uz;
+  double tn = px * dx + py * dy + pz * dz;
+  if (tn <= 0.0 || ll == 0.0) return px * px + py * py + pz * pz;
+  if (tn >= ll) {
+    double rx = qx - wx, ry = qy - wy, rz = qz - wz;
+    return rx * rx + ry * ry + rz * rz;
+  }
+  double t = tn * il;
+  double cx = ux + t * dx, cy = uy + t * dy, cz = uz + t * dz;
+  double rx = qx - cx, ry = qy - cy, rz = qz - cz;
+  return rx * rx + ry * ry + rz * rz;
+}
+
+__device__ inline double face_d2(const GFace& g, double qx, double qy, double qz) {
+  double vx = qx - g.ax, vy = qy - g.ay, vz = qz - g.az;
+  if (g.denom > 0.0 && g.inv_nn > 0.0) {
+    double e0x = g.bx - g.ax, e0y = g.by - g.ay, e0z = g.bz - g.az;
+    double e1x = g.cx - g.ax, e1y = g.cy - g.ay, e1z = g.cz - g.az;
+    double d20 = vx * e0x + vy * e0y + vz * e0z;
+    double d21 = vx * e1x + vy * e1y + vz * e1z;
+    double bb = g.d11 * d20 - g.d01 * d21;
+    double cc = g.d00 * d21 - g.d01 * d20;
+    if (bb >= 0.0 && cc >= 0.0 && bb + cc <= g.denom) {
+      double dp = g.nx * vx + g.ny * vy + g.nz * vz;
+      return (dp * dp) * g.inv_nn;
+    }
+  }
+  // edge j joins p_j and p_{j+1}; flip bit j set when p_{j+1} has the lower index
+  double d0 = (g.flip & 1) ? edge_d2(qx, qy, qz, g.bx, g.by, g.bz, g.ax, g.ay, g.az, g.ll0, g.il0)
+                           : edge_d2(qx, qy, qz, g.ax, g.ay, g.az, g.bx, g.by, g.bz, g.ll0, g.il0);
+  double d1 = (g.flip & 2) ? edge_d2(qx, qy, qz, g.cx, g.cy, g.cz, g.bx, g.by, g.bz, g.ll1, g.il1)
+                           : edge_d2(qx, qy, qz, g.bx, g.by, g.bz, g.cx, g.cy, g.cz, g.ll1, g.il1);
+  double d2 = (g.flip & 4) ? edge_d2(qx, qy, qz, g.ax, g.ay, g.az, g.cx, g.cy, g.cz, g.ll2, g.il2)
+                           : edge_d2(qx, qy, qz, g.cx, g.cy, g.cz, g.ax, g.ay, g.az, g.ll2, g.il2);
+  double m = d2 < d1 ? d2 : d1;
+  return m < d0 ? m : d0;
+}
+
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ inline double box_gap2(const double* __restrict__ tb, long long t, double lx, double ly, double lz,
+                                  double hx, double hy, double hz) {
+  double tl[3] = {tb[t * 6 + 0], tb[t * 6 + 1], tb[t * 6 + 2]};
+  double th[3] = {tb[t * 6 + 3], tb[t * 6 + 4], tb[t * 6 + 5]};
+  double ql[3] = {lx, ly, lz}, qh[3] = {hx, hy, hz};
+  double s = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    double g = dmax(0.0, dmax(tl[a] - qh[a], ql[a] - th[a]));
+    s += g * g;
+  }
+  return s;
+}
+
+// One wavefront (64 lanes) per 4x4x4 brick.
+__global__ void __launch_bounds__(64) k_grid_build(const GFace* __restrict__ faces, long long nf,
+                                                   const double* __restrict__ tb, long long ntiles, int N,
+                                                   int bricks_per_axis, double* __restrict__ dist64,
+                                                   float* __restrict__ dist32, int* __restrict__ nearest) {
+  __shared__ GFace tile[kTile];
+  const int lane = threadIdx.x;
+  long long b = blockIdx.x;
+  const int bx = (int)(b % bricks_per_axis);
+  const int by = (int)((b / bricks_per_axis) % bricks_per_axis);
+  const int bz = (int)(b / ((long long)bricks_per_axis * bricks_per_axis));
+  const int k = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), i = bz * 4 + (lane >> 4);
+  const bool valid = (k < N) && (j < N) && (i < N);
+  const double qx = (double)k / (double)N, qy = (double)j / (double)N, qz = (double)i / (double)N;
+  // brick AABB over valid points
+  const int k1 = min(bx * 4 + 3, N - 1), j1 = min(by * 4 + 3, N - 1), i1 = min(bz * 4 + 3, N - 1);
+  const double lx = (double)(bx * 4) / N, ly = (double)(by * 4) / N, lz = (double)(bz * 4) / N;
+  const double hx = (double)k1 / N, hy = (double)j1 / N, hz = (double)i1 / N;
+
+  double best = INFINITY;
+  int bidx = INT_MAX;
+
+  auto process_tile = [&](long long t) {
+    long long f = t * kTile + lane;
+    __syncthreads();
+    if (f < nf) tile[lane] = faces[f];
+    __syncthreads();
+    int cnt = (int)((nf - t * kTile) < kTile ? (nf - t * kTile) : kTile);
+    for (int q = 0; q < cnt; ++q) {
+      const GFace& g = tile[q];
+      double d = face_d2(g, qx, qy, qz);
+      if (d < best || (d == best && g.face < bidx)) { best = d; bidx = g.face; }
+    }
+  };
+
+  // 1. the tile closest to the brick first (tightens the cull bound early)
+  double mlb = INFINITY;
+  long long mt = -1;
+  for (long long c = 0; c < ntiles; c += 64) {
+    long long t = c + lane;
+    double lb = t < ntiles ? box_gap2(tb, t, lx, ly, lz, hx, hy, hz) : INFINITY;
+    if (lb < mlb) { mlb = lb; mt = t; }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    double ol = __shfl_xor(mlb, o, 64);
+    long long ot = __shfl_xor(mt, o, 64);
+    if (ol < mlb || (ol == mlb && ot < mt)) { mlb = ol; mt = ot; }
+  }
+  if (mt >= 0) process_tile(mt);
+
+  // 2. every other tile whose lower bound can still improve some lane
+  for (long long c = 0; c < ntiles; c += 64) {
+    long long t = c + lane;
+    double lb = t < ntiles ? box_gap2(tb, t, lx, ly, lz, hx, hy, hz) : INFINITY;
+    double bound = wave_max(valid ? best : -INFINITY);
+    bool need = (t < ntiles) && (t != mt) && !(lb > bound * (1.0 + 1e-9));
+    unsigned long long mask = __ballot(need);
+    while (mask) {
+      int l = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      double lbl = __shfl(lb, l, 64);
+      double bnd = wave_max(valid ? best : -INFINITY);
+      if (lbl > bnd * (1.0 + 1e-9)) continue;
+      process_tile(c + l);
+    }
+  }
+  if (valid) {
+    long long cell = ((long long)i * N + j) * N + k;
+    double s = sqrt(best);
+    if (dist64) dist64[cell] = s;
+    if (dist32) dist32[cell] = (float)s;
+    if (nearest) nearest[cell] = bidx;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Normalization (Mesh::Normalize, mesh.cc:66-96) on device, FP64.
+
+template <class T>
+__global__ void __launch_bounds__(1024) k_normalize_stats(const T* __restrict__ V, long long n,
+                                                          double* __restrict__ out /* scale, pos[3] */) {
+  __shared__ double smn[3][1024], smx[3][1024];
+  double mn[3] = {1e30, 1e30, 1e30}, mx[3] = {-1e30, -1e30, -1e30};
+  for (long long i = threadIdx.x; i < n; i += blockDim.x)
+    for (int a = 0; a < 3; ++a) {
+      double v = (double)V[i * 3 + a];
+      if (v < mn[a]) mn[a] = v;
+      if (v > mx[a]) mx[a] = v;
+    }
+  for (int a = 0; a < 3; ++a) { smn[a][threadIdx.x] = mn[a]; smx[a][threadIdx.x] = mx[a]; }
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int a = 0; a < 3; ++a) {
+        double o = smn[a][threadIdx.x + s];
+        if (o < smn[a][threadIdx.x]) smn[a][threadIdx.x] = o;
+        o = smx[a][threadIdx.x + s];
+        if (o > smx[a][threadIdx.x]) smx[a][threadIdx.x] = o;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double ex = smx[0][0] - smn[0][0], ey = smx[1][0] - smn[1][0], ez = smx[2][0] - smn[2][0];
+    double m = ez > ey ? ez : ey;           // std::max(b, c)
+    m = m > ex ? m : ex;                    // std::max(a, .)
+    double scale = m * 1.1;
+    out[0] = scale;
+    for (int a = 0; a < 3; ++a) out[1 + a] = smn[a][0] - 0.05 * scale;
+  }
+}
+
+template <class T>
+__global__ void k_apply_transform(const T* __restrict__ V, long long n, const double* __restrict__ sp,
+                                  double* __restrict__ out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * 3) return;
+  int a = (int)(i % 3);
+  out[i] = ((double)V[i] - sp[1 + a]) / sp[0];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+
+int grid_workspace_bytes(long long nf, size_t* bytes) {
+  long long ntiles = (nf + kTile - 1) / kTile;
+  *bytes = (size_t)nf * sizeof(GFace) * 2 + (size_t)nf * sizeof(unsigned) + (size_t)ntiles * 6 * sizeof(double) +
+           256 * 4;
+  return MO_OK;
+}
+
+int grid_build_launch(const double* Vn, long long nv, const int* F, long long nf, int N, double* dist64,
+                      float* dist32, int* nearest, void* workspace, hipStream_t stream, int* err_dev) {
+  MO_CHECK(N >= 2 && N <= 2048, MO_ERR_ARG, "grid resolution %d out of range [2, 2048]", N);
+  MO_CHECK(nf >= 1, MO_ERR_ARG, "distance field needs at least one triangle (got %lld)", nf);
+  char* w = (char*)workspace;
+  GFace* raw = (GFace*)w; w += (size_t)nf * sizeof(GFace);
+  GFace* sorted = (GFace*)w; w += (size_t)nf * sizeof(GFace);
+  double* tb = (double*)w;
+  long long ntiles = (nf + kTile - 1) / kTile;
+  w += (size_t)ntiles * 6 * sizeof(double);
+  unsigned* bin = (unsigned*)w;
+  const int T = 256;
+  hipLaunchKernelGGL(k_face_prep, dim3((unsigned)((nf + T - 1) / T)), dim3(T), 0, stream, Vn, nv, F, nf, raw, bin,
+                     err_dev);
+  hipLaunchKernelGGL(k_bin_sort, dim3(1), dim3(1024), 0, stream, raw, bin, nf, sorted);
+  hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((ntiles + T - 1) / T)), dim3(T), 0, stream, sorted, nf, tb);
+  int bpa = (N + 3) / 4;
+  long long nb = (long long)bpa * bpa * bpa;
+  hipLaunchKernelGGL(k_grid_build, dim3((unsigned)nb), dim3(64), 0, stream, sorted, nf, tb, ntiles, N, bpa, dist64,
+                     dist32, nearest);
+  MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+template <class T>
+int normalize_launch(const T* V, long long nv, double* sp_dev, double* Vn_dev, hipStream_t stream) {
+  hipLaunchKernelGGL(k_normalize_stats<T>, dim3(1), dim3(1024), 0, stream, V, nv, sp_dev);
+  if (Vn_dev && nv > 0) {
+    long long n3 = nv * 3;
+    hipLaunchKernelGGL(k_apply_transform<T>, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, stream, V, nv,
+                       sp_dev, Vn_dev);
+  }
+  MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+template <class T>
+int apply_transform_launch(const T* V, long long nv, const double* sp_dev, double* out, hipStream_t stream) {
+  if (nv <= 0) return MO_OK;
+  long long n3 = nv * 3;
+  hipLaunchKernelGGL(k_apply_transform<T>, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, stream, V, nv, sp_dev,
+                     out);
+  MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+template int normalize_launch<float>(const float*, long long, double*, double*, hipStream_t);
+template int normalize_launch<double>(const double*, long long, double*, double*, hipStream_t);
+template int apply_transform_launch<float>(const float*, long long, const double*, double*, hipStream_t);
+template int apply_transform_launch<double>(const double*, long long, const double*, double*, hipStream_t);
+
+}  // namespace mo

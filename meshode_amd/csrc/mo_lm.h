@@ -1,0 +1,65 @@
+// Batched Levenberg-Marquardt deformation solver (Deformer::Deform /
+// Deformer::DeformGraph, deformer.cc:18-92 and 370-442) on gfx950.
+#pragma once
+
+#include "mo_common.h"
+
+namespace mo {
+
+// One source/target pair of the batch.  All pointers are device pointers.
+struct LMPairDesc {
+  const float* grid;      // N^3 FP32 distance grid of the target
+  int N;
+  int nv;                 // source vertices
+  long long vbase;        // offset of this pair in the per-vertex arrays
+  long long ebase;        // offset of this pair in the adjacency entry array
+  float lambda;           // rigidity weight (EdgeLoss lambda)
+  int pad;
+};
+
+// Solver options; defaults restate Ceres' Solver::Options defaults that
+// deformer.cc:55-59 leaves untouched (max_num_iterations = 100).
+struct LMOptionsDev {
+  int max_num_iterations;
+  int max_consecutive_invalid_steps;
+  double function_tolerance;
+  double gradient_tolerance;
+  double parameter_tolerance;
+  double min_relative_decrease;
+  double initial_trust_region_radius;
+  double max_trust_region_radius;
+  double min_trust_region_radius;
+  double min_lm_diagonal;
+  double max_lm_diagonal;
+  int pcg_max_iterations;       // inner PCG iteration cap per LM step
+  float pcg_relative_tolerance; // ||r|| <= tol * ||r0|| stops PCG
+};
+
+struct LMResultDev {
+  double initial_cost;
+  double final_cost;
+  double vertex_cost;
+  double rigidity_cost;
+  int iterations;
+  int successful_steps;
+  int unsuccessful_steps;
+  int termination;      // 0 convergence, 1 iteration cap, 2 failure
+  long long pcg_iterations;
+};
+
+// Number of float4 scratch slots per vertex used by the solver.
+constexpr int kLMWorkSlots = 13;
+
+// Per-vertex arrays (all float4, indexed vbase + i).
+struct LMArrays {
+  const float4* x0;     // initial normalized position (xyz), w = sum of edge multiplicities
+  float4* u;            // displacement from x0 (solution), in/out
+  const int2* adj;      // (first entry, count) of the vertex's adjacency list
+  const unsigned* nbr;  // entry: local neighbour index (low 24 bits) | multiplicity << 24
+  float4* work;         // kLMWorkSlots float4 per vertex of scratch, slot-major
+};
+
+int lm_solve_launch(const LMPairDesc* pairs_dev, int npairs, const LMArrays& arrays, const LMOptionsDev& opt,
+                    LMResultDev* results_dev, long long total_vertices, int block_threads, hipStream_t s);
+
+}  // namespace mo

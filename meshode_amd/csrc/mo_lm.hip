@@ -1,0 +1,394 @@
+// Batched Levenberg-Marquardt solve of Deformer::Deform / DeformGraph
+// (deformer.cc:18-92, 370-442) on gfx950: one workgroup per pair, the whole
+// trust-region loop inside one persistent kernel.
+//
+// Problem (unchanged from the reference): per vertex u the DistanceLoss
+// residual d(x_u) (distanceloss.h:6-25), per directed edge (a, b) the
+// EdgeLoss residual lambda (x_a - x_b - v_ab) with v_ab = x0_a - x0_b
+// (edgeloss.h:8-33, deformer.cc:44).  Unknowns are kept as displacements
+// u = x - x0, so every edge residual is exactly lambda (u_a - u_b) and the
+// rigidity energy of all directed edges between u and w collapses to
+// 1/2 lambda^2 m_uw ||u_u - u_w||^2 with m_uw the number of such edges.  The
+// adjacency is stored once per vertex (neighbour, multiplicity).
+//
+// Minimizer: Ceres' trust-region loop with the Levenberg-Marquardt strategy
+// (Jacobi scaling fixed at iteration 0, LM diagonal clamp(colnorm^2(J s)) /
+// radius, rho test against min_relative_decrease, radius update
+// max(1/3, 1 - (2 rho - 1)^3), parameter / function / gradient tolerances,
+// 5 consecutive invalid steps = failure).  Ceres' exact sparse Cholesky
+// (SPARSE_NORMAL_CHOLESKY, deformer.cc:57) is replaced by a matrix-free
+// preconditioned conjugate gradient on
+//     (J^T J + D^2) delta = -J^T f,
+//     J^T J = blockdiag(g g^T) + lambda^2 (L_m (x) I_3)
+// in the unscaled variables (Jacobi scaling cancels under the 3x3
+// block-Jacobi preconditioner), with FP32 vectors and FP64 reductions.
+//
+// All scalar state of the trust region is replicated in every thread and
+// updated from block-wide reductions that every thread sums in the same
+// order, so control flow is uniform without any broadcast.
+#include "mo_common.h"
+#include "mo_lm.h"
+#include "mo_lookup.h"
+
+#include <cfloat>
+
+namespace mo {
+
+namespace {
+
+constexpr int kMaxWaves = 16;
+
+// Block-wide reduction of K sums followed by M maxima; every thread gets the
+// totals.  Two LDS buffers alternate so consecutive reductions need only the
+// one barrier inside each.
+template <int K, int M>
+__device__ inline void block_reduce(double (&v)[K + M], double* red, int& sel) {
+  for (int k = 0; k < K; ++k)
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+  for (int k = K; k < K + M; ++k)
+    for (int o = 32; o > 0; o >>= 1) { double t = __shfl_xor(v[k], o, 64); v[k] = t > v[k] ? t : v[k]; }
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double* buf = red + sel * (8 * kMaxWaves);
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < K + M; ++k) buf[k * kMaxWaves + w] = v[k];
+  __syncthreads();
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int q = 0; q < nw; ++q) s += buf[k * kMaxWaves + q];
+    v[k] = s;
+  }
+  for (int k = K; k < K + M; ++k) {
+    double s = buf[k * kMaxWaves];
+    for (int q = 1; q < nw; ++q) s = buf[k * kMaxWaves + q] > s ? buf[k * kMaxWaves + q] : s;
+    v[k] = s;
+  }
+  sel ^= 1;
+}
+
+__device__ inline unsigned nb_index(unsigned e) { return e & 0xFFFFFFu; }
+__device__ inline float nb_mult(unsigned e) { return (float)(e >> 24); }
+
+// Work slots (float4 each, SoA over all vertices of the batch).
+enum Slot { kG = 0, kGR, kS, kDG, kDT, kMI0, kMI1, kDL, kR, kZ, kP0, kP1, kQ, kNumSlots };
+static_assert(kNumSlots == kLMWorkSlots, "work slot count is part of the host contract");
+
+__global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict__ pairs, LMArrays A,
+                                                   LMOptionsDev opt, LMResultDev* __restrict__ res,
+                                                   long long totalV) {
+  __shared__ double red[2 * 8 * kMaxWaves];
+  const LMPairDesc P = pairs[blockIdx.x];
+  const int T = blockDim.x, tid = threadIdx.x;
+  const int nv = P.nv;
+  const long long vb = P.vbase;
+  const float l2 = P.lambda * P.lambda;
+  const float* __restrict__ grid = P.grid;
+  const int N = P.N;
+  const float4* __restrict__ X0 = A.x0 + vb;
+  float4* __restrict__ U = A.u + vb;
+  const int2* __restrict__ ADJ = A.adj + vb;
+  const unsigned* __restrict__ NB = A.nbr + P.ebase;
+  float4* __restrict__ W[kNumSlots];
+  for (int k = 0; k < kNumSlots; ++k) W[k] = A.work + (size_t)k * totalV + vb;
+  float4* __restrict__ G = W[kG];
+  float4* __restrict__ GR = W[kGR];
+  float4* __restrict__ S = W[kS];
+  float4* __restrict__ DG = W[kDG];
+  float4* __restrict__ DT = W[kDT];
+  float4* __restrict__ MI0 = W[kMI0];
+  float4* __restrict__ MI1 = W[kMI1];
+  float4* __restrict__ DL = W[kDL];
+  float4* __restrict__ R = W[kR];
+  float4* __restrict__ Z = W[kZ];
+  float4* __restrict__ PB[2] = {W[kP0], W[kP1]};
+  float4* __restrict__ Q = W[kQ];
+  int sel = 0;
+
+  // ---- evaluate residuals, gradient J^T f, cost at the current u --------
+  auto eval_jacobian = [&](bool first, double& cv, double& ce, double& gmax) {
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = tid; i < nv; i += T) {
+      const float4 x0 = X0[i];
+      const float4 ui = U[i];
+      const JetF J = lookup_jet(grid, N, x0.x + ui.x, x0.y + ui.y, x0.z + ui.z);
+      G[i] = make_float4(J.v0, J.v1, J.v2, J.a);
+      float rx = 0.f, ry = 0.f, rz = 0.f;
+      double e = 0.0;
+      const int2 a = ADJ[i];
+      for (int k = 0; k < a.y; ++k) {
+        const unsigned en = NB[a.x + k];
+        const float m = nb_mult(en);
+        const float4 uj = U[nb_index(en)];
+        const float dx = ui.x - uj.x, dy = ui.y - uj.y, dz = ui.z - uj.z;
+        rx += m * dx; ry += m * dy; rz += m * dz;
+        e += (double)m * ((double)dx * dx + (double)dy * dy + (double)dz * dz);
+      }
+      const float gx = J.a * J.v0 + l2 * rx, gy = J.a * J.v1 + l2 * ry, gz = J.a * J.v2 + l2 * rz;
+      GR[i] = make_float4(gx, gy, gz, 0.f);
+      v[0] += 0.5 * (double)J.a * (double)J.a;
+      v[1] += 0.25 * (double)l2 * e;
+      double gm = fabs((double)gx);
+      gm = fabs((double)gy) > gm ? fabs((double)gy) : gm;
+      gm = fabs((double)gz) > gm ? fabs((double)gz) : gm;
+      v[2] = gm > v[2] ? gm : v[2];
+      if (first) {
+        const float deg = x0.w;
+        const float c0 = J.v0 * J.v0 + l2 * deg, c1 = J.v1 * J.v1 + l2 * deg, c2 = J.v2 * J.v2 + l2 * deg;
+        S[i] = make_float4(1.0f / (1.0f + sqrtf(c0)), 1.0f / (1.0f + sqrtf(c1)), 1.0f / (1.0f + sqrtf(c2)), 0.f);
+      }
+    }
+    block_reduce<2, 1>(v, red, sel);
+    cv = v[0]; ce = v[1]; gmax = v[2];
+  };
+
+  // ---- LM diagonal, block-Jacobi preconditioner, PCG start -------------
+  auto setup_linear = [&](bool reuse, double radius, double& rz_out, double& rr_out) {
+    double v[2] = {0.0, 0.0};
+    const double inv_radius = 1.0 / radius;
+    for (int i = tid; i < nv; i += T) {
+      const float4 g = G[i];
+      const float deg = X0[i].w;
+      const float4 s = S[i];
+      const float ld = l2 * deg;
+      float4 dg;
+      if (!reuse) {
+        const double c[3] = {(double)g.x * g.x + ld, (double)g.y * g.y + ld, (double)g.z * g.z + ld};
+        const double sc[3] = {s.x, s.y, s.z};
+        float d3[3];
+        for (int k = 0; k < 3; ++k) {
+          double t = sc[k] * sc[k] * c[k];
+          t = t < opt.min_lm_diagonal ? opt.min_lm_diagonal : (t > opt.max_lm_diagonal ? opt.max_lm_diagonal : t);
+          d3[k] = (float)t;
+        }
+        dg = make_float4(d3[0], d3[1], d3[2], 0.f);
+        DG[i] = dg;
+      } else {
+        dg = DG[i];
+      }
+      // D^2 in unscaled variables: diag / (radius s^2)
+      double t0 = (double)dg.x / ((double)s.x * s.x) * inv_radius;
+      double t1 = (double)dg.y / ((double)s.y * s.y) * inv_radius;
+      double t2 = (double)dg.z / ((double)s.z * s.z) * inv_radius;
+      const double cap = 1e30;
+      const float4 dt = make_float4((float)(t0 < cap ? t0 : cap), (float)(t1 < cap ? t1 : cap),
+                                    (float)(t2 < cap ? t2 : cap), 0.f);
+      DT[i] = dt;
+      // M = g g^T + lambda^2 deg I + D^2, inverted in FP64
+      const double a = (double)g.x * g.x + ld + dt.x, b = (double)g.x * g.y, c = (double)g.x * g.z;
+      const double d = (double)g.y * g.y + ld + dt.y, e = (double)g.y * g.z, f = (double)g.z * g.z + ld + dt.z;
+      const double A0 = d * f - e * e, B0 = c * e - b * f, C0 = b * e - c * d;
+      const double det = a * A0 + b * B0 + c * C0;
+      float4 m0, m1;
+      if (det > 0.0 && det < INFINITY) {
+        const double id = 1.0 / det;
+        m0 = make_float4((float)(A0 * id), (float)(B0 * id), (float)(C0 * id), (float)((a * f - c * c) * id));
+        m1 = make_float4((float)((b * c - a * e) * id), (float)((a * d - b * b) * id), 0.f, 0.f);
+      } else {
+        m0 = make_float4(a > 0 ? (float)(1.0 / a) : 0.f, 0.f, 0.f, d > 0 ? (float)(1.0 / d) : 0.f);
+        m1 = make_float4(0.f, f > 0 ? (float)(1.0 / f) : 0.f, 0.f, 0.f);
+      }
+      MI0[i] = m0;
+      MI1[i] = m1;
+      const float4 gr = GR[i];
+      const float r0 = -gr.x, r1 = -gr.y, r2 = -gr.z;
+      const float z0 = m0.x * r0 + m0.y * r1 + m0.z * r2;
+      const float z1 = m0.y * r0 + m0.w * r1 + m1.x * r2;
+      const float z2 = m0.z * r0 + m1.x * r1 + m1.y * r2;
+      R[i] = make_float4(r0, r1, r2, 0.f);
+      Z[i] = make_float4(z0, z1, z2, 0.f);
+      DL[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      v[0] += (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
+      v[1] += (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
+    }
+    block_reduce<2, 0>(v, red, sel);
+    rz_out = v[0];
+    rr_out = v[1];
+  };
+
+  // ---- matrix-free PCG: (J^T J + D^2) delta = -J^T f -------------------
+  // Iteration k reads p_old from PB[k&1], writes p = z + beta p_old to
+  // PB[(k+1)&1] (own vertex) while gathering neighbours' p on the fly, so
+  // the p update needs no pass or barrier of its own.
+  auto pcg = [&](double rz, double rr0) -> int {
+    const double tol2 = (double)opt.pcg_relative_tolerance * opt.pcg_relative_tolerance * rr0;
+    double beta = 0.0;
+    int k = 0;
+    if (!(rr0 > 0.0)) return 0;
+    for (; k < opt.pcg_max_iterations; ++k) {
+      float4* __restrict__ Pold = PB[k & 1];
+      float4* __restrict__ Pnew = PB[(k + 1) & 1];
+      const float fb = (float)beta;
+      const bool first = (k == 0);
+      double v1[1] = {0.0};
+      for (int i = tid; i < nv; i += T) {
+        const float4 zi = Z[i];
+        float4 pi = zi;
+        if (!first) { const float4 po = Pold[i]; pi.x += fb * po.x; pi.y += fb * po.y; pi.z += fb * po.z; }
+        const float4 g = G[i];
+        const float4 dt = DT[i];
+        const float gp = g.x * pi.x + g.y * pi.y + g.z * pi.z;
+        float lx = 0.f, ly = 0.f, lz = 0.f;
+        const int2 a = ADJ[i];
+        for (int e = 0; e < a.y; ++e) {
+          const unsigned en = NB[a.x + e];
+          const unsigned j = nb_index(en);
+          const float m = nb_mult(en);
+          float4 pj = Z[j];
+          if (!first) { const float4 po = Pold[j]; pj.x += fb * po.x; pj.y += fb * po.y; pj.z += fb * po.z; }
+          lx += m * (pi.x - pj.x); ly += m * (pi.y - pj.y); lz += m * (pi.z - pj.z);
+        }
+        const float qx = g.x * gp + l2 * lx + dt.x * pi.x;
+        const float qy = g.y * gp + l2 * ly + dt.y * pi.y;
+        const float qz = g.z * gp + l2 * lz + dt.z * pi.z;
+        Pnew[i] = make_float4(pi.x, pi.y, pi.z, 0.f);
+        Q[i] = make_float4(qx, qy, qz, 0.f);
+        v1[0] += (double)pi.x * qx + (double)pi.y * qy + (double)pi.z * qz;
+      }
+      block_reduce<1, 0>(v1, red, sel);
+      const double pq = v1[0];
+      if (!(pq > 0.0)) break;
+      const double alpha = rz / pq;
+      const float fa = (float)alpha;
+      double v2[2] = {0.0, 0.0};
+      for (int i = tid; i < nv; i += T) {
+        const float4 pi = Pnew[i];
+        const float4 qi = Q[i];
+        float4 d = DL[i];
+        d.x += fa * pi.x; d.y += fa * pi.y; d.z += fa * pi.z;
+        DL[i] = d;
+        float4 r = R[i];
+        r.x -= fa * qi.x; r.y -= fa * qi.y; r.z -= fa * qi.z;
+        R[i] = r;
+        const float4 m0 = MI0[i], m1 = MI1[i];
+        const float z0 = m0.x * r.x + m0.y * r.y + m0.z * r.z;
+        const float z1 = m0.y * r.x + m0.w * r.y + m1.x * r.z;
+        const float z2 = m0.z * r.x + m1.x * r.y + m1.y * r.z;
+        Z[i] = make_float4(z0, z1, z2, 0.f);
+        v2[0] += (double)r.x * z0 + (double)r.y * z1 + (double)r.z * z2;
+        v2[1] += (double)r.x * r.x + (double)r.y * r.y + (double)r.z * r.z;
+      }
+      block_reduce<2, 0>(v2, red, sel);
+      beta = v2[0] / rz;
+      rz = v2[0];
+      if (v2[1] <= tol2) { ++k; break; }
+    }
+    return k;
+  };
+
+  // ---- model cost change, candidate cost, step and parameter norms ------
+  auto eval_candidate = [&](double& mc, double& ccv, double& cce, double& sn, double& xn) {
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = tid; i < nv; i += T) {
+      const float4 x0 = X0[i];
+      const float4 ui = U[i];
+      const float4 di = DL[i];
+      const float4 g = G[i];
+      const float gd = g.x * di.x + g.y * di.y + g.z * di.z;
+      v[0] += (double)gd * ((double)g.w + 0.5 * (double)gd);
+      const float cx = ui.x + di.x, cy = ui.y + di.y, cz = ui.z + di.z;
+      const float dc = lookup_f32(grid, N, x0.x + cx, x0.y + cy, x0.z + cz);
+      v[1] += 0.5 * (double)dc * (double)dc;
+      double mr = 0.0, ce = 0.0;
+      const int2 a = ADJ[i];
+      for (int e = 0; e < a.y; ++e) {
+        const unsigned en = NB[a.x + e];
+        const unsigned j = nb_index(en);
+        const float m = nb_mult(en);
+        const float4 uj = U[j];
+        const float4 dj = DL[j];
+        const float ex = ui.x - uj.x, ey = ui.y - uj.y, ez = ui.z - uj.z;
+        const float fx = di.x - dj.x, fy = di.y - dj.y, fz = di.z - dj.z;
+        mr += (double)m * ((double)fx * (ex + 0.5f * fx) + (double)fy * (ey + 0.5f * fy) +
+                           (double)fz * (ez + 0.5f * fz));
+        const float hx = cx - (uj.x + dj.x), hy = cy - (uj.y + dj.y), hz = cz - (uj.z + dj.z);
+        ce += (double)m * ((double)hx * hx + (double)hy * hy + (double)hz * hz);
+      }
+      v[0] += 0.5 * (double)l2 * mr;
+      v[2] += 0.25 * (double)l2 * ce;
+      v[3] += (double)di.x * di.x + (double)di.y * di.y + (double)di.z * di.z;
+      const float px = x0.x + ui.x, py = x0.y + ui.y, pz = x0.z + ui.z;
+      v[4] += (double)px * px + (double)py * py + (double)pz * pz;
+    }
+    block_reduce<5, 0>(v, red, sel);
+    mc = v[0]; ccv = v[1]; cce = v[2]; sn = sqrt(v[3]); xn = sqrt(v[4]);
+  };
+
+  // ---- trust-region loop (state replicated in every thread) -------------
+  double cv, ce, gmax;
+  eval_jacobian(true, cv, ce, gmax);
+  double cost = cv + ce;
+  const double initial_cost = cost;
+  double radius = opt.initial_trust_region_radius, decrease = 2.0;
+  bool reuse = false, last_success = true;
+  int iter = 0, invalid = 0, nsucc = 0, nfail = 0, term = 1;
+  long long pcg_total = 0;
+  while (true) {
+    if (iter >= opt.max_num_iterations) { term = 1; break; }
+    if (last_success && gmax <= opt.gradient_tolerance) { term = 0; break; }
+    if (radius <= opt.min_trust_region_radius) { term = 0; break; }
+    ++iter;
+    double rz, rr;
+    setup_linear(reuse, radius, rz, rr);
+    reuse = true;
+    pcg_total += pcg(rz, rr);
+    double mc, ccv, cce, sn, xn;
+    eval_candidate(mc, ccv, cce, sn, xn);
+    const double mcc = -mc;
+    const double ccost = ccv + cce;
+    if (!(mcc > 0.0) || !(ccost < INFINITY)) {
+      if (++invalid >= opt.max_consecutive_invalid_steps) { term = 2; break; }
+      radius /= decrease; decrease *= 2.0;
+      last_success = false; ++nfail;
+      continue;
+    }
+    invalid = 0;
+    if (sn <= opt.parameter_tolerance * (xn + opt.parameter_tolerance)) { term = 0; break; }
+    if (fabs(cost - ccost) <= opt.function_tolerance * cost) { term = 0; break; }
+    const double rho = (cost - ccost) / mcc;
+    if (rho > opt.min_relative_decrease) {
+      for (int i = tid; i < nv; i += T) {
+        float4 ui = U[i];
+        const float4 di = DL[i];
+        ui.x += di.x; ui.y += di.y; ui.z += di.z;
+        U[i] = ui;
+      }
+      __syncthreads();
+      eval_jacobian(false, cv, ce, gmax);
+      cost = cv + ce;
+      const double t = 2.0 * rho - 1.0;
+      const double f = 1.0 - t * t * t;
+      radius = radius / (f > 1.0 / 3.0 ? f : 1.0 / 3.0);
+      radius = radius < opt.max_trust_region_radius ? radius : opt.max_trust_region_radius;
+      decrease = 2.0; reuse = false; last_success = true; ++nsucc;
+    } else {
+      radius /= decrease; decrease *= 2.0; reuse = true; last_success = false; ++nfail;
+    }
+  }
+  if (tid == 0) {
+    LMResultDev r;
+    r.initial_cost = initial_cost;
+    r.final_cost = cv + ce;
+    r.vertex_cost = cv;
+    r.rigidity_cost = ce;
+    r.iterations = iter;
+    r.successful_steps = nsucc;
+    r.unsuccessful_steps = nfail;
+    r.termination = term;
+    r.pcg_iterations = pcg_total;
+    res[blockIdx.x] = r;
+  }
+}
+
+}  // namespace
+
+int lm_solve_launch(const LMPairDesc* pairs_dev, int npairs, const LMArrays& arrays, const LMOptionsDev& opt,
+                    LMResultDev* results_dev, long long total_vertices, int block_threads, hipStream_t s) {
+  if (npairs <= 0) return MO_OK;
+  MO_CHECK(block_threads >= 64 && block_threads <= 1024 && block_threads % 64 == 0, MO_ERR_ARG,
+           "block_threads must be a multiple of 64 in [64, 1024] (got %d)", block_threads);
+  hipLaunchKernelGGL(k_lm_solve, dim3(npairs), dim3(block_threads), 0, s, pairs_dev, arrays, opt, results_dev,
+                     total_vertices);
+  MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+}  // namespace mo

@@ -1,0 +1,125 @@
+"""Levenberg-Marquardt deformation on MI355X (Deformer, src/lib/deformer.h:11-30).
+
+``deform_batch`` runs Deformer::Deform (faces) / Deformer::DeformGraph
+(edges) for many pairs in one persistent launch; ``rigid_deform_batch`` runs
+the whole rigid_deform pipeline (rigid_deform.cc:49-61: normalize target,
+build its distance grid, move the source into the target frame, solve,
+return the result in the target's original frame) device-resident.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def default_options(**overrides) -> _lib.LMOptions:
+    o = _lib.default_lm_options()
+    for k, v in overrides.items():
+        if not hasattr(o, k):
+            raise AttributeError(f"unknown LM option {k!r}")
+        setattr(o, k, v)
+    return o
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        raise RuntimeError("deformation needs a ROCm GPU: torch.cuda.is_available() is False")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _as_dev(a, dtype):
+    if isinstance(a, np.ndarray):
+        a = torch.from_numpy(np.ascontiguousarray(a))
+    return a.to(device=_dev(), dtype=dtype).contiguous()
+
+
+def _arr(ctype, vals):
+    return (ctype * len(vals))(*vals)
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def deform_batch(param_ids, Vs, idxs, kinds, lambdas, options=None):
+    """Deformer::Deform / DeformGraph for each pair.
+
+    Vs: FP64 device tensors [nv,3] in the template's normalized frame,
+    updated in place.  idxs: int32 faces [n,3] (kind 0) or edges [n,2]
+    (kind 1).  Returns a list of stats dicts."""
+    n = len(param_ids)
+    Vs = list(Vs)
+    for V in Vs:
+        if V.dtype != torch.float64 or not V.is_cuda or not V.is_contiguous():
+            raise TypeError("deform_batch expects contiguous float64 device tensors")
+    I = [_as_dev(i, torch.int32) for i in idxs]
+    stats = (_lib.LMStats * n)()
+    opt = options or default_options()
+    _lib.call("mo_deform_batch", n,
+              _arr(C.c_int32, [int(p) for p in param_ids]),
+              _arr(C.c_void_p, [V.data_ptr() for V in Vs]),
+              _arr(C.c_int64, [V.shape[0] for V in Vs]),
+              _arr(C.c_void_p, [i.data_ptr() for i in I]),
+              _arr(C.c_int64, [i.shape[0] for i in I]),
+              _arr(C.c_int32, [int(k) for k in kinds]),
+              _arr(C.c_double, [float(l) for l in lambdas]),
+              C.byref(opt), stats, _stream())
+    return [s.as_dict() for s in stats]
+
+
+class PairBatch:
+    """Device-resident inputs of a rigid_deform batch (FP64 vertices, int32 faces)."""
+
+    def __init__(self, pairs, grid_resolution=64, lambdas=1.0):
+        self.src = [_as_dev(p[0], torch.float64) for p in pairs]
+        self.src_f = [_as_dev(p[1], torch.int32) for p in pairs]
+        self.tgt = [_as_dev(p[2], torch.float64) for p in pairs]
+        self.tgt_f = [_as_dev(p[3], torch.int32) for p in pairs]
+        n = len(pairs)
+        gr = grid_resolution if isinstance(grid_resolution, (list, tuple)) else [grid_resolution] * n
+        lam = lambdas if isinstance(lambdas, (list, tuple)) else [lambdas] * n
+        self.out = [torch.empty_like(s) for s in self.src]
+        self._args = (
+            n,
+            _arr(C.c_void_p, [t.data_ptr() for t in self.src]),
+            _arr(C.c_int64, [t.shape[0] for t in self.src]),
+            _arr(C.c_void_p, [t.data_ptr() for t in self.src_f]),
+            _arr(C.c_int64, [t.shape[0] for t in self.src_f]),
+            _arr(C.c_void_p, [t.data_ptr() for t in self.tgt]),
+            _arr(C.c_int64, [t.shape[0] for t in self.tgt]),
+            _arr(C.c_void_p, [t.data_ptr() for t in self.tgt_f]),
+            _arr(C.c_int64, [t.shape[0] for t in self.tgt_f]),
+            _arr(C.c_int32, [int(g) for g in gr]),
+            _arr(C.c_double, [float(l) for l in lam]),
+        )
+        self._out_ptrs = _arr(C.c_void_p, [t.data_ptr() for t in self.out])
+        self.n = n
+
+    def run(self, options=None):
+        opt = options or default_options()
+        stats = (_lib.LMStats * self.n)()
+        _lib.call("mo_rigid_deform_batch", *self._args, C.byref(opt), self._out_ptrs, stats, _stream())
+        return [s.as_dict() for s in stats]
+
+
+def rigid_deform_batch(pairs, grid_resolution=64, lambdas=1.0, options=None):
+    """Run the rigid_deform pipeline on [(src_V, src_F, tgt_V, tgt_F), ...].
+    Returns (list of deformed source vertex tensors in the target frame, stats)."""
+    b = PairBatch(pairs, grid_resolution, lambdas)
+    st = b.run(options)
+    return b.out, st
+
+
+def template_f64(V, F, grid_resolution):
+    """mo_template_create_f64: template from FP64 vertices (the CLI path,
+    Mesh::ReadOBJ keeps doubles).  Returns the param_id."""
+    Vd = _as_dev(V, torch.float64)
+    Fd = _as_dev(F, torch.int32)
+    pid = C.c_int32(-1)
+    _lib.call("mo_template_create_f64", C.c_void_p(Vd.data_ptr()), Vd.shape[0], C.c_void_p(Fd.data_ptr()),
+              Fd.shape[0], 0, int(grid_resolution), _stream(), C.byref(pid))
+    return int(pid.value)

@@ -1,0 +1,259 @@
+"""Drop-in replacement for the reference's ``pyDeform`` module on MI355X.
+
+Same function names, positional signatures and semantics as
+``PYBIND11_MODULE(pyDeform, m)`` (src/interface/pydeform.cc:14-39); every
+computation runs in libmeshode.so (HIP, gfx950) through its C ABI.
+
+Differences from the reference, all on inputs the reference leaves
+undefined:
+  * tensors may live on the CPU or on a ROCm device; results come back on
+    the input's device (the reference requires CPU tensors,
+    distance_layer.cc:21);
+  * non-contiguous tensors are handled through ``.contiguous()`` (the
+    reference reads ``storage().data()`` and ignores offsets and strides);
+  * wrong dtypes, unknown ``param_id`` values, out-of-range indices and
+    loss calls before the matching Store call raise instead of reading
+    out of bounds.
+Functions of the CAD / linear-solve path (``LoadCadMesh``, ``SolveLinear``,
+``Cad*``) are outside the deformation hot path and raise
+``NotImplementedError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from . import objio
+
+__all__ = [
+    "LoadMesh", "LoadCadMesh", "SaveMesh",
+    "InitializeDeformTemplate", "NormalizeByTemplate", "DenormalizeByTemplate", "SolveLinear",
+    "DistanceFieldLoss_forward", "DistanceFieldLoss_backward",
+    "RigidEdgeLoss_forward", "RigidEdgeLoss_backward", "StoreRigidityInformation",
+    "CadEdgeLoss_forward", "CadEdgeLoss_backward", "StoreCadInformation",
+    "GraphEdgeLoss_forward", "GraphEdgeLoss_backward", "StoreGraphInformation",
+]
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("pyDeform (MI355X) needs a ROCm GPU: torch.cuda.is_available() is False")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else C.c_void_p(0)
+
+
+def _check(t, dtype, name, width):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype} (got {t.dtype})")
+    if t.dim() != 2 or t.shape[1] != width:
+        if not (t.numel() == 0):
+            raise ValueError(f"{name} must have shape [n,{width}] (got {tuple(t.shape)})")
+
+
+def _to_dev(t):
+    """Contiguous device view of t (a copy when t is on the CPU)."""
+    dev = _device()
+    if t.device != dev:
+        return t.detach().to(dev).contiguous()
+    return t.detach().contiguous()
+
+
+# Device copies of CPU index tensors, keyed by storage pointer, size and
+# version counter, so that repeated calls with the same F/E pass the same
+# device buffer (the library caches its incidence lists by that buffer).
+_index_cache: dict = {}
+
+
+def _index_to_dev(t):
+    dev = _device()
+    if t.device == dev:
+        return t.contiguous()
+    key = (t.data_ptr(), t.numel(), t._version, str(t.dtype))
+    hit = _index_cache.get(key)
+    if hit is not None:
+        return hit
+    d = t.detach().to(dev).contiguous()
+    if len(_index_cache) > 32:
+        _index_cache.clear()
+    _index_cache[key] = d
+    return d
+
+
+def _nrows(t):
+    return int(t.shape[0]) if t.dim() > 0 else 0
+
+
+# ---------------------------------------------------------------------------
+# Mesh I/O (mesh_tensor.cc:87-100, 180-186; mesh.cc:14-64)
+
+def LoadMesh(filename):
+    """Returns [V float32 [n,3] (raw, not normalized), F int32 [m,3]]."""
+    V, F = objio.read_obj(filename)
+    return [torch.from_numpy(V.astype("float32")), torch.from_numpy(F.astype("int32"))]
+
+
+def SaveMesh(filename, tensorV, tensorF):
+    """Writes `v x y z` / 1-based `f a b c` lines (Mesh::WriteOBJ)."""
+    V = tensorV.detach().cpu().contiguous().double().numpy()
+    F = tensorF.detach().cpu().contiguous().numpy()
+    objio.write_obj(filename, V, F)
+
+
+def LoadCadMesh(filename):
+    raise NotImplementedError("LoadCadMesh (CAD subdivision + representative graph) is outside the "
+                              "deformation hot path of this build")
+
+
+def SolveLinear(*args):
+    raise NotImplementedError("SolveLinear (linear_layer.cc) is outside the deformation hot path of this build")
+
+
+def CadEdgeLoss_forward(*args):
+    raise NotImplementedError("CadEdgeLoss_* is outside the deformation hot path of this build")
+
+
+CadEdgeLoss_backward = CadEdgeLoss_forward
+StoreCadInformation = CadEdgeLoss_forward
+
+
+# ---------------------------------------------------------------------------
+# Templates (deform_params.cc:16-40, normalize.cc:5-45)
+
+def InitializeDeformTemplate(tensorV, tensorF, symmetry, grid_resolution):
+    """Normalizes the target and builds its grid_resolution^3 distance grid.
+    Returns the integer param_id."""
+    _check(tensorV, torch.float32, "tensorV", 3)
+    _check(tensorF, torch.int32, "tensorF", 3)
+    V = _to_dev(tensorV)
+    F = _to_dev(tensorF)
+    pid = C.c_int32(-1)
+    _lib.call("mo_template_create_f32", _ptr(V), _nrows(V), _ptr(F), _nrows(F), int(symmetry),
+              int(grid_resolution), _stream(), C.byref(pid))
+    return int(pid.value)
+
+
+def _inplace(name, tensorV, param_id):
+    _check(tensorV, torch.float32, "tensorV", 3)
+    V = _to_dev(tensorV)
+    _lib.call(name, _ptr(V), _nrows(V), int(param_id), _stream())
+    if V.data_ptr() != tensorV.data_ptr():
+        tensorV.data.copy_(V.to(tensorV.device).view_as(tensorV))
+
+
+def NormalizeByTemplate(tensorV, param_id):
+    _inplace("mo_normalize_by_template", tensorV, param_id)
+
+
+def DenormalizeByTemplate(tensorV, param_id):
+    _inplace("mo_denormalize_by_template", tensorV, param_id)
+
+
+# ---------------------------------------------------------------------------
+# Distance layer (distance_layer.cc:8-80)
+
+def _distance(name, tensorV, param_id, width):
+    _check(tensorV, torch.float32, "tensorV", 3)
+    V = _to_dev(tensorV)
+    n = _nrows(V)
+    out = torch.empty((n,) if width == 1 else (n, 3), dtype=torch.float32, device=V.device)
+    _lib.call(name, _ptr(V), n, int(param_id), _ptr(out), _stream())
+    return out.to(tensorV.device)
+
+
+def DistanceFieldLoss_forward(tensorV, param_id):
+    return _distance("mo_distance_loss_forward", tensorV, param_id, 1)
+
+
+def DistanceFieldLoss_backward(tensorV, param_id):
+    return _distance("mo_distance_loss_backward", tensorV, param_id, 3)
+
+
+# ---------------------------------------------------------------------------
+# Rigid (faces) and graph (edges) layers (rigid_layer.cc, graph_layer.cc)
+
+def _store(name, tensorV, tensorI, width, param_id):
+    _check(tensorV, torch.float32, "tensorV", 3)
+    _check(tensorI, torch.int32, "index tensor", width)
+    V = _to_dev(tensorV)
+    I = _index_to_dev(tensorI)
+    _lib.call(name, _ptr(V), _nrows(V), _ptr(I), _nrows(I), int(param_id), _stream())
+
+
+def _edge_fwd(name, tensorV, tensorI, width, param_id):
+    _check(tensorV, torch.float32, "tensorV", 3)
+    _check(tensorI, torch.int32, "index tensor", width)
+    V = _to_dev(tensorV)
+    I = _index_to_dev(tensorI)
+    nh = _nrows(I) * (3 if width == 3 else 1)
+    out = torch.empty((nh, 3), dtype=torch.float32, device=V.device)
+    _lib.call(name, _ptr(V), _nrows(V), _ptr(I), _nrows(I), int(param_id), _ptr(out), _stream())
+    return out.to(tensorV.device)
+
+
+def _edge_bwd(name, tensorV, tensorI, width, param_id):
+    _check(tensorV, torch.float32, "tensorV", 3)
+    _check(tensorI, torch.int32, "index tensor", width)
+    V = _to_dev(tensorV)
+    I = _index_to_dev(tensorI)
+    out = torch.empty((_nrows(V), 3), dtype=torch.float32, device=V.device)
+    _lib.call(name, _ptr(V), _nrows(V), _ptr(I), _nrows(I), int(param_id), _ptr(out), _stream())
+    return out.to(tensorV.device)
+
+
+def StoreRigidityInformation(tensorV, tensorF, param_id):
+    _store("mo_rigidity_store", tensorV, tensorF, 3, param_id)
+
+
+def RigidEdgeLoss_forward(tensorV, tensorF, param_id):
+    return _edge_fwd("mo_rigid_loss_forward", tensorV, tensorF, 3, param_id)
+
+
+def RigidEdgeLoss_backward(tensorV, tensorF, param_id):
+    return _edge_bwd("mo_rigid_loss_backward", tensorV, tensorF, 3, param_id)
+
+
+def StoreGraphInformation(tensorV, tensorE, param_id):
+    _store("mo_graph_store", tensorV, tensorE, 2, param_id)
+
+
+def GraphEdgeLoss_forward(tensorV, tensorE, param_id):
+    return _edge_fwd("mo_graph_loss_forward", tensorV, tensorE, 2, param_id)
+
+
+def GraphEdgeLoss_backward(tensorV, tensorE, param_id):
+    return _edge_bwd("mo_graph_loss_backward", tensorV, tensorE, 2, param_id)
+
+
+# ---------------------------------------------------------------------------
+# Extras of this build (no pybind counterpart): template grid access.
+
+def TemplateInfo(param_id):
+    """(grid_resolution, scale, trans[3]) of a template."""
+    import numpy as np
+    N = C.c_int32()
+    sc = C.c_double()
+    tr = np.zeros(3)
+    _lib.call("mo_template_info", int(param_id), C.byref(N), C.byref(sc), C.c_void_p(tr.ctypes.data))
+    return int(N.value), float(sc.value), tr
+
+
+def TemplateGrid(param_id):
+    """(dist64, dist32, nearest) device tensors of shape [N,N,N] ([z][y][x])."""
+    N, _, _ = TemplateInfo(param_id)
+    dev = _device()
+    d64 = torch.empty((N, N, N), dtype=torch.float64, device=dev)
+    d32 = torch.empty((N, N, N), dtype=torch.float32, device=dev)
+    nn = torch.empty((N, N, N), dtype=torch.int32, device=dev)
+    _lib.call("mo_template_grid", int(param_id), _ptr(d64), _ptr(d32), _ptr(nn), _stream())
+    return d64, d32, nn
