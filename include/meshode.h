@@ -138,9 +138,10 @@ typedef struct mo_lm_options {
   double min_trust_region_radius;        /* 1e-32 */
   double min_lm_diagonal;                /* 1e-6 */
   double max_lm_diagonal;                /* 1e32 */
-  int32_t pcg_max_iterations;            /* inner PCG cap per LM step */
-  float pcg_relative_tolerance;          /* ||r|| <= tol ||r0|| ends PCG */
-  int32_t block_threads;                 /* threads per pair (multiple of 64, <= 1024) */
+  int32_t pcg_max_iterations;            /* inner PCG cap per LM step (1000) */
+  float pcg_relative_tolerance;          /* ||r|| <= tol ||r0|| ends PCG (1e-5) */
+  int32_t block_threads;                 /* threads per pair (multiple of 64, <= 1024; 256) */
+  int32_t timing;                        /* nonzero: record phase times (mo_last_timings) */
 } mo_lm_options;
 
 typedef struct mo_lm_stats {
@@ -153,6 +154,8 @@ typedef struct mo_lm_stats {
   int32_t unsuccessful_steps;
   int32_t termination;   /* 0 convergence, 1 iteration cap, 2 failure */
   int64_t pcg_iterations;
+  int64_t clock_start;   /* device wall clock (ticks) when the pair's solve began */
+  int64_t clock_end;     /* ... and ended; see mo_last_timings for the rate */
 } mo_lm_stats;
 
 /* Fills the defaults listed above. */
@@ -184,6 +187,15 @@ int mo_rigid_deform_batch(int32_t npairs, const double* const* src, const int64_
                           const int64_t* tgt_nv, const int32_t* const* tgt_faces, const int64_t* tgt_nf,
                           const int32_t* grid_resolution, const double* lambda, const mo_lm_options* opt,
                           double* const* out, mo_lm_stats* stats, void* stream);
+
+/* Phase times of the calling thread's last mo_deform_batch /
+ * mo_rigid_deform_batch call made with opt->timing set, from HIP events on
+ * the call's stream: out[0] distance-grid builds (ms), out[1] topology and
+ * source setup (ms), out[2] the LM solver kernel (ms), out[3] whole call
+ * (ms), out[4] device wall clock (ticks) at the start of the call, out[5]
+ * wall clock rate (Hz).  No counterpart in the reference (its only timing
+ * is Ceres' progress output, deformer.cc:58). */
+int mo_last_timings(double* out6);
 
 #ifdef __cplusplus
 }

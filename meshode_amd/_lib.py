@@ -22,7 +22,7 @@ SYMBOLS = [
     "mo_distance_loss_forward", "mo_distance_loss_backward",
     "mo_rigidity_store", "mo_rigid_loss_forward", "mo_rigid_loss_backward",
     "mo_graph_store", "mo_graph_loss_forward", "mo_graph_loss_backward",
-    "mo_lm_default_options", "mo_deform_batch", "mo_rigid_deform_batch",
+    "mo_lm_default_options", "mo_deform_batch", "mo_rigid_deform_batch", "mo_last_timings",
 ]
 
 
@@ -50,6 +50,7 @@ class LMOptions(C.Structure):
         ("pcg_max_iterations", C.c_int32),
         ("pcg_relative_tolerance", C.c_float),
         ("block_threads", C.c_int32),
+        ("timing", C.c_int32),
     ]
 
 
@@ -64,6 +65,8 @@ class LMStats(C.Structure):
         ("unsuccessful_steps", C.c_int32),
         ("termination", C.c_int32),
         ("pcg_iterations", C.c_int64),
+        ("clock_start", C.c_int64),
+        ("clock_end", C.c_int64),
     ]
 
     def as_dict(self):
@@ -97,6 +100,7 @@ _SIGS = {
     "mo_deform_batch": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(LMOptions), _vp, _vp]),
     "mo_rigid_deform_batch": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                         C.POINTER(LMOptions), _vp, _vp, _vp]),
+    "mo_last_timings": (C.c_int, [_vp]),
 }
 
 
@@ -125,6 +129,16 @@ def call(name, *args):
     if rc != 0:
         raise MoError(name, rc, lib.mo_last_error().decode(errors="replace"))
     return rc
+
+
+def last_timings():
+    """Phase times of this thread's last batch call made with options.timing=1:
+    dict(grid_ms, setup_ms, lm_ms, total_ms, clock_start, clock_hz)."""
+    import numpy as np
+    out = np.zeros(6)
+    call("mo_last_timings", C.c_void_p(out.ctypes.data))
+    return dict(grid_ms=out[0], setup_ms=out[1], lm_ms=out[2], total_ms=out[3], clock_start=out[4],
+                clock_hz=out[5])
 
 
 def default_lm_options() -> LMOptions:

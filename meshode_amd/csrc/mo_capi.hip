@@ -24,6 +24,7 @@
 namespace mo {
 
 static thread_local std::string g_last_error;
+static thread_local double g_timings[6] = {0, 0, 0, 0, 0, 0};
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
@@ -309,6 +310,8 @@ int edge_backward(const float* V, int64_t nv, const int32_t* idx, int64_t n, int
 // ---------------------------------------------------------------------------
 // Batched LM pipeline.
 
+__global__ void k_stamp(long long* out) { *out = wall_clock64(); }
+
 __global__ void k_make_x0(const double* __restrict__ V, int nv, const float* __restrict__ deg, float4* __restrict__ x0,
                           float4* __restrict__ u) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -359,7 +362,53 @@ LMOptionsDev to_dev_options(const mo_lm_options& o) {
   return d;
 }
 
-int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_stats* stats, hipStream_t s) {
+// HIP events around the phases of a batch call (only when opt.timing is set).
+struct PhaseTimer {
+  bool on = false;
+  hipStream_t s = nullptr;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  long long* stamp = nullptr;   // device: wall clock at the start of the call
+  int start(bool enable, hipStream_t st) {
+    on = enable;
+    s = st;
+    if (!on) return MO_OK;
+    for (auto& e : ev) MO_HIP(hipEventCreate(&e));
+    MO_HIP(hipMallocAsync((void**)&stamp, sizeof(long long), s));
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, stamp);
+    MO_HIP(hipEventRecord(ev[0], s));
+    return MO_OK;
+  }
+  int mark(int i) {
+    if (on) MO_HIP(hipEventRecord(ev[i], s));
+    return MO_OK;
+  }
+  int finish() {
+    if (!on) return MO_OK;
+    MO_HIP(hipEventRecord(ev[4], s));
+    MO_HIP(hipEventSynchronize(ev[4]));
+    float ms[4];
+    MO_HIP(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
+    MO_HIP(hipEventElapsedTime(&ms[1], ev[1], ev[2]));
+    MO_HIP(hipEventElapsedTime(&ms[2], ev[2], ev[3]));
+    MO_HIP(hipEventElapsedTime(&ms[3], ev[0], ev[4]));
+    long long st = 0;
+    MO_HIP(hipMemcpy(&st, stamp, sizeof(long long), hipMemcpyDeviceToHost));
+    int dev = 0, rate_khz = 0;
+    MO_HIP(hipGetDevice(&dev));
+    MO_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+    for (int k = 0; k < 4; ++k) g_timings[k] = ms[k];
+    g_timings[4] = (double)st;
+    g_timings[5] = (double)rate_khz * 1000.0;
+    return MO_OK;
+  }
+  ~PhaseTimer() {
+    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    if (stamp) (void)hipFreeAsync(stamp, s);
+  }
+};
+
+int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_stats* stats, hipStream_t s,
+            PhaseTimer& timer) {
   const int np = (int)pairs.size();
   if (np == 0) return MO_OK;
   MO_CHECK(opt.block_threads >= 64 && opt.block_threads <= 1024 && opt.block_threads % 64 == 0, MO_ERR_ARG,
@@ -421,7 +470,9 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   }
   MO_HIP(hipGetLastError());
   LMArrays A{x0, u, adj, raw, work};
+  if ((rc = timer.mark(2))) return rc;
   if ((rc = lm_solve_launch(d_pairs, np, A, to_dev_options(opt), d_res, nvalloc, opt.block_threads, s))) return rc;
+  if ((rc = timer.mark(3))) return rc;
   for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
     if (c.nv == 0) continue;
@@ -448,6 +499,8 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
       o.unsuccessful_steps = r.unsuccessful_steps;
       o.termination = r.termination;
       o.pcg_iterations = r.pcg_iterations;
+      o.clock_start = r.clock_start;
+      o.clock_end = r.clock_end;
     }
   }
   return MO_OK;
@@ -594,6 +647,13 @@ MO_EXPORT void mo_lm_default_options(mo_lm_options* o) {
   o->pcg_max_iterations = 1000;
   o->pcg_relative_tolerance = 1e-5f;
   o->block_threads = 256;
+  o->timing = 0;
+}
+
+MO_EXPORT int mo_last_timings(double* out6) {
+  MO_CHECK(out6, MO_ERR_ARG, "null output");
+  for (int k = 0; k < 6; ++k) out6[k] = g_timings[k];
+  return MO_OK;
 }
 
 MO_EXPORT int mo_deform_batch(int32_t npairs, const int32_t* param_ids, double* const* V, const int64_t* nv,
@@ -612,7 +672,13 @@ MO_EXPORT int mo_deform_batch(int32_t npairs, const int32_t* param_ids, double* 
     MO_CHECK(idx[p] || nidx[p] == 0, MO_ERR_ARG, "pair %d: null index buffer", p);
     pairs[p] = CorePair{t->grid32.as<float>(), t->N, V[p], nv[p], idx[p], nidx[p], kind[p], lambda[p], nullptr, V[p]};
   }
-  return lm_core(pairs, o, stats, as_stream(stream));
+  PhaseTimer timer;
+  hipStream_t s = as_stream(stream);
+  int rc;
+  if ((rc = timer.start(o.timing != 0, s))) return rc;
+  if ((rc = timer.mark(1))) return rc;
+  if ((rc = lm_core(pairs, o, stats, s, timer))) return rc;
+  return timer.finish();
 }
 
 MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, const int64_t* src_nv,
@@ -644,6 +710,8 @@ MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, co
   bytes += al(gmax) + 256;
   Arena ar;
   int rc;
+  PhaseTimer timer;
+  if ((rc = timer.start(o.timing != 0, s))) return rc;
   if ((rc = ar.reserve(bytes, s))) return rc;
   int* err = ar.take<int>(1);
   void* gws = ar.take<char>(gmax);
@@ -661,6 +729,8 @@ MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, co
     if ((rc = apply_transform_launch<double>(src[p], src_nv[p], sp, sn, s))) return rc;
     pairs[p] = CorePair{g32, (int)N, sn, src_nv[p], src_faces[p], src_nf[p], 0, lambda[p], sp, out[p]};
   }
-  if ((rc = lm_core(pairs, o, stats, s))) return rc;
+  if ((rc = timer.mark(1))) return rc;
+  if ((rc = lm_core(pairs, o, stats, s, timer))) return rc;
+  if ((rc = timer.finish())) return rc;
   return check_err_flag(err, s, "target faces");
 }

@@ -5,12 +5,13 @@
 
 namespace mo {
 
-// FP64 face record for the point-triangle distance (200 bytes).
+// FP64 face record for the point-triangle distance (240 bytes).
 struct GFace {
   double ax, ay, az, bx, by, bz, cx, cy, cz;   // positions in face order
   double nx, ny, nz, inv_nn;                   // n = (b-a) x (c-a), 1/(n.n)
   double d00, d01, d11, denom;                 // Gram entries of (b-a, c-a)
   double ll0, ll1, ll2, il0, il1, il2;         // |edge j|^2 and its reciprocal
+  double lox, loy, loz, hix, hiy, hiz;         // face AABB (culling only)
   int flip;                                    // bit j: edge j endpoints swapped (index order)
   int face;                                    // original face index
 };

@@ -13,20 +13,25 @@
 // rounding, so results are bit-reproducible on the host.
 //
 // Kernel structure (HBM traffic is tiny; the build is FP64-VALU bound):
-//   1. k_face_prep   : per face, FP64 face record + AABB + Morton bin of the
-//                      centroid (16^3 bins).
-//   2. k_bin_sort    : one workgroup counting-sorts faces by bin (LDS
+//   1. k_face_bins   : per face, index check + Morton bin of the centroid
+//                      (16^3 bins).
+//   2. k_bin_sort    : one workgroup counting-sorts face indices by bin (LDS
 //                      histogram + scan) so 64-face tiles are spatially tight.
-//   3. k_tile_bounds : AABB per tile of 64 sorted faces.
-//   4. k_grid_build  : one wavefront per 4x4x4 brick of grid points.  The
-//                      wave first evaluates the tile closest to its brick,
-//                      then walks all tiles, culling every tile whose AABB is
-//                      farther from the brick AABB than the wave's current
-//                      worst best distance (ballot over 64 tiles at a time).
-//                      Surviving tiles are staged into LDS (one face record
-//                      per lane) and every lane tests its grid point against
-//                      all 64 faces.  Ties keep the lowest face index, so
-//                      the visiting order never changes the result.
+//   3. k_face_prep   : FP64 face record per face, written in sorted order.
+//   4. k_tile_bounds : AABB per tile of 32 sorted faces.
+//   5. k_grid_build  : one wavefront per 4x4x4 brick of grid points.  The
+//                      wave first evaluates the 4 tiles closest to its brick
+//                      (which seed every lane with a near-exact distance),
+//                      then walks all tiles 64 at a time: a brick-level
+//                      ballot drops tiles whose AABB is farther from the
+//                      brick than the wave's worst best distance, and each
+//                      remaining tile is kept only if some lane's own
+//                      point-to-AABB bound still reaches that lane's best.
+//                      For a kept tile the wave walks its 32 faces, whose
+//                      records are read with wave-uniform addresses (scalar
+//                      loads); a face is tested exactly only when its AABB
+//                      and plane bounds can still improve some lane.  Ties keep the lowest face index, so the
+//                      visiting order never changes the result.
 #include "mo_common.h"
 #include "mo_grid.h"
 
@@ -36,7 +41,8 @@ namespace mo {
 
 namespace {
 
-constexpr int kTile = 64;
+constexpr int kTile = 32;      // faces per culling tile
+constexpr int kSeedTiles = 4;  // nearest tiles evaluated before the culled walk
 constexpr int kBinBits = 4;                  // 16 bins per axis
 constexpr int kBins = 1 << (3 * kBinBits);   // 4096
 
@@ -50,17 +56,58 @@ __device__ inline unsigned spread3(unsigned v) {
 __device__ inline double dmin(double a, double b) { return b < a ? b : a; }
 __device__ inline double dmax(double a, double b) { return b > a ? b : a; }
 
-__global__ void k_face_prep(const double* __restrict__ V, long long nv, const int* __restrict__ F,
-                            long long nf, GFace* __restrict__ faces, unsigned* __restrict__ bin,
-                            int* __restrict__ err) {
+// Pass 1: validate indices and compute the centroid bin (Morton order of a
+// 16^3 lattice over [0,1]^3, clamped).
+__global__ void k_face_bins(const double* __restrict__ V, long long nv, const int* __restrict__ F, long long nf,
+                            unsigned* __restrict__ bin, int* __restrict__ err) {
   long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
+  double c[3] = {0.0, 0.0, 0.0};
+  for (int j = 0; j < 3; ++j) {
+    int vi = F[f * 3 + j];
+    if (vi < 0 || vi >= nv) { atomicOr(err, 1); vi = 0; }
+    for (int a = 0; a < 3; ++a) c[a] += V[(long long)vi * 3 + a];
+  }
+  unsigned b3[3];
+  for (int a = 0; a < 3; ++a) {
+    int b = (int)floor(c[a] * (1.0 / 3.0) * (1 << kBinBits));
+    b = b < 0 ? 0 : (b > (1 << kBinBits) - 1 ? (1 << kBinBits) - 1 : b);
+    b3[a] = (unsigned)b;
+  }
+  bin[f] = spread3(b3[0]) | (spread3(b3[1]) << 1) | (spread3(b3[2]) << 2);
+}
+
+// Pass 2: single-workgroup counting sort of face indices by bin.
+__global__ void __launch_bounds__(1024) k_bin_sort(const unsigned* __restrict__ bin, long long nf,
+                                                   int* __restrict__ perm) {
+  __shared__ unsigned hist[kBins];
+  for (int b = threadIdx.x; b < kBins; b += blockDim.x) hist[b] = 0;
+  __syncthreads();
+  for (long long f = threadIdx.x; f < nf; f += blockDim.x) atomicAdd(&hist[bin[f]], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned s = 0;
+    for (int b = 0; b < kBins; ++b) { unsigned c = hist[b]; hist[b] = s; s += c; }
+  }
+  __syncthreads();
+  for (long long f = threadIdx.x; f < nf; f += blockDim.x) {
+    unsigned pos = atomicAdd(&hist[bin[f]], 1u);
+    perm[pos] = (int)f;
+  }
+}
+
+// Pass 3: FP64 face record of face perm[pos], written at its sorted position.
+__global__ void k_face_prep(const double* __restrict__ V, long long nv, const int* __restrict__ F, long long nf,
+                            const int* __restrict__ perm, GFace* __restrict__ faces) {
+  long long pos = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= nf) return;
+  const long long f = perm[pos];
   GFace g;
   int idx[3];
   double p[3][3];
   for (int j = 0; j < 3; ++j) {
     int vi = F[f * 3 + j];
-    if (vi < 0 || vi >= nv) { atomicOr(err, 1); vi = 0; }
+    if (vi < 0 || vi >= nv) vi = 0;     // reported by k_face_bins
     idx[j] = vi;
     p[j][0] = V[(long long)vi * 3 + 0];
     p[j][1] = V[(long long)vi * 3 + 1];
@@ -91,37 +138,12 @@ __global__ void k_face_prep(const double* __restrict__ V, long long nv, const in
   }
   g.ll0 = ll[0]; g.ll1 = ll[1]; g.ll2 = ll[2];
   g.il0 = il[0]; g.il1 = il[1]; g.il2 = il[2];
+  g.lox = dmin(g.ax, dmin(g.bx, g.cx)); g.hix = dmax(g.ax, dmax(g.bx, g.cx));
+  g.loy = dmin(g.ay, dmin(g.by, g.cy)); g.hiy = dmax(g.ay, dmax(g.by, g.cy));
+  g.loz = dmin(g.az, dmin(g.bz, g.cz)); g.hiz = dmax(g.az, dmax(g.bz, g.cz));
   g.flip = flip;
   g.face = (int)f;
-  faces[f] = g;
-  // centroid bin (Morton order of a 16^3 lattice over [0,1]^3, clamped)
-  unsigned c[3];
-  for (int a = 0; a < 3; ++a) {
-    double m = (p[0][a] + p[1][a] + p[2][a]) * (1.0 / 3.0);
-    int b = (int)floor(m * (1 << kBinBits));
-    b = b < 0 ? 0 : (b > (1 << kBinBits) - 1 ? (1 << kBinBits) - 1 : b);
-    c[a] = (unsigned)b;
-  }
-  bin[f] = spread3(c[0]) | (spread3(c[1]) << 1) | (spread3(c[2]) << 2);
-}
-
-// Single-workgroup counting sort of face records by bin.
-__global__ void __launch_bounds__(1024) k_bin_sort(const GFace* __restrict__ in, const unsigned* __restrict__ bin,
-                                                   long long nf, GFace* __restrict__ out) {
-  __shared__ unsigned hist[kBins];
-  for (int b = threadIdx.x; b < kBins; b += blockDim.x) hist[b] = 0;
-  __syncthreads();
-  for (long long f = threadIdx.x; f < nf; f += blockDim.x) atomicAdd(&hist[bin[f]], 1u);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned s = 0;
-    for (int b = 0; b < kBins; ++b) { unsigned c = hist[b]; hist[b] = s; s += c; }
-  }
-  __syncthreads();
-  for (long long f = threadIdx.x; f < nf; f += blockDim.x) {
-    unsigned pos = atomicAdd(&hist[bin[f]], 1u);
-    out[pos] = in[f];
-  }
+  faces[pos] = g;
 }
 
 __global__ void k_tile_bounds(const GFace* __restrict__ faces, long long nf, double* __restrict__ tb) {
@@ -132,12 +154,9 @@ __global__ void k_tile_bounds(const GFace* __restrict__ faces, long long nf, dou
   long long e = (t + 1) * kTile < nf ? (t + 1) * kTile : nf;
   for (long long f = t * kTile; f < e; ++f) {
     const GFace& g = faces[f];
-    double xs[3] = {g.ax, g.bx, g.cx}, ys[3] = {g.ay, g.by, g.cy}, zs[3] = {g.az, g.bz, g.cz};
-    for (int j = 0; j < 3; ++j) {
-      lo[0] = dmin(lo[0], xs[j]); hi[0] = dmax(hi[0], xs[j]);
-      lo[1] = dmin(lo[1], ys[j]); hi[1] = dmax(hi[1], ys[j]);
-      lo[2] = dmin(lo[2], zs[j]); hi[2] = dmax(hi[2], zs[j]);
-    }
+    lo[0] = dmin(lo[0], g.lox); hi[0] = dmax(hi[0], g.hix);
+    lo[1] = dmin(lo[1], g.loy); hi[1] = dmax(hi[1], g.hiy);
+    lo[2] = dmin(lo[2], g.loz); hi[2] = dmax(hi[2], g.hiz);
   }
   for (int a = 0; a < 3; ++a) { tb[t * 6 + a] = lo[a]; tb[t * 6 + 3 + a] = hi[a]; }
 }
@@ -207,7 +226,6 @@ __global__ void __launch_bounds__(64) k_grid_build(const GFace* __restrict__ fac
                                                    const double* __restrict__ tb, long long ntiles, int N,
                                                    int bricks_per_axis, double* __restrict__ dist64,
                                                    float* __restrict__ dist32, int* __restrict__ nearest) {
-  __shared__ GFace tile[kTile];
   const int lane = threadIdx.x;
   long long b = blockIdx.x;
   const int bx = (int)(b % bricks_per_axis);
@@ -224,48 +242,75 @@ __global__ void __launch_bounds__(64) k_grid_build(const GFace* __restrict__ fac
   double best = INFINITY;
   int bidx = INT_MAX;
 
-  auto process_tile = [&](long long t) {
-    long long f = t * kTile + lane;
-    __syncthreads();
-    if (f < nf) tile[lane] = faces[f];
-    __syncthreads();
-    int cnt = (int)((nf - t * kTile) < kTile ? (nf - t * kTile) : kTile);
+  // Every lane tests the same face at the same time, so the face record is
+  // read with a wave-uniform address (scalar loads into SGPRs): no LDS
+  // staging and no barriers.
+  auto process_tile = [&](int t) {
+    const long long f0 = (long long)t * kTile;
+    const int cnt = (int)((nf - f0) < kTile ? (nf - f0) : kTile);
     for (int q = 0; q < cnt; ++q) {
-      const GFace& g = tile[q];
+      const GFace g = faces[f0 + q];
+      // face AABB bound: skip the exact test unless some lane can still
+      // improve or tie its best
+      double gx = dmax(0.0, dmax(g.lox - qx, qx - g.hix));
+      double gy = dmax(0.0, dmax(g.loy - qy, qy - g.hiy));
+      double gz = dmax(0.0, dmax(g.loz - qz, qz - g.hiz));
+      double lb = gx * gx + gy * gy + gz * gz;
+      // the face plane is a second lower bound (0 for degenerate faces)
+      const double dp = g.nx * (qx - g.ax) + g.ny * (qy - g.ay) + g.nz * (qz - g.az);
+      lb = dmax(lb, (dp * dp) * g.inv_nn);
+      if (!__ballot(valid && !(lb > best * (1.0 + 1e-9)))) continue;
       double d = face_d2(g, qx, qy, qz);
       if (d < best || (d == best && g.face < bidx)) { best = d; bidx = g.face; }
     }
   };
 
-  // 1. the tile closest to the brick first (tightens the cull bound early)
-  double mlb = INFINITY;
-  long long mt = -1;
-  for (long long c = 0; c < ntiles; c += 64) {
-    long long t = c + lane;
-    double lb = t < ntiles ? box_gap2(tb, t, lx, ly, lz, hx, hy, hz) : INFINITY;
-    if (lb < mlb) { mlb = lb; mt = t; }
+  // 1. the kSeedTiles tiles closest to the brick first: they seed every lane
+  //    with a near-exact distance, which makes the culling below tight
+  int seeds[kSeedTiles];
+  int nseed = 0;
+  for (int r = 0; r < kSeedTiles && r < ntiles; ++r) {
+    double mlb = INFINITY;
+    long long mt = -1;
+    for (long long c = 0; c < ntiles; c += 64) {
+      long long t = c + lane;
+      if (t >= ntiles) continue;
+      bool taken = false;
+      for (int q = 0; q < nseed; ++q) taken |= (seeds[q] == (int)t);
+      if (taken) continue;
+      double lb = box_gap2(tb, t, lx, ly, lz, hx, hy, hz);
+      if (lb < mlb) { mlb = lb; mt = t; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      double ol = __shfl_xor(mlb, o, 64);
+      long long ot = __shfl_xor(mt, o, 64);
+      if (ol < mlb || (ol == mlb && ot >= 0 && (mt < 0 || ot < mt))) { mlb = ol; mt = ot; }
+    }
+    if (mt < 0) break;
+    const int st = __builtin_amdgcn_readfirstlane((int)mt);
+    seeds[nseed++] = st;
+    process_tile(st);
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    double ol = __shfl_xor(mlb, o, 64);
-    long long ot = __shfl_xor(mt, o, 64);
-    if (ol < mlb || (ol == mlb && ot < mt)) { mlb = ol; mt = ot; }
-  }
-  if (mt >= 0) process_tile(mt);
 
-  // 2. every other tile whose lower bound can still improve some lane
+  // 2. every other tile that can still improve (or tie) some lane's best:
+  //    a brick-level prefilter (brick AABB to tile AABB against the wave's
+  //    worst best) selects candidates 64 at a time, then each candidate is
+  //    kept only if some lane's own point-to-AABB bound reaches its own best.
   for (long long c = 0; c < ntiles; c += 64) {
     long long t = c + lane;
+    bool taken = false;
+    for (int q = 0; q < nseed; ++q) taken |= (seeds[q] == (int)t);
     double lb = t < ntiles ? box_gap2(tb, t, lx, ly, lz, hx, hy, hz) : INFINITY;
     double bound = wave_max(valid ? best : -INFINITY);
-    bool need = (t < ntiles) && (t != mt) && !(lb > bound * (1.0 + 1e-9));
+    bool need = (t < ntiles) && !taken && !(lb > bound * (1.0 + 1e-9));
     unsigned long long mask = __ballot(need);
     while (mask) {
       int l = __ffsll((long long)mask) - 1;
       mask &= mask - 1;
-      double lbl = __shfl(lb, l, 64);
-      double bnd = wave_max(valid ? best : -INFINITY);
-      if (lbl > bnd * (1.0 + 1e-9)) continue;
-      process_tile(c + l);
+      const int tt = __builtin_amdgcn_readfirstlane((int)(c + l));
+      const double pl = box_gap2(tb, tt, qx, qy, qz, qx, qy, qz);
+      if (!__ballot(valid && !(pl > best * (1.0 + 1e-9)))) continue;
+      process_tile(tt);
     }
   }
   if (valid) {
@@ -329,8 +374,8 @@ __global__ void k_apply_transform(const T* __restrict__ V, long long n, const do
 
 int grid_workspace_bytes(long long nf, size_t* bytes) {
   long long ntiles = (nf + kTile - 1) / kTile;
-  *bytes = (size_t)nf * sizeof(GFace) * 2 + (size_t)nf * sizeof(unsigned) + (size_t)ntiles * 6 * sizeof(double) +
-           256 * 4;
+  *bytes = (size_t)nf * sizeof(GFace) + 256 + (size_t)nf * sizeof(unsigned) * 2 + 256 +
+           (size_t)ntiles * 6 * sizeof(double) + 256;
   return MO_OK;
 }
 
@@ -338,21 +383,22 @@ int grid_build_launch(const double* Vn, long long nv, const int* F, long long nf
                       float* dist32, int* nearest, void* workspace, hipStream_t stream, int* err_dev) {
   MO_CHECK(N >= 2 && N <= 2048, MO_ERR_ARG, "grid resolution %d out of range [2, 2048]", N);
   MO_CHECK(nf >= 1, MO_ERR_ARG, "distance field needs at least one triangle (got %lld)", nf);
+  auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
   char* w = (char*)workspace;
-  GFace* raw = (GFace*)w; w += (size_t)nf * sizeof(GFace);
-  GFace* sorted = (GFace*)w; w += (size_t)nf * sizeof(GFace);
+  GFace* faces = (GFace*)w; w += align((size_t)nf * sizeof(GFace));
+  unsigned* bin = (unsigned*)w; w += align((size_t)nf * sizeof(unsigned));
+  int* perm = (int*)w; w += align((size_t)nf * sizeof(int));
   double* tb = (double*)w;
   long long ntiles = (nf + kTile - 1) / kTile;
-  w += (size_t)ntiles * 6 * sizeof(double);
-  unsigned* bin = (unsigned*)w;
   const int T = 256;
-  hipLaunchKernelGGL(k_face_prep, dim3((unsigned)((nf + T - 1) / T)), dim3(T), 0, stream, Vn, nv, F, nf, raw, bin,
-                     err_dev);
-  hipLaunchKernelGGL(k_bin_sort, dim3(1), dim3(1024), 0, stream, raw, bin, nf, sorted);
-  hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((ntiles + T - 1) / T)), dim3(T), 0, stream, sorted, nf, tb);
+  const unsigned fb = (unsigned)((nf + T - 1) / T);
+  hipLaunchKernelGGL(k_face_bins, dim3(fb), dim3(T), 0, stream, Vn, nv, F, nf, bin, err_dev);
+  hipLaunchKernelGGL(k_bin_sort, dim3(1), dim3(1024), 0, stream, bin, nf, perm);
+  hipLaunchKernelGGL(k_face_prep, dim3(fb), dim3(T), 0, stream, Vn, nv, F, nf, perm, faces);
+  hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((ntiles + T - 1) / T)), dim3(T), 0, stream, faces, nf, tb);
   int bpa = (N + 3) / 4;
   long long nb = (long long)bpa * bpa * bpa;
-  hipLaunchKernelGGL(k_grid_build, dim3((unsigned)nb), dim3(64), 0, stream, sorted, nf, tb, ntiles, N, bpa, dist64,
+  hipLaunchKernelGGL(k_grid_build, dim3((unsigned)nb), dim3(64), 0, stream, faces, nf, tb, ntiles, N, bpa, dist64,
                      dist32, nearest);
   MO_HIP(hipGetLastError());
   return MO_OK;

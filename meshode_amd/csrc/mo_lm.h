@@ -45,6 +45,8 @@ struct LMResultDev {
   int unsuccessful_steps;
   int termination;      // 0 convergence, 1 iteration cap, 2 failure
   long long pcg_iterations;
+  long long clock_start;  // wall_clock64() at the start / end of the pair's solve
+  long long clock_end;
 };
 
 // Number of float4 scratch slots per vertex used by the solver.

@@ -102,6 +102,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
   float4* __restrict__ PB[2] = {W[kP0], W[kP1]};
   float4* __restrict__ Q = W[kQ];
   int sel = 0;
+  const long long clock_start = wall_clock64();
 
   // ---- evaluate residuals, gradient J^T f, cost at the current u --------
   auto eval_jacobian = [&](bool first, double& cv, double& ce, double& gmax) {
@@ -374,6 +375,8 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
     r.unsuccessful_steps = nfail;
     r.termination = term;
     r.pcg_iterations = pcg_total;
+    r.clock_start = clock_start;
+    r.clock_end = wall_clock64();
     res[blockIdx.x] = r;
   }
 }

@@ -70,37 +70,34 @@ def box_surface(center, extent, segs):
     """Closed, outward-oriented triangulated surface of an axis-aligned box
     with segs = (nx, ny, nz) segments per axis."""
     nx, ny, nz = (int(s) for s in segs)
+    n = np.array([nx, ny, nz])
     lo = np.asarray(center, float) - 0.5 * np.asarray(extent, float)
-    step = np.asarray(extent, float) / np.array([nx, ny, nz], float)
-    index = {}
-    verts = []
-
-    def vid(i, j, k):
-        key = (i, j, k)
-        if key not in index:
-            index[key] = len(verts)
-            verts.append(lo + step * np.array([i, j, k], float))
-        return index[key]
-
+    step = np.asarray(extent, float) / n
+    I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    on = (I == 0) | (I == nx) | (J == 0) | (J == ny) | (K == 0) | (K == nz)
+    idmap = -np.ones(on.shape, np.int64)
+    # vertex order: lattice order (i, j, k) restricted to the surface
+    idmap[on] = np.arange(int(on.sum()))
+    verts = lo + step * np.stack([I[on], J[on], K[on]], 1).astype(float)
     faces = []
-    n = (nx, ny, nz)
     for axis in range(3):
         u, v = (axis + 1) % 3, (axis + 2) % 3
+        A, B = np.meshgrid(np.arange(n[u]), np.arange(n[v]), indexing="ij")
+        A, B = A.reshape(-1), B.reshape(-1)
         for side in (0, 1):
-            for a in range(n[u]):
-                for b in range(n[v]):
-                    def P(da, db):
-                        c = [0, 0, 0]
-                        c[axis] = side * n[axis]
-                        c[u] = a + da
-                        c[v] = b + db
-                        return vid(*c)
-                    p00, p10, p11, p01 = P(0, 0), P(1, 0), P(1, 1), P(0, 1)
-                    if side == 1:
-                        faces += [(p00, p10, p11), (p00, p11, p01)]
-                    else:
-                        faces += [(p00, p11, p10), (p00, p01, p11)]
-    return np.array(verts), np.array(faces, np.int32)
+            def P(da, db):
+                c = [None, None, None]
+                c[axis] = np.full_like(A, side * n[axis])
+                c[u] = A + da
+                c[v] = B + db
+                return idmap[c[0], c[1], c[2]]
+            p00, p10, p11, p01 = P(0, 0), P(1, 0), P(1, 1), P(0, 1)
+            if side == 1:
+                t = np.stack([np.stack([p00, p10, p11], 1), np.stack([p00, p11, p01], 1)], 1)
+            else:
+                t = np.stack([np.stack([p00, p11, p10], 1), np.stack([p00, p01, p11], 1)], 1)
+            faces.append(t.reshape(-1, 3))
+    return verts, np.concatenate(faces).astype(np.int32)
 
 
 def _box_verts(s):

@@ -881,3 +881,116 @@ OR_API double mo_oracle_lm_cost(const double* grid, int N, const double* x, cons
     for (int k = 0; k < 3; ++k) P.rest[3 * h + k] = x0[3 * E[2 * h] + k] - x0[3 * E[2 * h + 1] + k];
   return lm_cost(P, x, vcost, ecost);
 }
+
+// ---------------------------------------------------------------------------
+// Grid build with an AABB tree over the faces, the way libigl's
+// point_mesh_squared_distance queries its AABB (mesh.cc:140).  Same per-face
+// arithmetic and tie rule as mo_oracle_build_grid, so the outputs are
+// identical; used as the CPU baseline's grid build.
+namespace {
+
+struct BvhNode {
+  double lo[3], hi[3];
+  int left, right;    // children (internal) or -1
+  int first, count;   // face range in the permuted order (leaf)
+};
+
+struct Bvh {
+  std::vector<BvhNode> nodes;
+  std::vector<int> perm;
+};
+
+int bvh_build(Bvh& B, const std::vector<FaceData>& faces, const std::vector<V3>& cen, int first, int count) {
+  BvhNode nd;
+  for (int a = 0; a < 3; ++a) { nd.lo[a] = INFINITY; nd.hi[a] = -INFINITY; }
+  double clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = first; i < first + count; ++i) {
+    const FaceData& f = faces[B.perm[i]];
+    for (int j = 0; j < 3; ++j) {
+      const double p[3] = {f.p[j].x, f.p[j].y, f.p[j].z};
+      for (int a = 0; a < 3; ++a) { nd.lo[a] = std::min(nd.lo[a], p[a]); nd.hi[a] = std::max(nd.hi[a], p[a]); }
+    }
+    const double c[3] = {cen[B.perm[i]].x, cen[B.perm[i]].y, cen[B.perm[i]].z};
+    for (int a = 0; a < 3; ++a) { clo[a] = std::min(clo[a], c[a]); chi[a] = std::max(chi[a], c[a]); }
+  }
+  nd.left = nd.right = -1;
+  nd.first = first;
+  nd.count = count;
+  int id = (int)B.nodes.size();
+  B.nodes.push_back(nd);
+  if (count <= 4) return id;
+  int ax = 0;
+  for (int a = 1; a < 3; ++a) if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+  int mid = first + count / 2;
+  std::nth_element(B.perm.begin() + first, B.perm.begin() + mid, B.perm.begin() + first + count, [&](int x, int y) {
+    const double cx = ax == 0 ? cen[x].x : (ax == 1 ? cen[x].y : cen[x].z);
+    const double cy = ax == 0 ? cen[y].x : (ax == 1 ? cen[y].y : cen[y].z);
+    return cx < cy || (cx == cy && x < y);
+  });
+  int l = bvh_build(B, faces, cen, first, mid - first);
+  int r = bvh_build(B, faces, cen, mid, first + count - mid);
+  B.nodes[id].left = l;
+  B.nodes[id].right = r;
+  return id;
+}
+
+inline double box_d2(const BvhNode& n, V3 q) {
+  const double p[3] = {q.x, q.y, q.z};
+  double s = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    double g = 0.0;
+    if (p[a] < n.lo[a]) g = n.lo[a] - p[a];
+    else if (p[a] > n.hi[a]) g = p[a] - n.hi[a];
+    s += g * g;
+  }
+  return s;
+}
+
+}  // namespace
+
+OR_API void mo_oracle_build_grid_bvh(const double* V, const int* F, long nf, int N, double* dist, float* dist32,
+                                     int* nearest) {
+  std::vector<FaceData> faces(nf);
+  std::vector<V3> cen(nf);
+  for (long f = 0; f < nf; ++f) {
+    faces[f] = make_face(V, F, f);
+    cen[f] = V3{(faces[f].p[0].x + faces[f].p[1].x + faces[f].p[2].x) / 3.0,
+                (faces[f].p[0].y + faces[f].p[1].y + faces[f].p[2].y) / 3.0,
+                (faces[f].p[0].z + faces[f].p[1].z + faces[f].p[2].z) / 3.0};
+  }
+  Bvh B;
+  B.perm.resize(nf);
+  for (long f = 0; f < nf; ++f) B.perm[f] = (int)f;
+  if (nf > 0) bvh_build(B, faces, cen, 0, (int)nf);
+  std::vector<int> stack;
+  const long total = (long)N * N * N;
+  for (long c = 0; c < total; ++c) {
+    int k = (int)(c % N), j = (int)((c / N) % N), i = (int)(c / ((long)N * N));
+    V3 q{(double)k / N, (double)j / N, (double)i / N};
+    double best = INFINITY;
+    int bi = -1;
+    stack.clear();
+    if (nf > 0) stack.push_back(0);
+    while (!stack.empty()) {
+      int id = stack.back();
+      stack.pop_back();
+      const BvhNode& n = B.nodes[id];
+      if (box_d2(n, q) > best * (1.0 + 1e-9)) continue;
+      if (n.left < 0) {
+        for (int t = n.first; t < n.first + n.count; ++t) {
+          int f = B.perm[t];
+          double d = point_face_d2(faces[f], q);
+          if (d < best || (d == best && f < bi)) { best = d; bi = f; }
+        }
+      } else {
+        double dl = box_d2(B.nodes[n.left], q), dr = box_d2(B.nodes[n.right], q);
+        if (dl <= dr) { stack.push_back(n.right); stack.push_back(n.left); }
+        else { stack.push_back(n.left); stack.push_back(n.right); }
+      }
+    }
+    double s = std::sqrt(best);
+    if (dist) dist[c] = s;
+    if (dist32) dist32[c] = (float)s;
+    if (nearest) nearest[c] = bi;
+  }
+}

@@ -269,3 +269,12 @@ def lm_cost(grid, N, x, x0, E, lam=1.0):
     tot = lib().mo_oracle_lm_cost(_p(grid), C.c_int(N), _p(x), _p(x0), C.c_int(len(x)), _p(E),
                                   C.c_int(len(E)), C.c_double(lam), C.byref(vc), C.byref(ec))
     return tot, vc.value, ec.value
+
+
+def build_grid_bvh(V, F, N):
+    """Same as build_grid (identical outputs) with an AABB tree over the faces."""
+    V = _f64(V); F = _i32(F)
+    n = N * N * N
+    d = np.zeros(n); d32 = np.zeros(n, np.float32); idx = np.zeros(n, np.int32)
+    lib().mo_oracle_build_grid_bvh(_p(V), _p(F), C.c_long(len(F)), C.c_int(N), _p(d), _p(d32), _p(idx))
+    return d, d32, idx
