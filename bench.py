@@ -64,7 +64,8 @@ def main():
     ap.add_argument("--verts", type=int, default=5000)
     ap.add_argument("--lam", type=float, default=1.0)
     ap.add_argument("--plain-pairs", type=int, default=8, help="pairs run one at a time for the plain GPU rate")
-    ap.add_argument("--block-threads", type=int, default=256)
+    ap.add_argument("--block-threads", type=int, default=1024, help="threads per pair of the global-memory kernel")
+    ap.add_argument("--kernel-variant", type=int, default=0, help="0: register/LDS kernel when a pair fits; 1: global")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -90,7 +91,7 @@ def main():
           flush=True)
 
     batch = deform.PairBatch(pairs, grid_resolution=args.grid, lambdas=args.lam)
-    opt = deform.default_options(block_threads=args.block_threads, timing=1)
+    opt = deform.default_options(block_threads=args.block_threads, timing=1, kernel_variant=args.kernel_variant)
     torch.cuda.synchronize()
 
     for w in range(args.warmup):
@@ -164,6 +165,8 @@ def main():
             "lambda": args.lam,
             "lm_max_iterations": 100,
             "block_threads": args.block_threads,
+            "kernel_variant": args.kernel_variant,
+            "lm_shapes": os.environ.get("MO_LM_SHAPES", "default"),
         },
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max())},

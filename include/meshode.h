@@ -140,8 +140,10 @@ typedef struct mo_lm_options {
   double max_lm_diagonal;                /* 1e32 */
   int32_t pcg_max_iterations;            /* inner PCG cap per LM step (1000) */
   float pcg_relative_tolerance;          /* ||r|| <= tol ||r0|| ends PCG (1e-5) */
-  int32_t block_threads;                 /* threads per pair (multiple of 64, <= 1024; 256) */
+  int32_t block_threads;                 /* threads per pair of the global-memory kernel (1024) */
   int32_t timing;                        /* nonzero: record phase times (mo_last_timings) */
+  int32_t kernel_variant;                /* 0: register/LDS kernel when a pair fits, else the
+                                            global-memory kernel; 1: global-memory kernel only */
 } mo_lm_options;
 
 typedef struct mo_lm_stats {

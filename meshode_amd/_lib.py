@@ -51,6 +51,7 @@ class LMOptions(C.Structure):
         ("pcg_relative_tolerance", C.c_float),
         ("block_threads", C.c_int32),
         ("timing", C.c_int32),
+        ("kernel_variant", C.c_int32),
     ]
 
 

@@ -407,13 +407,26 @@ struct PhaseTimer {
   }
 };
 
+int device_lds_limit() {
+  static int cached = -1;
+  if (cached < 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
+      cached = v;
+    else
+      cached = 65536;
+  }
+  return cached;
+}
+
 int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_stats* stats, hipStream_t s,
             PhaseTimer& timer) {
   const int np = (int)pairs.size();
   if (np == 0) return MO_OK;
   MO_CHECK(opt.block_threads >= 64 && opt.block_threads <= 1024 && opt.block_threads % 64 == 0, MO_ERR_ARG,
            "block_threads must be a multiple of 64 in [64, 1024]");
-  long long nvtot = 0, nedges_tot = 0, max_edges = 0;
+  long long nvtot = 0, nedges_tot = 0, max_edges = 0, max_nv = 0;
   std::vector<long long> vbase(np);
   for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
@@ -424,12 +437,14 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
     long long ne = c.kind == 0 ? 3 * c.nidx : c.nidx;
     nedges_tot += ne;
     max_edges = std::max(max_edges, ne);
+    max_nv = std::max(max_nv, c.nv);
   }
   const long long nvalloc = std::max<long long>(nvtot, 1);
   size_t bytes = al(nvalloc * sizeof(float4)) * 2 + al(nvalloc * sizeof(int2)) + al(nvalloc * sizeof(float)) +
                  al((size_t)kLMWorkSlots * nvalloc * sizeof(float4)) +
                  al(std::max<long long>(2 * nedges_tot, 1) * sizeof(unsigned)) + al(topo_workspace_bytes(nvalloc)) +
-                 al(np * sizeof(LMPairDesc)) + al(np * sizeof(TopoPairDesc)) + al(np * sizeof(LMResultDev)) + 256;
+                 al(np * sizeof(LMPairDesc)) + al(np * sizeof(TopoPairDesc)) + al(np * sizeof(LMResultDev)) +
+                 al(2 * np * sizeof(int)) + al(np * sizeof(int)) + 512;
   Arena ar;
   int rc;
   if ((rc = ar.reserve(bytes, s))) return rc;
@@ -443,25 +458,21 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   LMPairDesc* d_pairs = ar.take<LMPairDesc>(np);
   TopoPairDesc* d_topo = ar.take<TopoPairDesc>(np);
   LMResultDev* d_res = ar.take<LMResultDev>(np);
+  int* d_pstat = ar.take<int>(2 * np);
+  int* d_plist = ar.take<int>(np);
   int* err = ar.take<int>(1);
 
-  // largest pairs first: the hardware dispatches workgroups in order, so
-  // this is a longest-processing-time-first schedule
-  std::vector<int> order(np);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pairs[a].nv > pairs[b].nv; });
   std::vector<LMPairDesc> h_pairs(np);
   std::vector<TopoPairDesc> h_topo(np);
-  for (int q = 0; q < np; ++q) {
-    const int p = order[q];
+  for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
-    h_pairs[q] = LMPairDesc{c.grid, c.N, (int)c.nv, vbase[p], 0, (float)c.lambda, 0};
+    h_pairs[p] = LMPairDesc{c.grid, c.N, (int)c.nv, vbase[p], 0, (float)c.lambda, 0};
     h_topo[p] = TopoPairDesc{c.idx, c.kind == 0 ? 3 * c.nidx : c.nidx, vbase[p], (int)c.nv, c.kind};
   }
   MO_HIP(hipMemcpyAsync(d_pairs, h_pairs.data(), np * sizeof(LMPairDesc), hipMemcpyHostToDevice, s));
   MO_HIP(hipMemcpyAsync(d_topo, h_topo.data(), np * sizeof(TopoPairDesc), hipMemcpyHostToDevice, s));
   if ((rc = zero_fill(err, sizeof(int), s))) return rc;
-  if ((rc = topo_build_launch(d_topo, np, max_edges, nvtot, 2 * nedges_tot, raw, adj, deg, tws, err, s))) return rc;
+  if ((rc = topo_build_launch(d_topo, np, max_edges, max_nv, nvtot, raw, adj, deg, d_pstat, tws, err, s))) return rc;
   for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
     if (c.nv == 0) continue;
@@ -469,9 +480,55 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
                        deg + vbase[p], x0 + vbase[p], u + vbase[p]);
   }
   MO_HIP(hipGetLastError());
+  std::vector<int> pstat(2 * np);
+  int herr = 0;
+  MO_HIP(hipMemcpyAsync(pstat.data(), d_pstat, 2 * np * sizeof(int), hipMemcpyDeviceToHost, s));
+  MO_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  MO_HIP(hipStreamSynchronize(s));
+  MO_CHECK(herr == 0, MO_ERR_RANGE, "source topology: index out of range or too many vertices (flags 0x%x)", herr);
+
+  // Schedule: largest pairs first within every launch (workgroups are
+  // dispatched in order, so this is longest-processing-time-first).  Pairs
+  // that fit the register/LDS kernel go to its size buckets, the rest to the
+  // global-memory kernel.
+  std::vector<int> order(np);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pairs[a].nv > pairs[b].nv; });
+  const int nb = lm_lds_buckets();
+  const size_t lds_cap = (size_t)std::max(0, device_lds_limit() - 4096);
+  std::vector<std::vector<int>> bucket(nb + 1);
+  std::vector<size_t> bucket_lds(nb + 1, 0);
+  for (int p : order) {
+    const CorePair& c = pairs[p];
+    int b = -1;
+    if (opt.kernel_variant == 0 && c.nv > 0 && c.nv <= 32767 && pstat[2 * p + 1] <= 2) {
+      b = lm_lds_bucket_for((int)c.nv);
+      if (b >= 0 && lm_lds_bytes((int)c.nv, pstat[2 * p]) > lds_cap) b = -1;
+    }
+    if (b < 0) b = nb;
+    bucket[b].push_back(p);
+    if (b < nb) bucket_lds[b] = std::max(bucket_lds[b], lm_lds_bytes((int)c.nv, pstat[2 * p]));
+  }
+  std::vector<int> plist;
+  std::vector<int> start(nb + 2, 0);
+  for (int b = 0; b <= nb; ++b) {
+    start[b] = (int)plist.size();
+    plist.insert(plist.end(), bucket[b].begin(), bucket[b].end());
+  }
+  start[nb + 1] = (int)plist.size();
+  MO_HIP(hipMemcpyAsync(d_plist, plist.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
   LMArrays A{x0, u, adj, raw, work};
+  const LMOptionsDev dopt = to_dev_options(opt);
   if ((rc = timer.mark(2))) return rc;
-  if ((rc = lm_solve_launch(d_pairs, np, A, to_dev_options(opt), d_res, nvalloc, opt.block_threads, s))) return rc;
+  for (int b = 0; b < nb; ++b)
+    if (!bucket[b].empty())
+      if ((rc = lm_lds_launch(b, d_pairs, d_plist + start[b], (int)bucket[b].size(), A, dopt, d_res, nvalloc,
+                              bucket_lds[b], s)))
+        return rc;
+  if (!bucket[nb].empty())
+    if ((rc = lm_solve_launch(d_pairs, d_plist + start[nb], (int)bucket[nb].size(), A, dopt, d_res, nvalloc,
+                              opt.block_threads, s)))
+      return rc;
   if ((rc = timer.mark(3))) return rc;
   for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
@@ -481,15 +538,12 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   }
   MO_HIP(hipGetLastError());
   std::vector<LMResultDev> h_res(np);
-  int herr = 0;
   MO_HIP(hipMemcpyAsync(h_res.data(), d_res, np * sizeof(LMResultDev), hipMemcpyDeviceToHost, s));
-  MO_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
   MO_HIP(hipStreamSynchronize(s));
-  MO_CHECK(herr == 0, MO_ERR_RANGE, "source topology: index out of range or too many vertices (flags 0x%x)", herr);
   if (stats) {
-    for (int q = 0; q < np; ++q) {
-      const LMResultDev& r = h_res[q];
-      mo_lm_stats& o = stats[order[q]];
+    for (int p = 0; p < np; ++p) {
+      const LMResultDev& r = h_res[p];
+      mo_lm_stats& o = stats[p];
       o.initial_cost = r.initial_cost;
       o.final_cost = r.final_cost;
       o.vertex_cost = r.vertex_cost;
@@ -646,8 +700,9 @@ MO_EXPORT void mo_lm_default_options(mo_lm_options* o) {
   o->max_lm_diagonal = 1e32;
   o->pcg_max_iterations = 1000;
   o->pcg_relative_tolerance = 1e-5f;
-  o->block_threads = 256;
+  o->block_threads = 1024;
   o->timing = 0;
+  o->kernel_variant = 0;
 }
 
 MO_EXPORT int mo_last_timings(double* out6) {

@@ -27,10 +27,12 @@
 //                      brick than the wave's worst best distance, and each
 //                      remaining tile is kept only if some lane's own
 //                      point-to-AABB bound still reaches that lane's best.
-//                      For a kept tile the wave walks its 32 faces, whose
-//                      records are read with wave-uniform addresses (scalar
-//                      loads); a face is tested exactly only when its AABB
-//                      and plane bounds can still improve some lane.  Ties keep the lowest face index, so the
+//                      For a kept tile, one lane per face first drops the
+//                      faces whose AABB is out of reach of the whole brick;
+//                      the wave then walks the survivors, whose records are
+//                      read with wave-uniform addresses (scalar loads), and
+//                      tests a face exactly only when its AABB and plane
+//                      bounds can still improve some lane.  Ties keep the lowest face index, so the
 //                      visiting order never changes the result.
 #include "mo_common.h"
 #include "mo_grid.h"
@@ -248,10 +250,24 @@ __global__ void __launch_bounds__(64) k_grid_build(const GFace* __restrict__ fac
   auto process_tile = [&](int t) {
     const long long f0 = (long long)t * kTile;
     const int cnt = (int)((nf - f0) < kTile ? (nf - f0) : kTile);
-    for (int q = 0; q < cnt; ++q) {
+    // stage 1, one face per lane: drop faces whose AABB is farther from the
+    // brick AABB than the wave's worst best distance
+    const double bound = wave_max(valid ? best : -INFINITY);
+    bool cand = false;
+    if (lane < cnt) {
+      const GFace* gp = faces + f0 + lane;
+      const double ax = dmax(0.0, dmax(gp->lox - hx, lx - gp->hix));
+      const double ay = dmax(0.0, dmax(gp->loy - hy, ly - gp->hiy));
+      const double az = dmax(0.0, dmax(gp->loz - hz, lz - gp->hiz));
+      cand = !(ax * ax + ay * ay + az * az > bound * (1.0 + 1e-9));
+    }
+    unsigned long long m = __ballot(cand);
+    // stage 2, per surviving face: every lane's own AABB and plane bounds,
+    // then the exact test when some lane can still improve or tie its best
+    while (m) {
+      const int q = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
+      m &= m - 1;
       const GFace g = faces[f0 + q];
-      // face AABB bound: skip the exact test unless some lane can still
-      // improve or tie its best
       double gx = dmax(0.0, dmax(g.lox - qx, qx - g.hix));
       double gy = dmax(0.0, dmax(g.loy - qy, qy - g.hiy));
       double gz = dmax(0.0, dmax(g.loz - qz, qz - g.hiz));

@@ -61,7 +61,18 @@ struct LMArrays {
   float4* work;         // kLMWorkSlots float4 per vertex of scratch, slot-major
 };
 
-int lm_solve_launch(const LMPairDesc* pairs_dev, int npairs, const LMArrays& arrays, const LMOptionsDev& opt,
-                    LMResultDev* results_dev, long long total_vertices, int block_threads, hipStream_t s);
+// General kernel (vectors in global memory): pairs plist[0..npairs) of pairs_dev.
+int lm_solve_launch(const LMPairDesc* pairs_dev, const int* plist, int npairs, const LMArrays& arrays,
+                    const LMOptionsDev& opt, LMResultDev* results_dev, long long total_vertices, int block_threads,
+                    hipStream_t s);
+
+// Register/LDS-resident kernel (mo_lm_lds.hip) for pairs of at most 8192
+// vertices whose adjacency multiplicities are 1 or 2, in size buckets.
+int lm_lds_buckets();
+int lm_lds_bucket_for(int nv);                       // -1: too large
+size_t lm_lds_bytes(int nv, long long entries);      // dynamic LDS for one pair
+int lm_lds_launch(int bucket, const LMPairDesc* pairs_dev, const int* plist, int count, const LMArrays& arrays,
+                  const LMOptionsDev& opt, LMResultDev* results_dev, long long total_vertices, size_t lds_bytes,
+                  hipStream_t s);
 
 }  // namespace mo

@@ -28,6 +28,7 @@
 // order, so control flow is uniform without any broadcast.
 #include "mo_common.h"
 #include "mo_lm.h"
+#include "mo_lm_util.h"
 #include "mo_lookup.h"
 
 #include <cfloat>
@@ -36,47 +37,16 @@ namespace mo {
 
 namespace {
 
-constexpr int kMaxWaves = 16;
-
-// Block-wide reduction of K sums followed by M maxima; every thread gets the
-// totals.  Two LDS buffers alternate so consecutive reductions need only the
-// one barrier inside each.
-template <int K, int M>
-__device__ inline void block_reduce(double (&v)[K + M], double* red, int& sel) {
-  for (int k = 0; k < K; ++k)
-    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
-  for (int k = K; k < K + M; ++k)
-    for (int o = 32; o > 0; o >>= 1) { double t = __shfl_xor(v[k], o, 64); v[k] = t > v[k] ? t : v[k]; }
-  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  double* buf = red + sel * (8 * kMaxWaves);
-  if ((threadIdx.x & 63) == 0)
-    for (int k = 0; k < K + M; ++k) buf[k * kMaxWaves + w] = v[k];
-  __syncthreads();
-  for (int k = 0; k < K; ++k) {
-    double s = 0.0;
-    for (int q = 0; q < nw; ++q) s += buf[k * kMaxWaves + q];
-    v[k] = s;
-  }
-  for (int k = K; k < K + M; ++k) {
-    double s = buf[k * kMaxWaves];
-    for (int q = 1; q < nw; ++q) s = buf[k * kMaxWaves + q] > s ? buf[k * kMaxWaves + q] : s;
-    v[k] = s;
-  }
-  sel ^= 1;
-}
-
-__device__ inline unsigned nb_index(unsigned e) { return e & 0xFFFFFFu; }
-__device__ inline float nb_mult(unsigned e) { return (float)(e >> 24); }
-
 // Work slots (float4 each, SoA over all vertices of the batch).
 enum Slot { kG = 0, kGR, kS, kDG, kDT, kMI0, kMI1, kDL, kR, kZ, kP0, kP1, kQ, kNumSlots };
 static_assert(kNumSlots == kLMWorkSlots, "work slot count is part of the host contract");
 
-__global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict__ pairs, LMArrays A,
-                                                   LMOptionsDev opt, LMResultDev* __restrict__ res,
-                                                   long long totalV) {
+__global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict__ pairs,
+                                                   const int* __restrict__ plist, LMArrays A, LMOptionsDev opt,
+                                                   LMResultDev* __restrict__ res, long long totalV) {
   __shared__ double red[2 * 8 * kMaxWaves];
-  const LMPairDesc P = pairs[blockIdx.x];
+  const int pid = plist[blockIdx.x];
+  const LMPairDesc P = pairs[pid];
   const int T = blockDim.x, tid = threadIdx.x;
   const int nv = P.nv;
   const long long vb = P.vbase;
@@ -276,7 +246,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
   };
 
   // ---- model cost change, candidate cost, step and parameter norms ------
-  auto eval_candidate = [&](double& mc, double& ccv, double& cce, double& sn, double& xn) {
+  auto eval_candidate = [&]() -> LMCandidate {
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int i = tid; i < nv; i += T) {
       const float4 x0 = X0[i];
@@ -310,86 +280,37 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
       v[4] += (double)px * px + (double)py * py + (double)pz * pz;
     }
     block_reduce<5, 0>(v, red, sel);
-    mc = v[0]; ccv = v[1]; cce = v[2]; sn = sqrt(v[3]); xn = sqrt(v[4]);
+    return LMCandidate{-v[0], v[1] + v[2], sqrt(v[3]), sqrt(v[4])};
   };
 
-  // ---- trust-region loop (state replicated in every thread) -------------
-  double cv, ce, gmax;
-  eval_jacobian(true, cv, ce, gmax);
-  double cost = cv + ce;
-  const double initial_cost = cost;
-  double radius = opt.initial_trust_region_radius, decrease = 2.0;
-  bool reuse = false, last_success = true;
-  int iter = 0, invalid = 0, nsucc = 0, nfail = 0, term = 1;
-  long long pcg_total = 0;
-  while (true) {
-    if (iter >= opt.max_num_iterations) { term = 1; break; }
-    if (last_success && gmax <= opt.gradient_tolerance) { term = 0; break; }
-    if (radius <= opt.min_trust_region_radius) { term = 0; break; }
-    ++iter;
+  auto eval = [&](bool first, double& cv, double& ce, double& gmax) { eval_jacobian(first, cv, ce, gmax); };
+  auto solve = [&](bool reuse, double radius) -> int {
     double rz, rr;
     setup_linear(reuse, radius, rz, rr);
-    reuse = true;
-    pcg_total += pcg(rz, rr);
-    double mc, ccv, cce, sn, xn;
-    eval_candidate(mc, ccv, cce, sn, xn);
-    const double mcc = -mc;
-    const double ccost = ccv + cce;
-    if (!(mcc > 0.0) || !(ccost < INFINITY)) {
-      if (++invalid >= opt.max_consecutive_invalid_steps) { term = 2; break; }
-      radius /= decrease; decrease *= 2.0;
-      last_success = false; ++nfail;
-      continue;
+    return pcg(rz, rr);
+  };
+  auto accept = [&]() {
+    for (int i = tid; i < nv; i += T) {
+      float4 ui = U[i];
+      const float4 di = DL[i];
+      ui.x += di.x; ui.y += di.y; ui.z += di.z;
+      U[i] = ui;
     }
-    invalid = 0;
-    if (sn <= opt.parameter_tolerance * (xn + opt.parameter_tolerance)) { term = 0; break; }
-    if (fabs(cost - ccost) <= opt.function_tolerance * cost) { term = 0; break; }
-    const double rho = (cost - ccost) / mcc;
-    if (rho > opt.min_relative_decrease) {
-      for (int i = tid; i < nv; i += T) {
-        float4 ui = U[i];
-        const float4 di = DL[i];
-        ui.x += di.x; ui.y += di.y; ui.z += di.z;
-        U[i] = ui;
-      }
-      __syncthreads();
-      eval_jacobian(false, cv, ce, gmax);
-      cost = cv + ce;
-      const double t = 2.0 * rho - 1.0;
-      const double f = 1.0 - t * t * t;
-      radius = radius / (f > 1.0 / 3.0 ? f : 1.0 / 3.0);
-      radius = radius < opt.max_trust_region_radius ? radius : opt.max_trust_region_radius;
-      decrease = 2.0; reuse = false; last_success = true; ++nsucc;
-    } else {
-      radius /= decrease; decrease *= 2.0; reuse = true; last_success = false; ++nfail;
-    }
-  }
-  if (tid == 0) {
-    LMResultDev r;
-    r.initial_cost = initial_cost;
-    r.final_cost = cv + ce;
-    r.vertex_cost = cv;
-    r.rigidity_cost = ce;
-    r.iterations = iter;
-    r.successful_steps = nsucc;
-    r.unsuccessful_steps = nfail;
-    r.termination = term;
-    r.pcg_iterations = pcg_total;
-    r.clock_start = clock_start;
-    r.clock_end = wall_clock64();
-    res[blockIdx.x] = r;
-  }
+    __syncthreads();
+  };
+  lm_trust_region_loop(opt, eval, solve, eval_candidate, accept, res + pid, clock_start);
 }
 
 }  // namespace
 
-int lm_solve_launch(const LMPairDesc* pairs_dev, int npairs, const LMArrays& arrays, const LMOptionsDev& opt,
-                    LMResultDev* results_dev, long long total_vertices, int block_threads, hipStream_t s) {
+int lm_solve_launch(const LMPairDesc* pairs_dev, const int* plist, int npairs, const LMArrays& arrays,
+                    const LMOptionsDev& opt, LMResultDev* results_dev, long long total_vertices, int block_threads,
+                    hipStream_t s) {
   if (npairs <= 0) return MO_OK;
   MO_CHECK(block_threads >= 64 && block_threads <= 1024 && block_threads % 64 == 0, MO_ERR_ARG,
            "block_threads must be a multiple of 64 in [64, 1024] (got %d)", block_threads);
-  hipLaunchKernelGGL(k_lm_solve, dim3(npairs), dim3(block_threads), 0, s, pairs_dev, arrays, opt, results_dev,
-                     total_vertices);
+  hipLaunchKernelGGL(k_lm_solve, dim3(npairs), dim3(block_threads), 0, s, pairs_dev, plist, arrays, opt,
+                     results_dev, total_vertices);
   MO_HIP(hipGetLastError());
   return MO_OK;
 }

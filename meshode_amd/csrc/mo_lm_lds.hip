@@ -1,0 +1,409 @@
+// Register/LDS-resident variant of the batched LM solver (same algorithm as
+// mo_lm.hip: Deformer::Deform / DeformGraph, deformer.cc:18-92, 370-442,
+// with Ceres' trust-region LM restated and a matrix-free block-Jacobi PCG).
+//
+// One workgroup of T threads solves one pair whose nv <= T * VPT vertices
+// are statically owned (vertex i by thread i % T, slot i / T), so every
+// per-vertex vector of the inner PCG (delta, r, p, q/z, g, D^2) lives in
+// VGPRs across the whole solve.  The only vector neighbours read (u while
+// evaluating the Jacobian, p inside PCG, u + delta for the candidate) is
+// published to one float4-per-vertex LDS array, and the pair's adjacency is
+// copied once into LDS as uint16 entries (neighbour << 1 | multiplicity - 1),
+// so the PCG loop touches no global memory at all.
+//
+// Differences from the global kernel that do not change the algorithm:
+//   * the block-Jacobi preconditioner M = g g^T + diag(lambda^2 deg + D^2)
+//     is applied with the Sherman-Morrison formula each iteration instead of
+//     a stored inverse (saves 6 VGPRs per vertex);
+//   * the rigidity residuals are linear in x, so the rigidity part of the
+//     model cost change equals its actual change E_r(u + delta) - E_r(u);
+//     only u + delta has to be gathered for the candidate.
+#include "mo_common.h"
+#include "mo_lm.h"
+#include "mo_lm_util.h"
+#include "mo_lookup.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+namespace mo {
+
+namespace {
+
+enum GSlot { gGR = 1, gS = 2, gDG = 3 };   // global slots of LMArrays::work used here
+
+// Block-wide exclusive scan of one int per thread (T <= 1024).
+__device__ inline int block_exclusive_scan(int x, int* buf, int& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int v = x;
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  if (lane == 63) buf[w] = v;
+  __syncthreads();
+  int base = 0;
+  total = 0;
+  for (int q = 0; q < nw; ++q) {
+    if (q < w) base += buf[q];
+    total += buf[q];
+  }
+  __syncthreads();
+  return base + v - x;
+}
+
+template <int T, int VPT>
+__global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict__ pairs,
+                                                    const int* __restrict__ plist, LMArrays A, LMOptionsDev opt,
+                                                    LMResultDev* __restrict__ res, long long totalV) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[2 * 8 * kMaxWaves];
+  __shared__ int scan_buf[kMaxWaves];
+  const int pid = plist[blockIdx.x];
+  const LMPairDesc P = pairs[pid];
+  const int tid = threadIdx.x;
+  const int nv = P.nv;
+  const long long vb = P.vbase;
+  const float l2 = P.lambda * P.lambda;
+  const float* __restrict__ grid = P.grid;
+  const int N = P.N;
+  const float4* __restrict__ X0 = A.x0 + vb;
+  float4* __restrict__ U = A.u + vb;
+  const int2* __restrict__ ADJ = A.adj + vb;
+  const unsigned* __restrict__ NB = A.nbr + P.ebase;
+  float4* __restrict__ GR = A.work + (size_t)gGR * totalV + vb;
+  float4* __restrict__ S = A.work + (size_t)gS * totalV + vb;
+  float4* __restrict__ DG = A.work + (size_t)gDG * totalV + vb;
+  float4* VEC = (float4*)smem;
+  unsigned short* LCSR = (unsigned short*)(smem + (size_t)nv * sizeof(float4));
+  int sel = 0;
+  const long long clock_start = wall_clock64();
+
+  // ---- adjacency -> LDS (packed, compacted by a block scan) -------------
+  unsigned csr[VPT];   // (LDS offset << 16) | count
+  {
+    int cnt[VPT], gst[VPT], mine = 0;
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const int i = tid + k * T;
+      cnt[k] = 0;
+      gst[k] = 0;
+      if (i < nv) {
+        const int2 a = ADJ[i];
+        gst[k] = a.x;
+        cnt[k] = a.y;
+        mine += a.y;
+      }
+    }
+    int total;
+    int off = block_exclusive_scan(mine, scan_buf, total);
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      csr[k] = ((unsigned)off << 16) | (unsigned)cnt[k];
+      for (int e = 0; e < cnt[k]; ++e) {
+        const unsigned en = NB[gst[k] + e];
+        LCSR[off + e] = (unsigned short)((nb_index(en) << 1) | ((en >> 24) - 1u));
+      }
+      off += cnt[k];
+    }
+  }
+
+  // ---- per-vertex register state ----------------------------------------
+  float gx[VPT], gy[VPT], gz[VPT], ld[VPT];   // grad d, lambda^2 deg
+  float tx[VPT], ty[VPT], tz[VPT];      // LM diagonal D^2 (unscaled variables)
+  float rx[VPT], ry[VPT], rz_[VPT];     // PCG residual
+  float px[VPT], py[VPT], pz[VPT];      // PCG direction
+  float qx[VPT], qy[VPT], qz[VPT];      // A p, then the preconditioned residual z
+  float ex[VPT], ey[VPT], ez[VPT];      // step delta
+
+  // z = M^-1 r with M = g g^T + diag(c), c = lambda^2 deg + D^2 (Sherman-Morrison)
+  auto precond = [&](int k, float r0, float r1, float r2, float& z0, float& z1, float& z2) {
+    const float i0 = __builtin_amdgcn_rcpf(ld[k] + tx[k]), i1 = __builtin_amdgcn_rcpf(ld[k] + ty[k]),
+                i2 = __builtin_amdgcn_rcpf(ld[k] + tz[k]);
+    const float h0 = gx[k] * i0, h1 = gy[k] * i1, h2 = gz[k] * i2;
+    const float sig = 1.0f + gx[k] * h0 + gy[k] * h1 + gz[k] * h2;
+    const float c = (h0 * r0 + h1 * r1 + h2 * r2) * __builtin_amdgcn_rcpf(sig);
+    z0 = r0 * i0 - h0 * c;
+    z1 = r1 * i1 - h1 * c;
+    z2 = r2 * i2 - h2 * c;
+  };
+
+  auto eval = [&](bool first, double& cv, double& ce, double& gmax) {
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const int i = tid + k * T;
+      if (i < nv) VEC[i] = U[i];
+    }
+    __syncthreads();
+    double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const int i = tid + k * T;
+      if (i >= nv) continue;
+      const float4 x0 = X0[i];
+      const float4 ui = VEC[i];
+      const JetF J = lookup_jet(grid, N, x0.x + ui.x, x0.y + ui.y, x0.z + ui.z);
+      gx[k] = J.v0; gy[k] = J.v1; gz[k] = J.v2;
+      ld[k] = l2 * x0.w;
+      float sx = 0.f, sy = 0.f, sz = 0.f;
+      double e = 0.0;
+      const int o = (int)(csr[k] >> 16), c = (int)(csr[k] & 0xFFFFu);
+      for (int q = 0; q < c; ++q) {
+        const unsigned w = LCSR[o + q];
+        const float m = (float)((w & 1u) + 1u);
+        const float4 uj = VEC[w >> 1];
+        const float ax = ui.x - uj.x, ay = ui.y - uj.y, az = ui.z - uj.z;
+        sx += m * ax; sy += m * ay; sz += m * az;
+        e += (double)m * ((double)ax * ax + (double)ay * ay + (double)az * az);
+      }
+      const float g0 = J.a * J.v0 + l2 * sx, g1 = J.a * J.v1 + l2 * sy, g2 = J.a * J.v2 + l2 * sz;
+      GR[i] = make_float4(g0, g1, g2, J.a);   // J^T f and the residual d
+      v[0] += 0.5 * (double)J.a * (double)J.a;
+      v[1] += 0.25 * (double)l2 * e;
+      double gm = fabs((double)g0);
+      gm = fabs((double)g1) > gm ? fabs((double)g1) : gm;
+      gm = fabs((double)g2) > gm ? fabs((double)g2) : gm;
+      v[2] = gm > v[2] ? gm : v[2];
+      if (first) {
+        const float c0 = J.v0 * J.v0 + ld[k], c1 = J.v1 * J.v1 + ld[k], c2 = J.v2 * J.v2 + ld[k];
+        S[i] = make_float4(1.0f / (1.0f + sqrtf(c0)), 1.0f / (1.0f + sqrtf(c1)), 1.0f / (1.0f + sqrtf(c2)), 0.f);
+      }
+    }
+    block_reduce<2, 1>(v, red, sel);
+    cv = v[0]; ce = v[1]; gmax = v[2];
+  };
+
+  auto solve = [&](bool reuse, double radius) -> int {
+    const double inv_radius = 1.0 / radius;
+    double v[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const int i = tid + k * T;
+      if (i >= nv) continue;
+      const float4 s = S[i];
+      float4 dg;
+      if (!reuse) {
+        const double cc[3] = {(double)gx[k] * gx[k] + ld[k], (double)gy[k] * gy[k] + ld[k],
+                              (double)gz[k] * gz[k] + ld[k]};
+        const double sc[3] = {s.x, s.y, s.z};
+        float d3[3];
+        for (int a = 0; a < 3; ++a) {
+          double t = sc[a] * sc[a] * cc[a];
+          t = t < opt.min_lm_diagonal ? opt.min_lm_diagonal : (t > opt.max_lm_diagonal ? opt.max_lm_diagonal : t);
+          d3[a] = (float)t;
+        }
+        dg = make_float4(d3[0], d3[1], d3[2], 0.f);
+        DG[i] = dg;
+      } else {
+        dg = DG[i];
+      }
+      const double cap = 1e30;
+      const double t0 = (double)dg.x / ((double)s.x * s.x) * inv_radius;
+      const double t1 = (double)dg.y / ((double)s.y * s.y) * inv_radius;
+      const double t2 = (double)dg.z / ((double)s.z * s.z) * inv_radius;
+      tx[k] = (float)(t0 < cap ? t0 : cap);
+      ty[k] = (float)(t1 < cap ? t1 : cap);
+      tz[k] = (float)(t2 < cap ? t2 : cap);
+      const float4 gr = GR[i];
+      rx[k] = -gr.x; ry[k] = -gr.y; rz_[k] = -gr.z;
+      float z0, z1, z2;
+      precond(k, rx[k], ry[k], rz_[k], z0, z1, z2);
+      px[k] = z0; py[k] = z1; pz[k] = z2;
+      ex[k] = ey[k] = ez[k] = 0.f;
+      VEC[i] = make_float4(z0, z1, z2, 0.f);
+      v[0] += (double)rx[k] * z0 + (double)ry[k] * z1 + (double)rz_[k] * z2;
+      v[1] += (double)rx[k] * rx[k] + (double)ry[k] * ry[k] + (double)rz_[k] * rz_[k];
+    }
+    block_reduce<2, 0>(v, red, sel);   // also publishes p to LDS
+    double rz = v[0];
+    const double rr0 = v[1];
+    if (!(rr0 > 0.0)) return 0;
+    const double tol2 = (double)opt.pcg_relative_tolerance * opt.pcg_relative_tolerance * rr0;
+    int it = 0;
+    for (; it < opt.pcg_max_iterations; ++it) {
+      double w1[1] = {0.0};
+#pragma unroll
+      for (int k = 0; k < VPT; ++k) {
+        const int i = tid + k * T;
+        if (i >= nv) continue;
+        const float gp = gx[k] * px[k] + gy[k] * py[k] + gz[k] * pz[k];
+        float lx = 0.f, ly = 0.f, lz = 0.f;
+        const int o = (int)(csr[k] >> 16), c = (int)(csr[k] & 0xFFFFu);
+        for (int q = 0; q < c; ++q) {
+          const unsigned w = LCSR[o + q];
+          const float m = (float)((w & 1u) + 1u);
+          const float4 pj = VEC[w >> 1];
+          lx += m * (px[k] - pj.x); ly += m * (py[k] - pj.y); lz += m * (pz[k] - pj.z);
+        }
+        qx[k] = gx[k] * gp + l2 * lx + tx[k] * px[k];
+        qy[k] = gy[k] * gp + l2 * ly + ty[k] * py[k];
+        qz[k] = gz[k] * gp + l2 * lz + tz[k] * pz[k];
+        w1[0] += (double)px[k] * qx[k] + (double)py[k] * qy[k] + (double)pz[k] * qz[k];
+      }
+      block_reduce<1, 0>(w1, red, sel);
+      const double pq = w1[0];
+      if (!(pq > 0.0)) break;
+      const float fa = (float)(rz / pq);
+      double w2[2] = {0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < VPT; ++k) {
+        const int i = tid + k * T;
+        if (i >= nv) continue;
+        ex[k] += fa * px[k]; ey[k] += fa * py[k]; ez[k] += fa * pz[k];
+        rx[k] -= fa * qx[k]; ry[k] -= fa * qy[k]; rz_[k] -= fa * qz[k];
+        float z0, z1, z2;
+        precond(k, rx[k], ry[k], rz_[k], z0, z1, z2);
+        qx[k] = z0; qy[k] = z1; qz[k] = z2;
+        w2[0] += (double)rx[k] * z0 + (double)ry[k] * z1 + (double)rz_[k] * z2;
+        w2[1] += (double)rx[k] * rx[k] + (double)ry[k] * ry[k] + (double)rz_[k] * rz_[k];
+      }
+      block_reduce<2, 0>(w2, red, sel);
+      const float fb = (float)(w2[0] / rz);
+      rz = w2[0];
+      if (w2[1] <= tol2) { ++it; break; }
+#pragma unroll
+      for (int k = 0; k < VPT; ++k) {
+        const int i = tid + k * T;
+        if (i >= nv) continue;
+        px[k] = qx[k] + fb * px[k]; py[k] = qy[k] + fb * py[k]; pz[k] = qz[k] + fb * pz[k];
+        VEC[i] = make_float4(px[k], py[k], pz[k], 0.f);
+      }
+      __syncthreads();
+    }
+    return it;
+  };
+
+  double ce_now = 0.0;   // rigidity energy at the current x (set by eval)
+  auto candidate = [&]() -> LMCandidate {
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const int i = tid + k * T;
+      if (i < nv) {
+        const float4 ui = U[i];
+        VEC[i] = make_float4(ui.x + ex[k], ui.y + ey[k], ui.z + ez[k], 0.f);
+      }
+    }
+    __syncthreads();
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const int i = tid + k * T;
+      if (i >= nv) continue;
+      const float4 x0 = X0[i];
+      const float dres = GR[i].w;
+      const float gd = gx[k] * ex[k] + gy[k] * ey[k] + gz[k] * ez[k];
+      v[0] += (double)gd * ((double)dres + 0.5 * (double)gd);
+      const float4 wv = VEC[i];
+      const float wx = wv.x, wy = wv.y, wz = wv.z;
+      const float dc = lookup_f32(grid, N, x0.x + wx, x0.y + wy, x0.z + wz);
+      v[1] += 0.5 * (double)dc * (double)dc;
+      double e = 0.0;
+      const int o = (int)(csr[k] >> 16), c = (int)(csr[k] & 0xFFFFu);
+      for (int q = 0; q < c; ++q) {
+        const unsigned w = LCSR[o + q];
+        const float m = (float)((w & 1u) + 1u);
+        const float4 wj = VEC[w >> 1];
+        const float ax = wx - wj.x, ay = wy - wj.y, az = wz - wj.z;
+        e += (double)m * ((double)ax * ax + (double)ay * ay + (double)az * az);
+      }
+      v[2] += 0.25 * (double)l2 * e;
+      v[3] += (double)ex[k] * ex[k] + (double)ey[k] * ey[k] + (double)ez[k] * ez[k];
+      const float4 ui = U[i];
+      const float qx0 = x0.x + ui.x, qy0 = x0.y + ui.y, qz0 = x0.z + ui.z;
+      v[4] += (double)qx0 * qx0 + (double)qy0 * qy0 + (double)qz0 * qz0;
+    }
+    block_reduce<5, 0>(v, red, sel);
+    // rigidity residuals are linear: their model change equals the actual one
+    return LMCandidate{-v[0] - (v[2] - ce_now), v[1] + v[2], sqrt(v[3]), sqrt(v[4])};
+  };
+
+  auto eval_tracked = [&](bool first, double& cv, double& ce, double& gmax) {
+    eval(first, cv, ce, gmax);
+    ce_now = ce;
+  };
+  auto accept = [&]() {
+    // u <- u + delta: exactly the candidate published to LDS
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+      const int i = tid + k * T;
+      if (i < nv) U[i] = VEC[i];
+    }
+  };
+  lm_trust_region_loop(opt, eval_tracked, solve, candidate, accept, res + pid, clock_start);
+}
+
+template <int T, int VPT>
+int launch_bucket(const LMPairDesc* pairs, const int* plist, int count, const LMArrays& A, const LMOptionsDev& opt,
+                  LMResultDev* res, long long totalV, size_t lds_bytes, hipStream_t s) {
+  auto kern = k_lm_solve_lds<T, VPT>;
+  MO_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+  hipLaunchKernelGGL(kern, dim3(count), dim3(T), lds_bytes, s, pairs, plist, A, opt, res, totalV);
+  MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+}  // namespace
+
+// Shapes of the register/LDS kernel: (threads, vertices per thread).  A pair
+// of nv vertices can use any shape with T * VPT >= nv; the preference order
+// below is the tuned default (MO_LM_SHAPES="TxVPT,TxVPT,..." overrides it,
+// which is how the shape sweep in the benchmark tooling explores it).
+static const int kShapeT[] = {1024, 1024, 1024, 1024, 512, 512, 512, 512, 256, 256, 256};
+static const int kShapeVPT[] = {1, 2, 4, 5, 8, 10, 12, 16, 20, 24, 32};
+constexpr int kNumShapes = 11;
+
+static std::vector<int> shape_preference() {
+  static std::vector<int> pref;
+  static bool init = false;
+  if (!init) {
+    init = true;
+    const char* env = getenv("MO_LM_SHAPES");
+    if (env) {
+      std::string e(env);
+      size_t pos = 0;
+      while (pos < e.size()) {
+        size_t comma = e.find(',', pos);
+        std::string tok = e.substr(pos, comma == std::string::npos ? std::string::npos : comma - pos);
+        int t = 0, v = 0;
+        if (sscanf(tok.c_str(), "%dx%d", &t, &v) == 2)
+          for (int b = 0; b < kNumShapes; ++b)
+            if (kShapeT[b] == t && kShapeVPT[b] == v) pref.push_back(b);
+        if (comma == std::string::npos) break;
+        pos = comma + 1;
+      }
+    }
+    if (pref.empty())
+      for (int b = 0; b < kNumShapes; ++b) pref.push_back(b);
+  }
+  return pref;
+}
+
+int lm_lds_buckets() { return kNumShapes; }
+
+int lm_lds_bucket_for(int nv) {
+  for (int b : shape_preference())
+    if (nv <= kShapeT[b] * kShapeVPT[b]) return b;
+  return -1;
+}
+
+size_t lm_lds_bytes(int nv, long long entries) {
+  return (((size_t)nv * 16 + (size_t)entries * 2) + 15) & ~(size_t)15;
+}
+
+int lm_lds_launch(int bucket, const LMPairDesc* pairs, const int* plist, int count, const LMArrays& A,
+                  const LMOptionsDev& opt, LMResultDev* res, long long totalV, size_t lds_bytes, hipStream_t s) {
+  if (count <= 0) return MO_OK;
+  switch (bucket) {
+#define MO_SHAPE(ID, T, V) \
+    case ID: return launch_bucket<T, V>(pairs, plist, count, A, opt, res, totalV, lds_bytes, s);
+    MO_SHAPE(0, 1024, 1) MO_SHAPE(1, 1024, 2) MO_SHAPE(2, 1024, 4) MO_SHAPE(3, 1024, 5)
+    MO_SHAPE(4, 512, 8) MO_SHAPE(5, 512, 10) MO_SHAPE(6, 512, 12) MO_SHAPE(7, 512, 16)
+    MO_SHAPE(8, 256, 20) MO_SHAPE(9, 256, 24) MO_SHAPE(10, 256, 32)
+#undef MO_SHAPE
+    default: return fail(MO_ERR_ARG, "bad LDS solver shape %d", bucket);
+  }
+}
+
+}  // namespace mo

@@ -1,0 +1,131 @@
+// Device helpers shared by the LM solver kernels: block-wide FP64
+// reductions and the trust-region loop itself (Ceres' TrustRegionMinimizer
+// with the Levenberg-Marquardt strategy, restated; see mo_lm.hip).
+#pragma once
+
+#include "mo_common.h"
+#include "mo_lm.h"
+
+namespace mo {
+
+constexpr int kMaxWaves = 16;
+
+// Block-wide reduction of K sums followed by M maxima; every thread gets the
+// totals, summed in the same order by every thread (so all threads take the
+// same decisions).  Two LDS buffers alternate so consecutive reductions need
+// only the one barrier inside each.
+template <int K, int M>
+__device__ inline void block_reduce(double (&v)[K + M], double* red, int& sel) {
+  for (int k = 0; k < K; ++k)
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+  for (int k = K; k < K + M; ++k)
+    for (int o = 32; o > 0; o >>= 1) {
+      double t = __shfl_xor(v[k], o, 64);
+      v[k] = t > v[k] ? t : v[k];
+    }
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double* buf = red + sel * (8 * kMaxWaves);
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < K + M; ++k) buf[k * kMaxWaves + w] = v[k];
+  __syncthreads();
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int q = 0; q < nw; ++q) s += buf[k * kMaxWaves + q];
+    v[k] = s;
+  }
+  for (int k = K; k < K + M; ++k) {
+    double s = buf[k * kMaxWaves];
+    for (int q = 1; q < nw; ++q) s = buf[k * kMaxWaves + q] > s ? buf[k * kMaxWaves + q] : s;
+    v[k] = s;
+  }
+  sel ^= 1;
+}
+
+__device__ inline unsigned nb_index(unsigned e) { return e & 0xFFFFFFu; }
+__device__ inline float nb_mult(unsigned e) { return (float)(e >> 24); }
+
+// Candidate evaluation of one trust-region step.
+struct LMCandidate {
+  double model_change;   // model_cost_change = -(J delta).(f + J delta / 2)
+  double cost;           // cost at x + delta
+  double step_norm;      // ||delta||
+  double x_norm;         // ||x||
+};
+
+// The trust-region loop of Ceres' TrustRegionMinimizer with the
+// Levenberg-Marquardt strategy (restated; Ceres is not available here):
+//   eval(first, cost, gmax)          residuals, J^T f, cost at x (first: also
+//                                    the Jacobi scaling)
+//   solve(reuse_diagonal, radius)    LM step delta (returns PCG iterations)
+//   candidate()                      -> LMCandidate
+//   accept()                         x <- x + delta
+// Writes the per-pair result from thread 0.
+template <class Eval, class Solve, class Cand, class Accept>
+__device__ inline void lm_trust_region_loop(const LMOptionsDev& opt, Eval eval, Solve solve, Cand candidate,
+                                            Accept accept, LMResultDev* out, long long clock_start) {
+  double cv, ce, gmax;
+  eval(true, cv, ce, gmax);
+  double cost = cv + ce;
+  const double initial_cost = cost;
+  double radius = opt.initial_trust_region_radius, decrease = 2.0;
+  bool reuse = false, last_success = true;
+  int iter = 0, invalid = 0, nsucc = 0, nfail = 0, term = 1;
+  long long pcg_total = 0;
+  while (true) {
+    if (iter >= opt.max_num_iterations) { term = 1; break; }
+    if (last_success && gmax <= opt.gradient_tolerance) { term = 0; break; }
+    if (radius <= opt.min_trust_region_radius) { term = 0; break; }
+    ++iter;
+    pcg_total += solve(reuse, radius);
+    reuse = true;
+    const LMCandidate c = candidate();
+    if (!(c.model_change > 0.0) || !(c.cost < INFINITY)) {
+      if (++invalid >= opt.max_consecutive_invalid_steps) { term = 2; break; }
+      radius /= decrease;
+      decrease *= 2.0;
+      last_success = false;
+      ++nfail;
+      continue;
+    }
+    invalid = 0;
+    if (c.step_norm <= opt.parameter_tolerance * (c.x_norm + opt.parameter_tolerance)) { term = 0; break; }
+    if (fabs(cost - c.cost) <= opt.function_tolerance * cost) { term = 0; break; }
+    const double rho = (cost - c.cost) / c.model_change;
+    if (rho > opt.min_relative_decrease) {
+      accept();
+      eval(false, cv, ce, gmax);
+      cost = cv + ce;
+      const double t = 2.0 * rho - 1.0;
+      const double f = 1.0 - t * t * t;
+      radius = radius / (f > 1.0 / 3.0 ? f : 1.0 / 3.0);
+      radius = radius < opt.max_trust_region_radius ? radius : opt.max_trust_region_radius;
+      decrease = 2.0;
+      reuse = false;
+      last_success = true;
+      ++nsucc;
+    } else {
+      radius /= decrease;
+      decrease *= 2.0;
+      reuse = true;
+      last_success = false;
+      ++nfail;
+    }
+  }
+  if (threadIdx.x == 0) {
+    LMResultDev r;
+    r.initial_cost = initial_cost;
+    r.final_cost = cv + ce;
+    r.vertex_cost = cv;
+    r.rigidity_cost = ce;
+    r.iterations = iter;
+    r.successful_steps = nsucc;
+    r.unsuccessful_steps = nfail;
+    r.termination = term;
+    r.pcg_iterations = pcg_total;
+    r.clock_start = clock_start;
+    r.clock_end = wall_clock64();
+    *out = r;
+  }
+}
+
+}  // namespace mo

@@ -24,8 +24,10 @@ size_t topo_workspace_bytes(long long nvtot);
 // neighbour | multiplicity << 24; deg[v] = sum of multiplicities.
 // err_dev bit 0: edge index out of range; bit 1: neighbour index >= 2^24 or
 // multiplicity > 255.
-int topo_build_launch(const TopoPairDesc* pairs_dev, int npairs, long long max_edges, long long nvtot,
-                      long long raw_capacity, unsigned* raw, int2* adj, float* deg, void* ws, int* err_dev,
+// pstat[2p] = unique adjacency entries of pair p, pstat[2p+1] = largest
+// multiplicity.
+int topo_build_launch(const TopoPairDesc* pairs_dev, int npairs, long long max_edges, long long max_nv,
+                      long long nvtot, unsigned* raw, int2* adj, float* deg, int* pstat, void* ws, int* err_dev,
                       hipStream_t s);
 
 }  // namespace mo

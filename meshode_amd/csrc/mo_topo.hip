@@ -31,6 +31,11 @@ __device__ inline bool edge_of(const TopoPairDesc& P, long long h, int* a, int* 
   return *a != *b;
 }
 
+__global__ void k_zero_u32(unsigned* __restrict__ p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 __global__ void k_topo_count(const TopoPairDesc* __restrict__ pairs, unsigned* __restrict__ cnt,
                              int* __restrict__ err) {
   const TopoPairDesc P = pairs[blockIdx.y];
@@ -60,31 +65,40 @@ __global__ void k_topo_fill(const TopoPairDesc* __restrict__ pairs, const long l
   }
 }
 
-__global__ void k_topo_finish(long long nvtot, const long long* __restrict__ off, const unsigned* __restrict__ cnt,
-                              unsigned* __restrict__ raw, int2* __restrict__ adj, float* __restrict__ deg,
-                              int* __restrict__ err) {
-  long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nvtot) return;
-  const long long o = off[v];
-  const int n = (int)cnt[v];
-  unsigned* L = raw + o;
-  // insertion sort (lists are short: the vertex degree)
-  for (int i = 1; i < n; ++i) {
-    unsigned x = L[i];
-    int j = i - 1;
-    while (j >= 0 && L[j] > x) { L[j + 1] = L[j]; --j; }
-    L[j + 1] = x;
+__global__ void k_topo_finish(const TopoPairDesc* __restrict__ pairs, const long long* __restrict__ off,
+                              const unsigned* __restrict__ cnt, unsigned* __restrict__ raw, int2* __restrict__ adj,
+                              float* __restrict__ deg, int* __restrict__ pstat, int* __restrict__ err) {
+  const int p = blockIdx.y;
+  const TopoPairDesc P = pairs[p];
+  int uniq = 0, maxm = 0;
+  for (long long lv = (long long)blockIdx.x * blockDim.x + threadIdx.x; lv < P.nv;
+       lv += (long long)gridDim.x * blockDim.x) {
+    const long long v = P.vbase + lv;
+    const long long o = off[v];
+    const int n = (int)cnt[v];
+    unsigned* L = raw + o;
+    // insertion sort (lists are short: the vertex degree)
+    for (int i = 1; i < n; ++i) {
+      unsigned x = L[i];
+      int j = i - 1;
+      while (j >= 0 && L[j] > x) { L[j + 1] = L[j]; --j; }
+      L[j + 1] = x;
+    }
+    int u = 0;
+    for (int i = 0; i < n;) {
+      unsigned x = L[i];
+      int m = 0;
+      while (i < n && L[i] == x) { ++m; ++i; }
+      if (m > 255 || x > 0xFFFFFFu) { atomicOr(err, 2); m = m > 255 ? 255 : m; }
+      maxm = m > maxm ? m : maxm;
+      L[u++] = (x & 0xFFFFFFu) | ((unsigned)m << 24);
+    }
+    adj[v] = make_int2((int)o, u);
+    deg[v] = (float)n;
+    uniq += u;
   }
-  int u = 0;
-  for (int i = 0; i < n;) {
-    unsigned x = L[i];
-    int m = 0;
-    while (i < n && L[i] == x) { ++m; ++i; }
-    if (m > 255 || x > 0xFFFFFFu) { atomicOr(err, 2); m = m > 255 ? 255 : m; }
-    L[u++] = (x & 0xFFFFFFu) | ((unsigned)m << 24);
-  }
-  adj[v] = make_int2((int)o, u);
-  deg[v] = (float)n;
+  if (uniq) atomicAdd(&pstat[2 * p], uniq);
+  if (maxm) atomicMax(&pstat[2 * p + 1], maxm);
 }
 
 // Exclusive scan of unsigned counts into 64-bit offsets: per-block scan,
@@ -154,8 +168,8 @@ size_t topo_workspace_bytes(long long nvtot) {
   return (size_t)nvtot * (2 * sizeof(unsigned) + sizeof(long long)) + scan_workspace_bytes(nvtot) + 64;
 }
 
-int topo_build_launch(const TopoPairDesc* pairs_dev, int npairs, long long max_edges, long long nvtot,
-                      long long raw_capacity, unsigned* raw, int2* adj, float* deg, void* ws, int* err_dev,
+int topo_build_launch(const TopoPairDesc* pairs_dev, int npairs, long long max_edges, long long max_nv,
+                      long long nvtot, unsigned* raw, int2* adj, float* deg, int* pstat, void* ws, int* err_dev,
                       hipStream_t s) {
   if (npairs <= 0 || nvtot <= 0) return MO_OK;
   char* w = (char*)ws;
@@ -163,7 +177,12 @@ int topo_build_launch(const TopoPairDesc* pairs_dev, int npairs, long long max_e
   unsigned* cursor = (unsigned*)w; w += (size_t)nvtot * sizeof(unsigned);
   long long* off = (long long*)w; w += (size_t)nvtot * sizeof(long long);
   void* scan_ws = (void*)w;
-  MO_HIP(hipMemsetAsync(cnt, 0, (size_t)nvtot * sizeof(unsigned) * 2, s));
+  {
+    long long nz = nvtot * 2;   // cnt and cursor (adjacent)
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, cnt, nz);
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((2 * npairs + 255) / 256)), dim3(256), 0, s, (unsigned*)pstat,
+                       (long long)(2 * npairs));
+  }
   const int T = 256;
   long long bx = (max_edges + T - 1) / T;
   if (bx > 4096) bx = 4096;
@@ -171,10 +190,12 @@ int topo_build_launch(const TopoPairDesc* pairs_dev, int npairs, long long max_e
   hipLaunchKernelGGL(k_topo_count, dim3((unsigned)bx, (unsigned)npairs), dim3(T), 0, s, pairs_dev, cnt, err_dev);
   int rc = exclusive_scan_launch(cnt, nvtot, off, nullptr, scan_ws, s);
   if (rc) return rc;
-  (void)raw_capacity;
   hipLaunchKernelGGL(k_topo_fill, dim3((unsigned)bx, (unsigned)npairs), dim3(T), 0, s, pairs_dev, off, cursor, raw);
-  hipLaunchKernelGGL(k_topo_finish, dim3((unsigned)((nvtot + T - 1) / T)), dim3(T), 0, s, nvtot, off, cnt, raw, adj,
-                     deg, err_dev);
+  long long bv = (max_nv + T - 1) / T;
+  if (bv > 4096) bv = 4096;
+  if (bv < 1) bv = 1;
+  hipLaunchKernelGGL(k_topo_finish, dim3((unsigned)bv, (unsigned)npairs), dim3(T), 0, s, pairs_dev, off, cnt, raw,
+                     adj, deg, pstat, err_dev);
   MO_HIP(hipGetLastError());
   return MO_OK;
 }
