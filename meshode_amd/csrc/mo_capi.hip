@@ -163,19 +163,23 @@ int template_create(const T* V, int64_t nv, const int32_t* F, int64_t nf, int32_
   if ((rc = t->grid64.alloc(t->cells * sizeof(double)))) return rc;
   if ((rc = t->grid32.alloc(t->cells * sizeof(float)))) return rc;
   if ((rc = t->nearest.alloc(t->cells * sizeof(int)))) return rc;
-  size_t gws;
-  grid_workspace_bytes(nf, &gws);
+  GridSpec gs{nullptr, nv, F, nf, t->N, t->grid64.as<double>(), t->grid32.as<float>(), t->nearest.as<int>()};
+  const size_t gws = grid_batch_workspace_bytes(&gs, 1);
   Arena ar;
-  if ((rc = ar.reserve(al(nv * 3 * sizeof(double)) + al(8 * sizeof(double)) + al(gws) + 256, s))) return rc;
+  if ((rc = ar.reserve(al(nv * 3 * sizeof(double)) + al(8 * sizeof(double)) + al(xform_workspace_bytes(1)) + al(gws) +
+                           256,
+                       s)))
+    return rc;
   double* Vn = ar.take<double>(nv * 3);
   double* sp = ar.take<double>(8);
   int* err = (int*)(sp + 4);
+  void* xws = ar.take<char>(xform_workspace_bytes(1));
   void* ws = ar.take<char>(gws);
+  gs.Vn = Vn;
   if ((rc = zero_fill(err, sizeof(int), s))) return rc;
-  if ((rc = normalize_launch<T>(V, nv, sp, Vn, s))) return rc;
-  if ((rc = grid_build_launch(Vn, nv, F, nf, t->N, t->grid64.as<double>(), t->grid32.as<float>(),
-                              t->nearest.as<int>(), ws, s, err)))
-    return rc;
+  XformJob xj{(const void*)V, sizeof(T) == 8 ? 1 : 0, nv, sp, Vn};
+  if ((rc = xform_batch_launch(&xj, 1, true, xws, s))) return rc;
+  if ((rc = grid_build_batch_launch(&gs, 1, ws, s, err))) return rc;
   double host[5];
   MO_HIP(hipMemcpyAsync(host, sp, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
   MO_HIP(hipStreamSynchronize(s));
@@ -615,15 +619,15 @@ MO_EXPORT int mo_grid_build(const double* Vn, int64_t nv, const int32_t* F, int6
                             double* dist64, float* dist32, int32_t* nearest, void* stream) {
   MO_CHECK(Vn && F && nv > 0 && nf > 0, MO_ERR_ARG, "grid build needs vertices and faces");
   hipStream_t s = as_stream(stream);
-  size_t gws;
-  grid_workspace_bytes(nf, &gws);
+  GridSpec gs{Vn, nv, F, nf, grid_resolution, dist64, dist32, nearest};
+  const size_t gws = grid_batch_workspace_bytes(&gs, 1);
   Arena ar;
   int rc;
   if ((rc = ar.reserve(al(gws) + 256, s))) return rc;
   void* ws = ar.take<char>(gws);
   int* err = ar.take<int>(1);
   if ((rc = zero_fill(err, sizeof(int), s))) return rc;
-  if ((rc = grid_build_launch(Vn, nv, F, nf, grid_resolution, dist64, dist32, nearest, ws, s, err))) return rc;
+  if ((rc = grid_build_batch_launch(&gs, 1, ws, s, err))) return rc;
   return check_err_flag(err, s, "mo_grid_build");
 }
 
@@ -750,40 +754,46 @@ MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, co
   if (opt) o = *opt; else mo_lm_default_options(&o);
   hipStream_t s = as_stream(stream);
   // per-pair scratch: normalized target, (scale, pos), normalized source, FP32 grid
-  size_t bytes = 256;
-  size_t gmax = 0;
+  std::vector<GridSpec> gspecs(npairs);
+  size_t bytes = 512;
   for (int p = 0; p < npairs; ++p) {
-    MO_CHECK(tgt_nv[p] > 0 && tgt_nf[p] > 0, MO_ERR_ARG, "pair %d: empty target", p);
+    MO_CHECK(tgt[p] && tgt_faces[p] && tgt_nv[p] > 0 && tgt_nf[p] > 0, MO_ERR_ARG, "pair %d: empty target", p);
+    MO_CHECK(src_nv[p] >= 0 && (src[p] || src_nv[p] == 0), MO_ERR_ARG, "pair %d: bad source", p);
     MO_CHECK(grid_resolution[p] >= 2 && grid_resolution[p] <= 1024, MO_ERR_ARG, "pair %d: bad grid resolution", p);
     long long N = grid_resolution[p];
-    bytes += al(tgt_nv[p] * 3 * sizeof(double)) + al(8 * sizeof(double)) + al(src_nv[p] * 3 * sizeof(double)) +
-             al(N * N * N * sizeof(float));
-    size_t g;
-    grid_workspace_bytes(tgt_nf[p], &g);
-    gmax = std::max(gmax, g);
+    bytes += al(tgt_nv[p] * 3 * sizeof(double)) + al(8 * sizeof(double)) +
+             al(std::max<long long>(src_nv[p], 1) * 3 * sizeof(double)) + al(N * N * N * sizeof(float));
+    gspecs[p] = GridSpec{nullptr, tgt_nv[p], tgt_faces[p], tgt_nf[p], (int)N, nullptr, nullptr, nullptr};
   }
-  bytes += al(gmax) + 256;
+  const size_t gws = grid_batch_workspace_bytes(gspecs.data(), npairs);
+  bytes += al(gws) + 2 * al(xform_workspace_bytes(npairs));
   Arena ar;
   int rc;
   PhaseTimer timer;
   if ((rc = timer.start(o.timing != 0, s))) return rc;
   if ((rc = ar.reserve(bytes, s))) return rc;
   int* err = ar.take<int>(1);
-  void* gws = ar.take<char>(gmax);
+  void* gws_ptr = ar.take<char>(gws);
+  void* xws_t = ar.take<char>(xform_workspace_bytes(npairs));
+  void* xws_s = ar.take<char>(xform_workspace_bytes(npairs));
   if ((rc = zero_fill(err, sizeof(int), s))) return rc;
   std::vector<CorePair> pairs(npairs);
+  std::vector<XformJob> tj(npairs), sj(npairs);
   for (int p = 0; p < npairs; ++p) {
     const long long N = grid_resolution[p];
     double* tn = ar.take<double>(tgt_nv[p] * 3);
     double* sp = ar.take<double>(8);
     double* sn = ar.take<double>(std::max<long long>(src_nv[p], 1) * 3);
     float* g32 = ar.take<float>(N * N * N);
-    if ((rc = normalize_launch<double>(tgt[p], tgt_nv[p], sp, tn, s))) return rc;
-    if ((rc = grid_build_launch(tn, tgt_nv[p], tgt_faces[p], tgt_nf[p], (int)N, nullptr, g32, nullptr, gws, s, err)))
-      return rc;
-    if ((rc = apply_transform_launch<double>(src[p], src_nv[p], sp, sn, s))) return rc;
+    tj[p] = XformJob{(const void*)tgt[p], 1, tgt_nv[p], sp, tn};
+    sj[p] = XformJob{(const void*)src[p], 1, src_nv[p], sp, sn};
+    gspecs[p].Vn = tn;
+    gspecs[p].dist32 = g32;
     pairs[p] = CorePair{g32, (int)N, sn, src_nv[p], src_faces[p], src_nf[p], 0, lambda[p], sp, out[p]};
   }
+  if ((rc = xform_batch_launch(tj.data(), npairs, true, xws_t, s))) return rc;
+  if ((rc = grid_build_batch_launch(gspecs.data(), npairs, gws_ptr, s, err))) return rc;
+  if ((rc = xform_batch_launch(sj.data(), npairs, false, xws_s, s))) return rc;
   if ((rc = timer.mark(1))) return rc;
   if ((rc = lm_core(pairs, o, stats, s, timer))) return rc;
   if ((rc = timer.finish())) return rc;

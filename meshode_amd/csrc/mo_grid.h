@@ -16,20 +16,54 @@ struct GFace {
   int face;                                    // original face index
 };
 
-int grid_workspace_bytes(long long nf, size_t* bytes);
+// One distance grid to build: normalized FP64 target vertices [nv,3] and
+// int32 faces [nf,3] (device); any of dist64 / dist32 / nearest may be null.
+struct GridSpec {
+  const double* Vn;
+  long long nv;
+  const int* F;
+  long long nf;
+  int N;
+  double* dist64;
+  float* dist32;
+  int* nearest;
+};
 
-// Vn: normalized target vertices [nv,3] FP64 (device); F: [nf,3] int32
-// (device).  Any of dist64 / dist32 / nearest may be null.  err_dev: device
+// Device-side job record (workspace slices included).
+struct GridJob {
+  const double* V;
+  const int* F;
+  long long nv, nf, ntiles;
+  int N;
+  GFace* faces;
+  unsigned* bin;
+  int* perm;
+  double* tb;
+  double* dist64;
+  float* dist32;
+  int* nearest;
+};
+
+size_t grid_batch_workspace_bytes(const GridSpec* specs, int n);
+
+// Builds every grid of the batch with one launch per stage.  err_dev: device
 // int, bit 0 set when a face index is out of range.
-int grid_build_launch(const double* Vn, long long nv, const int* F, long long nf, int N, double* dist64,
-                      float* dist32, int* nearest, void* workspace, hipStream_t stream, int* err_dev);
+int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipStream_t stream, int* err_dev);
 
-// Mesh::Normalize: sp_dev <- (scale, pos[3]); Vn_dev (optional) <- normalized copy.
-template <class T>
-int normalize_launch(const T* V, long long nv, double* sp_dev, double* Vn_dev, hipStream_t stream);
+// Mesh::Normalize / Mesh::ApplyTransform jobs: V [n,3] (FP32 or FP64),
+// sp = (scale, pos[3]) in device memory, out = (V - pos) / scale (FP64).
+struct XformJob {
+  const void* V;
+  int f64;
+  long long n;
+  double* sp;
+  double* out;
+};
 
-// Mesh::ApplyTransform with device-resident (scale, pos).
-template <class T>
-int apply_transform_launch(const T* V, long long nv, const double* sp_dev, double* out, hipStream_t stream);
+size_t xform_workspace_bytes(int n);
+
+// compute_stats: also compute sp from V first (Mesh::Normalize); otherwise
+// sp is an input (Mesh::ApplyTransform).  out may be null when n == 0.
+int xform_batch_launch(const XformJob* jobs, int n, bool compute_stats, void* workspace, hipStream_t stream);
 
 }  // namespace mo

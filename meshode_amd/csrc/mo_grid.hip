@@ -37,7 +37,9 @@
 #include "mo_common.h"
 #include "mo_grid.h"
 
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace mo {
 
@@ -59,51 +61,71 @@ __device__ inline double dmin(double a, double b) { return b < a ? b : a; }
 __device__ inline double dmax(double a, double b) { return b > a ? b : a; }
 
 // Pass 1: validate indices and compute the centroid bin (Morton order of a
-// 16^3 lattice over [0,1]^3, clamped).
-__global__ void k_face_bins(const double* __restrict__ V, long long nv, const int* __restrict__ F, long long nf,
-                            unsigned* __restrict__ bin, int* __restrict__ err) {
-  long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
-  double c[3] = {0.0, 0.0, 0.0};
-  for (int j = 0; j < 3; ++j) {
-    int vi = F[f * 3 + j];
-    if (vi < 0 || vi >= nv) { atomicOr(err, 1); vi = 0; }
-    for (int a = 0; a < 3; ++a) c[a] += V[(long long)vi * 3 + a];
+// 16^3 lattice over [0,1]^3, clamped).  blockIdx.y = job.
+__global__ void k_face_bins(const GridJob* __restrict__ jobs, int* __restrict__ err) {
+  const GridJob J = jobs[blockIdx.y];
+  for (long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x; f < J.nf;
+       f += (long long)gridDim.x * blockDim.x) {
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < 3; ++j) {
+      int vi = J.F[f * 3 + j];
+      if (vi < 0 || vi >= J.nv) { atomicOr(err, 1); vi = 0; }
+      for (int a = 0; a < 3; ++a) c[a] += J.V[(long long)vi * 3 + a];
+    }
+    unsigned b3[3];
+    for (int a = 0; a < 3; ++a) {
+      int b = (int)floor(c[a] * (1.0 / 3.0) * (1 << kBinBits));
+      b = b < 0 ? 0 : (b > (1 << kBinBits) - 1 ? (1 << kBinBits) - 1 : b);
+      b3[a] = (unsigned)b;
+    }
+    J.bin[f] = spread3(b3[0]) | (spread3(b3[1]) << 1) | (spread3(b3[2]) << 2);
   }
-  unsigned b3[3];
-  for (int a = 0; a < 3; ++a) {
-    int b = (int)floor(c[a] * (1.0 / 3.0) * (1 << kBinBits));
-    b = b < 0 ? 0 : (b > (1 << kBinBits) - 1 ? (1 << kBinBits) - 1 : b);
-    b3[a] = (unsigned)b;
-  }
-  bin[f] = spread3(b3[0]) | (spread3(b3[1]) << 1) | (spread3(b3[2]) << 2);
 }
 
-// Pass 2: single-workgroup counting sort of face indices by bin.
-__global__ void __launch_bounds__(1024) k_bin_sort(const unsigned* __restrict__ bin, long long nf,
-                                                   int* __restrict__ perm) {
+// Pass 2: one workgroup per job counting-sorts face indices by bin (LDS
+// histogram, block-wide scan of the 4096 bins, LDS-atomic scatter).
+__global__ void __launch_bounds__(1024) k_bin_sort(const GridJob* __restrict__ jobs) {
+  const GridJob J = jobs[blockIdx.x];
   __shared__ unsigned hist[kBins];
+  __shared__ unsigned wsum[16];
   for (int b = threadIdx.x; b < kBins; b += blockDim.x) hist[b] = 0;
   __syncthreads();
-  for (long long f = threadIdx.x; f < nf; f += blockDim.x) atomicAdd(&hist[bin[f]], 1u);
+  for (long long f = threadIdx.x; f < J.nf; f += blockDim.x) atomicAdd(&hist[J.bin[f]], 1u);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned s = 0;
-    for (int b = 0; b < kBins; ++b) { unsigned c = hist[b]; hist[b] = s; s += c; }
+  // exclusive scan: each of the 1024 threads owns 4 consecutive bins
+  const int t = threadIdx.x;
+  unsigned v0 = hist[4 * t], v1 = hist[4 * t + 1], v2 = hist[4 * t + 2], v3 = hist[4 * t + 3];
+  unsigned mine = v0 + v1 + v2 + v3, incl = mine;
+  const int lane = t & 63, w = t >> 6;
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
   }
+  if (lane == 63) wsum[w] = incl;
   __syncthreads();
-  for (long long f = threadIdx.x; f < nf; f += blockDim.x) {
-    unsigned pos = atomicAdd(&hist[bin[f]], 1u);
-    perm[pos] = (int)f;
+  unsigned base = 0;
+  for (int q = 0; q < w; ++q) base += wsum[q];
+  unsigned ex = base + incl - mine;
+  hist[4 * t] = ex;
+  hist[4 * t + 1] = ex + v0;
+  hist[4 * t + 2] = ex + v0 + v1;
+  hist[4 * t + 3] = ex + v0 + v1 + v2;
+  __syncthreads();
+  for (long long f = threadIdx.x; f < J.nf; f += blockDim.x) {
+    unsigned pos = atomicAdd(&hist[J.bin[f]], 1u);
+    J.perm[pos] = (int)f;
   }
 }
 
 // Pass 3: FP64 face record of face perm[pos], written at its sorted position.
-__global__ void k_face_prep(const double* __restrict__ V, long long nv, const int* __restrict__ F, long long nf,
-                            const int* __restrict__ perm, GFace* __restrict__ faces) {
-  long long pos = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pos >= nf) return;
-  const long long f = perm[pos];
+__global__ void k_face_prep(const GridJob* __restrict__ jobs) {
+  const GridJob J = jobs[blockIdx.y];
+  const double* __restrict__ V = J.V;
+  const int* __restrict__ F = J.F;
+  const long long nv = J.nv;
+  for (long long pos = (long long)blockIdx.x * blockDim.x + threadIdx.x; pos < J.nf;
+       pos += (long long)gridDim.x * blockDim.x) {
+  const long long f = J.perm[pos];
   GFace g;
   int idx[3];
   double p[3][3];
@@ -145,22 +167,32 @@ __global__ void k_face_prep(const double* __restrict__ V, long long nv, const in
   g.loz = dmin(g.az, dmin(g.bz, g.cz)); g.hiz = dmax(g.az, dmax(g.bz, g.cz));
   g.flip = flip;
   g.face = (int)f;
-  faces[pos] = g;
+  J.faces[pos] = g;
+  }
 }
 
-__global__ void k_tile_bounds(const GFace* __restrict__ faces, long long nf, double* __restrict__ tb) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long ntiles = (nf + kTile - 1) / kTile;
-  if (t >= ntiles) return;
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  long long e = (t + 1) * kTile < nf ? (t + 1) * kTile : nf;
-  for (long long f = t * kTile; f < e; ++f) {
-    const GFace& g = faces[f];
-    lo[0] = dmin(lo[0], g.lox); hi[0] = dmax(hi[0], g.hix);
-    lo[1] = dmin(lo[1], g.loy); hi[1] = dmax(hi[1], g.hiy);
-    lo[2] = dmin(lo[2], g.loz); hi[2] = dmax(hi[2], g.hiz);
+// One wavefront per tile: lanes read the face AABBs, wave min/max.
+__global__ void __launch_bounds__(256) k_tile_bounds(const GridJob* __restrict__ jobs) {
+  const GridJob J = jobs[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= J.ntiles) return;
+  const long long f = t * kTile + lane;
+  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  if (lane < kTile && f < J.nf) {
+    const GFace* g = J.faces + f;
+    v[0] = g->lox; v[1] = g->loy; v[2] = g->loz; v[3] = g->hix; v[4] = g->hiy; v[5] = g->hiz;
   }
-  for (int a = 0; a < 3; ++a) { tb[t * 6 + a] = lo[a]; tb[t * 6 + 3 + a] = hi[a]; }
+  for (int o = 32; o > 0; o >>= 1)
+    for (int a = 0; a < 6; ++a) {
+      double y = __shfl_xor(v[a], o, 64);
+      v[a] = a < 3 ? dmin(v[a], y) : dmax(v[a], y);
+    }
+  if (lane < 6) {
+    double out = v[0];
+    for (int a = 1; a < 6; ++a) out = lane == a ? v[a] : out;
+    J.tb[t * 6 + lane] = out;
+  }
 }
 
 // Canonical edge squared distance (endpoints ordered by vertex index).
@@ -224,12 +256,20 @@ __device__ inline double box_gap2(const double* __restrict__ tb, long long t, do
 }
 
 // One wavefront (64 lanes) per 4x4x4 brick.
-__global__ void __launch_bounds__(64) k_grid_build(const GFace* __restrict__ faces, long long nf,
-                                                   const double* __restrict__ tb, long long ntiles, int N,
-                                                   int bricks_per_axis, double* __restrict__ dist64,
-                                                   float* __restrict__ dist32, int* __restrict__ nearest) {
+__global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ jobs) {
+  const GridJob& JJ = jobs[blockIdx.y];
+  const GFace* __restrict__ faces = JJ.faces;
+  const long long nf = JJ.nf;
+  const double* __restrict__ tb = JJ.tb;
+  const long long ntiles = JJ.ntiles;
+  const int N = JJ.N;
+  const int bricks_per_axis = (N + 3) / 4;
+  double* __restrict__ dist64 = JJ.dist64;
+  float* __restrict__ dist32 = JJ.dist32;
+  int* __restrict__ nearest = JJ.nearest;
   const int lane = threadIdx.x;
   long long b = blockIdx.x;
+  if (b >= (long long)bricks_per_axis * bricks_per_axis * bricks_per_axis) return;
   const int bx = (int)(b % bricks_per_axis);
   const int by = (int)((b / bricks_per_axis) % bricks_per_axis);
   const int bz = (int)(b / ((long long)bricks_per_axis * bricks_per_axis));
@@ -341,14 +381,18 @@ __global__ void __launch_bounds__(64) k_grid_build(const GFace* __restrict__ fac
 // ---------------------------------------------------------------------------
 // Normalization (Mesh::Normalize, mesh.cc:66-96) on device, FP64.
 
-template <class T>
-__global__ void __launch_bounds__(1024) k_normalize_stats(const T* __restrict__ V, long long n,
-                                                          double* __restrict__ out /* scale, pos[3] */) {
+__device__ inline double xf_read(const XformJob& J, long long i) {
+  return J.f64 ? ((const double*)J.V)[i] : (double)((const float*)J.V)[i];
+}
+
+// One workgroup per job: min / max per axis, then scale and pos.
+__global__ void __launch_bounds__(1024) k_normalize_stats(const XformJob* __restrict__ jobs) {
+  const XformJob J = jobs[blockIdx.x];
   __shared__ double smn[3][1024], smx[3][1024];
   double mn[3] = {1e30, 1e30, 1e30}, mx[3] = {-1e30, -1e30, -1e30};
-  for (long long i = threadIdx.x; i < n; i += blockDim.x)
+  for (long long i = threadIdx.x; i < J.n; i += blockDim.x)
     for (int a = 0; a < 3; ++a) {
-      double v = (double)V[i * 3 + a];
+      double v = xf_read(J, i * 3 + a);
       if (v < mn[a]) mn[a] = v;
       if (v > mx[a]) mx[a] = v;
     }
@@ -369,18 +413,19 @@ __global__ void __launch_bounds__(1024) k_normalize_stats(const T* __restrict__ 
     double m = ez > ey ? ez : ey;           // std::max(b, c)
     m = m > ex ? m : ex;                    // std::max(a, .)
     double scale = m * 1.1;
-    out[0] = scale;
-    for (int a = 0; a < 3; ++a) out[1 + a] = smn[a][0] - 0.05 * scale;
+    J.sp[0] = scale;
+    for (int a = 0; a < 3; ++a) J.sp[1 + a] = smn[a][0] - 0.05 * scale;
   }
 }
 
-template <class T>
-__global__ void k_apply_transform(const T* __restrict__ V, long long n, const double* __restrict__ sp,
-                                  double* __restrict__ out) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * 3) return;
-  int a = (int)(i % 3);
-  out[i] = ((double)V[i] - sp[1 + a]) / sp[0];
+// out = (V - pos) / scale with the job's (scale, pos); blockIdx.y = job.
+__global__ void k_apply_transform(const XformJob* __restrict__ jobs) {
+  const XformJob J = jobs[blockIdx.y];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < J.n * 3;
+       i += (long long)gridDim.x * blockDim.x) {
+    int a = (int)(i % 3);
+    J.out[i] = (xf_read(J, i) - J.sp[1 + a]) / J.sp[0];
+  }
 }
 
 }  // namespace
@@ -388,63 +433,71 @@ __global__ void k_apply_transform(const T* __restrict__ V, long long n, const do
 // ---------------------------------------------------------------------------
 // Host launchers.
 
-int grid_workspace_bytes(long long nf, size_t* bytes) {
-  long long ntiles = (nf + kTile - 1) / kTile;
-  *bytes = (size_t)nf * sizeof(GFace) + 256 + (size_t)nf * sizeof(unsigned) * 2 + 256 +
-           (size_t)ntiles * 6 * sizeof(double) + 256;
-  return MO_OK;
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t grid_job_workspace_bytes(long long nf) {
+  const long long ntiles = (nf + kTile - 1) / kTile;
+  return al256((size_t)nf * sizeof(GFace)) + al256((size_t)nf * sizeof(unsigned)) + al256((size_t)nf * sizeof(int)) +
+         al256((size_t)ntiles * 6 * sizeof(double));
 }
 
-int grid_build_launch(const double* Vn, long long nv, const int* F, long long nf, int N, double* dist64,
-                      float* dist32, int* nearest, void* workspace, hipStream_t stream, int* err_dev) {
-  MO_CHECK(N >= 2 && N <= 2048, MO_ERR_ARG, "grid resolution %d out of range [2, 2048]", N);
-  MO_CHECK(nf >= 1, MO_ERR_ARG, "distance field needs at least one triangle (got %lld)", nf);
-  auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
+int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipStream_t stream, int* err_dev) {
+  if (n <= 0) return MO_OK;
+  std::vector<GridJob> jobs(n);
   char* w = (char*)workspace;
-  GFace* faces = (GFace*)w; w += align((size_t)nf * sizeof(GFace));
-  unsigned* bin = (unsigned*)w; w += align((size_t)nf * sizeof(unsigned));
-  int* perm = (int*)w; w += align((size_t)nf * sizeof(int));
-  double* tb = (double*)w;
-  long long ntiles = (nf + kTile - 1) / kTile;
+  GridJob* d_jobs = (GridJob*)w;
+  w += al256((size_t)n * sizeof(GridJob));
+  long long max_nf = 0, max_tiles = 0, max_bricks = 0;
+  for (int q = 0; q < n; ++q) {
+    const GridSpec& sp = specs[q];
+    MO_CHECK(sp.N >= 2 && sp.N <= 2048, MO_ERR_ARG, "grid resolution %d out of range [2, 2048]", sp.N);
+    MO_CHECK(sp.nf >= 1, MO_ERR_ARG, "distance field needs at least one triangle (got %lld)", sp.nf);
+    GridJob& J = jobs[q];
+    J.V = sp.Vn; J.F = sp.F; J.nv = sp.nv; J.nf = sp.nf; J.N = sp.N;
+    J.ntiles = (sp.nf + kTile - 1) / kTile;
+    J.faces = (GFace*)w; w += al256((size_t)sp.nf * sizeof(GFace));
+    J.bin = (unsigned*)w; w += al256((size_t)sp.nf * sizeof(unsigned));
+    J.perm = (int*)w; w += al256((size_t)sp.nf * sizeof(int));
+    J.tb = (double*)w; w += al256((size_t)J.ntiles * 6 * sizeof(double));
+    J.dist64 = sp.dist64; J.dist32 = sp.dist32; J.nearest = sp.nearest;
+    const long long bpa = (sp.N + 3) / 4;
+    max_nf = std::max(max_nf, sp.nf);
+    max_tiles = std::max(max_tiles, J.ntiles);
+    max_bricks = std::max(max_bricks, bpa * bpa * bpa);
+  }
+  MO_HIP(hipMemcpyAsync(d_jobs, jobs.data(), (size_t)n * sizeof(GridJob), hipMemcpyHostToDevice, stream));
   const int T = 256;
-  const unsigned fb = (unsigned)((nf + T - 1) / T);
-  hipLaunchKernelGGL(k_face_bins, dim3(fb), dim3(T), 0, stream, Vn, nv, F, nf, bin, err_dev);
-  hipLaunchKernelGGL(k_bin_sort, dim3(1), dim3(1024), 0, stream, bin, nf, perm);
-  hipLaunchKernelGGL(k_face_prep, dim3(fb), dim3(T), 0, stream, Vn, nv, F, nf, perm, faces);
-  hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((ntiles + T - 1) / T)), dim3(T), 0, stream, faces, nf, tb);
-  int bpa = (N + 3) / 4;
-  long long nb = (long long)bpa * bpa * bpa;
-  hipLaunchKernelGGL(k_grid_build, dim3((unsigned)nb), dim3(64), 0, stream, faces, nf, tb, ntiles, N, bpa, dist64,
-                     dist32, nearest);
+  const unsigned fb = (unsigned)std::min<long long>((max_nf + T - 1) / T, 1024);
+  hipLaunchKernelGGL(k_face_bins, dim3(fb, n), dim3(T), 0, stream, d_jobs, err_dev);
+  hipLaunchKernelGGL(k_bin_sort, dim3(n), dim3(1024), 0, stream, d_jobs);
+  hipLaunchKernelGGL(k_face_prep, dim3(fb, n), dim3(T), 0, stream, d_jobs);
+  hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((max_tiles + 3) / 4), n), dim3(256), 0, stream, d_jobs);
+  hipLaunchKernelGGL(k_grid_build, dim3((unsigned)max_bricks, n), dim3(64), 0, stream, d_jobs);
   MO_HIP(hipGetLastError());
   return MO_OK;
 }
 
-template <class T>
-int normalize_launch(const T* V, long long nv, double* sp_dev, double* Vn_dev, hipStream_t stream) {
-  hipLaunchKernelGGL(k_normalize_stats<T>, dim3(1), dim3(1024), 0, stream, V, nv, sp_dev);
-  if (Vn_dev && nv > 0) {
-    long long n3 = nv * 3;
-    hipLaunchKernelGGL(k_apply_transform<T>, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, stream, V, nv,
-                       sp_dev, Vn_dev);
+size_t grid_batch_workspace_bytes(const GridSpec* specs, int n) {
+  size_t b = al256((size_t)n * sizeof(GridJob));
+  for (int q = 0; q < n; ++q) b += grid_job_workspace_bytes(specs[q].nf);
+  return b + 256;
+}
+
+size_t xform_workspace_bytes(int n) { return al256((size_t)n * sizeof(XformJob)) + 256; }
+
+int xform_batch_launch(const XformJob* jobs, int n, bool compute_stats, void* workspace, hipStream_t stream) {
+  if (n <= 0) return MO_OK;
+  XformJob* d_jobs = (XformJob*)workspace;
+  long long max_n = 0;
+  for (int q = 0; q < n; ++q) max_n = std::max(max_n, jobs[q].n);
+  MO_HIP(hipMemcpyAsync(d_jobs, jobs, (size_t)n * sizeof(XformJob), hipMemcpyHostToDevice, stream));
+  if (compute_stats) hipLaunchKernelGGL(k_normalize_stats, dim3(n), dim3(1024), 0, stream, d_jobs);
+  if (max_n > 0) {
+    const unsigned bx = (unsigned)std::min<long long>((max_n * 3 + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_apply_transform, dim3(bx, n), dim3(256), 0, stream, d_jobs);
   }
   MO_HIP(hipGetLastError());
   return MO_OK;
 }
-
-template <class T>
-int apply_transform_launch(const T* V, long long nv, const double* sp_dev, double* out, hipStream_t stream) {
-  if (nv <= 0) return MO_OK;
-  long long n3 = nv * 3;
-  hipLaunchKernelGGL(k_apply_transform<T>, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, stream, V, nv, sp_dev,
-                     out);
-  MO_HIP(hipGetLastError());
-  return MO_OK;
-}
-
-template int normalize_launch<float>(const float*, long long, double*, double*, hipStream_t);
-template int normalize_launch<double>(const double*, long long, double*, double*, hipStream_t);
-template int apply_transform_launch<float>(const float*, long long, const double*, double*, hipStream_t);
-template int apply_transform_launch<double>(const double*, long long, const double*, double*, hipStream_t);
 
 }  // namespace mo

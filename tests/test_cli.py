@@ -1,0 +1,66 @@
+"""rigid_deform CLI drop-in (rigid_deform.cc:12-63)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from meshode_amd import objio, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "meshode_amd", "bin", "rigid_deform")
+
+
+def test_usage_exits_zero():
+    """argc < 5 prints the usage line and exits 0 (rigid_deform.cc:13-18); no GPU needed."""
+    r = subprocess.run([CLI, "a.obj", "b.obj", "c.obj"], capture_output=True, text=True)
+    assert r.returncode == 0
+    assert r.stdout.startswith("./rigid_deform source.obj reference.obj output.obj [GRID_RESOLUTION=64]")
+
+
+def test_obj_roundtrip_rules(tmp_path):
+    """objio restates Mesh::ReadOBJ / WriteOBJ (mesh.cc:14-64)."""
+    p = tmp_path / "m.obj"
+    p.write_text("# c\nmtllib x\nv 1 2 3\nv 4 5 6\nvn 0 0 1\nv 7 8 9\nf 1//1 2//1 3//1\nf 3/1/2 2 1\nf 1 2\n")
+    V, F = objio.read_obj(str(p))
+    assert V.shape == (3, 3) and np.array_equal(V[2], [7, 8, 9])
+    assert F.tolist() == [[0, 1, 2], [2, 1, 0], [0, 1, 1]]      # missing token reuses the previous one
+    q = tmp_path / "o.obj"
+    objio.write_obj(str(q), V * 0.1234567891, F)
+    assert q.read_text().splitlines()[0] == "v 0.123457 0.246914 0.37037"
+    assert q.read_text().splitlines()[3] == "f 1 2 3"
+
+
+@pytest.mark.gpu
+def test_cli_matches_library_and_oracle(gpu, oracle, tmp_path):
+    print("[cli] case=box2000_n32", flush=True)
+    sV, sF, tV, tF = synth.box_union_pair(31, n_boxes=6, n_verts=2000)
+    src, tgt, out = tmp_path / "s.obj", tmp_path / "t.obj", tmp_path / "o.obj"
+    objio.write_obj(str(src), sV, sF)
+    objio.write_obj(str(tgt), tV, tF)
+    r = subprocess.run([CLI, str(src), str(tgt), str(out), "32", "5000", "1.0"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0] == "Source:\t\tNum vertices: 2000\tNum faces: %d" % len(sF)
+    assert lines[1] == "Reference:\tNum vertices: 2000\tNum faces: %d" % len(tF)
+    assert lines[-1] == "Deformed"
+    final = float(re.search(r"Final cost: (\S+)", r.stdout).group(1))
+    # the same pair through the library (inputs parsed back from the OBJ text)
+    from meshode_amd import deform
+    sV2, sF2 = objio.read_obj(str(src))
+    tV2, tF2 = objio.read_obj(str(tgt))
+    outs, st = deform.rigid_deform_batch([(sV2, sF2, tV2, tF2)], grid_resolution=32, lambdas=1.0)
+    Vo, Fo = objio.read_obj(str(out))
+    assert np.array_equal(Fo, sF2)
+    assert np.allclose(Vo, outs[0].cpu().numpy(), rtol=1e-5, atol=1e-6)
+    assert final == pytest.approx(st[0]["final_cost"], rel=1e-5)
+    # energy parity with the oracle on the same inputs
+    tn, sc, pos = oracle.normalize(tV2)
+    grid, _, _ = oracle.build_grid_bvh(tn, tF2, 32)
+    sn = oracle.apply_transform(sV2, sc, pos)
+    E = oracle.face_edges(sF2)
+    _, so, _ = oracle.lm_solve(grid, 32, sn, E, 1.0)
+    eg, _, _ = oracle.lm_cost(grid, 32, (outs[0].cpu().numpy() - pos) / sc, sn, E, 1.0)
+    assert abs(eg - so["final_cost"]) <= 1e-3 * so["final_cost"]

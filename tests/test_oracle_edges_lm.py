@@ -1,0 +1,164 @@
+"""Pins the oracle's edge-rigidity losses (rigid_layer.cc, graph_layer.cc)
+and its Levenberg-Marquardt restatement (deformer.cc:18-92) on the CPU.
+
+Edge losses: closed form, zero at rest and under translation, gradients that
+sum to zero and match central differences of 1/2 sum R.  LM: the first step
+is re-derived independently in numpy (Jacobian, Jacobi scaling, LM diagonal,
+dense solve, model cost change) and must match the oracle's trace; a linear
+residual converges to zero; a translated target is recovered; accepted steps
+decrease the cost; lambda = 0 leaves no rigidity cost."""
+import numpy as np
+import pytest
+
+from meshode_amd import synth
+
+
+def _tet():
+    V = np.array([[0.1, 0.2, 0.3], [0.4, 0.2, 0.3], [0.1, 0.6, 0.3], [0.2, 0.3, 0.7]], np.float32)
+    F = np.array([[0, 1, 2], [0, 3, 1], [1, 3, 2], [2, 3, 0]], np.int32)
+    return V, F
+
+
+def test_rigid_closed_form_and_invariants(oracle):
+    V, F = _tet()
+    off = oracle.store_rigidity(V, F)
+    assert np.array_equal(off[0], V[1] - V[0]) and np.array_equal(off[2], V[0] - V[2])
+    assert np.all(oracle.rigid_fwd(V, F, off) == 0)
+    assert np.all(oracle.rigid_bwd(V, F, off) == 0)
+    T = V + np.float32(0.25)
+    assert np.allclose(oracle.rigid_fwd(T, F, off), 0, atol=1e-12)
+    W = V.copy()
+    W[3] += np.array([0.05, -0.02, 0.01], np.float32)
+    fwd = oracle.rigid_fwd(W, F, off)
+    # half-edge (f=1, j=0): a=0, b=3: (V3' - V0 - (V3 - V0))^2 = delta^2
+    assert np.allclose(fwd[3], np.array([0.05, -0.02, 0.01]) ** 2, rtol=1e-5)
+    g = oracle.rigid_bwd(W, F, off)
+    assert np.allclose(g.sum(0), 0, atol=1e-6)
+
+
+def test_rigid_gradient_central_differences(oracle):
+    V, F = synth.icosphere(2)
+    V = (V * 0.3 + 0.5).astype(np.float64)
+    off = oracle.store_rigidity(V.astype(np.float32), F).astype(np.float64)
+    rng = np.random.default_rng(0)
+    W = V + rng.normal(0, 0.01, V.shape)
+
+    def loss(X):
+        a, b = F.reshape(-1), F[:, [1, 2, 0]].reshape(-1)
+        r = X[b] - X[a] - off
+        return 0.5 * np.sum(r * r)
+
+    g = oracle.rigid_bwd(W.astype(np.float32), F, off.astype(np.float32)).astype(np.float64)
+    h = 1e-6
+    for idx in rng.choice(len(V), 12, replace=False):
+        for k in range(3):
+            Wp, Wm = W.copy(), W.copy()
+            Wp[idx, k] += h
+            Wm[idx, k] -= h
+            fd = (loss(Wp) - loss(Wm)) / (2 * h)
+            assert g[idx, k] == pytest.approx(fd, rel=2e-3, abs=2e-5)
+
+
+def test_graph_loss_closed_form(oracle):
+    rng = np.random.default_rng(1)
+    V = rng.uniform(0, 1, (20, 3)).astype(np.float32)
+    E = np.array([[0, 1], [1, 2], [2, 0], [5, 5], [3, 4]], np.int32)
+    off = oracle.store_graph(V, E)
+    assert np.array_equal(off[0], V[1] - V[0])
+    W = V.copy()
+    W[1] += np.float32(0.1)
+    fwd = oracle.graph_fwd(W, E, off)
+    assert np.allclose(fwd[0], 0.01, rtol=1e-5) and np.allclose(fwd[1], 0.01, rtol=1e-5)
+    assert np.all(fwd[3] == 0)          # degenerate edge (a == b)
+    g = oracle.graph_bwd(W, E, off)
+    # vertex 1 is the head of edge 0 (+r) and the tail of edge 1 (-r)
+    assert np.allclose(g[1], 0.1 + 0.1, rtol=1e-5)
+    assert np.allclose(g.sum(0), 0, atol=1e-6)
+
+
+def _small_problem(oracle, N=10):
+    _, _, tV, tF = synth.displaced_sphere_pair(9, src_level=1, tgt_level=2)
+    sV, sF = synth.icosphere(1)
+    tn, sc, pos = oracle.normalize(tV)
+    grid, _, _ = oracle.build_grid(tn, tF, N)
+    sn = oracle.apply_transform(sV * 0.8, sc, pos)
+    return grid, sn, sF
+
+
+def test_first_lm_step_matches_dense_rederivation(oracle):
+    """Re-derive the first LM step of deformer.cc:73 (Ceres LM, exact solve)
+    in numpy and compare the model cost change and candidate cost."""
+    N = 10
+    grid, x0, F = _small_problem(oracle, N)
+    E = oracle.face_edges(F)
+    lam = 1.0
+    _, st, tr = oracle.lm_solve(grid, N, x0, E, lam, trace=True)
+    n = x0.size
+    d, g = oracle.lookup_f64(grid, N, x0)
+    # residuals and Jacobian: per vertex [d, 0, 0], per half-edge lam (x_a - x_b - v)
+    rows_f, rows_J = [], []
+    for u in range(len(x0)):
+        J = np.zeros((3, n)); J[0, 3 * u:3 * u + 3] = g[u]
+        rows_J.append(J); rows_f.append(np.array([d[u], 0.0, 0.0]))
+    for a, b in E:
+        J = np.zeros((3, n))
+        J[:, 3 * a:3 * a + 3] = lam * np.eye(3)
+        J[:, 3 * b:3 * b + 3] = -lam * np.eye(3)
+        rows_J.append(J); rows_f.append(np.zeros(3))     # v_h = x0_a - x0_b: zero residual at x0
+    J = np.concatenate(rows_J); f = np.concatenate(rows_f)
+    cn = (J * J).sum(0)
+    s = 1.0 / (1.0 + np.sqrt(cn))
+    Js = J * s
+    diag = np.clip((Js * Js).sum(0), 1e-6, 1e32)
+    radius = 1e4
+    A = Js.T @ Js + np.diag(diag / radius)
+    y = np.linalg.solve(A, Js.T @ f)
+    delta = s * (-y)
+    m = J @ delta
+    mcc = -m @ (f + m / 2)
+    cand, _, _ = oracle.lm_cost(grid, N, (x0.reshape(-1) + delta).reshape(-1, 3), x0, E, lam)
+    assert tr[0, 3] == pytest.approx(mcc, rel=1e-9)
+    assert tr[0, 2] == pytest.approx(cand, rel=1e-9)
+    assert tr[0, 1] == pytest.approx(0.5 * d @ d, rel=1e-12)
+    assert st["initial_cost"] == pytest.approx(0.5 * d @ d, rel=1e-12)
+
+
+def test_linear_residual_converges(oracle):
+    """One vertex, no edges, on an affine field (a linear residual):
+    Gauss-Newton reaches d = 0."""
+    N = 16
+    k = np.arange(N) / N
+    Z, Y, X = np.meshgrid(k, k, k, indexing="ij")
+    grid = (0.1 * (X - 0.5)).reshape(-1)
+    x, st, _ = oracle.lm_solve(grid, N, np.array([[0.6, 0.5, 0.5]]), np.zeros((0, 2), np.int32), 1.0)
+    assert st["final_cost"] < 1e-20
+    assert x[0, 0] == pytest.approx(0.5, abs=1e-9)
+    assert x[0, 1] == 0.5 and x[0, 2] == 0.5
+
+
+def test_translation_recovered_and_monotone(oracle):
+    V, F = synth.icosphere(2)
+    N = 32
+    tgt = V * 0.3 + 0.5
+    grid, _, _ = oracle.build_grid(tgt, F, N)
+    shift = np.array([0.02, -0.015, 0.01])
+    src = tgt + shift
+    E = oracle.face_edges(F)
+    x, st, tr = oracle.lm_solve(grid, N, src, E, 1.0, trace=True)
+    # the trilinear field is not zero on the surface (interpolated |d| has a
+    # floor of a fraction of a cell), so compare with the solve started on
+    # the target itself: same minimum, reached by translating back
+    _, st_t, _ = oracle.lm_solve(grid, N, tgt, E, 1.0)
+    assert st["final_cost"] == pytest.approx(st_t["final_cost"], rel=1e-3)
+    moved = (x - src).mean(0)
+    assert np.allclose(moved, -shift, atol=1e-3)
+    acc = tr[tr[:, 6] == 1]
+    assert np.all(acc[:, 2] < acc[:, 1])
+
+
+def test_lambda_zero_has_no_rigidity(oracle):
+    grid, x0, F = _small_problem(oracle)
+    E = oracle.face_edges(F)
+    x, st, _ = oracle.lm_solve(grid, 10, x0, E, 0.0)
+    assert st["rigidity_cost"] == 0.0
+    assert st["vertex_cost"] < st["initial_cost"]
