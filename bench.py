@@ -67,6 +67,12 @@ def main():
     ap.add_argument("--block-threads", type=int, default=1024, help="threads per pair of the global-memory kernel")
     ap.add_argument("--kernel-variant", type=int, default=0, help="0: register/LDS kernel when a pair fits; 1: global")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode"],
+                    help="batch: the headline line; serve / adam / ode: the serving, pyDeform-Adam and "
+                         "NeuralODE workloads (their own JSON line)")
+    ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of Poisson arrivals")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: requests per second")
+    ap.add_argument("--max-verts", type=int, default=8000, help="serve: largest source size")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -82,6 +88,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     _lib.load()
+    if args.mode != "batch":
+        print(json.dumps({"serve": bench_serve, "adam": bench_adam, "ode": bench_ode}[args.mode](args)), flush=True)
+        return
 
     t0 = time.perf_counter()
     distinct = [synth.box_union_pair(s, n_verts=args.verts) for s in range(min(args.distinct, args.pairs))]
@@ -210,6 +219,122 @@ def main():
                                         "oracle_iterations": so["iterations"],
                                         "gpu_iterations": stats_all[-1][0]["iterations"]}
     print(json.dumps(result), flush=True)
+
+
+def bench_serve(args):
+    """Poisson arrivals of config-1/config-2-like pairs of uneven sizes and
+    grids through the continuous-batching server; p50/p99 latency and pairs/s."""
+    import numpy as np
+    from meshode_amd import serve, synth
+    rng = np.random.default_rng(7)
+    pool = []
+    for i in range(24):
+        nv = int(np.exp(rng.uniform(np.log(2000), np.log(args.max_verts))))
+        if i % 2 == 0:
+            p = synth.box_union_pair(1000 + i, n_verts=max(nv, 1000))
+        else:
+            lvl = 3 if nv < 4000 else 4
+            p = synth.displaced_sphere_pair(1000 + i, src_level=lvl, tgt_level=lvl + 1)
+        grid = int(rng.choice([64, 128, 256]))
+        pool.append((p, grid))
+    n_req = int(args.rate * args.duration)
+    gaps = rng.exponential(1.0 / args.rate, n_req)
+    picks = rng.integers(0, len(pool), n_req)
+    srv = serve.DeformServer(mem_budget_bytes=200e9, max_batch=512, workers=2)
+    # warm the template cache and the kernels on every pool entry
+    warm = [srv.submit(serve.DeformRequest(p[0], p[1], i, p[2], p[3], grid=g)) for i, (p, g) in enumerate(pool)]
+    for f in warm:
+        f.result()
+    futs = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for gap, k in zip(gaps, picks):
+        t_next += gap
+        now = time.perf_counter()
+        if t_next > now:
+            time.sleep(t_next - now)
+        (p, g) = pool[k]
+        futs.append(srv.submit(serve.DeformRequest(p[0], p[1], int(k), p[2], p[3], grid=g)))
+    res = [f.result() for f in futs]
+    t1 = time.perf_counter()
+    srv.close()
+    lat = np.array([r.latency_s for r in res]) * 1e3
+    return {"metric": "serving: deformation pairs/sec with p50/p99 latency (Poisson arrivals)",
+            "value": len(res) / (t1 - t0), "unit": "pairs/s", "requests": len(res),
+            "offered_rate": args.rate, "duration_s": t1 - t0,
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+            "mean_batch": float(np.mean(srv.batches)) if srv.batches else 0.0,
+            "config": {"workload": "serve", "sources": "2k-%d verts log-uniform, box/sphere 50/50" % args.max_verts,
+                       "grids": [64, 128, 256], "distinct_targets": len(pool), "template_cache": "LRU, 200 GB"}}
+
+
+def bench_adam(args):
+    """Config 2: RigidLossLayer + Adam (lr 1e-3) on a 5000-vertex box union vs
+    its target at 128^3; steps/s of the forward/backward/Adam loop."""
+    import numpy as np
+    import torch
+    from meshode_amd import layers, synth
+    sV, sF, tV, tF = synth.box_union_pair(0, n_verts=5000)
+    src = torch.from_numpy(sV.astype(np.float32)).cuda()
+    F = torch.from_numpy(sF).cuda()
+    layer = layers.RigidLossLayer(src, F, torch.from_numpy(tV.astype(np.float32)).cuda(),
+                                  torch.from_numpy(tF).cuda(), grid_resolution=128)
+    V = torch.nn.Parameter(src.clone())
+    opt = torch.optim.Adam([V], lr=1e-3)
+    steps = max(args.steps, 1) * 100
+    for _ in range(20):
+        opt.zero_grad(); layer(V, F).backward(); opt.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        opt.zero_grad()
+        loss = layer(V, F)
+        loss.backward()
+        opt.step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"metric": "pyDeform Adam steps/sec (5000 verts, 128^3, rigidity 1)", "value": steps / dt,
+            "unit": "steps/s", "steps": steps, "final_loss": float(loss.item()),
+            "config": {"workload": "config 2: RigidLossLayer + Adam lr 1e-3", "verts": 5000, "grid": 128}}
+
+
+def bench_ode(args):
+    """Config 3: NeuralODE flow (3-256-256-256-3 softplus, RK4 x 16, adjoint)
+    pushing 50k surface points + the 5000 mesh vertices, loss on a 128^3 grid
+    + rigidity 0.1; optimizer steps/s."""
+    import numpy as np
+    import torch
+    from meshode_amd import neuralode, pyDeform, synth
+    sV, sF, tV, tF = synth.box_union_pair(0, n_verts=5000)
+    pid = pyDeform.InitializeDeformTemplate(torch.from_numpy(tV.astype(np.float32)), torch.from_numpy(tF), 0, 128)
+    V = torch.from_numpy(sV.astype(np.float32))
+    pyDeform.NormalizeByTemplate(V, pid)
+    rng = np.random.default_rng(0)
+    f = rng.integers(0, len(sF), 50000)
+    w = rng.dirichlet([1, 1, 1], 50000)
+    pts = (V.numpy()[sF[f]] * w[:, :, None]).sum(1).astype(np.float32)      # uniform-in-face samples
+    P = torch.cat([V, torch.from_numpy(pts)]).cuda()
+    E = torch.from_numpy(np.concatenate([sF[:, [0, 1]], sF[:, [1, 2]], sF[:, [2, 0]]]).astype(np.int32)).cuda()
+    model = neuralode.NeuralODEDeformer(P, E, pid, n_mesh=len(V), rho=0.1, steps=16)
+    opt = torch.optim.Adam(model.field.parameters(), lr=1e-3)
+    for _ in range(2):
+        opt.zero_grad(); model()[0].backward(); opt.step()
+    torch.cuda.synchronize()
+    steps = max(args.steps, 1) * 5
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        opt.zero_grad()
+        loss, _ = model()
+        loss.backward()
+        opt.step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    flops = 2 * P.shape[0] * (3 * 256 + 2 * 256 * 256 + 256 * 3) * 64 * 4     # fwd + recompute + 2x bwd
+    return {"metric": "NeuralODE optimizer steps/sec (55k points, RK4x16, 128^3)", "value": steps / dt,
+            "unit": "steps/s", "steps": steps, "final_loss": float(loss.item()),
+            "approx_mlp_tflops": flops * steps / dt / 1e12,
+            "config": {"workload": "config 3: velocity MLP 3-256-256-256-3 softplus, RK4x16, adjoint", "points":
+                       int(P.shape[0]), "grid": 128}}
 
 
 def _cpu_model():

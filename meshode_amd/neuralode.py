@@ -1,0 +1,152 @@
+"""NeuralODE deformation (the reference's neuralode_fast.py:7-64 and
+cad_neural_deform2.py:40-108, in the configuration of the config-3 workload).
+
+* ``VelocityMLP``: autonomous velocity field v(y) = MLP(y - 0.5),
+  3 -> 256 -> 256 -> 256 -> 3 with softplus, Xavier-uniform weights from a
+  seed, zero biases.  The reference centres its input the same way
+  (neuralode_fast.py:28-32; it also feeds t, which this field does not use).
+  The 256x256 layers are FP32 GEMMs (hipBLASLt, on the MFMA units).
+* ``odeint_rk4``: fixed-grid RK4 over t in [0, 1] with ``steps`` equal steps
+  using the 3/8-rule step that torchdiffeq's ``method="rk4"`` applies
+  (the reference calls ``odeint(..., method="rk4")``, neuralode_fast.py:55-60):
+      k1 = f(y); k2 = f(y + h k1/3); k3 = f(y + h (k2 - k1/3));
+      k4 = f(y + h (k1 - k2 + k3)); y' = y + h (k1 + 3 k2 + 3 k3 + k4) / 8.
+* ``RK4Adjoint``: the gradient of the discretized flow (discrete adjoint).
+  The forward keeps only the state at each step boundary (one checkpoint
+  per step); the backward recomputes one step at a time from its
+  checkpoint and back-propagates through its four stages, so memory is one
+  step's activations instead of all ``4 * steps`` evaluations.
+* ``DistanceRigidityLoss``: 1/2 sum d(y)^2 over all pushed points (the
+  pyDeform distance kernels) + 1/2 rho^2 sum ||(y_b - y_a) - (y0_b - y0_a)||^2
+  over the source mesh edges (the graph-edge kernels).
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+from torch import nn
+from torch.autograd import Function
+
+from . import pyDeform
+
+
+class VelocityMLP(nn.Module):
+    def __init__(self, hidden=256, seed=0, dtype=torch.float32):
+        super().__init__()
+        g = torch.Generator().manual_seed(seed)
+        dims = [3, hidden, hidden, hidden, 3]
+        self.weights = nn.ParameterList()
+        self.biases = nn.ParameterList()
+        for a, b in zip(dims[:-1], dims[1:]):
+            bound = math.sqrt(6.0 / (a + b))
+            self.weights.append(nn.Parameter((torch.rand(b, a, generator=g, dtype=torch.float64) * 2 - 1)
+                                             .mul_(bound).to(dtype)))
+            self.biases.append(nn.Parameter(torch.zeros(b, dtype=dtype)))
+
+    def forward(self, y):
+        h = y - 0.5
+        n = len(self.weights)
+        for i, (W, b) in enumerate(zip(self.weights, self.biases)):
+            h = torch.addmm(b, h, W.t())
+            if i < n - 1:
+                h = nn.functional.softplus(h)
+        return h
+
+
+def rk4_step(f, y, h):
+    """One 3/8-rule RK4 step (torchdiffeq's ``rk4`` fixed-grid step)."""
+    k1 = f(y)
+    k2 = f(y + h * k1 / 3.0)
+    k3 = f(y + h * (k2 - k1 / 3.0))
+    k4 = f(y + h * (k1 - k2 + k3))
+    return y + h * (k1 + 3.0 * (k2 + k3) + k4) * 0.125
+
+
+class RK4Adjoint(Function):
+    """y1 = Phi(y0; theta) for ``steps`` RK4 steps; backward by recomputation
+    from per-step checkpoints."""
+
+    @staticmethod
+    def forward(ctx, y0, steps, func, *params):
+        h = 1.0 / steps
+        ys = [y0.detach()]
+        with torch.no_grad():
+            y = y0.detach()
+            for _ in range(steps):
+                y = rk4_step(func, y, h)
+                ys.append(y)
+        ctx.func, ctx.steps, ctx.h = func, steps, h
+        ctx.save_for_backward(*ys[:-1])
+        ctx.n_params = len(params)
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        ys = ctx.saved_tensors
+        params = [p for p in ctx.func.parameters()]
+        gp = [torch.zeros_like(p) for p in params]
+        for n in reversed(range(ctx.steps)):
+            with torch.enable_grad():
+                y = ys[n].detach().requires_grad_(True)
+                y1 = rk4_step(ctx.func, y, ctx.h)
+                grads = torch.autograd.grad(y1, [y] + params, gy, allow_unused=True)
+            gy = grads[0]
+            for i, g in enumerate(grads[1:]):
+                if g is not None:
+                    gp[i] += g
+        return (gy, None, None, *gp)
+
+
+def odeint_rk4(func: nn.Module, y0, steps=16):
+    """Integrate dy/dt = func(y) over [0, 1] with ``steps`` RK4 steps."""
+    return RK4Adjoint.apply(y0, steps, func, *func.parameters())
+
+
+class _DistanceLoss(Function):
+    @staticmethod
+    def forward(ctx, y, pid):
+        ctx.save_for_backward(y)
+        ctx.pid = pid
+        return (pyDeform.DistanceFieldLoss_forward(y.contiguous(), pid) * 0.5).sum()
+
+    @staticmethod
+    def backward(ctx, g):
+        (y,) = ctx.saved_tensors
+        return g * pyDeform.DistanceFieldLoss_backward(y.contiguous(), ctx.pid), None
+
+
+class _GraphLoss(Function):
+    @staticmethod
+    def forward(ctx, y, E, pid):
+        ctx.save_for_backward(y, E)
+        ctx.pid = pid
+        return (pyDeform.GraphEdgeLoss_forward(y.contiguous(), E, pid) * 0.5).sum()
+
+    @staticmethod
+    def backward(ctx, g):
+        y, E = ctx.saved_tensors
+        return g * pyDeform.GraphEdgeLoss_backward(y.contiguous(), E, ctx.pid), None, None
+
+
+class NeuralODEDeformer(nn.Module):
+    """Pushes ``points`` (the first ``n_mesh`` are the source mesh vertices,
+    with edges ``E``) through the flow; loss = distance to the target grid
+    (all points) + rho^2 * rigidity (mesh edges)."""
+
+    def __init__(self, points, E, pid, n_mesh, rho=0.1, steps=16, hidden=256, seed=0):
+        super().__init__()
+        self.field = VelocityMLP(hidden, seed).to(points.device)
+        self.register_buffer("y0", points.detach().contiguous())
+        self.E = E.contiguous()
+        self.pid = pid
+        self.n_mesh = n_mesh
+        self.rho2 = rho * rho
+        self.steps = steps
+        pyDeform.StoreGraphInformation(self.y0[:n_mesh].contiguous(), self.E, pid)
+
+    def forward(self):
+        y1 = odeint_rk4(self.field, self.y0, self.steps)
+        lossD = _DistanceLoss.apply(y1, self.pid)
+        lossR = _GraphLoss.apply(y1[: self.n_mesh], self.E, self.pid)
+        return lossD + self.rho2 * lossR, y1

@@ -1,0 +1,68 @@
+"""Golden fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py
+from the oracle on seeded inputs).  CPU: the oracle still reproduces them
+exactly.  GPU: the HIP path reproduces them (bit-exact for the grid, the
+lookups and the losses; LM energy within 1e-3)."""
+import os
+
+import numpy as np
+import pytest
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return dict(np.load(os.path.join(G, name), allow_pickle=False))
+
+
+def test_oracle_reproduces_golden(oracle):
+    g = _load("grid_sphere_n12.npz")
+    tn, sc, pos = oracle.normalize(g["V"].astype(np.float64))
+    assert sc == g["scale"] and np.array_equal(pos, g["pos"])
+    d64, _, nn = oracle.build_grid(tn, g["F"], int(g["N"]))
+    assert np.array_equal(d64, g["dist64"]) and np.array_equal(nn, g["nearest"])
+    lk = _load("lookup_n12.npz")
+    d, gr = oracle.lookup_f32(g["dist64"], int(g["N"]), lk["P"])
+    assert np.array_equal(d, lk["d"]) and np.array_equal(gr, lk["g"])
+    assert np.array_equal(oracle.distance_loss_bwd(g["dist64"], int(g["N"]), lk["P"]), lk["bwd"])
+    e = _load("edges_ico2.npz")
+    off = oracle.store_rigidity(e["V0"], e["F"])
+    assert np.array_equal(oracle.rigid_bwd(e["V1"], e["F"], off), e["rigid_bwd"])
+    lm = _load("lm_sphere_n24.npz")
+    qn, qs, qp = oracle.normalize(lm["tgt"])
+    grid, _, _ = oracle.build_grid(qn, lm["tgt_f"], int(lm["N"]))
+    x, st, _ = oracle.lm_solve(grid, int(lm["N"]), lm["x0"], oracle.face_edges(lm["src_f"]), 1.0)
+    assert np.array_equal(x, lm["x"]) and st["final_cost"] == lm["final_cost"]
+
+
+@pytest.mark.gpu
+def test_gpu_reproduces_golden(gpu, oracle):
+    import torch
+    from meshode_amd import deform, pyDeform
+    print("[golden] grid / lookups / edges / lm", flush=True)
+    g = _load("grid_sphere_n12.npz")
+    N = int(g["N"])
+    pid = pyDeform.InitializeDeformTemplate(torch.from_numpy(g["V"]), torch.from_numpy(g["F"]), 0, N)
+    d64, _, nn = [t.cpu().numpy().reshape(-1) for t in pyDeform.TemplateGrid(pid)]
+    assert np.array_equal(d64, g["dist64"]) and np.array_equal(nn, g["nearest"])
+    lk = _load("lookup_n12.npz")
+    P = torch.from_numpy(lk["P"]).cuda()
+    assert np.array_equal(pyDeform.DistanceFieldLoss_forward(P, pid).cpu().numpy(), lk["fwd"])
+    assert np.array_equal(pyDeform.DistanceFieldLoss_backward(P, pid).cpu().numpy(), lk["bwd"])
+    e = _load("edges_ico2.npz")
+    pyDeform.StoreRigidityInformation(torch.from_numpy(e["V0"]), torch.from_numpy(e["F"]), pid)
+    V1 = torch.from_numpy(e["V1"]).cuda()
+    Fd = torch.from_numpy(e["F"]).cuda()
+    assert np.array_equal(pyDeform.RigidEdgeLoss_forward(V1, Fd, pid).cpu().numpy(), e["rigid_fwd"])
+    assert np.array_equal(pyDeform.RigidEdgeLoss_backward(V1, Fd, pid).cpu().numpy(), e["rigid_bwd"])
+    pyDeform.StoreGraphInformation(torch.from_numpy(e["V0"]), torch.from_numpy(e["E"]), pid)
+    Ed = torch.from_numpy(e["E"]).cuda()
+    assert np.array_equal(pyDeform.GraphEdgeLoss_forward(V1, Ed, pid).cpu().numpy(), e["graph_fwd"])
+    assert np.array_equal(pyDeform.GraphEdgeLoss_backward(V1, Ed, pid).cpu().numpy(), e["graph_bwd"])
+    lm = _load("lm_sphere_n24.npz")
+    outs, st = deform.rigid_deform_batch([(lm["src"], lm["src_f"], lm["tgt"], lm["tgt_f"])],
+                                         grid_resolution=int(lm["N"]))
+    qn, qs, qp = oracle.normalize(lm["tgt"])
+    grid, _, _ = oracle.build_grid(qn, lm["tgt_f"], int(lm["N"]))
+    eg, _, _ = oracle.lm_cost(grid, int(lm["N"]), (outs[0].cpu().numpy() - qp) / qs, lm["x0"],
+                              oracle.face_edges(lm["src_f"]), 1.0)
+    assert abs(eg - float(lm["final_cost"])) <= 1e-3 * float(lm["final_cost"])

@@ -1,0 +1,134 @@
+"""NeuralODE flow (config 3): the 3/8-rule RK4 integrator, the checkpointed
+discrete adjoint and the full deformer loss.
+
+CPU (FP64, pure torch): one step on a linear field equals the degree-4 Taylor
+polynomial of exp(hA) (exact for 4-stage 4th-order RK); one step on a
+nonlinear field equals a generic explicit-RK evaluation of the 3/8-rule
+Butcher tableau; the adjoint equals autograd through the unrolled steps and
+central differences.  GPU: FP32 forward and gradients against FP64, and the
+deformer's loss against the oracle's distance and edge losses."""
+import numpy as np
+import pytest
+import torch
+
+from meshode_amd import neuralode as NO
+
+
+def test_linear_field_taylor():
+    torch.manual_seed(0)
+    A = torch.randn(3, 3, dtype=torch.float64) * 0.7
+    f = lambda y: y @ A.t()        # noqa: E731
+    y = torch.randn(5, 3, dtype=torch.float64)
+    h = 0.3
+    M = h * A
+    P = torch.eye(3, dtype=torch.float64) + M + M @ M / 2 + M @ M @ M / 6 + M @ M @ M @ M / 24
+    assert torch.allclose(NO.rk4_step(f, y, h), y @ P.t(), rtol=1e-14, atol=1e-14)
+
+
+def test_three_eighths_tableau():
+    """Generic explicit RK with Kutta's 3/8-rule tableau."""
+    c = [0.0, 1 / 3, 2 / 3, 1.0]
+    a = [[], [1 / 3], [-1 / 3, 1.0], [1.0, -1.0, 1.0]]
+    b = [1 / 8, 3 / 8, 3 / 8, 1 / 8]
+    f = lambda y: torch.sin(3 * y) + y * y      # noqa: E731
+    y = torch.tensor([[0.3, -0.7, 1.1]], dtype=torch.float64)
+    h = 0.4
+    k = []
+    for i in range(4):
+        yi = y + h * sum(a[i][j] * k[j] for j in range(i)) if i else y
+        k.append(f(yi))
+    expect = y + h * sum(bi * ki for bi, ki in zip(b, k))
+    assert torch.allclose(NO.rk4_step(f, y, h), expect, rtol=1e-15, atol=1e-15)
+
+
+def _tiny(seed=1):
+    m = NO.VelocityMLP(hidden=8, seed=seed, dtype=torch.float64)
+    y0 = torch.rand(7, 3, dtype=torch.float64, generator=torch.Generator().manual_seed(seed))
+    return m, y0
+
+
+def test_adjoint_equals_unrolled_autograd():
+    m, y0 = _tiny()
+    y0a = y0.clone().requires_grad_(True)
+    out = NO.odeint_rk4(m, y0a, steps=4)
+    w = torch.linspace(-1, 1, out.numel(), dtype=torch.float64).view_as(out)
+    (out * w).sum().backward()
+    ga = [y0a.grad] + [p.grad.clone() for p in m.parameters()]
+    for p in m.parameters():
+        p.grad = None
+    y0b = y0.clone().requires_grad_(True)
+    y = y0b
+    for _ in range(4):
+        y = NO.rk4_step(m, y, 0.25)
+    (y * w).sum().backward()
+    gb = [y0b.grad] + [p.grad for p in m.parameters()]
+    assert torch.allclose(out, y.detach(), rtol=0, atol=0)
+    for a_, b_ in zip(ga, gb):
+        assert torch.allclose(a_, b_, rtol=1e-12, atol=1e-14)
+
+
+def test_adjoint_central_differences():
+    m, y0 = _tiny(2)
+    w = torch.randn(7, 3, dtype=torch.float64, generator=torch.Generator().manual_seed(3))
+
+    def loss():
+        return (NO.odeint_rk4(m, y0, steps=3) * w).sum()
+
+    l0 = loss()
+    l0.backward()
+    W1 = m.weights[1]
+    g = W1.grad.clone()
+    eps = 1e-6
+    for (i, j) in [(0, 0), (3, 5), (7, 2)]:
+        with torch.no_grad():
+            W1[i, j] += eps
+        lp = loss().item()
+        with torch.no_grad():
+            W1[i, j] -= 2 * eps
+        lm = loss().item()
+        with torch.no_grad():
+            W1[i, j] += eps
+        assert g[i, j].item() == pytest.approx((lp - lm) / (2 * eps), rel=1e-6, abs=1e-10)
+
+
+@pytest.mark.gpu
+def test_gpu_flow_and_loss_against_fp64(gpu, oracle):
+    from meshode_amd import pyDeform, synth
+    print("[ode] case=box1000_points2000", flush=True)
+    sV, sF, tV, tF = synth.box_union_pair(71, n_boxes=4, n_verts=1000)
+    N = 32
+    pid = pyDeform.InitializeDeformTemplate(torch.from_numpy(tV.astype(np.float32)), torch.from_numpy(tF), 0, N)
+    grid = pyDeform.TemplateGrid(pid)[0].cpu().numpy().reshape(-1)
+    V = torch.from_numpy(sV.astype(np.float32))
+    pyDeform.NormalizeByTemplate(V, pid)
+    # points: mesh vertices + random points near the surface
+    rng = np.random.default_rng(0)
+    extra = (V.numpy()[rng.integers(0, len(V), 1000)] + rng.normal(0, 0.01, (1000, 3))).astype(np.float32)
+    pts = torch.cat([V, torch.from_numpy(extra)]).cuda()
+    E = torch.from_numpy(np.concatenate([sF[:, [0, 1]], sF[:, [1, 2]], sF[:, [2, 0]]]).astype(np.int32)).cuda()
+    model = NO.NeuralODEDeformer(pts, E, pid, n_mesh=len(V), rho=0.1, steps=16, seed=5)
+    loss, y1 = model()
+    loss.backward()
+    # FP64 reference of the flow with the same parameters
+    m64 = NO.VelocityMLP(hidden=256, seed=5, dtype=torch.float64)
+    with torch.no_grad():
+        y64 = NO.odeint_rk4(m64, pts.double().cpu(), steps=16)
+    assert torch.allclose(y1.detach().double().cpu(), y64, rtol=0, atol=2e-5)
+    # loss against the oracle's layers evaluated on the GPU result
+    y = y1.detach().cpu().numpy()
+    d = oracle.distance_loss_fwd(grid, N, y)
+    off = oracle.store_graph(V.numpy(), E.cpu().numpy())
+    r = oracle.graph_fwd(y[: len(V)], E.cpu().numpy(), off)
+    ref = 0.5 * d.astype(np.float64).sum() + 0.01 * 0.5 * r.astype(np.float64).sum()
+    assert loss.item() == pytest.approx(ref, rel=1e-4)
+    # parameter gradients against the FP64 adjoint with the oracle's loss gradient
+    gd = oracle.distance_loss_bwd(grid, N, y).astype(np.float64)
+    gr = oracle.graph_bwd(y[: len(V)], E.cpu().numpy(), off).astype(np.float64)
+    gy = gd.copy()
+    gy[: len(V)] += 0.01 * gr
+    y0 = pts.double().cpu()
+    out = NO.odeint_rk4(m64, y0, steps=16)
+    out.backward(torch.from_numpy(gy))
+    for p32, p64 in zip(model.field.parameters(), m64.parameters()):
+        a, b = p32.grad.double().cpu(), p64.grad
+        assert torch.linalg.norm(a - b) <= 2e-3 * torch.linalg.norm(b) + 1e-9
