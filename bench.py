@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of Poisson arrivals")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: requests per second")
     ap.add_argument("--max-verts", type=int, default=8000, help="serve: largest source size")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one plain 256 MiB device copy (known bytes) to scale HBM counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -88,6 +90,13 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     _lib.load()
+    if args.copy_calibration:
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     if args.mode != "batch":
         print(json.dumps({"serve": bench_serve, "adam": bench_adam, "ode": bench_ode}[args.mode](args)), flush=True)
         return
