@@ -31,6 +31,7 @@ extern "C" {
 #define MO_ERR_STATE (-3)
 #define MO_ERR_RANGE (-4)
 #define MO_ERR_NOMEM (-5)
+#define MO_ERR_NOCONV (-6)
 
 /* Library version (major * 10000 + minor * 100 + patch). */
 int mo_version(void);
@@ -198,6 +199,64 @@ int mo_rigid_deform_batch(int32_t npairs, const double* const* src, const int64_
  * wall clock rate (Hz).  No counterpart in the reference (its only timing
  * is Ceres' progress output, deformer.cc:58). */
 int mo_last_timings(double* out6);
+
+/* ---------------------------------------------------------------------------
+ * CAD chain (cad_deform.cc:21-120, LoadCadMesh mesh_tensor.cc:102-178).
+ * The preprocessing is host work, as in the reference; arrays of the
+ * mo_cad_* and mo_delaunay* functions are HOST arrays. */
+
+/* Mesh::RemoveDegenerated + Mesh::MergeDuplex (mesh.cc:320-333, 181-232),
+ * Subdivision::Subdivide(len_subdivide) (subdivision.cc:28-250),
+ * ComputeGeometryNeighbors(step_neighbors) (:252-341) and
+ * ComputeRepresentativeGraph(step_graph) (:343-399) of the CAD mesh V
+ * [nv,3] FP64 / F [nf,3] (raw coordinates).  The reference uses 2e-2,
+ * 1.5e-2, 1e-2 (cad_deform.cc:46-48).  threads <= 0 picks the host's core
+ * count (at most 16).  *handle identifies the result until mo_cad_destroy. */
+int mo_cad_prepare(const double* V, int64_t nv, const int32_t* F, int64_t nf, double len_subdivide,
+                   double step_neighbors, double step_graph, int32_t threads, int64_t* handle);
+
+/* sizes7: subdivided vertices, subdivided faces, geometry-neighbour pairs,
+ * representative vertices, representative edges, cleaned CAD vertices,
+ * cleaned CAD faces. */
+int mo_cad_sizes(int64_t handle, int64_t* sizes7);
+
+/* Copies the results (any pointer may be NULL): subdivided mesh V [n,3]
+ * FP64 and F [m,3]; E [e,2] the geometry-neighbour pairs
+ * (Subdivision::Neighbors); ref [n] the vertex -> representative map
+ * (Vertex2Graph); GV [g,3] FP64 and GE [h,2] the representative graph
+ * (GraphV, GraphE); the cleaned CAD mesh.  Pairs are (i < j), sorted, as the
+ * reference's std::set iterates them. */
+int mo_cad_export(int64_t handle, double* V, int32_t* F, int32_t* E, int32_t* ref, double* GV, int32_t* GE,
+                  double* clean_V, int32_t* clean_F);
+
+int mo_cad_destroy(int64_t handle);
+
+/* Delaunay2D (delaunay.cc:50-74): triangles (counter-clockwise) of the 2D
+ * points xy [n,2]; *ntri receives the count, tris [cap,3]. */
+int mo_delaunay2(const double* xy, int64_t n, int32_t* tris, int64_t cap, int64_t* ntri);
+
+/* Delaunay3D (delaunay.cc:76-101): simplices of the 3D points xyz [n,3];
+ * *dim = 3 (tetrahedra), 2 (coplanar: triangles), 1 (collinear: segments),
+ * 0/-1 (nothing); *count simplices of dim+1 indices, simplices [cap,4]. */
+int mo_delaunay3(const double* xyz, int64_t n, int32_t* simplices, int64_t cap, int64_t* count, int32_t* dim);
+
+/* Mesh::ApplyTransform (mesh.cc:98-105; Subdivision::ApplyTransform,
+ * subdivision.cc:19-26) with template param_id's normalization: V [nv,3]
+ * FP64 device array in place, v = (v - trans) / scale; inverse != 0 maps
+ * back, v = v * scale + trans (Mesh::WriteOBJ, mesh.cc:46-64). */
+int mo_apply_transform_f64(double* V, int64_t nv, int32_t param_id, int32_t inverse, void* stream);
+
+/* LinearEstimation (linear.cc:11-112; Subdivision::LinearSolve,
+ * subdivision.cc:462-471; pyDeform SolveLinear with_rot = 0,
+ * linear_layer.cc:5-87): V [nv,3] FP64 rest positions in, solution out;
+ * F [nf,3], E [ne,2] (geometry-neighbour pairs), ref [nv], GV [ngv,3] FP64
+ * (deformed representative vertices); all device arrays.  The SPD system is
+ * solved by FP64 conjugate gradients on the device to a relative residual
+ * of 1e-13 (MO_ERR_NOCONV otherwise).  *iterations (host, optional)
+ * receives the CG iteration count.  Synchronizes `stream`. */
+int mo_linear_estimation(double* V, int64_t nv, const int32_t* F, int64_t nf, const int32_t* E, int64_t ne,
+                         const int32_t* ref, const double* GV, int64_t ngv, double rigidity, int32_t* iterations,
+                         void* stream);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,8 @@ SYMBOLS = [
     "mo_rigidity_store", "mo_rigid_loss_forward", "mo_rigid_loss_backward",
     "mo_graph_store", "mo_graph_loss_forward", "mo_graph_loss_backward",
     "mo_lm_default_options", "mo_deform_batch", "mo_rigid_deform_batch", "mo_last_timings",
+    "mo_cad_prepare", "mo_cad_sizes", "mo_cad_export", "mo_cad_destroy", "mo_delaunay2", "mo_delaunay3",
+    "mo_apply_transform_f64", "mo_linear_estimation",
 ]
 
 
@@ -102,6 +104,14 @@ _SIGS = {
     "mo_rigid_deform_batch": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                         C.POINTER(LMOptions), _vp, _vp, _vp]),
     "mo_last_timings": (C.c_int, [_vp]),
+    "mo_cad_prepare": (C.c_int, [_vp, _i64, _vp, _i64, _f64, _f64, _f64, _i32, C.POINTER(_i64)]),
+    "mo_cad_sizes": (C.c_int, [_i64, _vp]),
+    "mo_cad_export": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mo_cad_destroy": (C.c_int, [_i64]),
+    "mo_delaunay2": (C.c_int, [_vp, _i64, _vp, _i64, C.POINTER(_i64)]),
+    "mo_delaunay3": (C.c_int, [_vp, _i64, _vp, _i64, C.POINTER(_i64), C.POINTER(_i32)]),
+    "mo_apply_transform_f64": (C.c_int, [_vp, _i64, _i32, _i32, _vp]),
+    "mo_linear_estimation": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _f64, C.POINTER(_i32), _vp]),
 }
 
 

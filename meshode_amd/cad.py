@@ -1,0 +1,80 @@
+"""CAD chain of cad_deform (cad_deform.cc:21-120) on the C ABI.
+
+* ``prepare``: RemoveDegenerated + MergeDuplex + Subdivide + geometry
+  neighbours + representative graph (host code in libmeshode, as in the
+  reference, mesh_tensor.cc:102-115).
+* ``delaunay2`` / ``delaunay3``: the triangulations standing in for CGAL's
+  (delaunay.cc:50-101).
+* ``linear_estimation``: LinearEstimation (linear.cc:11-112), solved on the
+  GPU.
+* ``cad_deform``: the whole non-flow cad_deform pipeline on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+def prepare(V, F, len_subdivide=2e-2, step_neighbors=1.5e-2, step_graph=1e-2, threads=0):
+    """dict(V, F, E, ref, GV, GE, clean_V, clean_F) of numpy arrays (FP64
+    vertices, int32 indices); E/GE are (i < j) pairs in sorted order."""
+    V = np.ascontiguousarray(V, np.float64).reshape(-1, 3)
+    F = np.ascontiguousarray(F, np.int32).reshape(-1, 3)
+    h = C.c_int64()
+    _lib.call("mo_cad_prepare", _p(V), len(V), _p(F), len(F), len_subdivide, step_neighbors, step_graph,
+              int(threads), C.byref(h))
+    try:
+        n = np.zeros(7, np.int64)
+        _lib.call("mo_cad_sizes", h.value, _p(n))
+        out = dict(V=np.zeros((n[0], 3)), F=np.zeros((n[1], 3), np.int32), E=np.zeros((n[2], 2), np.int32),
+                   ref=np.zeros(n[0], np.int32), GV=np.zeros((n[3], 3)), GE=np.zeros((n[4], 2), np.int32),
+                   clean_V=np.zeros((n[5], 3)), clean_F=np.zeros((n[6], 3), np.int32))
+        _lib.call("mo_cad_export", h.value, *[_p(out[k]) for k in ("V", "F", "E", "ref", "GV", "GE", "clean_V",
+                                                                   "clean_F")])
+    finally:
+        _lib.call("mo_cad_destroy", h.value)
+    return out
+
+
+def delaunay2(xy):
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    cap = 2 * len(xy) + 8
+    out = np.zeros((cap, 3), np.int32)
+    n = C.c_int64()
+    _lib.call("mo_delaunay2", _p(xy), len(xy), _p(out), cap, C.byref(n))
+    return out[: n.value]
+
+
+def delaunay3(xyz):
+    """(dim, simplices [k, dim+1])."""
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    cap = 8 * len(xyz) * len(xyz) + 16
+    out = np.zeros(cap * 4, np.int32)
+    n, dim = C.c_int64(), C.c_int32()
+    _lib.call("mo_delaunay3", _p(xyz), len(xyz), _p(out), cap, C.byref(n), C.byref(dim))
+    d = dim.value
+    if d < 1:
+        return d, np.zeros((0, max(d + 1, 1)), np.int32)
+    return d, out[: n.value * (d + 1)].reshape(-1, d + 1)
+
+
+def linear_estimation(V, F, E, ref, GV, rigidity=2.0):
+    """LinearEstimation on the GPU.  V [n,3] FP64 CUDA tensor updated in place;
+    F, E, ref int32 CUDA tensors; GV [g,3] FP64 CUDA tensor.  Returns the CG
+    iteration count."""
+    import torch
+    for name, t in (("V", V), ("F", F), ("E", E), ("ref", ref), ("GV", GV)):
+        if not t.is_cuda or not t.is_contiguous():
+            raise ValueError(f"linear_estimation: {name} must be a contiguous device tensor")
+    it = C.c_int32()
+    _lib.call("mo_linear_estimation", C.c_void_p(V.data_ptr()), V.shape[0], C.c_void_p(F.data_ptr()), F.shape[0],
+              C.c_void_p(E.data_ptr()), E.shape[0], C.c_void_p(ref.data_ptr()), C.c_void_p(GV.data_ptr()),
+              GV.shape[0], float(rigidity), C.byref(it), C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    return it.value

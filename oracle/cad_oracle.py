@@ -1,0 +1,574 @@
+"""CPU restatement of the reference's CAD preprocessing chain and of
+LinearEstimation, for small inputs.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker.  The product
+path (meshode_amd) never imports this module.
+
+Restates (reference files under src/lib/):
+  * Mesh::RemoveDegenerated       mesh.cc:320-333
+  * Mesh::MergeDuplex             mesh.cc:181-232
+  * Subdivision::Subdivide        subdivision.cc:28-129
+  * DelaunaySubdivision           subdivision.cc:131-250
+  * ComputeGeometryNeighbors      subdivision.cc:252-341
+  * ComputeRepresentativeGraph    subdivision.cc:343-399
+  * LinearEstimation              linear.cc:11-112 (assembled in the
+                                  reference's insertion order, solved with
+                                  scipy's sparse LU in place of Eigen's LDLT)
+  * Delaunay2D / Delaunay3D       delaunay.cc:50-101.  CGAL is absent; its
+    triangulation is restated as its published definition: the Delaunay
+    triangulation under the symbolic perturbation of Devillers & Teillaud
+    (CGTA 2011) -- a point exactly on a circum-circle/sphere is decided by
+    the lexicographic order of the points -- enumerated by brute force
+    (every simplex whose perturbed circum-ball is empty), with exact
+    predicates (float filter, then fractions.Fraction).
+
+Scalar arithmetic is Python float (IEEE double) in the reference's
+operation order; three-term dot products and norms sum left to right.
+Orders the reference leaves unspecified (CGAL's face iteration,
+unordered_set iteration) are fixed as in the product: sub-faces listed
+with their smallest vertex first, sorted; cell members in increasing index.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+from fractions import Fraction as Fr
+
+import numpy as np
+
+
+# ---------------------------------------------------------------------------
+# exact predicates
+def _sign(x):
+    return (x > 0) - (x < 0)
+
+
+def _det(M):
+    n = len(M)
+    if n == 1:
+        return M[0][0]
+    if n == 2:
+        return M[0][0] * M[1][1] - M[0][1] * M[1][0]
+    return sum((-1) ** j * M[0][j] * _det([r[:j] + r[j + 1:] for r in M[1:]]) for j in range(n) if M[0][j] != 0)
+
+
+def orient2(a, b, c):
+    """>0: a, b, c counter-clockwise."""
+    l = (a[0] - c[0]) * (b[1] - c[1])
+    r = (a[1] - c[1]) * (b[0] - c[0])
+    if abs(l - r) > 1e-12 * (abs(l) + abs(r)):
+        return _sign(l - r)
+    A, B, Cc = [[Fr(x) for x in p] for p in (a, b, c)]
+    return _sign((A[0] - Cc[0]) * (B[1] - Cc[1]) - (A[1] - Cc[1]) * (B[0] - Cc[0]))
+
+
+def incircle(a, b, c, d):
+    """>0: d inside the circle through the counter-clockwise a, b, c."""
+    rows = []
+    for p in (a, b, c):
+        q = [p[0] - d[0], p[1] - d[1]]
+        rows.append(q + [q[0] * q[0] + q[1] * q[1]])
+    v = float(np.linalg.det(np.array(rows)))
+    perm = float(np.abs(np.array(rows)).sum(axis=1).prod())  # Hadamard-type bound
+    if abs(v) > 1e-10 * perm:
+        return _sign(v)
+    rows = []
+    for p in (a, b, c):
+        q = [Fr(p[0]) - Fr(d[0]), Fr(p[1]) - Fr(d[1])]
+        rows.append(q + [q[0] * q[0] + q[1] * q[1]])
+    return _sign(_det(rows))
+
+
+def orient3(a, b, c, d):
+    """>0: d on the side of plane abc that (b-a) x (c-a) points to."""
+    M = [[b[i] - a[i] for i in range(3)], [c[i] - a[i] for i in range(3)], [d[i] - a[i] for i in range(3)]]
+    v = float(np.linalg.det(np.array(M)))
+    perm = float(np.abs(np.array(M)).sum(axis=1).prod()) + 1e-300
+    if abs(v) > 1e-10 * perm:
+        return _sign(v)
+    M = [[Fr(p[i]) - Fr(a[i]) for i in range(3)] for p in (b, c, d)]
+    return _sign(_det(M))
+
+
+def insphere(a, b, c, d, e):
+    """>0: e inside the sphere through a, b, c, d (orient3(a,b,c,d) > 0)."""
+    rows = []
+    for p in (a, b, c, d):
+        q = [p[i] - e[i] for i in range(3)]
+        rows.append(q + [q[0] * q[0] + q[1] * q[1] + q[2] * q[2]])
+    v = float(np.linalg.det(np.array(rows)))
+    perm = float(np.abs(np.array(rows)).sum(axis=1).prod()) + 1e-300
+    if abs(v) > 1e-10 * perm:
+        return -_sign(v)
+    rows = []
+    for p in (a, b, c, d):
+        q = [Fr(p[i]) - Fr(e[i]) for i in range(3)]
+        rows.append(q + [q[0] * q[0] + q[1] * q[1] + q[2] * q[2]])
+    return -_sign(_det(rows))
+
+
+def _lex_key(p):
+    return tuple(p)
+
+
+def incircle_pert(P, t, p):
+    """Perturbed in-circle of point index p against CCW triangle t."""
+    s = incircle(P[t[0]], P[t[1]], P[t[2]], P[p])
+    if s:
+        return s
+    pts = sorted([t[0], t[1], t[2], p], key=lambda i: _lex_key(P[i]))
+    for q in reversed(pts):
+        if q == p:
+            return -1
+        w = [p if v == q else v for v in t]
+        o = orient2(P[w[0]], P[w[1]], P[w[2]])
+        if o:
+            return o
+    return -1
+
+
+def insphere_pert(P, t, p):
+    s = insphere(P[t[0]], P[t[1]], P[t[2]], P[t[3]], P[p])
+    if s:
+        return s
+    pts = sorted(list(t) + [p], key=lambda i: _lex_key(P[i]))
+    for q in reversed(pts):
+        if q == p:
+            return -1
+        w = [p if v == q else v for v in t]
+        o = orient3(*[P[i] for i in w])
+        if o:
+            return o
+    return -1
+
+
+def _unique(P):
+    seen, out = {}, []
+    for i, p in enumerate(P):
+        k = tuple(p)
+        if k not in seen:
+            seen[k] = i
+            out.append(i)
+    return out
+
+
+def _batch_incircle_pert(P, A, t, others):
+    """incircle_pert for every point of `others`: float determinants in one
+    batch, exact re-evaluation (with the perturbation) where a value is within
+    a safe bound of zero."""
+    if not others:
+        return np.zeros(0, int)
+    T = A[list(t)]
+    D = A[others]
+    Q = T[None, :, :] - D[:, None, :]
+    M = np.concatenate([Q, (Q * Q).sum(-1, keepdims=True)], axis=-1)
+    v = np.linalg.det(M)
+    perm = np.abs(M).sum(-1).prod(-1)
+    sgn = np.sign(v).astype(int)
+    for k in np.nonzero(np.abs(v) <= 1e-10 * perm)[0]:
+        sgn[k] = incircle_pert(P, t, others[k])
+    return sgn
+
+
+def delaunay2_bruteforce(xy):
+    """Set of CCW triangles (as rotated tuples, smallest index first) of the
+    perturbed Delaunay triangulation of the points xy [n,2] (exact
+    duplicates: lowest index)."""
+    A = np.asarray(xy, np.float64)
+    P = [tuple(map(float, p)) for p in A]
+    ids = _unique(P)
+    out = set()
+    for a, b, c in itertools.combinations(ids, 3):
+        o = orient2(P[a], P[b], P[c])
+        if o == 0:
+            continue
+        t = (a, b, c) if o > 0 else (a, c, b)
+        others = [d for d in ids if d not in t]
+        if (_batch_incircle_pert(P, A, t, others) < 0).all():
+            out.add(_rot3(t))
+    return out
+
+
+def _rot3(t):
+    r = min(range(3), key=lambda i: t[i])
+    return (t[r], t[(r + 1) % 3], t[(r + 2) % 3])
+
+
+def _collinear3(a, b, c):
+    return all(orient2((a[i], a[j]), (b[i], b[j]), (c[i], c[j])) == 0 for i, j in ((0, 1), (1, 2), (2, 0)))
+
+
+def _apex(P, ids):
+    a, b, c = (P[i] for i in ids[:3])
+    off = 2.0 ** (math.frexp(max(abs(x) for i in range(len(P)) for x in P[i]) + 1.0)[1] + 1)
+    for k in range(3):
+        q = list(a)
+        q[k] = a[k] + off
+        if orient3(a, b, c, q):
+            return tuple(q)
+    raise AssertionError("collinear frame")
+
+
+def delaunay3_bruteforce(xyz):
+    """(dim, set of simplices as sorted tuples) of the perturbed Delaunay
+    triangulation of the points xyz [n,3]; dim 2 (coplanar set: triangles
+    in its plane), 1 (collinear: consecutive segments)."""
+    P = [tuple(map(float, p)) for p in np.asarray(xyz, np.float64)]
+    ids = _unique(P)
+    if len(ids) < 2:
+        return (0 if ids else -1), set()
+    i0, i1 = ids[0], ids[1]
+    rest = [i for i in ids[2:] if not _collinear3(P[i0], P[i1], P[i])]
+    if not rest:
+        s = sorted(ids, key=lambda i: P[i])
+        return 1, {tuple(sorted(p)) for p in zip(s[:-1], s[1:])}
+    i2 = rest[0]
+    if all(orient3(P[i0], P[i1], P[i2], P[i]) == 0 for i in ids):
+        q = _apex(P, [i0, i1, i2])
+        out = set()
+        for a, b, c in itertools.combinations(ids, 3):
+            o = orient3(P[a], P[b], P[c], q)
+            if o == 0:
+                continue
+            t = (a, b, c) if o > 0 else (a, c, b)
+
+            def inc(t, d):
+                s = insphere(P[t[0]], P[t[1]], P[t[2]], q, P[d]) * orient3(P[t[0]], P[t[1]], P[t[2]], q)
+                if s:
+                    return s
+                pts = sorted(list(t) + [d], key=lambda i: P[i])
+                for r in reversed(pts):
+                    if r == d:
+                        return -1
+                    w = [d if v == r else v for v in t]
+                    oo = orient3(P[w[0]], P[w[1]], P[w[2]], q)
+                    if oo:
+                        return oo
+                return -1
+            if all(inc(t, d) < 0 for d in ids if d not in t):
+                out.add(tuple(sorted(t)))
+        return 2, out
+    A = np.asarray(P, np.float64)
+    out = set()
+    for t in itertools.combinations(ids, 4):
+        o = orient3(*[P[i] for i in t])
+        if o == 0:
+            continue
+        tt = t if o > 0 else (t[1], t[0], t[2], t[3])
+        others = [e for e in ids if e not in t]
+        if not others:
+            out.add(tuple(sorted(t)))
+            continue
+        Q = A[list(tt)][None, :, :] - A[others][:, None, :]
+        M = np.concatenate([Q, (Q * Q).sum(-1, keepdims=True)], axis=-1)
+        v = np.linalg.det(M)
+        perm = np.abs(M).sum(-1).prod(-1)
+        sgn = -np.sign(v).astype(int)
+        for k in np.nonzero(np.abs(v) <= 1e-10 * perm)[0]:
+            sgn[k] = insphere_pert(P, tt, others[k])
+        if (sgn < 0).all():
+            out.add(tuple(sorted(t)))
+    return 3, out
+
+
+def simplex_edges(dim, simplices):
+    e = set()
+    for s in simplices:
+        for a, b in itertools.combinations(s, 2):
+            e.add((min(a, b), max(a, b)))
+    return e
+
+
+# ---------------------------------------------------------------------------
+# mesh clean-up
+def _trunc(x):
+    if not (-2147483649.0 < x < 2147483648.0):
+        return -2147483648
+    return int(x)
+
+
+def _sub(a, b):
+    return (a[0] - b[0], a[1] - b[1], a[2] - b[2])
+
+
+def _dot(a, b):
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+
+
+def _norm(a):
+    return math.sqrt(_dot(a, a))
+
+
+def _cross(a, b):
+    return (a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0])
+
+
+def remove_degenerated(V, F):
+    V = [tuple(map(float, v)) for v in V]
+    out = []
+    for f in F:
+        a, b, c = V[f[0]], V[f[1]], V[f[2]]
+        if _norm(_cross(_sub(b, a), _sub(c, a))) > 0:
+            out.append(tuple(int(x) for x in f))
+    return V, out
+
+
+def merge_duplex(V, F):
+    grid, shrink, NV = {}, [], []
+    for v in V:
+        k = (_trunc(v[0] * 1e6), _trunc(v[1] * 1e6), _trunc(v[2] * 1e6))
+        if k not in grid:
+            grid[k] = len(NV)
+            NV.append(v)
+        shrink.append(grid[k])
+    seen, NF = set(), []
+    for f in F:
+        g = tuple(shrink[i] for i in f)
+        if g[0] == g[1] or g[1] == g[2] or g[2] == g[0]:
+            continue
+        k = tuple(sorted(g))
+        if k in seen:
+            continue
+        seen.add(k)
+        NF.append(g)
+    return NV, NF
+
+
+# ---------------------------------------------------------------------------
+# subdivision
+def _area2(ax, ay, bx, by, cx, cy):
+    return (cx - ax) * (by - ay) - (bx - ax) * (cy - ay)
+
+
+def _div(a, s):
+    return (a[0] / s, a[1] / s, a[2] / s)
+
+
+def _delaunay_subdivision(boundary, V, face, length):
+    v0, v1, v2 = V[face[0]], V[face[1]], V[face[2]]
+    n = _cross(_sub(v1, v0), _sub(v2, v0))
+    n = _div(n, _norm(n))
+    curved = {}
+    for i in range(3):
+        x, y = V[face[i]], V[face[(i + 1) % 3]]
+        d = _cross(n, _sub(y, x))
+        d = _div(d, _norm(d))
+        c = tuple((y[k] + x[k]) * 0.5 + 1e3 * d[k] for k in range(3))
+        l = _norm(_sub(c, x))
+        for p in boundary[i]:
+            dd = _sub(V[p], c)
+            dn = _norm(dd)
+            curved[p] = tuple(dd[k] / dn * l + c[k] for k in range(3))
+    min_axis = 0
+    for i in (1, 2):
+        if abs(n[i]) < abs(n[min_axis]):
+            min_axis = 1
+    e = [0.0, 0.0, 0.0]
+    e[min_axis] = 1.0
+    tx = _cross(tuple(e), n)
+    tx = _div(tx, _norm(tx))
+    ty = _cross(n, tx)
+    vind = sorted(curved)
+    pts = [curved[p] for p in vind]
+    v1x, v1y = _dot(_sub(v1, v0), tx) / length, _dot(_sub(v1, v0), ty) / length
+    v2x, v2y = _dot(_sub(v2, v0), tx) / length, _dot(_sub(v2, v0), ty) / length
+    f32 = float(np.float32(0.999999))
+    minX, minY = _trunc(min(0.0, min(v1x, v2x))), _trunc(min(0.0, min(v1y, v2y)))
+    maxX = _trunc(max(0.0, max(v1x, v2x)) + f32)
+    maxY = _trunc(max(0.0, max(v1y, v2y)) + f32)
+    tri_inv = 1.0 / _area2(0.0, 0.0, v1x, v1y, v2x, v2y)
+    for py in range(minY, maxY + 1):
+        for px in range(minX, maxX + 1):
+            beta = _area2(0.0, 0.0, float(px), float(py), v2x, v2y) * tri_inv
+            gamma = _area2(0.0, 0.0, v1x, v1y, float(px), float(py)) * tri_inv
+            alpha = 1.0 - beta - gamma
+            if 0.0 < alpha < 1.0 and 0.0 < beta < 1.0 and 0.0 < gamma < 1.0:
+                sx, sy = px * length, py * length
+                r = tuple(v0[k] + tx[k] * sx + ty[k] * sy for k in range(3))
+                vind.append(len(V))
+                V.append(r)
+                pts.append(r)
+    xy = [(_dot(_sub(p, v0), tx), _dot(_sub(p, v0), ty)) for p in pts]
+    p2k = {}
+    for i, (x, y) in enumerate(xy):
+        p2k[(_trunc(x * 1e7), _trunc(y * 1e7))] = i
+    faces = []
+    for t in delaunay2_bruteforce(xy):
+        v = [vind[p2k[(_trunc(xy[k][0] * 1e7), _trunc(xy[k][1] * 1e7))]] for k in t]
+        if any(_norm(_sub(V[v[j]], V[v[k]])) > 3 * length for j in range(3) for k in range(j + 1, 3)):
+            continue
+        faces.append(_rot3(tuple(v)))
+    return sorted(faces)
+
+
+def subdivide(V, F, length):
+    """Returns (V_sub list, F_sub list, internal flags)."""
+    V = list(V)
+    v_num = len(V)
+    e2f = {}
+    for i, f in enumerate(F):
+        for j in range(3):
+            e2f.setdefault(tuple(sorted((f[j], f[(j + 1) % 3]))), []).append(i)
+    group = [-1] * len(F)
+    ng = 0
+    for i in range(len(F)):
+        if group[i] >= 0:
+            continue
+        group[i] = ng
+        q = [i]
+        for f in q:
+            for j in range(3):
+                for g in e2f[tuple(sorted((F[f][j], F[f][(j + 1) % 3])))]:
+                    if group[g] < 0:
+                        group[g] = ng
+                        q.append(g)
+        ng += 1
+    order = sorted(range(len(F)), key=lambda i: (group[i], i))
+    split, boundary, out = {}, set(), []
+    for fi in order:
+        face = F[fi]
+        lists = []
+        for j in range(3):
+            a, b = face[j], face[(j + 1) % 3]
+            h = tuple(sorted((a, b)))
+            if h not in split:
+                pa = V[a]
+                diff = _sub(V[b], pa)
+                ns = _trunc(_norm(diff) / length + 1)
+                diff = _div(diff, float(ns))
+                idx = [a]
+                for t in range(1, ns):
+                    idx.append(len(V))
+                    boundary.add(len(V))
+                    V.append(tuple(pa[k] + diff[k] * t for k in range(3)))
+                idx.append(b)
+                split[h] = idx
+            lists.append(split[h])
+        out.extend(_delaunay_subdivision(lists, V, face, length))
+    internal = [i >= v_num and i not in boundary for i in range(len(V))]
+    return V, out, internal
+
+
+def geometry_neighbors(V, F, step):
+    cells = {}
+    for i, v in enumerate(V):
+        for j in range(8):
+            d = (step if j & 4 else 0.0, step if j & 2 else 0.0, step if j & 1 else 0.0)
+            k = tuple(_trunc((v[c] + d[c]) / step) for c in range(3))
+            l = cells.setdefault(k, [])
+            if not l or l[-1] != i:
+                l.append(i)
+    pairs = set()
+    for l in cells.values():
+        if len(l) == 2:
+            pairs.add((min(l), max(l)))
+        elif len(l) >= 3:
+            pts = [V[i] for i in l]
+            p2k = {}
+            keys = [tuple(_trunc(p[c] * 1e7) for c in range(3)) for p in pts]
+            for i, k in zip(l, keys):
+                p2k[k] = i
+            dim, simp = delaunay3_bruteforce(pts)
+            for a, b in simplex_edges(dim, simp):
+                x, y = p2k[keys[a]], p2k[keys[b]]
+                if x != y:
+                    pairs.add((min(x, y), max(x, y)))
+    for f in F:
+        for j in range(3):
+            a, b = f[j], f[(j + 1) % 3]
+            if a != b:
+                pairs.add((min(a, b), max(a, b)))
+    return sorted(pairs)
+
+
+def representative_graph(V, F, neighbors, step):
+    cells = {}
+    for i, v in enumerate(V):
+        cells.setdefault(tuple(_trunc(v[c] / step) for c in range(3)), []).append(i)
+    ref = [0] * len(V)
+    GV = []
+    for g, k in enumerate(sorted(cells)):
+        p = [0.0, 0.0, 0.0]
+        for i in cells[k]:
+            for c in range(3):
+                p[c] += V[i][c]
+            ref[i] = g
+        GV.append(tuple(p[c] / float(len(cells[k])) for c in range(3)))
+    E = set()
+    for a, b in list(neighbors) + [(f[j], f[(j + 1) % 3]) for f in F for j in range(3)]:
+        x, y = ref[a], ref[b]
+        if x != y:
+            E.add((min(x, y), max(x, y)))
+    return ref, GV, sorted(E)
+
+
+def prepare(V, F, length=2e-2, step_neighbors=1.5e-2, step_graph=1e-2):
+    V, F = remove_degenerated(V, F)
+    V, F = merge_duplex(V, F)
+    clean = (list(V), list(F))
+    VS, FS, internal = subdivide(V, F, length)
+    E = geometry_neighbors(VS, FS, step_neighbors)
+    ref, GV, GE = representative_graph(VS, FS, E, step_graph)
+    return dict(clean=clean, V=VS, F=FS, internal=internal, E=E, ref=ref, GV=GV, GE=GE)
+
+
+# ---------------------------------------------------------------------------
+# LinearEstimation
+def linear_system(V, F, E, ref, GV, rigidity=2.0):
+    """(A as a dict {(r, c): value} summed in the reference's order, B [n,3])."""
+    V = np.asarray(V, np.float64)
+    GV = np.asarray(GV, np.float64)
+    n = len(V)
+    A = {}
+
+    def add(x, y, w):
+        A[(x, y)] = A.get((x, y), 0.0) + w if (x, y) in A else w
+
+    B = np.zeros((n, 3))
+    cells = {}
+    for i, r in enumerate(ref):
+        cells.setdefault(int(r), []).append(i)
+    for g in range(max(cells) + 1):
+        m = cells.get(g, [])
+        if not m:
+            continue
+        w = 1.0 / len(m)
+        w2 = w * w
+        for a in m:
+            for b in m:
+                add(a, b, w2)
+        for a in m:
+            for c in range(3):
+                B[a, c] += w * GV[g, c]
+    for i in range(n):
+        add(i, i, 1e-6)
+        for c in range(3):
+            B[i, c] += 1e-6 * GV[ref[i], c]
+
+    def edge(a, b):
+        d = V[a] - V[b]
+        reg = rigidity * 2e-2 / (math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + 1e-8)
+        reg *= reg
+        add(a, a, reg)
+        add(a, b, -reg)
+        add(b, a, -reg)
+        add(b, b, reg)
+        for c in range(3):
+            B[a, c] += reg * (V[a, c] - V[b, c])
+            B[b, c] += reg * (V[b, c] - V[a, c])
+    for f in F:
+        for j in range(3):
+            edge(int(f[j]), int(f[(j + 1) % 3]))
+    for a, b in E:
+        edge(int(a), int(b))
+    return A, B
+
+
+def linear_estimation(V, F, E, ref, GV, rigidity=2.0):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    A, B = linear_system(V, F, E, ref, GV, rigidity)
+    n = len(V)
+    keys = list(A)
+    M = sp.csc_matrix(([A[k] for k in keys], ([k[0] for k in keys], [k[1] for k in keys])), shape=(n, n))
+    return np.stack([spl.spsolve(M, B[:, c]) for c in range(3)], axis=1)

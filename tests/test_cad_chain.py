@@ -1,0 +1,170 @@
+"""CAD preprocessing chain (cad_deform.cc:43-51, LoadCadMesh
+mesh_tensor.cc:102-115): host code of libmeshode against the oracle.
+
+* Delaunay: against scipy's Qhull on random points (a unique
+  triangulation), and against the oracle's brute-force enumeration of the
+  perturbed Delaunay simplices on degenerate sets (lattices, cospherical box
+  corners, coplanar and collinear sets), plus insertion-order invariance and
+  the convex-hull area/volume.
+* The chain: cleaned mesh, subdivided vertices (bit-exact), sub-faces,
+  geometry-neighbour pairs, representative graph (bit-exact) on a general
+  tetrahedron, axis-aligned and rotated boxes and a mesh with duplicate
+  vertices, degenerate and duplicate faces.
+
+No GPU: these entry points are host code.
+"""
+import numpy as np
+import pytest
+from scipy.spatial import ConvexHull, Delaunay
+
+from meshode_amd import cad
+
+from oracle import cad_oracle  # noqa: E402
+
+
+def _tri_set(T):
+    return {tuple(sorted(int(x) for x in t)) for t in T}
+
+
+def _rot_set(T):
+    return {cad_oracle._rot3(tuple(int(x) for x in t)) for t in T}
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_delaunay2_matches_qhull_random(seed):
+    P = np.random.default_rng(seed).random((200, 2))
+    assert _tri_set(cad.delaunay2(P)) == _tri_set(Delaunay(P).simplices)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_delaunay3_matches_qhull_random(seed):
+    P = np.random.default_rng(seed).random((80, 3))
+    dim, T = cad.delaunay3(P)
+    assert dim == 3 and _tri_set(T) == _tri_set(Delaunay(P).simplices)
+
+
+def test_delaunay2_ccw_and_hull_area():
+    rng = np.random.default_rng(7)
+    P = np.round(rng.random((60, 2)) * 5) / 5
+    P = np.unique(P, axis=0)
+    T = cad.delaunay2(P)
+    a = P[T[:, 1]] - P[T[:, 0]]
+    b = P[T[:, 2]] - P[T[:, 0]]
+    cr = a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]
+    assert (cr > 0).all()
+    assert cr.sum() / 2 == pytest.approx(ConvexHull(P).volume, rel=1e-12)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_delaunay2_degenerate_against_bruteforce(seed):
+    rng = np.random.default_rng(100 + seed)
+    P = np.round(rng.random((24, 2)) * 4) / 4        # many cocircular quadruples
+    got = _rot_set(cad.delaunay2(P))
+    assert got == cad_oracle.delaunay2_bruteforce(P)
+    perm = rng.permutation(len(P))
+    again = _rot_set(perm[cad.delaunay2(P[perm])])
+    uniq = {tuple(p) for p in P}
+    if len(uniq) == len(P):                         # duplicates keep the lowest index
+        assert again == got
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_delaunay3_degenerate_against_bruteforce(seed):
+    rng = np.random.default_rng(200 + seed)
+    P = np.unique(np.round(rng.random((14, 3)) * 3) / 3, axis=0)
+    dim, T = cad.delaunay3(P)
+    odim, osimp = cad_oracle.delaunay3_bruteforce(P)
+    assert dim == odim and _tri_set(T) == osimp
+    perm = rng.permutation(len(P))
+    d2, T2 = cad.delaunay3(P[perm])
+    assert d2 == dim and _tri_set(perm[T2]) == _tri_set(T)
+    vol = sum(abs(np.linalg.det(P[t[1:]] - P[t[0]])) / 6 for t in T)
+    assert vol == pytest.approx(ConvexHull(P).volume, rel=1e-12)
+
+
+def test_delaunay3_lower_dimensions():
+    g = np.stack(np.meshgrid(np.arange(4), np.arange(3)), -1).reshape(-1, 2).astype(float) * 0.02
+    P = np.c_[g, np.full(len(g), 0.5)]                       # coplanar lattice
+    dim, T = cad.delaunay3(P)
+    assert dim == 2 and len(T) == 2 * 3 * 2
+    odim, osimp = cad_oracle.delaunay3_bruteforce(P)
+    assert odim == 2 and _tri_set(T) == osimp
+    L = np.outer(np.arange(5), [0.25, 0.5, 1.0])[[3, 0, 4, 1, 2]]       # exactly collinear
+    dim, S = cad.delaunay3(L)
+    assert dim == 1 and _tri_set(S) == {(1, 3), (3, 4), (0, 4), (0, 2)} and len(S) == 4
+    dim, S = cad.delaunay3(np.zeros((3, 3)))
+    assert dim == 0 and len(S) == 0
+    # box corners, face centres and centre: many cospherical sets
+    c = np.array(np.meshgrid([0, 1], [0, 1], [0, 1])).reshape(3, -1).T.astype(float)
+    P = np.concatenate([c, [[.5, .5, 0], [.5, .5, 1], [0, .5, .5], [1, .5, .5], [.5, 0, .5], [.5, 1, .5], [.5, .5, .5]]])
+    dim, T = cad.delaunay3(P)
+    assert dim == 3 and _tri_set(T) == cad_oracle.delaunay3_bruteforce(P)[1]
+
+
+def _box(lo, hi):
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    V = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]])
+    F = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4], [1, 2, 6], [1, 6, 5],
+                  [2, 3, 7], [2, 7, 6], [3, 0, 4], [3, 4, 7]])
+    return V, F
+
+
+def _cases():
+    tet = (np.array([[0, 0, 0], [0.07, 0.01, 0.005], [0.015, 0.06, 0.01], [0.02, 0.015, 0.05]]),
+           np.array([[0, 2, 1], [0, 1, 3], [1, 2, 3], [0, 3, 2]]))
+    th = 0.3
+    R = (np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+         @ np.array([[1, 0, 0], [0, np.cos(0.7), -np.sin(0.7)], [0, np.sin(0.7), np.cos(0.7)]]))
+    bV, bF = _box((0, 0, 0), (0.05, 0.04, 0.03))
+    V, F = _box((0.1, 0.2, 0.3), (0.14, 0.23, 0.32))
+    dirty = (np.concatenate([V, V[:2] + 1e-8, [[0.12, 0.21, 0.31]]]),
+             np.concatenate([F, [[8, 2, 1]], [[1, 2, 0]], [[0, 1, 1]], [[0, 10, 10]]]))
+    return {"tetra": tet, "box": _box((0.1, 0.2, 0.3), (0.15, 0.24, 0.33)),
+            "box_negative": _box((-0.02, -0.01, 0.0), (0.03, 0.03, 0.025)),
+            "rotated_box": (bV @ R.T + 0.1, bF), "dirty": dirty}
+
+
+@pytest.mark.parametrize("name", list(_cases()))
+def test_cad_prepare_against_oracle(name):
+    V, F = _cases()[name]
+    r = cad.prepare(V, F)
+    o = cad_oracle.prepare(V.tolist(), F.tolist())
+    np.testing.assert_array_equal(r["clean_V"], np.array(o["clean"][0]).reshape(-1, 3))
+    np.testing.assert_array_equal(r["clean_F"], np.array(o["clean"][1]).reshape(-1, 3))
+    np.testing.assert_array_equal(r["V"], np.array(o["V"]))             # bit-exact
+    np.testing.assert_array_equal(r["F"], np.array(o["F"]).reshape(-1, 3))
+    np.testing.assert_array_equal(r["E"], np.array(o["E"]).reshape(-1, 2))
+    np.testing.assert_array_equal(r["ref"], np.array(o["ref"]))
+    np.testing.assert_array_equal(r["GV"], np.array(o["GV"]))           # bit-exact
+    np.testing.assert_array_equal(r["GE"], np.array(o["GE"]).reshape(-1, 2))
+
+
+def test_subdivision_covers_each_face():
+    """Sub-faces have the parent's orientation; on a general tetrahedron no
+    sub-face is dropped, so their areas sum to the surface area."""
+    V, F = _cases()["tetra"]
+    r = cad.prepare(V, F)
+    SV, SF = r["V"], r["F"]
+    n_sub = np.cross(SV[SF[:, 1]] - SV[SF[:, 0]], SV[SF[:, 2]] - SV[SF[:, 0]])
+    n_par = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    area = np.linalg.norm(n_par, axis=1).sum() / 2
+    assert np.linalg.norm(n_sub, axis=1).sum() / 2 == pytest.approx(area, rel=1e-6)
+    # every sub-face normal agrees with one parent normal
+    cos = (n_sub / np.linalg.norm(n_sub, axis=1, keepdims=True)) @ (n_par / np.linalg.norm(n_par, axis=1,
+                                                                                            keepdims=True)).T
+    assert (cos.max(axis=1) > 1 - 1e-6).all()
+
+
+def test_representative_graph_invariants():
+    V, F = _cases()["rotated_box"]
+    r = cad.prepare(V, F)
+    ref, GV = r["ref"], r["GV"]
+    for g in range(len(GV)):
+        m = np.nonzero(ref == g)[0]
+        assert len(m) > 0
+        np.testing.assert_allclose(GV[g], r["V"][m].mean(axis=0), rtol=0, atol=1e-15)
+    # graph edges are exactly the neighbour pairs mapped through ref
+    e = {(min(a, b), max(a, b)) for a, b in ref[r["E"]] if a != b}
+    assert e == {tuple(x) for x in r["GE"]}
