@@ -34,6 +34,14 @@ def lm_algorithmic_bytes(nv, nh, lm_iters, pcg_iters):
     return lm_iters * (96 * nv + 44 * nh) + pcg_iters * (184 * nv + 4 * nh)
 
 
+# HBM bytes per LM-solver launch measured with rocprofv3 counter passes
+# (FETCH_SIZE x 1.99975 + WRITE_SIZE x 1.0, factors from a 256 MiB copy) on
+# the default workload; profiles/k_lm_solve_lds_hbm_traffic_batch512.txt.
+# Keyed by (pairs, distinct pairs, source verts, grid); other workloads
+# report null.
+MEASURED_LM_TRAFFIC = {(512, 128, 5000, 64): 300.42e9}
+
+
 def grid_algorithmic_bytes(N, nf_t):
     return 4 * N ** 3 + 36 * nf_t
 
@@ -196,13 +204,13 @@ def main():
         "pcg_iterations_per_lm_iteration": float(np.sum([s["pcg_iterations"] for s in last]) /
                                                  max(1, np.sum([s["iterations"] for s in last]))),
         "roofline": {
-            "kernel": "k_lm_solve",
+            "kernel": "k_lm_solve_lds",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": MEASURED_LM_TRAFFIC.get((args.pairs, len(distinct), args.verts, args.grid)),
             "algorithmic_bytes_per_launch": float(np.mean(lm_bytes)),
         },
     }

@@ -16,6 +16,12 @@ struct GFace {
   int face;                                    // original face index
 };
 
+// Compact culling record of a face (112 bytes): AABB and plane.
+struct GCull {
+  double lox, loy, loz, hix, hiy, hiz;
+  double nx, ny, nz, ax, ay, az, inv_nn, pad;
+};
+
 // One distance grid to build: normalized FP64 target vertices [nv,3] and
 // int32 faces [nf,3] (device); any of dist64 / dist32 / nearest may be null.
 struct GridSpec {
@@ -36,6 +42,7 @@ struct GridJob {
   long long nv, nf, ntiles;
   int N;
   GFace* faces;
+  GCull* cull;
   unsigned* bin;
   int* perm;
   double* tb;

@@ -168,6 +168,7 @@ __global__ void k_face_prep(const GridJob* __restrict__ jobs) {
   g.flip = flip;
   g.face = (int)f;
   J.faces[pos] = g;
+  J.cull[pos] = GCull{g.lox, g.loy, g.loz, g.hix, g.hiy, g.hiz, g.nx, g.ny, g.nz, g.ax, g.ay, g.az, g.inv_nn, 0.0};
   }
 }
 
@@ -180,7 +181,7 @@ __global__ void __launch_bounds__(256) k_tile_bounds(const GridJob* __restrict__
   const long long f = t * kTile + lane;
   double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   if (lane < kTile && f < J.nf) {
-    const GFace* g = J.faces + f;
+    const GCull* g = J.cull + f;
     v[0] = g->lox; v[1] = g->loy; v[2] = g->loz; v[3] = g->hix; v[4] = g->hiy; v[5] = g->hiz;
   }
   for (int o = 32; o > 0; o >>= 1)
@@ -259,6 +260,7 @@ __device__ inline double box_gap2(const double* __restrict__ tb, long long t, do
 __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ jobs) {
   const GridJob& JJ = jobs[blockIdx.y];
   const GFace* __restrict__ faces = JJ.faces;
+  const GCull* __restrict__ cull = JJ.cull;
   const long long nf = JJ.nf;
   const double* __restrict__ tb = JJ.tb;
   const long long ntiles = JJ.ntiles;
@@ -295,29 +297,39 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
     const double bound = wave_max(valid ? best : -INFINITY);
     bool cand = false;
     if (lane < cnt) {
-      const GFace* gp = faces + f0 + lane;
+      const GCull* gp = cull + f0 + lane;
       const double ax = dmax(0.0, dmax(gp->lox - hx, lx - gp->hix));
       const double ay = dmax(0.0, dmax(gp->loy - hy, ly - gp->hiy));
       const double az = dmax(0.0, dmax(gp->loz - hz, lz - gp->hiz));
       cand = !(ax * ax + ay * ay + az * az > bound * (1.0 + 1e-9));
     }
     unsigned long long m = __ballot(cand);
-    // stage 2, per surviving face: every lane's own AABB and plane bounds,
-    // then the exact test when some lane can still improve or tie its best
-    while (m) {
-      const int q = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
-      m &= m - 1;
-      const GFace g = faces[f0 + q];
-      double gx = dmax(0.0, dmax(g.lox - qx, qx - g.hix));
-      double gy = dmax(0.0, dmax(g.loy - qy, qy - g.hiy));
-      double gz = dmax(0.0, dmax(g.loz - qz, qz - g.hiz));
+    // stage 2, per surviving face: every lane's own AABB and plane bounds
+    // (compact record, the next one prefetched), then the exact test with
+    // the full record when some lane can still improve or tie its best
+    int q = m ? __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1) : -1;
+    if (q >= 0) m &= m - 1;
+    GCull c;
+    if (q >= 0) c = cull[f0 + q];
+    while (q >= 0) {
+      const int qn = m ? __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1) : -1;
+      if (qn >= 0) m &= m - 1;
+      GCull cn;
+      if (qn >= 0) cn = cull[f0 + qn];
+      double gx = dmax(0.0, dmax(c.lox - qx, qx - c.hix));
+      double gy = dmax(0.0, dmax(c.loy - qy, qy - c.hiy));
+      double gz = dmax(0.0, dmax(c.loz - qz, qz - c.hiz));
       double lb = gx * gx + gy * gy + gz * gz;
       // the face plane is a second lower bound (0 for degenerate faces)
-      const double dp = g.nx * (qx - g.ax) + g.ny * (qy - g.ay) + g.nz * (qz - g.az);
-      lb = dmax(lb, (dp * dp) * g.inv_nn);
-      if (!__ballot(valid && !(lb > best * (1.0 + 1e-9)))) continue;
-      double d = face_d2(g, qx, qy, qz);
-      if (d < best || (d == best && g.face < bidx)) { best = d; bidx = g.face; }
+      const double dp = c.nx * (qx - c.ax) + c.ny * (qy - c.ay) + c.nz * (qz - c.az);
+      lb = dmax(lb, (dp * dp) * c.inv_nn);
+      if (__ballot(valid && !(lb > best * (1.0 + 1e-9)))) {
+        const GFace g = faces[f0 + q];
+        double d = face_d2(g, qx, qy, qz);
+        if (d < best || (d == best && g.face < bidx)) { best = d; bidx = g.face; }
+      }
+      q = qn;
+      c = cn;
     }
   };
 
@@ -437,8 +449,8 @@ static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 size_t grid_job_workspace_bytes(long long nf) {
   const long long ntiles = (nf + kTile - 1) / kTile;
-  return al256((size_t)nf * sizeof(GFace)) + al256((size_t)nf * sizeof(unsigned)) + al256((size_t)nf * sizeof(int)) +
-         al256((size_t)ntiles * 6 * sizeof(double));
+  return al256((size_t)nf * sizeof(GFace)) + al256((size_t)nf * sizeof(GCull)) + al256((size_t)nf * sizeof(unsigned)) +
+         al256((size_t)nf * sizeof(int)) + al256((size_t)ntiles * 6 * sizeof(double));
 }
 
 int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipStream_t stream, int* err_dev) {
@@ -456,6 +468,7 @@ int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipSt
     J.V = sp.Vn; J.F = sp.F; J.nv = sp.nv; J.nf = sp.nf; J.N = sp.N;
     J.ntiles = (sp.nf + kTile - 1) / kTile;
     J.faces = (GFace*)w; w += al256((size_t)sp.nf * sizeof(GFace));
+    J.cull = (GCull*)w; w += al256((size_t)sp.nf * sizeof(GCull));
     J.bin = (unsigned*)w; w += al256((size_t)sp.nf * sizeof(unsigned));
     J.perm = (int*)w; w += al256((size_t)sp.nf * sizeof(int));
     J.tb = (double*)w; w += al256((size_t)J.ntiles * 6 * sizeof(double));
