@@ -78,3 +78,43 @@ def linear_estimation(V, F, E, ref, GV, rigidity=2.0):
               C.c_void_p(E.data_ptr()), E.shape[0], C.c_void_p(ref.data_ptr()), C.c_void_p(GV.data_ptr()),
               GV.shape[0], float(rigidity), C.byref(it), C.c_void_p(torch.cuda.current_stream().cuda_stream))
     return it.value
+
+
+def cad_deform(cad_V, cad_F, ref_V, ref_F, grid_resolution=64, lam=1.0, symmetry=0, options=None):
+    """The non-flow cad_deform pipeline (cad_deform.cc:30-120) in-process:
+    host preprocessing, then on the GPU the reference's grid, DeformGraph on
+    the representative graph and LinearEstimation.  Returns dict(V = deformed
+    subdivided vertices in the reference's original frame (FP64 numpy),
+    F, stats (DeformGraph summary), GV = deformed graph vertices (normalized
+    frame), cg_iterations, prep = the preprocessing arrays)."""
+    import torch
+    from . import deform, objio  # noqa: F401
+    ref_V = np.asarray(ref_V, np.float64).reshape(-1, 3)
+    ref_F = np.asarray(ref_F, np.int32).reshape(-1, 3)
+    if symmetry:
+        n = len(ref_V)
+        ref_V = np.concatenate([ref_V, ref_V * np.array([-1.0, 1.0, 1.0])])
+        ref_F = np.concatenate([ref_F, ref_F + n]).astype(np.int32)
+    prep = prepare(cad_V, cad_F)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    pid = deform.template_f64(ref_V, ref_F, grid_resolution)
+    try:
+        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        V = torch.from_numpy(prep["V"]).to(dev).contiguous()
+        GV = torch.from_numpy(prep["GV"]).to(dev).contiguous()
+        F = torch.from_numpy(prep["F"]).to(dev).contiguous()
+        E = torch.from_numpy(prep["E"]).to(dev).contiguous()
+        R = torch.from_numpy(prep["ref"]).to(dev).contiguous()
+        GE = torch.from_numpy(prep["GE"]).to(dev).contiguous()
+        _lib.call("mo_apply_transform_f64", C.c_void_p(V.data_ptr()), V.shape[0], pid, 0, stream)
+        _lib.call("mo_apply_transform_f64", C.c_void_p(GV.data_ptr()), GV.shape[0], pid, 0, stream)
+        GV0 = GV.clone()
+        stats = deform.deform_batch([pid], [GV], [GE], [1], [lam], options=options)[0]
+        it = linear_estimation(V, F, E, R, GV, 2.0)
+        _lib.call("mo_apply_transform_f64", C.c_void_p(V.data_ptr()), V.shape[0], pid, 1, stream)
+        torch.cuda.synchronize()
+        return dict(V=V.cpu().numpy(), F=prep["F"], stats=stats, GV=GV.cpu().numpy(), GV0=GV0.cpu().numpy(),
+                    cg_iterations=it, prep=prep, pid=pid)
+    except BaseException:
+        _lib.call("mo_template_destroy", pid)
+        raise

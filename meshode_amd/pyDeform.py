@@ -14,9 +14,11 @@ undefined:
   * wrong dtypes, unknown ``param_id`` values, out-of-range indices and
     loss calls before the matching Store call raise instead of reading
     out of bounds.
-Functions of the CAD / linear-solve path (``LoadCadMesh``, ``SolveLinear``,
-``Cad*``) are outside the deformation hot path and raise
-``NotImplementedError``.
+``LoadCadMesh`` runs the CAD preprocessing chain (host code in libmeshode,
+as in the reference) and ``SolveLinear`` (with_rot = 0) solves
+LinearEstimation on the GPU.  ``SolveLinear`` with with_rot = 1 and the
+``Cad*`` layer functions (cad_layer.cc, whose only Python caller is broken,
+cad_loss_layer.py:10-11) raise ``NotImplementedError``.
 """
 from __future__ import annotations
 
@@ -111,12 +113,42 @@ def SaveMesh(filename, tensorV, tensorF):
 
 
 def LoadCadMesh(filename):
-    raise NotImplementedError("LoadCadMesh (CAD subdivision + representative graph) is outside the "
-                              "deformation hot path of this build")
+    """mesh_tensor.cc:102-178: read the OBJ, RemoveDegenerated, MergeDuplex,
+    Subdivide(2e-2), ComputeGeometryNeighbors(1.5e-2),
+    ComputeRepresentativeGraph(1e-2) (host code in libmeshode).  Returns
+    [V f32 [n,3] (raw coordinates), F i32 [m,3], E i32 [e,2] (geometry
+    neighbour pairs), Src2Graph i32 [n,1], GraphV f32 [g,3], GraphE i32
+    [h,2]] as CPU tensors, like the reference."""
+    from . import cad
+    V, F = objio.read_obj(filename)
+    r = cad.prepare(V, F)
+    return [torch.from_numpy(r["V"].astype("float32")), torch.from_numpy(r["F"]),
+            torch.from_numpy(r["E"]), torch.from_numpy(r["ref"].reshape(-1, 1).copy()),
+            torch.from_numpy(r["GV"].astype("float32")), torch.from_numpy(r["GE"])]
 
 
-def SolveLinear(*args):
-    raise NotImplementedError("SolveLinear (linear_layer.cc) is outside the deformation hot path of this build")
+def SolveLinear(tensorV, tensorF, tensorE, tensorRef, tensorGraphV, rigidity, with_rot=0):
+    """linear_layer.cc:5-87 with with_rot = 0: LinearEstimation
+    (linear.cc:11-112) solved on the GPU; tensorV (float32 [n,3]) is
+    overwritten with the solution (FP64 solve, stored as float32) on its own
+    device.  The reference binding has no default for with_rot, so its
+    6-argument call from graph_loss_layer.py's Finalize raises; here
+    with_rot defaults to 0.  with_rot = 1 (LinearEstimationWithRot,
+    linear.cc:114-230) is not provided."""
+    if with_rot:
+        raise NotImplementedError("SolveLinear with_rot=1 (LinearEstimationWithRot) is not provided")
+    from . import cad
+    dev = _device()
+    V = tensorV.detach().to(dev).reshape(-1, 3).double().contiguous()
+    F = tensorF.detach().to(dev, torch.int32).reshape(-1, 3).contiguous()
+    E = tensorE.detach().to(dev, torch.int32).reshape(-1, 2).contiguous()
+    R = tensorRef.detach().to(dev, torch.int32).reshape(-1).contiguous()
+    GV = tensorGraphV.detach().to(dev).reshape(-1, 3).double().contiguous()
+    if R.numel() != V.shape[0]:
+        raise ValueError("SolveLinear: one graph reference per vertex is required")
+    cad.linear_estimation(V, F, E, R, GV, float(rigidity))
+    with torch.no_grad():
+        tensorV.copy_(V.to(tensorV.device, torch.float32).reshape(tensorV.shape))
 
 
 def CadEdgeLoss_forward(*args):

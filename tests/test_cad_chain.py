@@ -168,3 +168,36 @@ def test_representative_graph_invariants():
     # graph edges are exactly the neighbour pairs mapped through ref
     e = {(min(a, b), max(a, b)) for a, b in ref[r["E"]] if a != b}
     assert e == {tuple(x) for x in r["GE"]}
+
+
+def test_oracle_linear_estimation_closed_form():
+    """Two vertices in their own clusters joined by one face-less edge pair:
+    the 2x2 system per axis solved by hand."""
+    V = np.array([[0.0, 0.0, 0.0], [0.03, 0.0, 0.0]])
+    GV = np.array([[0.01, 0.02, 0.0], [0.05, -0.01, 0.03]])
+    ref = np.array([0, 1])
+    E = np.array([[0, 1]])
+    X = cad_oracle.linear_estimation(V, np.zeros((0, 3), int), E, ref, GV, rigidity=1.5)
+    w = (1.5 * 2e-2 / (0.03 + 1e-8)) ** 2
+    a = 1.0 + 1e-6
+    for c in range(3):
+        d = V[0, c] - V[1, c]
+        M = np.array([[a + w, -w], [-w, a + w]])
+        b = np.array([a * GV[0, c] + w * d, a * GV[1, c] - w * d])
+        np.testing.assert_allclose(X[:, c], np.linalg.solve(M, b), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("name", ["tetra", "rotated_box"])
+def test_oracle_linear_estimation_translation(name):
+    """Moving every graph vertex by t moves the minimizer by t."""
+    V, F = _cases()[name]
+    r = cad.prepare(V, F)
+    GV = r["GV"] + np.random.default_rng(1).normal(0, 1e-3, r["GV"].shape)
+    t = np.array([0.01, -0.02, 0.005])
+    X = cad_oracle.linear_estimation(r["V"], r["F"], r["E"], r["ref"], GV)
+    Xt = cad_oracle.linear_estimation(r["V"], r["F"], r["E"], r["ref"], GV + t)
+    np.testing.assert_allclose(Xt, X + t, rtol=0, atol=1e-12)
+    # at rest (graph vertices = cluster means) the solution moves by at most
+    # the 1e-6 pull towards the cluster centres
+    X0 = cad_oracle.linear_estimation(r["V"], r["F"], r["E"], r["ref"], r["GV"])
+    assert np.abs(X0 - r["V"]).max() < 1e-6
