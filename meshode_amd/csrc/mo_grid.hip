@@ -29,10 +29,11 @@
 //                      point-to-AABB bound still reaches that lane's best.
 //                      For a kept tile, one lane per face first drops the
 //                      faces whose AABB is out of reach of the whole brick;
-//                      the wave then walks the survivors, whose records are
-//                      read with wave-uniform addresses (scalar loads), and
-//                      tests a face exactly only when its AABB and plane
-//                      bounds can still improve some lane.  Ties keep the lowest face index, so the
+//                      the survivors' records are copied into LDS with
+//                      coalesced loads, and the wave walks them (one LDS
+//                      address for all lanes), testing a face exactly only
+//                      when its AABB and plane bounds can still improve
+//                      some lane.  Ties keep the lowest face index, so the
 //                      visiting order never changes the result.
 #include "mo_common.h"
 #include "mo_grid.h"
@@ -258,6 +259,10 @@ __device__ inline double box_gap2(const double* __restrict__ tb, long long t, do
 
 // One wavefront (64 lanes) per 4x4x4 brick.
 __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ jobs) {
+  // survivors of a tile's brick-level cull: their full records, copied with
+  // coalesced vector loads, then read by the whole wave at one LDS address
+  __shared__ GFace s_face[kTile];
+  __shared__ int s_list[kTile];
   const GridJob& JJ = jobs[blockIdx.y];
   const GFace* __restrict__ faces = JJ.faces;
   const GCull* __restrict__ cull = JJ.cull;
@@ -286,9 +291,6 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
   double best = INFINITY;
   int bidx = INT_MAX;
 
-  // Every lane tests the same face at the same time, so the face record is
-  // read with a wave-uniform address (scalar loads into SGPRs): no LDS
-  // staging and no barriers.
   auto process_tile = [&](int t) {
     const long long f0 = (long long)t * kTile;
     const int cnt = (int)((nf - f0) < kTile ? (nf - f0) : kTile);
@@ -303,59 +305,66 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
       const double az = dmax(0.0, dmax(gp->loz - hz, lz - gp->hiz));
       cand = !(ax * ax + ay * ay + az * az > bound * (1.0 + 1e-9));
     }
-    unsigned long long m = __ballot(cand);
-    // stage 2, per surviving face: every lane's own AABB and plane bounds
-    // (compact record, the next one prefetched), then the exact test with
-    // the full record when some lane can still improve or tie its best
-    int q = m ? __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1) : -1;
-    if (q >= 0) m &= m - 1;
-    GCull c;
-    if (q >= 0) c = cull[f0 + q];
-    while (q >= 0) {
-      const int qn = m ? __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1) : -1;
-      if (qn >= 0) m &= m - 1;
-      GCull cn;
-      if (qn >= 0) cn = cull[f0 + qn];
-      double gx = dmax(0.0, dmax(c.lox - qx, qx - c.hix));
-      double gy = dmax(0.0, dmax(c.loy - qy, qy - c.hiy));
-      double gz = dmax(0.0, dmax(c.loz - qz, qz - c.hiz));
+    const unsigned long long m = __ballot(cand);
+    const int ns = __popcll(m);
+    if (ns == 0) return;
+    // compact the survivors (in face order) and stage their records in LDS
+    const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    if (cand) s_list[rank] = lane;
+    __syncthreads();
+    constexpr int kChunks = (int)(sizeof(GFace) / sizeof(uint4));
+    static_assert(sizeof(GFace) % sizeof(uint4) == 0, "face records are copied in 16-byte chunks");
+    const uint4* __restrict__ src = reinterpret_cast<const uint4*>(faces + f0);
+    uint4* dst = reinterpret_cast<uint4*>(s_face);
+    for (int c = lane; c < ns * kChunks; c += 64) {
+      const int r = c / kChunks, w = c - r * kChunks;
+      dst[c] = src[s_list[r] * kChunks + w];
+    }
+    __syncthreads();
+    // stage 2, per surviving face: every lane's own AABB and plane bounds,
+    // then the exact test when some lane can still improve or tie its best
+    for (int r = 0; r < ns; ++r) {
+      const GFace& g = s_face[r];
+      double gx = dmax(0.0, dmax(g.lox - qx, qx - g.hix));
+      double gy = dmax(0.0, dmax(g.loy - qy, qy - g.hiy));
+      double gz = dmax(0.0, dmax(g.loz - qz, qz - g.hiz));
       double lb = gx * gx + gy * gy + gz * gz;
       // the face plane is a second lower bound (0 for degenerate faces)
-      const double dp = c.nx * (qx - c.ax) + c.ny * (qy - c.ay) + c.nz * (qz - c.az);
-      lb = dmax(lb, (dp * dp) * c.inv_nn);
+      const double dp = g.nx * (qx - g.ax) + g.ny * (qy - g.ay) + g.nz * (qz - g.az);
+      lb = dmax(lb, (dp * dp) * g.inv_nn);
       if (__ballot(valid && !(lb > best * (1.0 + 1e-9)))) {
-        const GFace g = faces[f0 + q];
         double d = face_d2(g, qx, qy, qz);
         if (d < best || (d == best && g.face < bidx)) { best = d; bidx = g.face; }
       }
-      q = qn;
-      c = cn;
     }
+    __syncthreads();  // the next tile overwrites s_face / s_list
   };
 
-  // 1. the kSeedTiles tiles closest to the brick first: they seed every lane
-  //    with a near-exact distance, which makes the culling below tight
+  // 1. seeds: one scan of the tile AABBs; each lane keeps its nearest tile,
+  //    and the kSeedTiles nearest of those are evaluated first.  They give
+  //    every lane a near-exact distance, which makes the culling below tight
+  //    (the choice of seeds affects speed only).
+  double mlb = INFINITY;
+  long long mt = -1;
+  for (long long c = 0; c < ntiles; c += 64) {
+    const long long t = c + lane;
+    if (t >= ntiles) continue;
+    const double lb = box_gap2(tb, t, lx, ly, lz, hx, hy, hz);
+    if (lb < mlb) { mlb = lb; mt = t; }
+  }
   int seeds[kSeedTiles];
   int nseed = 0;
-  for (int r = 0; r < kSeedTiles && r < ntiles; ++r) {
-    double mlb = INFINITY;
-    long long mt = -1;
-    for (long long c = 0; c < ntiles; c += 64) {
-      long long t = c + lane;
-      if (t >= ntiles) continue;
-      bool taken = false;
-      for (int q = 0; q < nseed; ++q) taken |= (seeds[q] == (int)t);
-      if (taken) continue;
-      double lb = box_gap2(tb, t, lx, ly, lz, hx, hy, hz);
-      if (lb < mlb) { mlb = lb; mt = t; }
-    }
+  for (int r = 0; r < kSeedTiles; ++r) {
+    double ol = mlb;
+    long long ot = mt;
     for (int o = 32; o > 0; o >>= 1) {
-      double ol = __shfl_xor(mlb, o, 64);
-      long long ot = __shfl_xor(mt, o, 64);
-      if (ol < mlb || (ol == mlb && ot >= 0 && (mt < 0 || ot < mt))) { mlb = ol; mt = ot; }
+      const double xl = __shfl_xor(ol, o, 64);
+      const long long xt = __shfl_xor(ot, o, 64);
+      if (xl < ol || (xl == ol && xt >= 0 && (ot < 0 || xt < ot))) { ol = xl; ot = xt; }
     }
-    if (mt < 0) break;
-    const int st = __builtin_amdgcn_readfirstlane((int)mt);
+    if (ot < 0) break;
+    const int st = __builtin_amdgcn_readfirstlane((int)ot);
+    if (mt == st) { mlb = INFINITY; mt = -1; }
     seeds[nseed++] = st;
     process_tile(st);
   }
