@@ -3,9 +3,12 @@
 // with Ceres' trust-region LM restated and a matrix-free block-Jacobi PCG).
 //
 // One workgroup of T threads solves one pair whose nv <= T * VPT vertices
-// are statically owned (vertex i by thread i % T, slot i / T), so every
-// per-vertex vector of the inner PCG (delta, r, p, q/z, g, D^2) lives in
-// VGPRs across the whole solve.  The only vector neighbours read (u while
+// are statically owned (vertex i by thread i % T, slot i / T), so the
+// per-vertex vectors the PCG iterates on (r, p, q/z and the distance
+// gradient g) live in VGPRs across the whole solve; the step delta and the
+// LM diagonal are kept per vertex in global memory (read/updated once per
+// PCG iteration, coalesced), which keeps the register footprint inside the
+// 128 (T = 1024) or 256 (T = 512) VGPRs a thread has.  The only vector neighbours read (u while
 // evaluating the Jacobian, p inside PCG, u + delta for the candidate) is
 // published to one float4-per-vertex LDS array, and the pair's adjacency is
 // copied once into LDS as uint16 entries (neighbour << 1 | multiplicity - 1),
@@ -32,7 +35,7 @@ namespace mo {
 
 namespace {
 
-enum GSlot { gGR = 1, gS = 2, gDG = 3 };   // global slots of LMArrays::work used here
+enum GSlot { gGR = 1, gS = 2, gDG = 3, gE = 4, gT = 5 };   // global slots of LMArrays::work used here
 
 // Block-wide exclusive scan of one int per thread (T <= 1024).
 __device__ inline int block_exclusive_scan(int x, int* buf, int& total) {
@@ -54,6 +57,15 @@ __device__ inline int block_exclusive_scan(int x, int* buf, int& total) {
   return base + v - x;
 }
 
+// An index the optimizer cannot see through: per-vertex addresses are then
+// formed at each use (base + index) instead of being hoisted out of the
+// solver loops, where they would hold dozens of registers for the whole
+// kernel.
+__device__ inline int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 template <int T, int VPT>
 __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict__ pairs,
                                                     const int* __restrict__ plist, LMArrays A, LMOptionsDev opt,
@@ -61,6 +73,9 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[2 * 8 * kMaxWaves];
   __shared__ int scan_buf[kMaxWaves];
+  const double o_min_diag = opt.min_lm_diagonal, o_max_diag = opt.max_lm_diagonal;
+  const float o_pcg_tol = opt.pcg_relative_tolerance;
+  const int o_pcg_max = opt.pcg_max_iterations;
   const int pid = plist[blockIdx.x];
   const LMPairDesc P = pairs[pid];
   const int tid = threadIdx.x;
@@ -76,6 +91,8 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
   float4* __restrict__ GR = A.work + (size_t)gGR * totalV + vb;
   float4* __restrict__ S = A.work + (size_t)gS * totalV + vb;
   float4* __restrict__ DG = A.work + (size_t)gDG * totalV + vb;
+  float4* __restrict__ EV = A.work + (size_t)gE * totalV + vb;
+  float4* __restrict__ TV = A.work + (size_t)gT * totalV + vb;
   float4* VEC = (float4*)smem;
   unsigned short* LCSR = (unsigned short*)(smem + (size_t)nv * sizeof(float4));
   int sel = 0;
@@ -87,7 +104,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
     int cnt[VPT], gst[VPT], mine = 0;
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = tid + k * T;
+      const int i = opaque(tid + k * T);
       cnt[k] = 0;
       gst[k] = 0;
       if (i < nv) {
@@ -111,17 +128,15 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
   }
 
   // ---- per-vertex register state ----------------------------------------
-  float gx[VPT], gy[VPT], gz[VPT], ld[VPT];   // grad d, lambda^2 deg
-  float tx[VPT], ty[VPT], tz[VPT];      // LM diagonal D^2 (unscaled variables)
+  float gx[VPT], gy[VPT], gz[VPT];   // grad d
   float rx[VPT], ry[VPT], rz_[VPT];     // PCG residual
   float px[VPT], py[VPT], pz[VPT];      // PCG direction
   float qx[VPT], qy[VPT], qz[VPT];      // A p, then the preconditioned residual z
-  float ex[VPT], ey[VPT], ez[VPT];      // step delta
 
   // z = M^-1 r with M = g g^T + diag(c), c = lambda^2 deg + D^2 (Sherman-Morrison)
-  auto precond = [&](int k, float r0, float r1, float r2, float& z0, float& z1, float& z2) {
-    const float i0 = __builtin_amdgcn_rcpf(ld[k] + tx[k]), i1 = __builtin_amdgcn_rcpf(ld[k] + ty[k]),
-                i2 = __builtin_amdgcn_rcpf(ld[k] + tz[k]);
+  auto precond = [&](int k, const float4 t4, float r0, float r1, float r2, float& z0, float& z1, float& z2) {
+    const float i0 = __builtin_amdgcn_rcpf(t4.w + t4.x), i1 = __builtin_amdgcn_rcpf(t4.w + t4.y),
+                i2 = __builtin_amdgcn_rcpf(t4.w + t4.z);
     const float h0 = gx[k] * i0, h1 = gy[k] * i1, h2 = gz[k] * i2;
     const float sig = 1.0f + gx[k] * h0 + gy[k] * h1 + gz[k] * h2;
     const float c = (h0 * r0 + h1 * r1 + h2 * r2) * __builtin_amdgcn_rcpf(sig);
@@ -133,23 +148,24 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
   auto eval = [&](bool first, double& cv, double& ce, double& gmax) {
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = tid + k * T;
+      const int i = opaque(tid + k * T);
       if (i < nv) VEC[i] = U[i];
     }
     __syncthreads();
     double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = tid + k * T;
+      const int i = opaque(tid + k * T);
       if (i >= nv) continue;
       const float4 x0 = X0[i];
       const float4 ui = VEC[i];
       const JetF J = lookup_jet(grid, N, x0.x + ui.x, x0.y + ui.y, x0.z + ui.z);
       gx[k] = J.v0; gy[k] = J.v1; gz[k] = J.v2;
-      ld[k] = l2 * x0.w;
+      const float ldk = l2 * x0.w;
       float sx = 0.f, sy = 0.f, sz = 0.f;
       double e = 0.0;
-      const int o = (int)(csr[k] >> 16), c = (int)(csr[k] & 0xFFFFu);
+      const int cs = opaque((int)csr[k]);
+      const int o = (int)((unsigned)cs >> 16), c = cs & 0xFFFF;
       for (int q = 0; q < c; ++q) {
         const unsigned w = LCSR[o + q];
         const float m = (float)((w & 1u) + 1u);
@@ -167,7 +183,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       gm = fabs((double)g2) > gm ? fabs((double)g2) : gm;
       v[2] = gm > v[2] ? gm : v[2];
       if (first) {
-        const float c0 = J.v0 * J.v0 + ld[k], c1 = J.v1 * J.v1 + ld[k], c2 = J.v2 * J.v2 + ld[k];
+        const float c0 = J.v0 * J.v0 + ldk, c1 = J.v1 * J.v1 + ldk, c2 = J.v2 * J.v2 + ldk;
         S[i] = make_float4(1.0f / (1.0f + sqrtf(c0)), 1.0f / (1.0f + sqrtf(c1)), 1.0f / (1.0f + sqrtf(c2)), 0.f);
       }
     }
@@ -180,18 +196,19 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
     double v[2] = {0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = tid + k * T;
+      const int i = opaque(tid + k * T);
       if (i >= nv) continue;
       const float4 s = S[i];
+      const float ldk = l2 * X0[i].w;
       float4 dg;
       if (!reuse) {
-        const double cc[3] = {(double)gx[k] * gx[k] + ld[k], (double)gy[k] * gy[k] + ld[k],
-                              (double)gz[k] * gz[k] + ld[k]};
+        const double cc[3] = {(double)gx[k] * gx[k] + ldk, (double)gy[k] * gy[k] + ldk,
+                              (double)gz[k] * gz[k] + ldk};
         const double sc[3] = {s.x, s.y, s.z};
         float d3[3];
         for (int a = 0; a < 3; ++a) {
           double t = sc[a] * sc[a] * cc[a];
-          t = t < opt.min_lm_diagonal ? opt.min_lm_diagonal : (t > opt.max_lm_diagonal ? opt.max_lm_diagonal : t);
+          t = t < o_min_diag ? o_min_diag : (t > o_max_diag ? o_max_diag : t);
           d3[a] = (float)t;
         }
         dg = make_float4(d3[0], d3[1], d3[2], 0.f);
@@ -203,15 +220,15 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       const double t0 = (double)dg.x / ((double)s.x * s.x) * inv_radius;
       const double t1 = (double)dg.y / ((double)s.y * s.y) * inv_radius;
       const double t2 = (double)dg.z / ((double)s.z * s.z) * inv_radius;
-      tx[k] = (float)(t0 < cap ? t0 : cap);
-      ty[k] = (float)(t1 < cap ? t1 : cap);
-      tz[k] = (float)(t2 < cap ? t2 : cap);
+      const float4 t4 = make_float4((float)(t0 < cap ? t0 : cap), (float)(t1 < cap ? t1 : cap),
+                                    (float)(t2 < cap ? t2 : cap), ldk);
+      TV[i] = t4;
       const float4 gr = GR[i];
       rx[k] = -gr.x; ry[k] = -gr.y; rz_[k] = -gr.z;
       float z0, z1, z2;
-      precond(k, rx[k], ry[k], rz_[k], z0, z1, z2);
+      precond(k, t4, rx[k], ry[k], rz_[k], z0, z1, z2);
       px[k] = z0; py[k] = z1; pz[k] = z2;
-      ex[k] = ey[k] = ez[k] = 0.f;
+      EV[i] = make_float4(0.f, 0.f, 0.f, 0.f);
       VEC[i] = make_float4(z0, z1, z2, 0.f);
       v[0] += (double)rx[k] * z0 + (double)ry[k] * z1 + (double)rz_[k] * z2;
       v[1] += (double)rx[k] * rx[k] + (double)ry[k] * ry[k] + (double)rz_[k] * rz_[k];
@@ -220,26 +237,28 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
     double rz = v[0];
     const double rr0 = v[1];
     if (!(rr0 > 0.0)) return 0;
-    const double tol2 = (double)opt.pcg_relative_tolerance * opt.pcg_relative_tolerance * rr0;
+    const double tol2 = (double)o_pcg_tol * o_pcg_tol * rr0;
     int it = 0;
-    for (; it < opt.pcg_max_iterations; ++it) {
+    for (; it < o_pcg_max; ++it) {
       double w1[1] = {0.0};
 #pragma unroll
       for (int k = 0; k < VPT; ++k) {
-        const int i = tid + k * T;
+        const int i = opaque(tid + k * T);
         if (i >= nv) continue;
         const float gp = gx[k] * px[k] + gy[k] * py[k] + gz[k] * pz[k];
         float lx = 0.f, ly = 0.f, lz = 0.f;
-        const int o = (int)(csr[k] >> 16), c = (int)(csr[k] & 0xFFFFu);
+        const int cs = opaque((int)csr[k]);
+      const int o = (int)((unsigned)cs >> 16), c = cs & 0xFFFF;
         for (int q = 0; q < c; ++q) {
           const unsigned w = LCSR[o + q];
           const float m = (float)((w & 1u) + 1u);
           const float4 pj = VEC[w >> 1];
           lx += m * (px[k] - pj.x); ly += m * (py[k] - pj.y); lz += m * (pz[k] - pj.z);
         }
-        qx[k] = gx[k] * gp + l2 * lx + tx[k] * px[k];
-        qy[k] = gy[k] * gp + l2 * ly + ty[k] * py[k];
-        qz[k] = gz[k] * gp + l2 * lz + tz[k] * pz[k];
+        const float4 t4 = TV[i];
+        qx[k] = gx[k] * gp + l2 * lx + t4.x * px[k];
+        qy[k] = gy[k] * gp + l2 * ly + t4.y * py[k];
+        qz[k] = gz[k] * gp + l2 * lz + t4.z * pz[k];
         w1[0] += (double)px[k] * qx[k] + (double)py[k] * qy[k] + (double)pz[k] * qz[k];
       }
       block_reduce<1, 0>(w1, red, sel);
@@ -249,12 +268,14 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       double w2[2] = {0.0, 0.0};
 #pragma unroll
       for (int k = 0; k < VPT; ++k) {
-        const int i = tid + k * T;
+        const int i = opaque(tid + k * T);
         if (i >= nv) continue;
-        ex[k] += fa * px[k]; ey[k] += fa * py[k]; ez[k] += fa * pz[k];
+        float4 e4 = EV[i];
+        e4.x += fa * px[k]; e4.y += fa * py[k]; e4.z += fa * pz[k];
+        EV[i] = e4;
         rx[k] -= fa * qx[k]; ry[k] -= fa * qy[k]; rz_[k] -= fa * qz[k];
         float z0, z1, z2;
-        precond(k, rx[k], ry[k], rz_[k], z0, z1, z2);
+        precond(k, TV[i], rx[k], ry[k], rz_[k], z0, z1, z2);
         qx[k] = z0; qy[k] = z1; qz[k] = z2;
         w2[0] += (double)rx[k] * z0 + (double)ry[k] * z1 + (double)rz_[k] * z2;
         w2[1] += (double)rx[k] * rx[k] + (double)ry[k] * ry[k] + (double)rz_[k] * rz_[k];
@@ -265,7 +286,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       if (w2[1] <= tol2) { ++it; break; }
 #pragma unroll
       for (int k = 0; k < VPT; ++k) {
-        const int i = tid + k * T;
+        const int i = opaque(tid + k * T);
         if (i >= nv) continue;
         px[k] = qx[k] + fb * px[k]; py[k] = qy[k] + fb * py[k]; pz[k] = qz[k] + fb * pz[k];
         VEC[i] = make_float4(px[k], py[k], pz[k], 0.f);
@@ -279,28 +300,31 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
   auto candidate = [&]() -> LMCandidate {
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = tid + k * T;
+      const int i = opaque(tid + k * T);
       if (i < nv) {
         const float4 ui = U[i];
-        VEC[i] = make_float4(ui.x + ex[k], ui.y + ey[k], ui.z + ez[k], 0.f);
+        const float4 e4 = EV[i];
+        VEC[i] = make_float4(ui.x + e4.x, ui.y + e4.y, ui.z + e4.z, 0.f);
       }
     }
     __syncthreads();
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = tid + k * T;
+      const int i = opaque(tid + k * T);
       if (i >= nv) continue;
       const float4 x0 = X0[i];
       const float dres = GR[i].w;
-      const float gd = gx[k] * ex[k] + gy[k] * ey[k] + gz[k] * ez[k];
+      const float4 e4 = EV[i];
+      const float gd = gx[k] * e4.x + gy[k] * e4.y + gz[k] * e4.z;
       v[0] += (double)gd * ((double)dres + 0.5 * (double)gd);
       const float4 wv = VEC[i];
       const float wx = wv.x, wy = wv.y, wz = wv.z;
       const float dc = lookup_f32(grid, N, x0.x + wx, x0.y + wy, x0.z + wz);
       v[1] += 0.5 * (double)dc * (double)dc;
       double e = 0.0;
-      const int o = (int)(csr[k] >> 16), c = (int)(csr[k] & 0xFFFFu);
+      const int cs = opaque((int)csr[k]);
+      const int o = (int)((unsigned)cs >> 16), c = cs & 0xFFFF;
       for (int q = 0; q < c; ++q) {
         const unsigned w = LCSR[o + q];
         const float m = (float)((w & 1u) + 1u);
@@ -309,7 +333,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
         e += (double)m * ((double)ax * ax + (double)ay * ay + (double)az * az);
       }
       v[2] += 0.25 * (double)l2 * e;
-      v[3] += (double)ex[k] * ex[k] + (double)ey[k] * ey[k] + (double)ez[k] * ez[k];
+      v[3] += (double)e4.x * e4.x + (double)e4.y * e4.y + (double)e4.z * e4.z;
       const float4 ui = U[i];
       const float qx0 = x0.x + ui.x, qy0 = x0.y + ui.y, qz0 = x0.z + ui.z;
       v[4] += (double)qx0 * qx0 + (double)qy0 * qy0 + (double)qz0 * qz0;
@@ -327,7 +351,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
     // u <- u + delta: exactly the candidate published to LDS
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = tid + k * T;
+      const int i = opaque(tid + k * T);
       if (i < nv) U[i] = VEC[i];
     }
   };

@@ -61,7 +61,7 @@ struct LMCandidate {
 //   accept()                         x <- x + delta
 // Writes the per-pair result from thread 0.
 template <class Eval, class Solve, class Cand, class Accept>
-__device__ inline void lm_trust_region_loop(const LMOptionsDev& opt, Eval eval, Solve solve, Cand candidate,
+__device__ inline void lm_trust_region_loop(const LMOptionsDev opt, Eval eval, Solve solve, Cand candidate,
                                             Accept accept, LMResultDev* out, long long clock_start) {
   double cv, ce, gmax;
   eval(true, cv, ce, gmax);
