@@ -39,7 +39,7 @@ def lm_algorithmic_bytes(nv, nh, lm_iters, pcg_iters):
 # the default workload; profiles/k_lm_solve_lds_hbm_traffic_batch512.txt.
 # Keyed by (pairs, distinct pairs, source verts, grid); other workloads
 # report null.
-MEASURED_LM_TRAFFIC = {(512, 128, 5000, 64): 300.42e9}
+MEASURED_LM_TRAFFIC = {(512, 128, 5000, 64): 270.58e9}
 
 
 def grid_algorithmic_bytes(N, nf_t):
