@@ -145,6 +145,7 @@ typedef struct mo_lm_options {
   int32_t timing;                        /* nonzero: record phase times (mo_last_timings) */
   int32_t kernel_variant;                /* 0: register/LDS kernel when a pair fits, else the
                                             global-memory kernel; 1: global-memory kernel only */
+  int32_t max_successful_steps;          /* > 0: stop after that many accepted steps (0) */
 } mo_lm_options;
 
 typedef struct mo_lm_stats {
@@ -165,8 +166,10 @@ typedef struct mo_lm_stats {
 void mo_lm_default_options(mo_lm_options* opt);
 
 /* Deformer::Deform (deformer.cc:18-92; kind 0, idx = faces [n,3]) or
- * Deformer::DeformGraph (deformer.cc:370-442; kind 1, idx = edges [n,2]) for
- * a batch of pairs in one persistent launch.  Pair p deforms V[p] ([nv[p],3]
+ * Deformer::DeformGraph (deformer.cc:370-442; kind 1, idx = edges [n,2]) or
+ * the same with AdaptiveEdgeLoss residuals (edgeloss.h:35-61; kind 2, idx =
+ * edges [n,2], weight lambda * 2e-2 / (|v| + 1e-8) per edge) for a batch of
+ * pairs in one persistent launch.  Pair p deforms V[p] ([nv[p],3]
  * FP64, already in the template's normalized frame, updated in place)
  * against the grid of template param_ids[p] with rigidity lambda[p].
  * param_ids, V, nv, idx, nidx, kind, lambda and stats are HOST arrays of
@@ -190,6 +193,26 @@ int mo_rigid_deform_batch(int32_t npairs, const double* const* src, const int64_
                           const int64_t* tgt_nv, const int32_t* const* tgt_faces, const int64_t* tgt_nf,
                           const int32_t* grid_resolution, const double* lambda, const mo_lm_options* opt,
                           double* const* out, mo_lm_stats* stats, void* stream);
+
+/* Deformer::DeformSubdivision with a TerminateWhenSuccessCallback
+ * (deformer.cc:173-263, callback.h:9-39, as cad_deform.cc:87-114 runs it):
+ * distance residuals on every vertex of the subdivided mesh plus
+ * AdaptiveEdgeLoss residuals on the geometry-neighbour pairs E [ne,2] and
+ * the face half-edges of F [nf,3], rest offsets from V_rest; the solve is
+ * repeated from the current positions until two consecutive final costs
+ * differ by less than 1e-6 (at most max_solves times), each solve stopping
+ * at the callback's second successful step.  V [nv,3] FP64 (device; start
+ * positions in, result out; may equal V_rest only before the first call).
+ * At the first successful step of a solve the callback receives the
+ * positions the solve started from (Ceres hands the user's parameter blocks
+ * back only when a solve ends), as a device pointer valid during the call.
+ * stats (host, optional) receives the last solve's summary, *solves (host,
+ * optional) the number of solves.  Synchronizes `stream`. */
+typedef void (*mo_flow_callback)(const double* V, int64_t nv, void* user);
+int mo_deform_subdivision(int32_t param_id, const double* V_rest, double* V, int64_t nv, const int32_t* F,
+                          int64_t nf, const int32_t* E, int64_t ne, double lambda, const mo_lm_options* opt,
+                          int32_t max_solves, mo_flow_callback callback, void* user, mo_lm_stats* stats,
+                          int32_t* solves, void* stream);
 
 /* Phase times of the calling thread's last mo_deform_batch /
  * mo_rigid_deform_batch call made with opt->timing set, from HIP events on

@@ -24,7 +24,7 @@ SYMBOLS = [
     "mo_graph_store", "mo_graph_loss_forward", "mo_graph_loss_backward",
     "mo_lm_default_options", "mo_deform_batch", "mo_rigid_deform_batch", "mo_last_timings",
     "mo_cad_prepare", "mo_cad_sizes", "mo_cad_export", "mo_cad_destroy", "mo_delaunay2", "mo_delaunay3",
-    "mo_apply_transform_f64", "mo_linear_estimation",
+    "mo_apply_transform_f64", "mo_linear_estimation", "mo_deform_subdivision",
 ]
 
 
@@ -54,6 +54,7 @@ class LMOptions(C.Structure):
         ("block_threads", C.c_int32),
         ("timing", C.c_int32),
         ("kernel_variant", C.c_int32),
+        ("max_successful_steps", C.c_int32),
     ]
 
 
@@ -112,7 +113,12 @@ _SIGS = {
     "mo_delaunay3": (C.c_int, [_vp, _i64, _vp, _i64, C.POINTER(_i64), C.POINTER(_i32)]),
     "mo_apply_transform_f64": (C.c_int, [_vp, _i64, _i32, _i32, _vp]),
     "mo_linear_estimation": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _f64, C.POINTER(_i32), _vp]),
+    "mo_deform_subdivision": (C.c_int, [_i32, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _f64, C.POINTER(LMOptions), _i32,
+                                        _vp, _vp, _vp, C.POINTER(_i32), _vp]),
 }
+
+# void (*)(const double* V, int64_t nv, void* user)
+FlowCallback = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_void_p)
 
 
 def load(path: str = LIB_PATH):

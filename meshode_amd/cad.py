@@ -80,6 +80,49 @@ def linear_estimation(V, F, E, ref, GV, rigidity=2.0):
     return it.value
 
 
+_hip = None
+
+
+def _copy_to_host(ptr, n):
+    """[n,3] FP64 device array at ptr -> numpy (hipMemcpy, device to host)."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemcpy.restype = C.c_int
+    out = np.empty((n, 3))
+    rc = _hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), out.nbytes, 2)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed ({rc})")
+    return out
+
+
+def deform_subdivision(pid, V_rest, V, F, E, lam=1.0, options=None, max_solves=100000):
+    """Deformer::DeformSubdivision with the flow callback (deformer.cc:
+    173-263, cad_deform.cc:87-114): V (FP64 device tensor, normalized frame)
+    is updated in place; returns (stats, snapshots) where snapshots are the
+    positions handed to the callback (numpy, normalized frame); cad_deform
+    appends the final state after the loop."""
+    import torch
+    from . import deform
+    snaps = []
+
+    def cb(ptr, n, user):
+        snaps.append(_copy_to_host(ptr, n))
+
+    cfn = _lib.FlowCallback(cb)
+    st = _lib.LMStats()
+    nsol = C.c_int32()
+    opt = options or deform.default_options()
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.call("mo_deform_subdivision", int(pid), C.c_void_p(V_rest.data_ptr()), C.c_void_p(V.data_ptr()),
+              V.shape[0], C.c_void_p(F.data_ptr()), F.shape[0], C.c_void_p(E.data_ptr()), E.shape[0], float(lam),
+              C.byref(opt), int(max_solves), cfn, None, C.byref(st), C.byref(nsol), stream)
+    d = st.as_dict()
+    d["solves"] = nsol.value
+    return d, snaps
+
+
 def cad_deform(cad_V, cad_F, ref_V, ref_F, grid_resolution=64, lam=1.0, symmetry=0, options=None):
     """The non-flow cad_deform pipeline (cad_deform.cc:30-120) in-process:
     host preprocessing, then on the GPU the reference's grid, DeformGraph on

@@ -16,13 +16,16 @@
 // subdivision.cc:462-471, on the GPU); "Deformed" (:116) and the subdivided
 // mesh written in the reference's original frame (:118, mesh.cc:46-64).
 // With flow_output (:76-114) the subdivided mesh itself is deformed
-// (Deformer::DeformSubdivision) and its trajectory written as polylines.
+// (Deformer::DeformSubdivision, AdaptiveEdgeLoss, repeated solves that stop
+// at the callback's second successful step) and the recorded positions are
+// written as one polyline per vertex; no LinearSolve in that mode.
 // Ceres' per-iteration progress table is replaced by one summary line.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -112,25 +115,66 @@ int main(int argc, char** argv) {
   mo_lm_options opt;
   mo_lm_default_options(&opt);
   if (argc > 8) {
-    std::cerr << "cad_deform: flow output (Deformer::DeformSubdivision) is not available in this build" << std::endl;
-    return 1;
+    // Deformer::DeformSubdivision on the subdivided mesh, recording the
+    // flow at each solve's first successful step (cad_deform.cc:87-114)
+    struct Flow {
+      std::vector<std::vector<double>> snaps;
+    } flow;
+    auto record = [](const double* Vd, int64_t nv, void* user) {
+      std::vector<double> h((size_t)nv * 3);
+      if (hipMemcpy(h.data(), Vd, h.size() * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess)
+        static_cast<Flow*>(user)->snaps.push_back(std::move(h));
+    };
+    double* dRest = nullptr;
+    HIPCHK(hipMalloc(&dRest, V.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(dRest, dV, V.size() * sizeof(double), hipMemcpyDeviceToDevice));
+    mo_lm_stats st;
+    int32_t nsolve = 0;
+    if (mo_deform_subdivision(pid, dRest, dV, n[0], dF, n[1], dE, n[2], lambda, &opt, 100000, record, &flow, &st,
+                              &nsolve, nullptr) != MO_OK)
+      return die("DeformSubdivision");
+    std::vector<double> last(V.size());
+    HIPCHK(hipMemcpy(last.data(), dV, last.size() * sizeof(double), hipMemcpyDeviceToHost));
+    flow.snaps.push_back(last);
+    std::cout << "LM: " << nsolve << " solves, last " << st.iterations << " iterations (" << st.successful_steps
+              << " successful), " << termination_name(st.termination) << std::endl;
+    std::cout << "Vertices cost: " << st.vertex_cost << std::endl;
+    std::cout << "Rigidity cost: " << st.rigidity_cost << std::endl;
+    std::cout << "Final cost: " << st.vertex_cost + st.rigidity_cost << std::endl;
+    int32_t gN = 0;
+    double scale = 1, pos[3] = {0, 0, 0};
+    if (mo_template_info(pid, &gN, &scale, pos) != MO_OK) return die("template");
+    std::ofstream os(argv[8]);
+    long long top = 1;
+    const size_t ns = flow.snaps.size();
+    for (int64_t i = 0; i < n[0]; ++i) {
+      for (size_t j = 0; j < ns; ++j) {
+        const double* v = &flow.snaps[j][3 * i];
+        os << "v " << v[0] * scale + pos[0] << " " << v[1] * scale + pos[1] << " " << v[2] * scale + pos[2] << "\n";
+      }
+      for (size_t j = 0; j + 1 < ns; ++j) os << "l " << top + (long long)j << " " << top + (long long)j + 1 << "\n";
+      top += (long long)ns;
+    }
+    os.close();
+    (void)hipFree(dRest);
+  } else {
+    // Deformer::DeformGraph on the representative graph
+    mo_lm_stats st;
+    const int32_t kind = 1;
+    double* gvs[1] = {dGV};
+    const int32_t* ges[1] = {dGE};
+    if (mo_deform_batch(1, &pid, gvs, &n[3], ges, &n[4], &kind, &lambda, &opt, &st, nullptr) != MO_OK)
+      return die("DeformGraph");
+    std::cout << "LM: " << st.iterations << " iterations (" << st.successful_steps << " successful), "
+              << st.pcg_iterations << " PCG iterations, " << termination_name(st.termination) << std::endl;
+    std::cout << "Vertices cost: " << st.vertex_cost << std::endl;
+    std::cout << "Rigidity cost: " << st.rigidity_cost << std::endl;
+    std::cout << "Final cost: " << st.vertex_cost + st.rigidity_cost << std::endl;
+    // Subdivision::LinearSolve (default rigidity 2, linear.h:8-14)
+    int32_t cg_it = 0;
+    if (mo_linear_estimation(dV, n[0], dF, n[1], dE, n[2], dR, dGV, n[3], 2.0, &cg_it, nullptr) != MO_OK)
+      return die("LinearSolve");
   }
-  // Deformer::DeformGraph on the representative graph
-  mo_lm_stats st;
-  const int32_t kind = 1;
-  double* gvs[1] = {dGV};
-  const int32_t* ges[1] = {dGE};
-  if (mo_deform_batch(1, &pid, gvs, &n[3], ges, &n[4], &kind, &lambda, &opt, &st, nullptr) != MO_OK)
-    return die("DeformGraph");
-  std::cout << "LM: " << st.iterations << " iterations (" << st.successful_steps << " successful), "
-            << st.pcg_iterations << " PCG iterations, " << termination_name(st.termination) << std::endl;
-  std::cout << "Vertices cost: " << st.vertex_cost << std::endl;
-  std::cout << "Rigidity cost: " << st.rigidity_cost << std::endl;
-  std::cout << "Final cost: " << st.vertex_cost + st.rigidity_cost << std::endl;
-  // Subdivision::LinearSolve (default rigidity 2, linear.h:8-14)
-  int32_t cg_it = 0;
-  if (mo_linear_estimation(dV, n[0], dF, n[1], dE, n[2], dR, dGV, n[3], 2.0, &cg_it, nullptr) != MO_OK)
-    return die("LinearSolve");
   std::cout << "Deformed" << std::endl;
 
   if (mo_apply_transform_f64(dV, n[0], pid, 1, nullptr) != MO_OK) return die("transform");

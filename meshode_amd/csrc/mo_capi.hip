@@ -316,12 +316,43 @@ int edge_backward(const float* V, int64_t nv, const int32_t* idx, int64_t n, int
 
 __global__ void k_stamp(long long* out) { *out = wall_clock64(); }
 
+// x0 = rest position (w: sum of edge weights), u = start - rest (0 when the
+// solve starts at the rest shape).
 __global__ void k_make_x0(const double* __restrict__ V, int nv, const float* __restrict__ deg, float4* __restrict__ x0,
-                          float4* __restrict__ u) {
+                          float4* __restrict__ u, const double* __restrict__ xs) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   x0[i] = make_float4((float)V[i * 3], (float)V[i * 3 + 1], (float)V[i * 3 + 2], deg[i]);
-  u[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (xs)
+    u[i] = make_float4((float)(xs[i * 3] - V[i * 3]), (float)(xs[i * 3 + 1] - V[i * 3 + 1]),
+                       (float)(xs[i * 3 + 2] - V[i * 3 + 2]), 0.f);
+  else
+    u[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// AdaptiveEdgeLoss weights (edgeloss.h:35-61): every adjacency entry of a
+// kind-2 pair gets m * (lambda * 2e-2 / (|V_i - V_j| + 1e-8))^2 (m merged
+// directed edges between the two vertices, all with the same rest offset),
+// and the vertex's degree becomes the sum of its weights.  V is the FP64
+// rest shape; adj / deg are this pair's slices.
+__global__ void k_adaptive_weights(const double* __restrict__ V, int nv, const int2* __restrict__ adj,
+                                   const unsigned* __restrict__ nbr, double lambda, float* __restrict__ wt,
+                                   float* __restrict__ deg) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int2 a = adj[i];
+  double dsum = 0.0;
+  for (int k = 0; k < a.y; ++k) {
+    const unsigned en = nbr[a.x + k];
+    const int j = (int)(en & 0xFFFFFFu);
+    const double m = (double)(en >> 24);
+    const double d0 = V[i * 3] - V[j * 3], d1 = V[i * 3 + 1] - V[j * 3 + 1], d2 = V[i * 3 + 2] - V[j * 3 + 2];
+    const double l = lambda * (2e-2 / (sqrt(d0 * d0 + d1 * d1 + d2 * d2) + 1e-8));
+    const double w = m * (l * l);
+    wt[a.x + k] = (float)w;
+    dsum += w;
+  }
+  deg[i] = (float)dsum;
 }
 
 // out = x0_64 + u (normalized frame), or (x0_64 + u) * scale + pos when sp
@@ -346,6 +377,7 @@ struct CorePair {
   double lambda;
   const double* sp;    // device (scale, pos) for denormalized output, or null
   double* out;
+  const double* xstart = nullptr;  // start positions (device FP64), null = the rest shape x0
 };
 
 LMOptionsDev to_dev_options(const mo_lm_options& o) {
@@ -363,6 +395,7 @@ LMOptionsDev to_dev_options(const mo_lm_options& o) {
   d.max_lm_diagonal = o.max_lm_diagonal;
   d.pcg_max_iterations = o.pcg_max_iterations;
   d.pcg_relative_tolerance = o.pcg_relative_tolerance;
+  d.max_successful_steps = o.max_successful_steps;
   return d;
 }
 
@@ -435,7 +468,8 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
     MO_CHECK(c.nv >= 0 && c.nv < (1 << 24), MO_ERR_ARG, "pair %d: vertex count %lld outside [0, 2^24)", p, c.nv);
-    MO_CHECK(c.kind == 0 || c.kind == 1, MO_ERR_ARG, "pair %d: kind must be 0 (faces) or 1 (edges)", p);
+    MO_CHECK(c.kind >= 0 && c.kind <= 2, MO_ERR_ARG,
+             "pair %d: kind must be 0 (faces), 1 (edges) or 2 (edges, adaptive weights)", p);
     vbase[p] = nvtot;
     nvtot += c.nv;
     long long ne = c.kind == 0 ? 3 * c.nidx : c.nidx;
@@ -446,7 +480,8 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   const long long nvalloc = std::max<long long>(nvtot, 1);
   size_t bytes = al(nvalloc * sizeof(float4)) * 2 + al(nvalloc * sizeof(int2)) + al(nvalloc * sizeof(float)) +
                  al((size_t)kLMWorkSlots * nvalloc * sizeof(float4)) +
-                 al(std::max<long long>(2 * nedges_tot, 1) * sizeof(unsigned)) + al(topo_workspace_bytes(nvalloc)) +
+                 al(std::max<long long>(2 * nedges_tot, 1) * sizeof(unsigned)) +
+                 al(std::max<long long>(2 * nedges_tot, 1) * sizeof(float)) + al(topo_workspace_bytes(nvalloc)) +
                  al(np * sizeof(LMPairDesc)) + al(np * sizeof(TopoPairDesc)) + al(np * sizeof(LMResultDev)) +
                  al(2 * np * sizeof(int)) + al(np * sizeof(int)) + 512;
   Arena ar;
@@ -458,6 +493,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   float* deg = ar.take<float>(nvalloc);
   float4* work = ar.take<float4>((size_t)kLMWorkSlots * nvalloc);
   unsigned* raw = ar.take<unsigned>(std::max<long long>(2 * nedges_tot, 1));
+  float* wt = ar.take<float>(std::max<long long>(2 * nedges_tot, 1));
   void* tws = ar.take<char>(topo_workspace_bytes(nvalloc));
   LMPairDesc* d_pairs = ar.take<LMPairDesc>(np);
   TopoPairDesc* d_topo = ar.take<TopoPairDesc>(np);
@@ -470,8 +506,9 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   std::vector<TopoPairDesc> h_topo(np);
   for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
-    h_pairs[p] = LMPairDesc{c.grid, c.N, (int)c.nv, vbase[p], 0, (float)c.lambda, 0};
-    h_topo[p] = TopoPairDesc{c.idx, c.kind == 0 ? 3 * c.nidx : c.nidx, vbase[p], (int)c.nv, c.kind};
+    const bool weighted = c.kind == 2;
+    h_pairs[p] = LMPairDesc{c.grid, c.N, (int)c.nv, vbase[p], 0, weighted ? 1.0f : (float)c.lambda, weighted ? 1 : 0};
+    h_topo[p] = TopoPairDesc{c.idx, c.kind == 0 ? 3 * c.nidx : c.nidx, vbase[p], (int)c.nv, c.kind == 0 ? 0 : 1};
   }
   MO_HIP(hipMemcpyAsync(d_pairs, h_pairs.data(), np * sizeof(LMPairDesc), hipMemcpyHostToDevice, s));
   MO_HIP(hipMemcpyAsync(d_topo, h_topo.data(), np * sizeof(TopoPairDesc), hipMemcpyHostToDevice, s));
@@ -480,8 +517,12 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
     if (c.nv == 0) continue;
-    hipLaunchKernelGGL(k_make_x0, dim3((unsigned)((c.nv + 255) / 256)), dim3(256), 0, s, c.x0, (int)c.nv,
-                       deg + vbase[p], x0 + vbase[p], u + vbase[p]);
+    const dim3 grid((unsigned)((c.nv + 255) / 256));
+    if (c.kind == 2)
+      hipLaunchKernelGGL(k_adaptive_weights, grid, dim3(256), 0, s, c.x0, (int)c.nv, adj + vbase[p], raw, c.lambda,
+                         wt, deg + vbase[p]);
+    hipLaunchKernelGGL(k_make_x0, grid, dim3(256), 0, s, c.x0, (int)c.nv, deg + vbase[p], x0 + vbase[p],
+                       u + vbase[p], c.xstart);
   }
   MO_HIP(hipGetLastError());
   std::vector<int> pstat(2 * np);
@@ -505,7 +546,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   for (int p : order) {
     const CorePair& c = pairs[p];
     int b = -1;
-    if (opt.kernel_variant == 0 && c.nv > 0 && c.nv <= 32767 && pstat[2 * p + 1] <= 2) {
+    if (opt.kernel_variant == 0 && c.kind != 2 && c.nv > 0 && c.nv <= 32767 && pstat[2 * p + 1] <= 2) {
       b = lm_lds_bucket_for((int)c.nv);
       if (b >= 0 && lm_lds_bytes((int)c.nv, pstat[2 * p]) > lds_cap) b = -1;
     }
@@ -521,7 +562,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   }
   start[nb + 1] = (int)plist.size();
   MO_HIP(hipMemcpyAsync(d_plist, plist.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
-  LMArrays A{x0, u, adj, raw, work};
+  LMArrays A{x0, u, adj, raw, work, wt};
   const LMOptionsDev dopt = to_dev_options(opt);
   if ((rc = timer.mark(2))) return rc;
   for (int b = 0; b < nb; ++b)
@@ -707,6 +748,7 @@ MO_EXPORT void mo_lm_default_options(mo_lm_options* o) {
   o->block_threads = 1024;
   o->timing = 0;
   o->kernel_variant = 0;
+  o->max_successful_steps = 0;
 }
 
 MO_EXPORT int mo_last_timings(double* out6) {
@@ -798,4 +840,80 @@ MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, co
   if ((rc = lm_core(pairs, o, stats, s, timer))) return rc;
   if ((rc = timer.finish())) return rc;
   return check_err_flag(err, s, "target faces");
+}
+
+namespace {
+
+// DeformSubdivision's residual order: the geometry-neighbour pairs, then
+// every face half-edge (deformer.cc:194-226).
+__global__ void k_subdivision_edges(const int32_t* __restrict__ E, long long ne, const int32_t* __restrict__ F,
+                                    long long nf, int32_t* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ne) {
+    out[2 * i] = E[2 * i];
+    out[2 * i + 1] = E[2 * i + 1];
+  } else if (i < ne + 3 * nf) {
+    const long long h = i - ne, f = h / 3;
+    const int j = (int)(h - 3 * f);
+    out[2 * i] = F[3 * f + j];
+    out[2 * i + 1] = F[3 * f + (j + 1) % 3];
+  }
+}
+
+}  // namespace
+
+MO_EXPORT int mo_deform_subdivision(int32_t param_id, const double* V_rest, double* V, int64_t nv, const int32_t* F,
+                                    int64_t nf, const int32_t* E, int64_t ne, double lambda,
+                                    const mo_lm_options* opt, int32_t max_solves, mo_flow_callback callback,
+                                    void* user, mo_lm_stats* stats, int32_t* solves, void* stream) {
+  Template* t = get_template(param_id);
+  MO_CHECK(t, MO_ERR_ARG, "unknown param_id %d", param_id);
+  MO_CHECK(nv >= 0 && nf >= 0 && ne >= 0 && max_solves > 0, MO_ERR_ARG, "mo_deform_subdivision: bad sizes");
+  MO_CHECK((V_rest && V) || nv == 0, MO_ERR_ARG, "mo_deform_subdivision: null vertex buffer");
+  MO_CHECK((F || nf == 0) && (E || ne == 0), MO_ERR_ARG, "mo_deform_subdivision: null index buffer");
+  if (solves) *solves = 0;
+  if (nv == 0) return MO_OK;
+  mo_lm_options o;
+  if (opt) o = *opt; else mo_lm_default_options(&o);
+  hipStream_t s = as_stream(stream);
+  const long long nh = ne + 3 * nf;
+  Arena ar;
+  int rc;
+  if ((rc = ar.reserve(al(std::max<long long>(nh, 1) * 2 * sizeof(int32_t)) + al(nv * 3 * sizeof(double)) + 512, s)))
+    return rc;
+  int32_t* edges = ar.take<int32_t>(std::max<long long>(nh, 1) * 2);
+  double* start = ar.take<double>(nv * 3);
+  if (nh > 0) {
+    hipLaunchKernelGGL(k_subdivision_edges, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, E, (long long)ne, F,
+                       (long long)nf, edges);
+    MO_HIP(hipGetLastError());
+  }
+  // TerminateWhenSuccessCallback keeps its counter across solves: a solve
+  // that starts with the counter at 0 calls back at its first successful
+  // step and stops at its second; one that starts at 1 stops at its first.
+  int counter = 0;
+  double prev = 1e30;
+  mo_lm_stats st{};
+  int n = 0;
+  for (; n < max_solves;) {
+    MO_HIP(hipMemcpyAsync(start, V, nv * 3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    CorePair c{t->grid32.as<float>(), t->N, V_rest, nv, edges, nh, 2, lambda, nullptr, V, start};
+    mo_lm_options oi = o;
+    oi.max_successful_steps = counter == 0 ? 2 : 1;
+    PhaseTimer timer;
+    if ((rc = lm_core(std::vector<CorePair>{c}, oi, &st, s, timer))) return rc;
+    ++n;
+    const int k = st.successful_steps;
+    if (counter == 0) {
+      if (k >= 1 && callback) callback(start, nv, user);
+      counter = k == 1 ? 1 : 0;
+    } else {
+      counter = k >= 1 ? 0 : 1;
+    }
+    if (std::fabs(prev - st.final_cost) < 1e-6) break;
+    prev = st.final_cost;
+  }
+  if (stats) *stats = st;
+  if (solves) *solves = n;
+  return MO_OK;
 }

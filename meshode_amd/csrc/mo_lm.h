@@ -13,8 +13,8 @@ struct LMPairDesc {
   int nv;                 // source vertices
   long long vbase;        // offset of this pair in the per-vertex arrays
   long long ebase;        // offset of this pair in the adjacency entry array
-  float lambda;           // rigidity weight (EdgeLoss lambda)
-  int pad;
+  float lambda;           // rigidity weight (EdgeLoss lambda); 1 for weighted pairs
+  int flags;              // bit 0: per-entry edge weights in LMArrays::wt
 };
 
 // Solver options; defaults restate Ceres' Solver::Options defaults that
@@ -33,6 +33,7 @@ struct LMOptionsDev {
   double max_lm_diagonal;
   int pcg_max_iterations;       // inner PCG iteration cap per LM step
   float pcg_relative_tolerance; // ||r|| <= tol * ||r0|| stops PCG
+  int max_successful_steps;     // > 0: stop after that many accepted steps
 };
 
 struct LMResultDev {
@@ -59,6 +60,7 @@ struct LMArrays {
   const int2* adj;      // (first entry, count) of the vertex's adjacency list
   const unsigned* nbr;  // entry: local neighbour index (low 24 bits) | multiplicity << 24
   float4* work;         // kLMWorkSlots float4 per vertex of scratch, slot-major
+  const float* wt;      // per adjacency entry: squared edge weight (weighted pairs only)
 };
 
 // General kernel (vectors in global memory): pairs plist[0..npairs) of pairs_dev.

@@ -51,6 +51,8 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
   const int nv = P.nv;
   const long long vb = P.vbase;
   const float l2 = P.lambda * P.lambda;
+  const bool weighted = (P.flags & 1) != 0;
+  const float* __restrict__ WT = A.wt + P.ebase;
   const float* __restrict__ grid = P.grid;
   const int N = P.N;
   const float4* __restrict__ X0 = A.x0 + vb;
@@ -87,7 +89,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
       const int2 a = ADJ[i];
       for (int k = 0; k < a.y; ++k) {
         const unsigned en = NB[a.x + k];
-        const float m = nb_mult(en);
+        const float m = weighted ? WT[a.x + k] : nb_mult(en);
         const float4 uj = U[nb_index(en)];
         const float dx = ui.x - uj.x, dy = ui.y - uj.y, dz = ui.z - uj.z;
         rx += m * dx; ry += m * dy; rz += m * dz;
@@ -202,7 +204,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
         for (int e = 0; e < a.y; ++e) {
           const unsigned en = NB[a.x + e];
           const unsigned j = nb_index(en);
-          const float m = nb_mult(en);
+          const float m = weighted ? WT[a.x + e] : nb_mult(en);
           float4 pj = Z[j];
           if (!first) { const float4 po = Pold[j]; pj.x += fb * po.x; pj.y += fb * po.y; pj.z += fb * po.z; }
           lx += m * (pi.x - pj.x); ly += m * (pi.y - pj.y); lz += m * (pi.z - pj.z);
@@ -263,7 +265,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
       for (int e = 0; e < a.y; ++e) {
         const unsigned en = NB[a.x + e];
         const unsigned j = nb_index(en);
-        const float m = nb_mult(en);
+        const float m = weighted ? WT[a.x + e] : nb_mult(en);
         const float4 uj = U[j];
         const float4 dj = DL[j];
         const float ex = ui.x - uj.x, ey = ui.y - uj.y, ez = ui.z - uj.z;

@@ -103,6 +103,7 @@ __device__ inline void lm_trust_region_loop(const LMOptionsDev opt, Eval eval, S
       reuse = false;
       last_success = true;
       ++nsucc;
+      if (opt.max_successful_steps > 0 && nsucc >= opt.max_successful_steps) { term = 0; break; }
     } else {
       radius /= decrease;
       decrease *= 2.0;

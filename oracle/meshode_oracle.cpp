@@ -641,7 +641,7 @@ struct LMProblem {
   int nv; int ne;
   const int* E;
   std::vector<double> rest;   // 3 per edge
-  double lambda;
+  std::vector<double> lam;    // per-edge residual weight (EdgeLoss lambda, or AdaptiveEdgeLoss's)
 };
 
 double lm_cost(const LMProblem& P, const double* x, double* vcost, double* ecost) {
@@ -655,7 +655,7 @@ double lm_cost(const LMProblem& P, const double* x, double* vcost, double* ecost
     if (a == b) continue;
     double s = 0.0;
     for (int k = 0; k < 3; ++k) {
-      double r = ((x[3 * a + k] - x[3 * b + k]) - P.rest[3 * h + k]) * P.lambda;
+      double r = ((x[3 * a + k] - x[3 * b + k]) - P.rest[3 * h + k]) * P.lam[h];
       s += r * r;
     }
     ce += 0.5 * s;
@@ -669,7 +669,6 @@ double lm_cost(const LMProblem& P, const double* x, double* vcost, double* ecost
 // squared column norms of J.
 void lm_jacobian(const LMProblem& P, const double* x, std::vector<double>& d, std::vector<double>& g,
                  std::vector<double>& grad, std::vector<double>& cn) {
-  const double l2 = P.lambda * P.lambda;
   std::fill(grad.begin(), grad.end(), 0.0);
   std::fill(cn.begin(), cn.end(), 0.0);
   for (int u = 0; u < P.nv; ++u) {
@@ -686,12 +685,13 @@ void lm_jacobian(const LMProblem& P, const double* x, std::vector<double>& d, st
   for (int h = 0; h < P.ne; ++h) {
     int a = P.E[2 * h], b = P.E[2 * h + 1];
     if (a == b) continue;
+    const double l = P.lam[h];
     for (int k = 0; k < 3; ++k) {
-      double r = ((x[3 * a + k] - x[3 * b + k]) - P.rest[3 * h + k]) * P.lambda;
-      grad[3 * a + k] += P.lambda * r;
-      grad[3 * b + k] -= P.lambda * r;
-      cn[3 * a + k] += l2;
-      cn[3 * b + k] += l2;
+      double r = ((x[3 * a + k] - x[3 * b + k]) - P.rest[3 * h + k]) * l;
+      grad[3 * a + k] += l * r;
+      grad[3 * b + k] -= l * r;
+      cn[3 * a + k] += l * l;
+      cn[3 * b + k] += l * l;
     }
   }
 }
@@ -712,38 +712,52 @@ OR_API void mo_oracle_lm_default_options(OracleLMOptions* o) {
   o->max_lm_diagonal = 1e32;
 }
 
-// Deformer::Deform / DeformGraph.  x: [nv,3] FP64 positions in the target's
-// normalized frame, updated in place.  E: [ne,2] directed edges (a,b) in
-// residual order.  trace (optional, 8 doubles per iteration, up to
-// max_num_iterations rows): iteration, cost, candidate cost, model cost
-// change, rho, radius after the step, accepted flag, step norm.
-OR_API int mo_oracle_lm_solve(const double* grid, int N, double* x, int nv, const int* E, int ne,
-                              double lambda, const OracleLMOptions* opt, OracleLMStats* st, double* trace) {
+// Deformer::Deform / DeformGraph / DeformSubdivision.  x: [nv,3] FP64
+// positions in the target's normalized frame (start point, updated in
+// place); x0: the positions the rest offsets v_h = x0_a - x0_b are taken
+// from (x itself when null).  E: [ne,2] directed edges (a,b) in residual
+// order.  lam_e (optional, [ne]): per-edge residual weight; AdaptiveEdgeLoss
+// (edgeloss.h:35-61) uses lambda * 2e-2 / (|v_h| + 1e-8), EdgeLoss the
+// constant lambda (when lam_e is null).  max_successful > 0 stops after that
+// many successful steps, as TerminateWhenSuccessCallback does after its
+// second one (callback.h:22-36).  trace (optional, 8 doubles per iteration,
+// up to max_num_iterations rows): iteration, cost, candidate cost, model
+// cost change, rho, radius after the step, accepted flag, step norm.
+OR_API int mo_oracle_lm_solve_ex(const double* grid, int N, double* x, const double* x0, int nv, const int* E,
+                                 int ne, const double* lam_e, double lambda, int max_successful,
+                                 const OracleLMOptions* opt, OracleLMStats* st, double* trace) {
+  if (!x0) x0 = x;
   LMProblem P;
-  P.grid = grid; P.N = N; P.nv = nv; P.ne = ne; P.E = E; P.lambda = lambda;
+  P.grid = grid; P.N = N; P.nv = nv; P.ne = ne; P.E = E;
   P.rest.resize((size_t)3 * ne);
   for (int h = 0; h < ne; ++h)
-    for (int k = 0; k < 3; ++k) P.rest[3 * h + k] = x[3 * E[2 * h] + k] - x[3 * E[2 * h + 1] + k];
+    for (int k = 0; k < 3; ++k) P.rest[3 * h + k] = x0[3 * E[2 * h] + k] - x0[3 * E[2 * h + 1] + k];
+  P.lam.assign(ne, lambda);
+  if (lam_e) P.lam.assign(lam_e, lam_e + ne);
 
   const int n = 3 * nv;
   // vertex adjacency with multiplicities
-  std::vector<std::vector<std::pair<int, int>>> adj(nv);
-  std::vector<int> deg(nv, 0);
+  // vertex adjacency with summed squared edge weights
+  std::vector<std::vector<std::pair<int, double>>> adj(nv);
+  std::vector<double> deg(nv, 0.0);
   for (int h = 0; h < ne; ++h) {
     int a = E[2 * h], b = E[2 * h + 1];
     if (a == b) continue;
-    deg[a]++; deg[b]++;
-    adj[a].push_back({b, 1});
-    adj[b].push_back({a, 1});
+    const double w = P.lam[h] * P.lam[h];
+    deg[a] += w; deg[b] += w;
+    adj[a].push_back({b, w});
+    adj[b].push_back({a, w});
   }
   std::vector<int> rp(nv + 1, 0), ci;
-  std::vector<int> cw;
+  std::vector<double> cw;
   for (int u = 0; u < nv; ++u) {
     auto& L = adj[u];
-    std::sort(L.begin(), L.end());
-    std::vector<std::pair<int, int>> M;
+    std::stable_sort(L.begin(), L.end(), [](const std::pair<int, double>& p, const std::pair<int, double>& q) {
+      return p.first < q.first;
+    });
+    std::vector<std::pair<int, double>> M;
     for (auto& e : L) {
-      if (!M.empty() && M.back().first == e.first) M.back().second++;
+      if (!M.empty() && M.back().first == e.first) M.back().second += e.second;
       else M.push_back(e);
     }
     L.swap(M);
@@ -766,7 +780,6 @@ OR_API int mo_oracle_lm_solve(const double* grid, int N, double* x, int nv, cons
   S.vals.assign(tot, 0.0);
 
   std::vector<double> d(nv), g(n), grad(n), cn(n), scale(n), diag(n), rhs(n), delta(n), cand(n);
-  const double l2 = lambda * lambda;
   double cost = lm_cost(P, x, nullptr, nullptr);
   st->initial_cost = cost;
   lm_jacobian(P, x, d, g, grad, cn);
@@ -792,14 +805,14 @@ OR_API int mo_oracle_lm_solve(const double* grid, int N, double* x, int nv, cons
       for (int k = 0; k < 3; ++k) {
         int row = 3 * pu + k;
         for (int l = 0; l <= k; ++l) {
-          double h = g[3 * u + k] * g[3 * u + l] + (k == l ? l2 * deg[u] : 0.0);
+          double h = g[3 * u + k] * g[3 * u + l] + (k == l ? deg[u] : 0.0);
           double v = scale[3 * u + k] * scale[3 * u + l] * h;
           if (k == l) v += diag[3 * u + k] / radius;
           S.at(row, 3 * pu + l) = v;
         }
         for (int e = rp[u]; e < rp[u + 1]; ++e) {
           int w = ci[e], pw = perm[w];
-          if (pw < pu) S.at(row, 3 * pw + k) = -scale[3 * u + k] * scale[3 * w + k] * l2 * cw[e];
+          if (pw < pu) S.at(row, 3 * pw + k) = -scale[3 * u + k] * scale[3 * w + k] * cw[e];
         }
       }
     }
@@ -823,9 +836,10 @@ OR_API int mo_oracle_lm_solve(const double* grid, int N, double* x, int nv, cons
       for (int h = 0; h < ne; ++h) {
         int a = E[2 * h], b = E[2 * h + 1];
         if (a == b) continue;
+        const double l = P.lam[h];
         for (int k = 0; k < 3; ++k) {
-          double r = ((x[3 * a + k] - x[3 * b + k]) - P.rest[3 * h + k]) * lambda;
-          double m = lambda * (delta[3 * a + k] - delta[3 * b + k]);
+          double r = ((x[3 * a + k] - x[3 * b + k]) - P.rest[3 * h + k]) * l;
+          double m = l * (delta[3 * a + k] - delta[3 * b + k]);
           acc += m * (r + 0.5 * m);
         }
       }
@@ -857,6 +871,7 @@ OR_API int mo_oracle_lm_solve(const double* grid, int N, double* x, int nv, cons
       radius = std::min(opt->max_trust_region_radius, radius);
       decrease = 2.0; reuse = false; last_success = true; ++nsucc;
       if (trace) { double* t = trace + 8 * (iter - 1); t[5] = radius; t[6] = 1; }
+      if (max_successful > 0 && nsucc >= max_successful) { term = 0; break; }
     } else {
       radius /= decrease; decrease *= 2.0; reuse = true; last_success = false; ++nfail;
       if (trace) { double* t = trace + 8 * (iter - 1); t[5] = radius; }
@@ -870,16 +885,29 @@ OR_API int mo_oracle_lm_solve(const double* grid, int N, double* x, int nv, cons
   return 0;
 }
 
+OR_API int mo_oracle_lm_solve(const double* grid, int N, double* x, int nv, const int* E, int ne,
+                              double lambda, const OracleLMOptions* opt, OracleLMStats* st, double* trace) {
+  return mo_oracle_lm_solve_ex(grid, N, x, nullptr, nv, E, ne, nullptr, lambda, 0, opt, st, trace);
+}
+
 // Cost terms of Deformer::Deform's final Evaluate calls (deformer.cc:76-91)
 // at x with rest offsets taken from x0.
-OR_API double mo_oracle_lm_cost(const double* grid, int N, const double* x, const double* x0, int nv,
-                                const int* E, int ne, double lambda, double* vcost, double* ecost) {
+OR_API double mo_oracle_lm_cost_ex(const double* grid, int N, const double* x, const double* x0, int nv,
+                                   const int* E, int ne, const double* lam_e, double lambda, double* vcost,
+                                   double* ecost) {
   LMProblem P;
-  P.grid = grid; P.N = N; P.nv = nv; P.ne = ne; P.E = E; P.lambda = lambda;
+  P.grid = grid; P.N = N; P.nv = nv; P.ne = ne; P.E = E;
+  P.lam.assign(ne, lambda);
+  if (lam_e) P.lam.assign(lam_e, lam_e + ne);
   P.rest.resize((size_t)3 * ne);
   for (int h = 0; h < ne; ++h)
     for (int k = 0; k < 3; ++k) P.rest[3 * h + k] = x0[3 * E[2 * h] + k] - x0[3 * E[2 * h + 1] + k];
   return lm_cost(P, x, vcost, ecost);
+}
+
+OR_API double mo_oracle_lm_cost(const double* grid, int N, const double* x, const double* x0, int nv,
+                                const int* E, int ne, double lambda, double* vcost, double* ecost) {
+  return mo_oracle_lm_cost_ex(grid, N, x, x0, nv, E, ne, nullptr, lambda, vcost, ecost);
 }
 
 // ---------------------------------------------------------------------------

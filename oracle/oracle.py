@@ -63,6 +63,8 @@ def lib():
         _lib = C.CDLL(_LIB_PATH)
         _lib.mo_oracle_lm_solve.restype = C.c_int
         _lib.mo_oracle_lm_cost.restype = C.c_double
+        _lib.mo_oracle_lm_cost_ex.restype = C.c_double
+        _lib.mo_oracle_lm_solve_ex.restype = C.c_int
     return _lib
 
 
@@ -260,6 +262,40 @@ def lm_solve(grid, N, V, E, lam=1.0, options: LMOptions | None = None, trace=Fal
     if tr is not None:
         tr = tr[: stats["iterations"]]
     return x, stats, tr
+
+
+def lm_solve_ex(grid, N, x, x0, E, lam_e=None, lam=1.0, max_successful=0, options: LMOptions | None = None):
+    """Deformer::DeformSubdivision-style solve (deformer.cc:173-263): start
+    at x, rest offsets from x0, per-edge weights lam_e (AdaptiveEdgeLoss,
+    edgeloss.h:35-61) or the constant lam, and at most max_successful
+    successful steps (TerminateWhenSuccessCallback, callback.h:22-36).
+    Returns (x_out, stats)."""
+    grid = _f64(grid); xo = _f64(x).copy(); x0 = _f64(x0); E = _i32(E)
+    le = None if lam_e is None else _f64(lam_e)
+    opt = options or default_lm_options()
+    st = LMStats()
+    lib().mo_oracle_lm_solve_ex(_p(grid), C.c_int(N), _p(xo), _p(x0), C.c_int(len(xo)), _p(E), C.c_int(len(E)),
+                                _p(le) if le is not None else None, C.c_double(lam), C.c_int(max_successful),
+                                C.byref(opt), C.byref(st), None)
+    return xo, {f: getattr(st, f) for f, _ in LMStats._fields_}
+
+
+def lm_cost_ex(grid, N, x, x0, E, lam_e=None, lam=1.0):
+    grid = _f64(grid); x = _f64(x); x0 = _f64(x0); E = _i32(E)
+    le = None if lam_e is None else _f64(lam_e)
+    vc = C.c_double(); ec = C.c_double()
+    tot = lib().mo_oracle_lm_cost_ex(_p(grid), C.c_int(N), _p(x), _p(x0), C.c_int(len(x)), _p(E), C.c_int(len(E)),
+                                     _p(le) if le is not None else None, C.c_double(lam), C.byref(vc), C.byref(ec))
+    return tot, vc.value, ec.value
+
+
+def adaptive_lambda(V, E, lam=1.0):
+    """AdaptiveEdgeLoss's weight (edgeloss.h:36-40): lambda * 2e-2 / (|v| + 1e-8),
+    v = V[a] - V[b] (FP64)."""
+    V = np.asarray(V, np.float64)
+    E = np.asarray(E)
+    v = V[E[:, 0]] - V[E[:, 1]]
+    return lam * (2e-2 / (np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2]) + 1e-8))
 
 
 def lm_cost(grid, N, x, x0, E, lam=1.0):

@@ -127,3 +127,112 @@ def test_cad_deform_cli(gpu, tmp_path):
     # usage line and exit 0 with too few arguments
     p = subprocess.run([exe, src], capture_output=True, text=True, timeout=60)
     assert p.returncode == 0 and p.stdout.startswith("./cad_deform cad.obj reference.obj output.obj")
+
+
+@pytest.mark.gpu
+def test_adaptive_edge_lm_against_oracle(gpu, oracle):
+    """kind 2 (AdaptiveEdgeLoss weights) through mo_deform_batch: full solve
+    and a solve capped at two successful steps, against the oracle."""
+    from meshode_amd import deform, pyDeform
+    print("[cad] case=adaptive-weight LM sphere", flush=True)
+    sV, sF, tV, tF = synth.displaced_sphere_pair(5, src_level=2, tgt_level=3)
+    N = 32
+    pid = deform.template_f64(tV, tF, N)
+    try:
+        _, scale, pos = pyDeform.TemplateInfo(pid)
+        grid = pyDeform.TemplateGrid(pid)[0].cpu().numpy().reshape(-1)
+        sn = (sV - pos) / scale
+        E = oracle.face_edges(sF)
+        lam = oracle.adaptive_lambda(sn, E, 1.0)
+        for cap in (0, 2):
+            V = torch.from_numpy(sn.copy()).cuda()
+            st = deform.deform_batch([pid], [V], [torch.from_numpy(E).cuda()], [2], [1.0],
+                                     options=deform.default_options(max_successful_steps=cap))[0]
+            xo, so = oracle.lm_solve_ex(grid, N, sn, sn, E, lam_e=lam, max_successful=cap)
+            eg, _, _ = oracle.lm_cost_ex(grid, N, V.cpu().numpy(), sn, E, lam)
+            assert st["final_cost"] == pytest.approx(eg, rel=1e-5)
+            assert eg == pytest.approx(so["final_cost"], rel=1e-3 if cap == 0 else 5e-3)
+            if cap:
+                assert st["successful_steps"] == cap == so["successful_steps"]
+    finally:
+        from meshode_amd import _lib
+        _lib.call("mo_template_destroy", pid)
+
+
+def _oracle_subdivision_loop(oracle, grid, N, Vn, E, lam_e):
+    """The repeated-solve loop of Deformer::DeformSubdivision with the
+    TerminateWhenSuccessCallback (deformer.cc:229-243, callback.h:22-36)."""
+    counter, prev, snaps, x = 0, 1e30, [], Vn.copy()
+    for _ in range(100000):
+        xn, st = oracle.lm_solve_ex(grid, N, x, Vn, E, lam_e=lam_e, max_successful=2 if counter == 0 else 1)
+        k = st["successful_steps"]
+        if counter == 0:
+            if k >= 1:
+                snaps.append(x.copy())
+            counter = 1 if k == 1 else 0
+        else:
+            counter = 0 if k >= 1 else 1
+        x = xn
+        if abs(prev - st["final_cost"]) < 1e-6:
+            break
+        prev = st["final_cost"]
+    return x, st, snaps
+
+
+@pytest.mark.gpu
+def test_deform_subdivision_flow_against_oracle(gpu, oracle):
+    from meshode_amd import deform, pyDeform
+    print("[cad] case=DeformSubdivision flow box -> sphere", flush=True)
+    V, F = _cad_box((0.2, 0.15, 0.1), 0.3)
+    tV, tF = _target()
+    r = cad.prepare(V, F)
+    N = 32
+    pid = deform.template_f64(tV, tF, N)
+    try:
+        _, scale, pos = pyDeform.TemplateInfo(pid)
+        grid = pyDeform.TemplateGrid(pid)[0].cpu().numpy().reshape(-1)
+        Vn = (r["V"] - pos) / scale
+        Eall = np.concatenate([r["E"], np.stack([r["F"][:, [0, 1, 2]].ravel(), r["F"][:, [1, 2, 0]].ravel()], 1)])
+        lam = oracle.adaptive_lambda(Vn, Eall, 1.0)
+        rest = torch.from_numpy(Vn).cuda()
+        Vd = rest.clone()
+        st, snaps = cad.deform_subdivision(pid, rest, Vd, torch.from_numpy(r["F"]).cuda(),
+                                           torch.from_numpy(r["E"]).cuda(), 1.0)
+        xo, so, osnaps = _oracle_subdivision_loop(oracle, grid, N, Vn, Eall, lam)
+        assert len(snaps) >= 1 and np.array_equal(snaps[0], Vn)     # the first callback sees the start
+        eg, _, _ = oracle.lm_cost_ex(grid, N, Vd.cpu().numpy(), Vn, Eall, lam)
+        assert st["final_cost"] == pytest.approx(eg, rel=1e-5)
+        assert eg == pytest.approx(so["final_cost"], rel=1e-3)
+        print("[cad] solves", st["solves"], "snapshots gpu/oracle", len(snaps), len(osnaps), flush=True)
+    finally:
+        from meshode_amd import _lib
+        _lib.call("mo_template_destroy", pid)
+
+
+@pytest.mark.gpu
+def test_cad_deform_cli_flow_output(gpu, tmp_path):
+    exe = os.path.join(ROOT, "meshode_amd", "bin", "cad_deform")
+    if not os.path.exists(exe):
+        pytest.skip("cad_deform not built")
+    print("[cad] case=cad_deform CLI flow", flush=True)
+    V, F = _cad_box((0.2, 0.15, 0.1), 0.3)
+    tV, tF = _target()
+    src, tgt = str(tmp_path / "cad.obj"), str(tmp_path / "ref.obj")
+    out, flow = str(tmp_path / "out.obj"), str(tmp_path / "flow.obj")
+    objio.write_obj(src, V, F)
+    objio.write_obj(tgt, tV, tF)
+    p = subprocess.run([exe, src, tgt, out, "32", "5000", "1.0", "0", flow], capture_output=True, text=True,
+                       timeout=600)
+    assert p.returncode == 0, p.stderr
+    assert p.stdout.splitlines()[-1] == "Deformed"
+    Vo, Fo = objio.read_obj(out)
+    lines = open(flow).read().splitlines()
+    nv = len(Vo)
+    nvl = sum(1 for l in lines if l.startswith("v "))
+    nll = sum(1 for l in lines if l.startswith("l "))
+    assert nvl % nv == 0
+    ns = nvl // nv
+    assert ns >= 2 and nll == nv * (ns - 1)
+    # the last recorded position of each vertex is the output mesh (6 digits)
+    P = np.array([[float(x) for x in l.split()[1:]] for l in lines if l.startswith("v ")]).reshape(nv, ns, 3)
+    np.testing.assert_allclose(P[:, -1], Vo, rtol=2e-6, atol=1e-6)

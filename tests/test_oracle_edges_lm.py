@@ -162,3 +162,31 @@ def test_lambda_zero_has_no_rigidity(oracle):
     x, st, _ = oracle.lm_solve(grid, 10, x0, E, 0.0)
     assert st["rigidity_cost"] == 0.0
     assert st["vertex_cost"] < st["initial_cost"]
+
+
+def test_oracle_weighted_lm_and_step_cap(oracle):
+    """mo_oracle_lm_solve_ex: constant per-edge weights reproduce the plain
+    solve; max_successful stops after exactly that many accepted steps;
+    AdaptiveEdgeLoss's weight is lambda * 2e-2 / (|v| + 1e-8)."""
+    from meshode_amd import synth
+    sV, sF, tV, tF = synth.displaced_sphere_pair(3, src_level=2, tgt_level=3)
+    tn, sc, pos = oracle.normalize(tV)
+    N = 24
+    g, _, _ = oracle.build_grid(tn, tF, N)
+    sn = oracle.apply_transform(sV, sc, pos)
+    E = oracle.face_edges(sF)
+    x1, s1, _ = oracle.lm_solve(g, N, sn, E, 0.7)
+    x2, s2 = oracle.lm_solve_ex(g, N, sn, sn, E, lam_e=np.full(len(E), 0.7))
+    np.testing.assert_array_equal(x1, x2)
+    for k in (1, 2, 3):
+        _, st = oracle.lm_solve_ex(g, N, sn, sn, E, lam=0.7, max_successful=k)
+        assert st["successful_steps"] == k and st["termination"] == 0
+    lam = oracle.adaptive_lambda(sn, E, 1.5)
+    v = sn[E[:, 0]] - sn[E[:, 1]]
+    np.testing.assert_allclose(lam, 1.5 * 2e-2 / (np.linalg.norm(v, axis=1) + 1e-8), rtol=1e-15)
+    # starting away from the rest shape: the rest offsets still come from x0
+    xs = sn + 0.01
+    tot, vc, ec = oracle.lm_cost_ex(g, N, xs, sn, E, lam)
+    assert ec == pytest.approx(0.0, abs=1e-20)       # a pure translation keeps every edge at rest
+    x3, st3 = oracle.lm_solve_ex(g, N, xs, sn, E, lam_e=lam)
+    assert st3["final_cost"] <= tot
