@@ -281,6 +281,30 @@ int mo_linear_estimation(double* V, int64_t nv, const int32_t* F, int64_t nf, co
                          const int32_t* ref, const double* GV, int64_t ngv, double rigidity, int32_t* iterations,
                          void* stream);
 
+/* LinearEstimationWithRot (linear.cc:114-230; linear.h:14-15; pyDeform
+ * SolveLinear with_rot = 1, linear_layer.cc:76-78): V [nv,3] FP64 rest
+ * positions in, solution out; F [nf,3] int32; TV [nv,3] FP64 target
+ * positions; all device arrays.  Per-vertex frames s_i R_i (scaled polar
+ * factor of the neighbourhood covariance) and the right-hand side are built
+ * on the device, then the SPD system (unit diagonal plus the face-edge
+ * Laplacian with weights 2e-2 rigidity / |e|) is solved by FP64 CG to a
+ * relative residual of 1e-13 (MO_ERR_NOCONV otherwise).  Vertices joined
+ * by a zero-length face edge (the reference's weight is infinite there and
+ * its right-hand side NaN) are solved as one unknown, the limit as the
+ * edge length goes to 0.  MO_ERR_ARG for a face index out of range.
+ * Synchronizes `stream`. */
+int mo_linear_estimation_rot(double* V, int64_t nv, const int32_t* F, int64_t nf, const double* TV,
+                             double rigidity, int32_t* iterations, void* stream);
+
+/* Nearest source point of each target point (ReverseLossLayer's
+ * cKDTree(src).query(tar, k = 1), reverse_loss_layer.py:18-19): src [ns,3],
+ * tar [nt,3] FP32 device arrays; index [nt] int32 and, if non-null,
+ * dist2 [nt] FP64 squared distances (device); tar and index may be null
+ * when nt = 0.  Distances are FP64 from the
+ * FP32 coordinates, (dx*dx + dy*dy) + dz*dz; ties go to the lowest index. */
+int mo_nearest_f32(const float* src, int64_t ns, const float* tar, int64_t nt, int32_t* index, double* dist2,
+                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,8 @@
   reference, mesh_tensor.cc:102-115).
 * ``delaunay2`` / ``delaunay3``: the triangulations standing in for CGAL's
   (delaunay.cc:50-101).
+* ``linear_estimation_rot``: LinearEstimationWithRot (linear.cc:114-230),
+  frames, right-hand side and CG on the GPU.
 * ``linear_estimation``: LinearEstimation (linear.cc:11-112), solved on the
   GPU.
 * ``cad_deform``: the whole non-flow cad_deform pipeline on the GPU.
@@ -77,6 +79,23 @@ def linear_estimation(V, F, E, ref, GV, rigidity=2.0):
     _lib.call("mo_linear_estimation", C.c_void_p(V.data_ptr()), V.shape[0], C.c_void_p(F.data_ptr()), F.shape[0],
               C.c_void_p(E.data_ptr()), E.shape[0], C.c_void_p(ref.data_ptr()), C.c_void_p(GV.data_ptr()),
               GV.shape[0], float(rigidity), C.byref(it), C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    return it.value
+
+
+def linear_estimation_rot(V, F, TV, rigidity=1.0):
+    """LinearEstimationWithRot on the GPU.  V [n,3] FP64 CUDA tensor (rest
+    positions) updated in place with the solution; F [m,3] int32 and TV [n,3]
+    FP64 CUDA tensors.  Returns the CG iteration count."""
+    import torch
+    for name, t, dt, w in (("V", V, torch.float64, 3), ("F", F, torch.int32, 3), ("TV", TV, torch.float64, 3)):
+        if not t.is_cuda or not t.is_contiguous() or t.dtype != dt or t.dim() != 2 or t.shape[1] != w:
+            raise ValueError(f"linear_estimation_rot: {name} must be a contiguous [n,{w}] {dt} device tensor")
+    if TV.shape[0] != V.shape[0]:
+        raise ValueError("linear_estimation_rot: TV needs one row per vertex")
+    it = C.c_int32()
+    _lib.call("mo_linear_estimation_rot", C.c_void_p(V.data_ptr()), V.shape[0], C.c_void_p(F.data_ptr()),
+              F.shape[0], C.c_void_p(TV.data_ptr()), float(rigidity), C.byref(it),
+              C.c_void_p(torch.cuda.current_stream().cuda_stream))
     return it.value
 
 

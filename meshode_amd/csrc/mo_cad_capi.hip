@@ -167,3 +167,12 @@ MO_EXPORT int mo_linear_estimation(double* V, int64_t nv, const int32_t* F, int6
   if (iterations) *iterations = it;
   return rc;
 }
+
+MO_EXPORT int mo_linear_estimation_rot(double* V, int64_t nv, const int32_t* F, int64_t nf, const double* TV,
+                                       double rigidity, int32_t* iterations, void* stream) {
+  MO_CHECK(V && TV && nv > 0 && nf >= 0 && (F || nf == 0), MO_ERR_ARG, "mo_linear_estimation_rot: bad arguments");
+  int it = 0;
+  const int rc = linear_estimation_rot(V, nv, F, nf, TV, rigidity, 100000, 1e-13, &it, (hipStream_t)stream);
+  if (iterations) *iterations = it;
+  return rc;
+}

@@ -15,8 +15,8 @@ undefined:
     loss calls before the matching Store call raise instead of reading
     out of bounds.
 ``LoadCadMesh`` runs the CAD preprocessing chain (host code in libmeshode,
-as in the reference) and ``SolveLinear`` (with_rot = 0) solves
-LinearEstimation on the GPU.  ``SolveLinear`` with with_rot = 1 and the
+as in the reference) and ``SolveLinear`` solves LinearEstimation
+(with_rot = 0) or LinearEstimationWithRot (with_rot = 1) on the GPU.  The
 ``Cad*`` layer functions (cad_layer.cc, whose only Python caller is broken,
 cad_loss_layer.py:10-11) raise ``NotImplementedError``.
 """
@@ -128,25 +128,30 @@ def LoadCadMesh(filename):
 
 
 def SolveLinear(tensorV, tensorF, tensorE, tensorRef, tensorGraphV, rigidity, with_rot=0):
-    """linear_layer.cc:5-87 with with_rot = 0: LinearEstimation
-    (linear.cc:11-112) solved on the GPU; tensorV (float32 [n,3]) is
-    overwritten with the solution (FP64 solve, stored as float32) on its own
+    """linear_layer.cc:5-87.  with_rot = 0: LinearEstimation (linear.cc:11-112)
+    with E, the per-vertex graph references and the graph vertices.
+    with_rot = 1: LinearEstimationWithRot (linear.cc:114-230); tensorGraphV
+    holds one target position per vertex and E / references are unused, as
+    in the reference.  Both are solved on the GPU in FP64; tensorV (float32
+    [n,3]) is overwritten with the solution, stored as float32, on its own
     device.  The reference binding has no default for with_rot, so its
     6-argument call from graph_loss_layer.py's Finalize raises; here
-    with_rot defaults to 0.  with_rot = 1 (LinearEstimationWithRot,
-    linear.cc:114-230) is not provided."""
-    if with_rot:
-        raise NotImplementedError("SolveLinear with_rot=1 (LinearEstimationWithRot) is not provided")
+    with_rot defaults to 0."""
     from . import cad
     dev = _device()
     V = tensorV.detach().to(dev).reshape(-1, 3).double().contiguous()
     F = tensorF.detach().to(dev, torch.int32).reshape(-1, 3).contiguous()
-    E = tensorE.detach().to(dev, torch.int32).reshape(-1, 2).contiguous()
-    R = tensorRef.detach().to(dev, torch.int32).reshape(-1).contiguous()
     GV = tensorGraphV.detach().to(dev).reshape(-1, 3).double().contiguous()
-    if R.numel() != V.shape[0]:
-        raise ValueError("SolveLinear: one graph reference per vertex is required")
-    cad.linear_estimation(V, F, E, R, GV, float(rigidity))
+    if with_rot:
+        if GV.shape[0] != V.shape[0]:
+            raise ValueError("SolveLinear(with_rot=1): one target position per vertex is required")
+        cad.linear_estimation_rot(V, F, GV, float(rigidity))
+    else:
+        E = tensorE.detach().to(dev, torch.int32).reshape(-1, 2).contiguous()
+        R = tensorRef.detach().to(dev, torch.int32).reshape(-1).contiguous()
+        if R.numel() != V.shape[0]:
+            raise ValueError("SolveLinear: one graph reference per vertex is required")
+        cad.linear_estimation(V, F, E, R, GV, float(rigidity))
     with torch.no_grad():
         tensorV.copy_(V.to(tensorV.device, torch.float32).reshape(tensorV.shape))
 

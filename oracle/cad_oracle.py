@@ -15,6 +15,13 @@ Restates (reference files under src/lib/):
                                   reference's insertion order, solved with
                                   scipy's sparse LU in place of Eigen's LDLT,
                                   then refined with edge-difference residuals)
+  * LinearEstimationWithRot       linear.cc:114-230 (frames from numpy's SVD,
+                                  U V^T as the reference's JacobiSVD product;
+                                  assembled in its order, sparse LU solve
+                                  refined with edge-difference residuals;
+                                  coincident vertices joined by a face edge,
+                                  where the reference divides by 0, solved
+                                  as one unknown: the |e| -> 0 limit)
   * Delaunay2D / Delaunay3D       delaunay.cc:50-101.  CGAL is absent; its
     triangulation is restated as its published definition: the Delaunay
     triangulation under the symbolic perturbation of Devillers & Teillaud
@@ -613,3 +620,160 @@ def linear_estimation_direct(V, F, E, ref, GV, rigidity=2.0):
     """The assembled system solved directly (no refinement), as the
     reference's LDLT does."""
     return linear_estimation(V, F, E, ref, GV, rigidity, refine=0)
+
+
+# ---------------------------------------------------------------------------
+# LinearEstimationWithRot
+def rot_frames(V, F, TV):
+    """(scale [n], R [n,3,3]) of linear.cc:137-166: per vertex, over its
+    face-edge neighbours in increasing index order (std::set),
+    cov = sum (TV_p - TV_i)(V_p - V_i)^T, scale = sum|d2| / (sum|d1| + 1e-8),
+    R = U V^T of the SVD of cov."""
+    V = np.asarray(V, np.float64)
+    TV = np.asarray(TV, np.float64)
+    n = len(V)
+    links = [set() for _ in range(n)]
+    for f in np.asarray(F).reshape(-1, 3):
+        for j in range(3):
+            a, b = int(f[j]), int(f[(j + 1) % 3])
+            links[a].add(b)
+            links[b].add(a)
+    cov = np.zeros((n, 3, 3))
+    scale = np.zeros(n)
+    for i in range(n):
+        lo = lc = 0.0
+        for p in sorted(links[i]):
+            d1 = V[p] - V[i]
+            d2 = TV[p] - TV[i]
+            lo += _norm(d1)
+            lc += _norm(d2)
+            cov[i] += np.outer(d2, d1)
+        scale[i] = lc / (lo + 1e-8)
+    U, _, Vt = np.linalg.svd(cov)
+    return scale, U @ Vt
+
+
+def coincident_classes(V, F):
+    """Class of each vertex under face edges of zero length (numbered by the
+    lowest member).  The reference's weight 2e-2/|e| is infinite on such an
+    edge and its right-hand side NaN; the solve treats each class as one
+    unknown, the limit of its system as |e| -> 0."""
+    V = np.asarray(V, np.float64)
+    n = len(V)
+    parent = list(range(n))
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+    for f in np.asarray(F).reshape(-1, 3):
+        for k in range(3):
+            a, b = int(f[k]), int(f[(k + 1) % 3])
+            if a != b and _norm(V[b] - V[a]) == 0.0:
+                ra, rb = find(a), find(b)
+                if ra != rb:
+                    parent[max(ra, rb)] = min(ra, rb)
+    ids, cls = {}, np.empty(n, np.int64)
+    for i in range(n):
+        cls[i] = ids.setdefault(find(i), len(ids))
+    return cls
+
+
+def linear_system_rot(V, F, TV, rigidity):
+    """(A dict, B [n,3]) of linear.cc:168-202 in the reference's order: unit
+    diagonal and TV, then for every face half-edge (v_j, v_j+1) and both
+    directions (v0, v1): reg = 2e-2 / |V_v1 - V_v0| * rigidity,
+    off = scale_v0 R_v0 (V_v1 - V_v0), A += reg (e_v0 - e_v1)(e_v0 - e_v1)^T,
+    B_v0 -= reg off, B_v1 += reg off.  Half-edges of zero length are left
+    out (their terms are infinite; see ``coincident_classes``)."""
+    V = np.asarray(V, np.float64)
+    TV = np.asarray(TV, np.float64)
+    n = len(V)
+    scale, R = rot_frames(V, F, TV)
+    A = {}
+
+    def add(x, y, w):
+        A[(x, y)] = A[(x, y)] + w if (x, y) in A else w
+
+    B = np.zeros((n, 3))
+    for i in range(n):
+        add(i, i, 1.0)
+        B[i] += TV[i]
+    regulation = 1.0
+    for f in np.asarray(F).reshape(-1, 3):
+        for k in range(3):
+            e = (int(f[k]), int(f[(k + 1) % 3]))
+            for j in range(2):
+                v0, v1 = e[j], e[1 - j]
+                off1 = V[v1] - V[v0]
+                if _norm(off1) == 0.0:
+                    continue
+                reg = regulation * 2e-2 / _norm(off1) * rigidity
+                off1 = scale[v0] * (R[v0] @ off1)
+                add(v0, v0, reg)
+                add(v0, v1, -reg)
+                add(v1, v0, -reg)
+                add(v1, v1, reg)
+                B[v0] += -reg * off1
+                B[v1] += reg * off1
+    return A, B
+
+
+def _rot_pairs(V, F, rigidity):
+    """Directed half-edge terms (v0, v1, reg) of nonzero length."""
+    V = np.asarray(V, np.float64)
+    P = []
+    for f in np.asarray(F).reshape(-1, 3):
+        for k in range(3):
+            e = (int(f[k]), int(f[(k + 1) % 3]))
+            for j in range(2):
+                v0, v1 = e[j], e[1 - j]
+                ln = _norm(V[v1] - V[v0])
+                if ln != 0.0:
+                    P.append((v0, v1, 1.0 * 2e-2 / ln * rigidity))
+    if not P:
+        return np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0)
+    a = np.array(P)
+    return a[:, 0].astype(np.int64), a[:, 1].astype(np.int64), a[:, 2]
+
+
+def linear_apply_rot(pairs, cls, Xc):
+    """P^T A P x evaluated from edge differences reg (x_v0 - x_v1), so the
+    equal and opposite terms of near-coincident pairs (weights up to
+    2e-2 / 5.6e-17) cancel exactly."""
+    v0, v1, reg = pairs
+    X = Xc[cls]
+    Y = X.copy()
+    T = reg[:, None] * (X[v0] - X[v1])
+    np.add.at(Y, v0, T)
+    np.add.at(Y, v1, -T)
+    Yc = np.zeros_like(Xc)
+    np.add.at(Yc, cls, Y)
+    return Yc
+
+
+def linear_estimation_rot(V, F, TV, rigidity=1.0, refine=12):
+    """The solution of ``linear_system_rot`` with each class of coincident
+    vertices as one unknown (P^T A P x = P^T B, X = P x for the membership
+    matrix P): scipy's sparse LU in place of Eigen's SimplicialLDLT, refined
+    with residuals from ``linear_apply_rot``.  Without the refinement the
+    direct solve is off by ~1e-2 where the subdivision left vertices 1e-17
+    apart (the assembled rows have lost their soft terms)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    A, B = linear_system_rot(V, F, TV, rigidity)
+    n = len(V)
+    keys = list(A)
+    M = sp.csc_matrix(([A[k] for k in keys], ([k[0] for k in keys], [k[1] for k in keys])), shape=(n, n))
+    cls = coincident_classes(V, F)
+    nc = int(cls.max()) + 1 if n else 0
+    P = sp.csc_matrix((np.ones(n), (np.arange(n), cls)), shape=(n, nc))
+    lu = spl.splu(sp.csc_matrix(P.T @ M @ P))
+    PB = P.T @ B
+    x = np.stack([lu.solve(PB[:, c]) for c in range(3)], axis=1)
+    pairs = _rot_pairs(V, F, rigidity)
+    for _ in range(refine):
+        R = PB - linear_apply_rot(pairs, cls, x)
+        x = x + np.stack([lu.solve(R[:, c]) for c in range(3)], axis=1)
+    return x[cls]
