@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--block-threads", type=int, default=1024, help="threads per pair of the global-memory kernel")
     ap.add_argument("--kernel-variant", type=int, default=0, help="0: register/LDS kernel when a pair fits; 1: global")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode"],
+    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "neural2"],
                     help="batch: the headline line; serve / adam / ode: the serving, pyDeform-Adam and "
                          "NeuralODE workloads (their own JSON line)")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of Poisson arrivals")
@@ -106,7 +106,8 @@ def main():
         torch.cuda.synchronize()
         del a, b
     if args.mode != "batch":
-        print(json.dumps({"serve": bench_serve, "adam": bench_adam, "ode": bench_ode}[args.mode](args)), flush=True)
+        print(json.dumps({"serve": bench_serve, "adam": bench_adam, "ode": bench_ode,
+                          "neural2": bench_neural2}[args.mode](args)), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -352,6 +353,124 @@ def bench_ode(args):
             "approx_mlp_tflops": flops * steps / dt / 1e12,
             "config": {"workload": "config 3: velocity MLP 3-256-256-256-3 softplus, RK4x16, adjoint", "points":
                        int(P.shape[0]), "grid": 128}}
+
+
+def _neural2_meshes():
+    """Source and target CAD meshes: 12-box unions of 2000 input vertices
+    each, through LoadCadMesh (subdivision 2e-2, graph cells 1e-2)."""
+    import os
+    import tempfile
+
+    from meshode_amd import objio, pyDeform, synth
+    sV, sF, tV, tF = synth.box_union_pair(0, n_verts=2000)
+    d = tempfile.mkdtemp()
+    out = []
+    for name, V, F in (("source", sV, sF), ("target", tV, tF)):
+        p = os.path.join(d, name + ".obj")
+        objio.write_obj(p, V, F)
+        out.append(pyDeform.LoadCadMesh(p))
+    return out
+
+
+def bench_neural2(args):
+    """NeuralODE bidirectional CAD deformation (cad_neural_deform2.py):
+    iterations/s of the Adam loop (flow forward on graph 1, inverse on graph
+    2, GraphLoss2 both ways, ReverseLoss both ways, backward, Adam), plus the
+    final LinearEstimationWithRot on the source mesh."""
+    import numpy as np
+    import torch
+    from meshode_amd.cad_neural_deform2 import NeuralDeform2
+    m1, m2 = _neural2_meshes()
+    raw = [m[4].clone() for m in (m1, m2)]
+    torch.manual_seed(0)
+    d = NeuralDeform2(m1, m2, 0.1, torch.device("cuda"))
+    for _ in range(5):
+        d.step()
+    torch.cuda.synchronize()
+    iters = max(args.steps, 1) * 40
+    t0 = time.perf_counter()
+    rows = [d.step() for _ in range(iters)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    last = torch.stack(rows[-1:]).cpu().numpy()[0]
+    t1 = time.perf_counter()
+    V = d.finalize()
+    torch.cuda.synchronize()
+    t_fin = time.perf_counter() - t1
+    out = {"metric": "cad_neural_deform2 Adam iterations/sec", "value": iters / dt, "unit": "iterations/s",
+           "n_gpus": 1, "steps": iters, "warmup": 5, "ms_per_step": 1e3 * dt / iters, "higher_is_better": True,
+           "dtype": "f32 (flow, losses); f64 (nearest distances, LinearEstimationWithRot)", "data": "synthetic",
+           "final_losses": [float(x) for x in last], "finalize_ms": 1e3 * t_fin,
+           "finite_output": bool(np.isfinite(V.numpy()).all()),
+           "config": {"workload": "cad_neural_deform2: 12-box unions (2000 input verts) through LoadCadMesh",
+                      "source_verts": int(m1[0].shape[0]), "graph_verts": [int(m1[4].shape[0]), int(m2[4].shape[0])],
+                      "graph_edges": [int(m1[5].shape[0]), int(m2[5].shape[0])], "grid": 64, "rigidity": 0.1}}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = _neural2_cpu_baseline(m1, m2, raw, d)
+    return out
+
+
+def _neural2_cpu_baseline(m1, m2, raw, d, iters=3):
+    """The same loop as the reference runs it on the host: torch CPU flow and
+    Adam, the oracle's C restatement of the pyDeform layers, scipy cKDTree
+    for ReverseLossLayer (as reverse_loss_layer.py:18-19 does)."""
+    import numpy as np
+    import torch
+    from scipy.spatial import cKDTree
+    from torch.autograd import Function
+
+    from meshode_amd import pyDeform
+    from meshode_amd.layers.neuralode_fast import NeuralODE
+    from oracle import oracle as O
+    O.build()
+    pid1, pid2 = int(d.param_id1), int(d.param_id2)
+    info = [pyDeform.TemplateInfo(p) for p in (pid1, pid2)]
+    grids = [pyDeform.TemplateGrid(p)[0].cpu().numpy().reshape(-1) for p in (pid1, pid2)]
+    G = [torch.from_numpy(O.normalize_by_template_f32(raw[k].numpy(), info[k][1], info[k][2])) for k in (0, 1)]
+    E = [m1[5].numpy(), m2[5].numpy()]
+    off = [O.store_graph(G[k].numpy(), E[k]) for k in (0, 1)]
+    r2 = torch.tensor(0.1 * 0.1).tolist()
+
+    class GL2(Function):
+        @staticmethod
+        def forward(ctx, V, k_grid, k_edges):
+            v = V.detach().numpy()
+            N = info[k_grid][0]
+            D = O.distance_loss_fwd(grids[k_grid], N, v) * np.float32(0.5)
+            R = O.graph_fwd(v, E[k_edges], off[k_edges]) * np.float32(0.5)
+            ctx.save_for_backward(V)
+            ctx.k = (k_grid, k_edges)
+            return torch.from_numpy(D).sum() + torch.from_numpy(R).sum() * r2
+
+        @staticmethod
+        def backward(ctx, g):
+            (V,) = ctx.saved_tensors
+            k_grid, k_edges = ctx.k
+            v = V.detach().numpy()
+            gD = torch.from_numpy(O.distance_loss_bwd(grids[k_grid], info[k_grid][0], v))
+            gR = torch.from_numpy(O.graph_bwd(v, E[k_edges], off[k_edges]))
+            return g * (gD + gR * r2), None, None
+
+    def rev(src, tar):
+        _, ii = cKDTree(src.detach().numpy()).query(tar.numpy(), k=1, workers=4)
+        x = src[torch.from_numpy(ii)] - tar
+        return 0.5 * (x * x).sum()
+
+    torch.manual_seed(0)
+    func = NeuralODE(torch.device("cpu"))
+    opt = torch.optim.Adam(func.parameters(), lr=1e-3)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        opt.zero_grad()
+        a = func.forward(G[0])
+        b = func.inverse(G[1])
+        loss = GL2.apply(a, 1, 0) + rev(a, G[1]) + GL2.apply(b, 0, 1) + rev(b, G[0])
+        loss.backward()
+        opt.step()
+    dt = time.perf_counter() - t0
+    return {"value": iters / dt, "unit": "iterations/s", "cores": torch.get_num_threads(), "kind": "port",
+            "sample": "%d iterations of the same loop on the host (torch CPU flow, oracle layers, cKDTree with 4 "
+                      "workers)" % iters}
 
 
 def _cpu_model():

@@ -37,57 +37,78 @@ from .layers.neuralode_fast import NeuralODE
 from .layers.reverse_loss_layer import ReverseLossLayer
 
 
+class NeuralDeform2:
+    """The driver's state: layers, flow, optimizer and device copies of the
+    graphs (cad_neural_deform2.py:39-63).  ``step`` runs one iteration of the
+    bidirectional loop (:68-80) and returns its four losses as a device
+    tensor; ``finalize`` runs :96-108 and returns the deformed source
+    vertices (float32, CPU) in the target's original frame."""
+
+    def __init__(self, mesh1, mesh2, rigidity=0.1, device=torch.device("cuda")):
+        self.mesh1, self.mesh2 = mesh1, mesh2
+        V1, F1, E1, V2G1, GV1, GE1 = mesh1
+        V2, F2, E2, V2G2, GV2, GE2 = mesh2
+        self.device = device
+        self.graph_loss = GraphLoss2Layer(V1, F1, GV1, GE1, V2, F2, GV2, GE2, rigidity, device)
+        self.param_id1 = self.graph_loss.param_id1
+        self.param_id2 = self.graph_loss.param_id2
+        self.reverse_loss = ReverseLossLayer()
+        self.func = NeuralODE(device)
+        self.optimizer = optim.Adam(self.func.parameters(), lr=1e-3)
+        self.GV1_origin = GV1.clone().to(device)
+        self.GV2_origin = GV2.clone().to(device)
+        self.GV1_device = GV1.to(device)
+        self.GV2_device = GV2.to(device)
+        self.GE1_device = GE1.to(device)
+        self.GE2_device = GE2.to(device)
+
+    def step(self):
+        self.optimizer.zero_grad()
+        GV1_deformed = self.func.forward(self.GV1_device)
+        GV2_deformed = self.func.inverse(self.GV2_device)
+        loss1_forward = self.graph_loss(GV1_deformed, self.GE1_device, self.GV2_device, self.GE2_device, 0)
+        loss1_backward = self.reverse_loss(GV1_deformed, self.GV2_origin, self.device)
+        loss2_forward = self.graph_loss(self.GV1_device, self.GE1_device, GV2_deformed, self.GE2_device, 1)
+        loss2_backward = self.reverse_loss(GV2_deformed, self.GV1_origin, self.device)
+        loss = loss1_forward + loss1_backward + loss2_forward + loss2_backward
+        loss.backward()
+        self.optimizer.step()
+        return torch.stack([loss1_forward, loss1_backward, loss2_forward, loss2_backward]).detach()
+
+    def finalize(self):
+        V1, F1, E1 = self.mesh1[0], self.mesh1[1], self.mesh1[2]
+        with torch.no_grad():
+            self.GV1_deformed = self.func.forward(self.GV1_device)
+            # the source mesh through the flow, in the source template's frame
+            V1_copy = V1.clone()
+            pyDeform.NormalizeByTemplate(V1_copy, self.param_id1.tolist())
+            V1_origin = V1_copy.clone()
+            self.V1_pushed = self.func.forward(V1_copy.to(self.device))
+        src_to_src = torch.from_numpy(np.arange(V1_origin.shape[0]).astype("int32"))
+        pyDeform.SolveLinear(V1_origin, F1, E1, src_to_src, self.V1_pushed, 1, 1)
+        pyDeform.DenormalizeByTemplate(V1_origin, self.param_id2.tolist())
+        return V1_origin
+
+
 def deform(mesh1, mesh2, rigidity=0.1, device=torch.device("cuda"), niter=1000, log=print):
     """mesh1 / mesh2: LoadCadMesh bundles [V, F, E, V2G, GV, GE] (CPU
     tensors; GV is normalized in place, as the reference's layer does).
     Returns dict(V, F, losses [niter, 4], func, optimizer, param_id1,
-    param_id2, GV1_deformed)."""
-    V1, F1, E1, V2G1, GV1, GE1 = mesh1
-    V2, F2, E2, V2G2, GV2, GE2 = mesh2
-    graph_loss = GraphLoss2Layer(V1, F1, GV1, GE1, V2, F2, GV2, GE2, rigidity, device)
-    param_id1 = graph_loss.param_id1
-    param_id2 = graph_loss.param_id2
-    reverse_loss = ReverseLossLayer()
-    func = NeuralODE(device)
-    optimizer = optim.Adam(func.parameters(), lr=1e-3)
-    GV1_origin = GV1.clone().to(device)
-    GV2_origin = GV2.clone().to(device)
-    GV1_device = GV1.to(device)
-    GV2_device = GV2.to(device)
-    GE1_device = GE1.to(device)
-    GE2_device = GE2.to(device)
-    n1, n2 = GV1.shape[0], GV2.shape[0]
+    param_id2, GV1_deformed, V1_pushed)."""
+    d = NeuralDeform2(mesh1, mesh2, rigidity, device)
+    n1, n2 = mesh1[4].shape[0], mesh2[4].shape[0]
     losses = []
     for it in range(niter):
-        optimizer.zero_grad()
-        GV1_deformed = func.forward(GV1_device)
-        GV2_deformed = func.inverse(GV2_device)
-        loss1_forward = graph_loss(GV1_deformed, GE1_device, GV2_device, GE2_device, 0)
-        loss1_backward = reverse_loss(GV1_deformed, GV2_origin, device)
-        loss2_forward = graph_loss(GV1_device, GE1_device, GV2_deformed, GE2_device, 1)
-        loss2_backward = reverse_loss(GV2_deformed, GV1_origin, device)
-        loss = loss1_forward + loss1_backward + loss2_forward + loss2_backward
-        loss.backward()
-        optimizer.step()
-        row = (loss1_forward.item(), loss1_backward.item(), loss2_forward.item(), loss2_backward.item())
+        row = d.step().tolist()
         losses.append(row)
         if log is not None:
             log("iter=%d, loss1_forward=%.6f loss1_backward=%.6f loss2_forward=%.6f loss2_backward=%.6f"
                 % (it, math.sqrt(row[0] / n1), math.sqrt(row[1] / n2), math.sqrt(row[2] / n2),
                    math.sqrt(row[3] / n1)))
-    with torch.no_grad():
-        GV1_deformed = func.forward(GV1_device)
-        # the source mesh through the flow, in the source template's frame
-        V1_copy = V1.clone()
-        pyDeform.NormalizeByTemplate(V1_copy, param_id1.tolist())
-        V1_origin = V1_copy.clone()
-        V1_pushed = func.forward(V1_copy.to(device))
-    src_to_src = torch.from_numpy(np.arange(V1_origin.shape[0]).astype("int32"))
-    pyDeform.SolveLinear(V1_origin, F1, E1, src_to_src, V1_pushed, 1, 1)
-    pyDeform.DenormalizeByTemplate(V1_origin, param_id2.tolist())
-    return dict(V=V1_origin, F=F1, losses=np.array(losses, np.float64).reshape(-1, 4), func=func,
-                optimizer=optimizer, param_id1=param_id1, param_id2=param_id2, GV1_deformed=GV1_deformed,
-                V1_pushed=V1_pushed)
+    V = d.finalize()
+    return dict(V=V, F=mesh1[1], losses=np.array(losses, np.float64).reshape(-1, 4), func=d.func,
+                optimizer=d.optimizer, param_id1=d.param_id1, param_id2=d.param_id2,
+                GV1_deformed=d.GV1_deformed, V1_pushed=d.V1_pushed)
 
 
 def main(argv=None):
