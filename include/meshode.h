@@ -281,6 +281,20 @@ int mo_linear_estimation(double* V, int64_t nv, const int32_t* F, int64_t nf, co
                          const int32_t* ref, const double* GV, int64_t ngv, double rigidity, int32_t* iterations,
                          void* stream);
 
+/* Deformer::DeformWithRot (deformer.cc:94-171; rigid_rot_deform.cc:55-56):
+ * positions and per-vertex angle-axis rotations, DistanceLoss per vertex and
+ * EdgeLossWithRot per face half-edge (a, b) = (F[f][j], F[f][j+1]),
+ *   lambda (rotate(w_a, p_a - p_b) - (V_a - V_b)),  w_a - w_b
+ * (edgeloss.h:64-98; half-edges with a == b are skipped), by the LM loop of
+ * mo_deform_batch (Ceres' defaults, max 100 iterations).  V [nv,3] FP64
+ * device array in the template's normalized frame: the rest shape in, the
+ * solution out.  rot [nv,3] FP64 device array or null: the start rotations
+ * (zeros when null, as the reference) in, the final ones out.  F [nf,3]
+ * device.  opt.block_threads above 256 is clamped to 256.  Synchronizes
+ * `stream`. */
+int mo_deform_rot(int32_t param_id, double* V, int64_t nv, const int32_t* F, int64_t nf, double lambda,
+                  const mo_lm_options* opt, double* rot, mo_lm_stats* stats, void* stream);
+
 /* LinearEstimationWithRot (linear.cc:114-230; linear.h:14-15; pyDeform
  * SolveLinear with_rot = 1, linear_layer.cc:76-78): V [nv,3] FP64 rest
  * positions in, solution out; F [nf,3] int32; TV [nv,3] FP64 target

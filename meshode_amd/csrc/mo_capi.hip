@@ -7,6 +7,7 @@
 #include "mo_grid.h"
 #include "mo_layers.h"
 #include "mo_lm.h"
+#include "mo_lm_rot.h"
 #include "mo_topo.h"
 
 #include <algorithm>
@@ -915,5 +916,127 @@ MO_EXPORT int mo_deform_subdivision(int32_t param_id, const double* V_rest, doub
   }
   if (stats) *stats = st;
   if (solves) *solves = n;
+  return MO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Deformer::DeformWithRot (deformer.cc:94-171).
+
+namespace {
+
+__global__ void k_rot_init(const double* __restrict__ V, const double* __restrict__ rot, int nv,
+                           float4* __restrict__ x0, float4* __restrict__ u, float4* __restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  x0[i] = make_float4((float)V[3 * i], (float)V[3 * i + 1], (float)V[3 * i + 2], 0.f);
+  u[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  w[i] = rot ? make_float4((float)rot[3 * i], (float)rot[3 * i + 1], (float)rot[3 * i + 2], 0.f)
+             : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_rot_finish(double* __restrict__ V, double* __restrict__ rot, int nv, const float4* __restrict__ u,
+                             const float4* __restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const float4 d = u[i];
+  V[3 * i] += (double)d.x;
+  V[3 * i + 1] += (double)d.y;
+  V[3 * i + 2] += (double)d.z;
+  if (rot) {
+    const float4 r = w[i];
+    rot[3 * i] = r.x;
+    rot[3 * i + 1] = r.y;
+    rot[3 * i + 2] = r.z;
+  }
+}
+
+}  // namespace
+
+MO_EXPORT int mo_deform_rot(int32_t param_id, double* V, int64_t nv, const int32_t* F, int64_t nf, double lambda,
+                            const mo_lm_options* opt, double* rot, mo_lm_stats* stats, void* stream) {
+  Template* t = get_template(param_id);
+  MO_CHECK(t, MO_ERR_ARG, "mo_deform_rot: unknown param_id %d", param_id);
+  MO_CHECK(V && nv > 0 && nv < (1LL << 24) && nf >= 0 && nf < (1LL << 28) && (F || nf == 0), MO_ERR_ARG,
+           "mo_deform_rot: bad arguments (nv %lld, nf %lld)", (long long)nv, (long long)nf);
+  mo_lm_options o;
+  if (opt) o = *opt; else mo_lm_default_options(&o);
+  MO_CHECK(o.block_threads >= 64 && o.block_threads <= 1024 && o.block_threads % 64 == 0, MO_ERR_ARG,
+           "block_threads must be a multiple of 64 in [64, 1024]");
+  hipStream_t s = as_stream(stream);
+  // half-edges in residual order (face, j), skipping a == b (Ceres rejects
+  // a residual block that names one parameter block twice)
+  std::vector<int32_t> hF((size_t)nf * 3);
+  if (nf) MO_HIP(hipMemcpyAsync(hF.data(), F, hF.size() * 4, hipMemcpyDeviceToHost, s));
+  MO_HIP(hipStreamSynchronize(s));
+  for (size_t k = 0; k < hF.size(); ++k)
+    MO_CHECK(hF[k] >= 0 && hF[k] < nv, MO_ERR_ARG, "mo_deform_rot: face index %d out of range", hF[k]);
+  std::vector<int2> hab;
+  hab.reserve(hF.size());
+  for (int64_t f = 0; f < nf; ++f)
+    for (int j = 0; j < 3; ++j) {
+      const int a = hF[3 * f + j], b = hF[3 * f + (j + 1) % 3];
+      if (a != b) hab.push_back(make_int2(a, b));
+    }
+  const int n = (int)nv, nh = (int)hab.size();
+  std::vector<int> nout(n, 0), nin(n, 0);
+  for (const int2& e : hab) { nout[e.x]++; nin[e.y]++; }
+  std::vector<int4> vadj(n);
+  std::vector<int> vlist(2 * (size_t)nh);
+  {
+    int pos = 0;
+    for (int i = 0; i < n; ++i) {
+      vadj[i] = make_int4(pos, nout[i], nin[i], 0);
+      pos += nout[i] + nin[i];
+    }
+    std::vector<int> fo(n), fi(n);
+    for (int i = 0; i < n; ++i) { fo[i] = vadj[i].x; fi[i] = vadj[i].x + vadj[i].y; }
+    for (int h = 0; h < nh; ++h) {
+      vlist[fo[hab[h].x]++] = h;
+      vlist[fi[hab[h].y]++] = h;
+    }
+  }
+  DevPtr d_x0, d_u, d_w, d_adj, d_list, d_hab, d_vw, d_hw, d_pair, d_res;
+  if (int e_ = d_x0.alloc((size_t)n * 16)) return e_;
+  if (int e_ = d_u.alloc((size_t)n * 16)) return e_;
+  if (int e_ = d_w.alloc((size_t)n * 16)) return e_;
+  if (int e_ = d_adj.alloc((size_t)n * 16)) return e_;
+  if (int e_ = d_list.alloc(vlist.size() * 4)) return e_;
+  if (int e_ = d_hab.alloc((size_t)nh * 8)) return e_;
+  if (int e_ = d_vw.alloc((size_t)n * 16 * kRotVSlots)) return e_;
+  if (int e_ = d_hw.alloc((size_t)nh * 16 * kRotHSlots)) return e_;
+  if (int e_ = d_pair.alloc(sizeof(RotPairDesc))) return e_;
+  if (int e_ = d_res.alloc(sizeof(LMResultDev))) return e_;
+  MO_HIP(hipMemcpyAsync(d_adj.p, vadj.data(), (size_t)n * 16, hipMemcpyHostToDevice, s));
+  if (!vlist.empty()) MO_HIP(hipMemcpyAsync(d_list.p, vlist.data(), vlist.size() * 4, hipMemcpyHostToDevice, s));
+  if (nh) MO_HIP(hipMemcpyAsync(d_hab.p, hab.data(), (size_t)nh * 8, hipMemcpyHostToDevice, s));
+  RotPairDesc pd{t->grid32.as<float>(), t->N, n, nh, 0, 0, 0, (float)lambda};
+  MO_HIP(hipMemcpyAsync(d_pair.p, &pd, sizeof(pd), hipMemcpyHostToDevice, s));
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_rot_init, dim3(blocks), dim3(256), 0, s, V, rot, n, d_x0.as<float4>(), d_u.as<float4>(),
+                     d_w.as<float4>());
+  MO_HIP(hipGetLastError());
+  RotArrays A{d_x0.as<float4>(), d_u.as<float4>(), d_w.as<float4>(), d_adj.as<int4>(), d_list.as<int>(),
+              d_hab.as<int2>(), d_vw.as<float4>(), d_hw.as<float4>()};
+  int rc = lm_rot_launch(d_pair.as<RotPairDesc>(), 1, A, to_dev_options(o), d_res.as<LMResultDev>(), n,
+                         nh ? nh : 1, std::min(o.block_threads, 256), s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_rot_finish, dim3(blocks), dim3(256), 0, s, V, rot, n, d_u.as<float4>(), d_w.as<float4>());
+  MO_HIP(hipGetLastError());
+  LMResultDev r;
+  MO_HIP(hipMemcpyAsync(&r, d_res.p, sizeof(r), hipMemcpyDeviceToHost, s));
+  MO_HIP(hipStreamSynchronize(s));
+  if (stats) {
+    stats->initial_cost = r.initial_cost;
+    stats->final_cost = r.final_cost;
+    stats->vertex_cost = r.vertex_cost;
+    stats->rigidity_cost = r.rigidity_cost;
+    stats->iterations = r.iterations;
+    stats->successful_steps = r.successful_steps;
+    stats->unsuccessful_steps = r.unsuccessful_steps;
+    stats->termination = r.termination;
+    stats->pcg_iterations = r.pcg_iterations;
+    stats->clock_start = r.clock_start;
+    stats->clock_end = r.clock_end;
+  }
   return MO_OK;
 }

@@ -1,7 +1,8 @@
 """Levenberg-Marquardt deformation on MI355X (Deformer, src/lib/deformer.h:11-30).
 
 ``deform_batch`` runs Deformer::Deform (faces) / Deformer::DeformGraph
-(edges) for many pairs in one persistent launch; ``rigid_deform_batch`` runs
+(edges) for many pairs in one persistent launch; ``deform_rot`` runs
+Deformer::DeformWithRot (positions and per-vertex rotations); ``rigid_deform_batch`` runs
 the whole rigid_deform pipeline (rigid_deform.cc:49-61: normalize target,
 build its distance grid, move the source into the target frame, solve,
 return the result in the target's original frame) device-resident.
@@ -69,6 +70,27 @@ def deform_batch(param_ids, Vs, idxs, kinds, lambdas, options=None):
               _arr(C.c_double, [float(l) for l in lambdas]),
               C.byref(opt), stats, _stream())
     return [s.as_dict() for s in stats]
+
+
+def deform_rot(param_id, V, F, lam=1.0, W=None, options=None):
+    """Deformer::DeformWithRot (deformer.cc:94-171) on the GPU.
+
+    V: FP64 device tensor [nv,3] in the template's normalized frame (rest
+    shape in, solution out).  F: int32 faces.  W: optional FP64 device
+    tensor [nv,3] of start rotations (zeros otherwise), updated in place.
+    Returns (stats dict, rotations [nv,3] FP64 device tensor)."""
+    if V.dtype != torch.float64 or not V.is_cuda or not V.is_contiguous() or V.dim() != 2 or V.shape[1] != 3:
+        raise TypeError("deform_rot expects a contiguous [n,3] float64 device tensor")
+    Fd = _as_dev(F, torch.int32).reshape(-1, 3)
+    if W is None:
+        W = torch.zeros_like(V)
+    elif W.dtype != torch.float64 or not W.is_cuda or not W.is_contiguous() or W.shape != V.shape:
+        raise TypeError("deform_rot: W must be a contiguous float64 device tensor shaped like V")
+    st = _lib.LMStats()
+    opt = options or default_options()
+    _lib.call("mo_deform_rot", int(param_id), C.c_void_p(V.data_ptr()), V.shape[0], C.c_void_p(Fd.data_ptr()),
+              Fd.shape[0], float(lam), C.byref(opt), C.c_void_p(W.data_ptr()), C.byref(st), _stream())
+    return st.as_dict(), W
 
 
 class PairBatch:

@@ -39,6 +39,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <vector>
 
 #define OR_API extern "C" __attribute__((visibility("default")))
@@ -908,6 +909,394 @@ OR_API double mo_oracle_lm_cost_ex(const double* grid, int N, const double* x, c
 OR_API double mo_oracle_lm_cost(const double* grid, int N, const double* x, const double* x0, int nv,
                                 const int* E, int ne, double lambda, double* vcost, double* ecost) {
   return mo_oracle_lm_cost_ex(grid, N, x, x0, nv, E, ne, nullptr, lambda, vcost, ecost);
+}
+
+// ---------------------------------------------------------------------------
+// Deformer::DeformWithRot (deformer.cc:94-171): unknowns x = positions [nv,3]
+// and angle-axis rotations w [nv,3] (zero at the start, deformer.cc:114);
+// residual blocks DistanceLoss(p_i) and, per face half-edge (a, b) =
+// (F[f][j], F[f][j+1]) with a != b, EdgeLossWithRot (edgeloss.h:64-90):
+//   r[0:3] = lambda (AngleAxisRotatePoint(w_a, p_a - p_b) - v),  v = x0_a - x0_b
+//   r[3:6] = w_a - w_b.
+// AngleAxisRotatePoint restates Ceres' published rotation.h (Ceres itself is
+// absent): Rodrigues' formula when |w|^2 > DBL_EPSILON, else p + w x p.  The
+// Jacobians are taken by forward-mode differentiation (JetK) of the same
+// expressions, as Ceres' AutoDiffCostFunction does.  The LM loop and the
+// exact sparse solve are those of mo_oracle_lm_solve_ex with 6 unknowns per
+// vertex (p then w).
+namespace {
+
+template <int K>
+struct JetK {
+  double a;
+  double v[K];
+  JetK() : a(0) { for (int k = 0; k < K; ++k) v[k] = 0; }
+  explicit JetK(double s) : a(s) { for (int k = 0; k < K; ++k) v[k] = 0; }
+  JetK(double s, int i) : a(s) { for (int k = 0; k < K; ++k) v[k] = 0; v[i] = 1; }
+};
+template <int K> inline JetK<K> operator+(const JetK<K>& f, const JetK<K>& g) {
+  JetK<K> r;
+  r.a = f.a + g.a;
+  for (int k = 0; k < K; ++k) r.v[k] = f.v[k] + g.v[k];
+  return r;
+}
+template <int K> inline JetK<K> operator-(const JetK<K>& f, const JetK<K>& g) {
+  JetK<K> r; r.a = f.a - g.a; for (int k = 0; k < K; ++k) r.v[k] = f.v[k] - g.v[k];
+  return r;
+}
+template <int K> inline JetK<K> operator*(const JetK<K>& f, const JetK<K>& g) {
+  JetK<K> r; r.a = f.a * g.a; for (int k = 0; k < K; ++k) r.v[k] = f.a * g.v[k] + f.v[k] * g.a;
+  return r;
+}
+template <int K> inline JetK<K> operator/(const JetK<K>& f, const JetK<K>& g) {
+  JetK<K> r; r.a = f.a / g.a;
+  const double ig = 1.0 / g.a;
+  for (int k = 0; k < K; ++k) r.v[k] = (f.v[k] - r.a * g.v[k]) * ig;
+  return r;
+}
+template <int K> inline JetK<K> sqrt(const JetK<K>& f) {
+  JetK<K> r; r.a = std::sqrt(f.a); const double h = 1.0 / (2.0 * r.a);
+  for (int k = 0; k < K; ++k) r.v[k] = f.v[k] * h;
+  return r;
+}
+template <int K> inline JetK<K> cos(const JetK<K>& f) {
+  JetK<K> r; r.a = std::cos(f.a); const double d = -std::sin(f.a);
+  for (int k = 0; k < K; ++k) r.v[k] = f.v[k] * d;
+  return r;
+}
+template <int K> inline JetK<K> sin(const JetK<K>& f) {
+  JetK<K> r; r.a = std::sin(f.a); const double d = std::cos(f.a);
+  for (int k = 0; k < K; ++k) r.v[k] = f.v[k] * d;
+  return r;
+}
+template <int K> inline double scalar_of(const JetK<K>& j) { return j.a; }
+
+template <class T>
+void angle_axis_rotate_point(const T w[3], const T pt[3], T out[3]) {
+  const T theta2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  if (scalar_of(theta2) > std::numeric_limits<double>::epsilon()) {
+    using std::sqrt; using std::cos; using std::sin;
+    const T theta = sqrt(theta2);
+    const T c = cos(theta), s = sin(theta);
+    const T one(1.0);
+    const T ti = one / theta;
+    const T u[3] = {w[0] * ti, w[1] * ti, w[2] * ti};
+    const T uxp[3] = {u[1] * pt[2] - u[2] * pt[1], u[2] * pt[0] - u[0] * pt[2], u[0] * pt[1] - u[1] * pt[0]};
+    const T tmp = (u[0] * pt[0] + u[1] * pt[1] + u[2] * pt[2]) * (one - c);
+    for (int k = 0; k < 3; ++k) out[k] = pt[k] * c + uxp[k] * s + u[k] * tmp;
+  } else {
+    const T wxp[3] = {w[1] * pt[2] - w[2] * pt[1], w[2] * pt[0] - w[0] * pt[2], w[0] * pt[1] - w[1] * pt[0]};
+    for (int k = 0; k < 3; ++k) out[k] = pt[k] + wxp[k];
+  }
+}
+
+// EdgeLossWithRot::operator() on (p1, p2, rot1, rot2)
+template <class T>
+void edge_rot_residual(const T* p1, const T* p2, const T* r1, const T* r2, const double* v, double lambda,
+                       T* res) {
+  T p[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
+  T q[3];
+  angle_axis_rotate_point(r1, p, q);
+  for (int k = 0; k < 3; ++k) res[k] = (q[k] - T(v[k])) * T(lambda);
+  for (int k = 0; k < 3; ++k) res[3 + k] = (r1[k] - r2[k]) * T(1.0);
+}
+
+struct RotProblem {
+  const double* grid; int N; int nv;
+  std::vector<int> ha, hb;        // half-edges (a != b) in residual order
+  std::vector<double> rest;       // v per half-edge
+  double lambda;
+};
+
+// x: [6 nv] (p_i at 6i..6i+2, w_i at 6i+3..6i+5)
+double rot_cost(const RotProblem& P, const double* x, double* vcost, double* ecost) {
+  double cv = 0.0, ce = 0.0;
+  for (int u = 0; u < P.nv; ++u) {
+    const double* p = x + 6 * u;
+    double d = distance_double_t<double>(P.grid, P.N, p, p[0], p[1], p[2]);
+    cv += 0.5 * d * d;
+  }
+  for (size_t h = 0; h < P.ha.size(); ++h) {
+    const int a = P.ha[h], b = P.hb[h];
+    double r[6];
+    edge_rot_residual<double>(x + 6 * a, x + 6 * b, x + 6 * a + 3, x + 6 * b + 3, &P.rest[3 * h], P.lambda, r);
+    double s2 = 0.0;
+    for (int k = 0; k < 6; ++k) s2 += r[k] * r[k];
+    ce += 0.5 * s2;
+  }
+  if (vcost) *vcost = cv;
+  if (ecost) *ecost = ce;
+  return cv + ce;
+}
+
+// Per-block Jacobians at x: distance value d and gradient g (3 per vertex);
+// edge residuals (6 per half-edge) and their 6x12 Jacobians over
+// (p_a, p_b, w_a, w_b).
+void rot_jacobian(const RotProblem& P, const double* x, std::vector<double>& d, std::vector<double>& g,
+                  std::vector<double>& er, std::vector<double>& ej) {
+  for (int u = 0; u < P.nv; ++u) {
+    const double* p = x + 6 * u;
+    Jet<double> pj[3] = {Jet<double>(p[0], 0), Jet<double>(p[1], 1), Jet<double>(p[2], 2)};
+    Jet<double> r = distance_double_t<Jet<double>>(P.grid, P.N, pj, p[0], p[1], p[2]);
+    d[u] = r.a;
+    for (int k = 0; k < 3; ++k) g[3 * u + k] = r.v[k];
+  }
+  for (size_t h = 0; h < P.ha.size(); ++h) {
+    const int a = P.ha[h], b = P.hb[h];
+    JetK<12> p1[3], p2[3], r1[3], r2[3], res[6];
+    for (int k = 0; k < 3; ++k) {
+      p1[k] = JetK<12>(x[6 * a + k], k);
+      p2[k] = JetK<12>(x[6 * b + k], 3 + k);
+      r1[k] = JetK<12>(x[6 * a + 3 + k], 6 + k);
+      r2[k] = JetK<12>(x[6 * b + 3 + k], 9 + k);
+    }
+    edge_rot_residual<JetK<12>>(p1, p2, r1, r2, &P.rest[3 * h], P.lambda, res);
+    for (int r = 0; r < 6; ++r) {
+      er[6 * h + r] = res[r].a;
+      for (int c = 0; c < 12; ++c) ej[72 * h + 12 * r + c] = res[r].v[c];
+    }
+  }
+}
+
+// unknown index of column c (0..11) of a half-edge block: (vertex, local 0..5)
+inline void rot_col(int c, int a, int b, int* v, int* l) {
+  if (c < 3) { *v = a; *l = c; }
+  else if (c < 6) { *v = b; *l = c - 3; }
+  else if (c < 9) { *v = a; *l = 3 + (c - 6); }
+  else { *v = b; *l = 3 + (c - 9); }
+}
+
+}  // namespace
+
+// x: [nv,3] FP64 positions in the target's normalized frame (start and rest
+// shape, updated in place); w: [nv,3] rotations (in/out; zeros as in the
+// reference).  F: [nf,3].
+OR_API int mo_oracle_lm_rot_solve(const double* grid, int N, double* xp, double* w, int nv, const int* F, int nf,
+                                  double lambda, const OracleLMOptions* opt, OracleLMStats* st) {
+  RotProblem P;
+  P.grid = grid; P.N = N; P.nv = nv; P.lambda = lambda;
+  for (int f = 0; f < nf; ++f)
+    for (int j = 0; j < 3; ++j) {
+      const int a = F[3 * f + j], b = F[3 * f + (j + 1) % 3];
+      if (a == b) continue;
+      P.ha.push_back(a);
+      P.hb.push_back(b);
+      for (int k = 0; k < 3; ++k) P.rest.push_back(xp[3 * a + k] - xp[3 * b + k]);
+    }
+  const int nh = (int)P.ha.size();
+  const int n = 6 * nv;
+  std::vector<double> x(n);
+  for (int u = 0; u < nv; ++u)
+    for (int k = 0; k < 3; ++k) { x[6 * u + k] = xp[3 * u + k]; x[6 * u + 3 + k] = w[3 * u + k]; }
+  // vertex adjacency for the ordering and the envelope
+  std::vector<std::vector<int>> adj(nv);
+  for (int h = 0; h < nh; ++h) { adj[P.ha[h]].push_back(P.hb[h]); adj[P.hb[h]].push_back(P.ha[h]); }
+  std::vector<int> rp(nv + 1, 0), ci;
+  for (int u = 0; u < nv; ++u) {
+    std::sort(adj[u].begin(), adj[u].end());
+    adj[u].erase(std::unique(adj[u].begin(), adj[u].end()), adj[u].end());
+    for (int v : adj[u]) ci.push_back(v);
+    rp[u + 1] = (int)ci.size();
+  }
+  std::vector<int> perm = rcm_order(nv, rp, ci);
+  Skyline S;
+  S.n = n;
+  S.first.resize(n);
+  S.start.resize(n + 1);
+  for (int u = 0; u < nv; ++u) {
+    int pu = perm[u], mn = pu;
+    for (int k = rp[u]; k < rp[u + 1]; ++k) mn = std::min(mn, perm[ci[k]]);
+    for (int k = 0; k < 6; ++k) S.first[6 * pu + k] = 6 * mn;
+  }
+  size_t tot = 0;
+  for (int i = 0; i < n; ++i) { S.start[i] = tot; tot += (size_t)(i - S.first[i] + 1); }
+  S.start[n] = tot;
+  S.vals.assign(tot, 0.0);
+
+  std::vector<double> d(nv), g(3 * (size_t)nv), er(6 * (size_t)nh), ej(72 * (size_t)nh);
+  std::vector<double> grad(n), cn(n), scale(n), diag(n), rhs(n), delta(n), cand(n);
+  auto gradient_and_norms = [&]() {
+    std::fill(grad.begin(), grad.end(), 0.0);
+    std::fill(cn.begin(), cn.end(), 0.0);
+    for (int u = 0; u < nv; ++u)
+      for (int k = 0; k < 3; ++k) {
+        grad[6 * u + k] += d[u] * g[3 * u + k];
+        cn[6 * u + k] += g[3 * u + k] * g[3 * u + k];
+      }
+    for (int h = 0; h < nh; ++h)
+      for (int c = 0; c < 12; ++c) {
+        int v, l;
+        rot_col(c, P.ha[h], P.hb[h], &v, &l);
+        double sg = 0.0, sn = 0.0;
+        for (int r = 0; r < 6; ++r) {
+          const double J = ej[72 * h + 12 * r + c];
+          sg += J * er[6 * h + r];
+          sn += J * J;
+        }
+        grad[6 * v + l] += sg;
+        cn[6 * v + l] += sn;
+      }
+  };
+  double cost = rot_cost(P, x.data(), nullptr, nullptr);
+  st->initial_cost = cost;
+  rot_jacobian(P, x.data(), d, g, er, ej);
+  gradient_and_norms();
+  for (int c = 0; c < n; ++c) scale[c] = 1.0 / (1.0 + std::sqrt(cn[c]));
+
+  double radius = opt->initial_trust_region_radius, decrease = 2.0;
+  bool reuse = false, last_success = true;
+  int iter = 0, invalid = 0, nsucc = 0, nfail = 0, term = 1;
+  auto gmax = [&]() { double m = 0; for (int c = 0; c < n; ++c) m = std::max(m, std::fabs(grad[c])); return m; };
+  auto idx = [&](int v, int l) { return 6 * perm[v] + l; };
+  while (true) {
+    if (iter >= opt->max_num_iterations) { term = 1; break; }
+    if (last_success && gmax() <= opt->gradient_tolerance) { term = 0; break; }
+    if (radius <= opt->min_trust_region_radius) { term = 0; break; }
+    ++iter;
+    if (!reuse)
+      for (int c = 0; c < n; ++c)
+        diag[c] = std::min(std::max(scale[c] * scale[c] * cn[c], opt->min_lm_diagonal), opt->max_lm_diagonal);
+    reuse = true;
+    // assemble S J^T J S + D^2 / radius (lower triangle, permuted order)
+    std::fill(S.vals.begin(), S.vals.end(), 0.0);
+    auto add = [&](int v1, int l1, int v2, int l2, double val) {
+      int r = idx(v1, l1), c = idx(v2, l2);
+      const double sv = scale[6 * v1 + l1] * scale[6 * v2 + l2] * val;
+      if (c > r) return;            // each unordered pair is added from both orders
+      S.at(r, c) += sv;
+    };
+    for (int u = 0; u < nv; ++u)
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) add(u, k, u, l, g[3 * u + k] * g[3 * u + l]);
+    for (int h = 0; h < nh; ++h)
+      for (int c1 = 0; c1 < 12; ++c1) {
+        int v1, l1;
+        rot_col(c1, P.ha[h], P.hb[h], &v1, &l1);
+        for (int c2 = 0; c2 < 12; ++c2) {
+          int v2, l2;
+          rot_col(c2, P.ha[h], P.hb[h], &v2, &l2);
+          double sum = 0.0;
+          for (int r = 0; r < 6; ++r) sum += ej[72 * h + 12 * r + c1] * ej[72 * h + 12 * r + c2];
+          add(v1, l1, v2, l2, sum);
+        }
+      }
+    for (int u = 0; u < nv; ++u)
+      for (int l = 0; l < 6; ++l) S.at(idx(u, l), idx(u, l)) += diag[6 * u + l] / radius;
+    bool ok = S.factor();
+    double mcc = -1.0;
+    if (ok) {
+      for (int u = 0; u < nv; ++u)
+        for (int l = 0; l < 6; ++l) rhs[idx(u, l)] = scale[6 * u + l] * grad[6 * u + l];
+      S.solve(rhs);
+      for (int u = 0; u < nv; ++u)
+        for (int l = 0; l < 6; ++l) delta[6 * u + l] = scale[6 * u + l] * (-rhs[idx(u, l)]);
+      for (int c = 0; c < n; ++c) if (!std::isfinite(delta[c])) ok = false;
+    }
+    if (ok) {
+      double acc = 0.0;
+      for (int u = 0; u < nv; ++u) {
+        double m = 0.0;
+        for (int k = 0; k < 3; ++k) m += g[3 * u + k] * delta[6 * u + k];
+        acc += m * (d[u] + 0.5 * m);
+      }
+      for (int h = 0; h < nh; ++h)
+        for (int r = 0; r < 6; ++r) {
+          double m = 0.0;
+          for (int c = 0; c < 12; ++c) {
+            int v, l;
+            rot_col(c, P.ha[h], P.hb[h], &v, &l);
+            m += ej[72 * h + 12 * r + c] * delta[6 * v + l];
+          }
+          acc += m * (er[6 * h + r] + 0.5 * m);
+        }
+      mcc = -acc;
+    }
+    if (!ok || !(mcc > 0.0)) {
+      ++invalid;
+      if (invalid >= opt->max_consecutive_invalid_steps) { term = 2; break; }
+      radius /= decrease; decrease *= 2.0;
+      last_success = false; ++nfail;
+      continue;
+    }
+    invalid = 0;
+    for (int c = 0; c < n; ++c) cand[c] = x[c] + delta[c];
+    double ccost = rot_cost(P, cand.data(), nullptr, nullptr);
+    double sn = 0.0, xn = 0.0;
+    for (int c = 0; c < n; ++c) { double t = x[c] - cand[c]; sn += t * t; xn += x[c] * x[c]; }
+    sn = std::sqrt(sn); xn = std::sqrt(xn);
+    double rho = (cost - ccost) / mcc;
+    if (sn <= opt->parameter_tolerance * (xn + opt->parameter_tolerance)) { term = 0; break; }
+    if (std::fabs(cost - ccost) <= opt->function_tolerance * cost) { term = 0; break; }
+    if (rho > opt->min_relative_decrease) {
+      x = cand;
+      cost = ccost;
+      rot_jacobian(P, x.data(), d, g, er, ej);
+      gradient_and_norms();
+      radius = radius / std::max(1.0 / 3.0, 1.0 - std::pow(2.0 * rho - 1.0, 3));
+      radius = std::min(opt->max_trust_region_radius, radius);
+      decrease = 2.0; reuse = false; last_success = true; ++nsucc;
+    } else {
+      radius /= decrease; decrease *= 2.0; reuse = true; last_success = false; ++nfail;
+    }
+  }
+  double vc, ec;
+  st->final_cost = rot_cost(P, x.data(), &vc, &ec);
+  st->vertex_cost = vc; st->rigidity_cost = ec;
+  st->iterations = iter; st->successful_steps = nsucc; st->unsuccessful_steps = nfail;
+  st->termination = term;
+  for (int u = 0; u < nv; ++u)
+    for (int k = 0; k < 3; ++k) { xp[3 * u + k] = x[6 * u + k]; w[3 * u + k] = x[6 * u + 3 + k]; }
+  return 0;
+}
+
+// J^T f of DeformWithRot at (x, w) with rest offsets from x0: grad [nv,6]
+// (d/dp then d/dw per vertex), from the JetK Jacobians.
+OR_API void mo_oracle_lm_rot_gradient(const double* grid, int N, const double* xp, const double* w,
+                                      const double* x0, int nv, const int* F, int nf, double lambda, double* grad) {
+  RotProblem P;
+  P.grid = grid; P.N = N; P.nv = nv; P.lambda = lambda;
+  for (int f = 0; f < nf; ++f)
+    for (int j = 0; j < 3; ++j) {
+      const int a = F[3 * f + j], b = F[3 * f + (j + 1) % 3];
+      if (a == b) continue;
+      P.ha.push_back(a);
+      P.hb.push_back(b);
+      for (int k = 0; k < 3; ++k) P.rest.push_back(x0[3 * a + k] - x0[3 * b + k]);
+    }
+  const int nh = (int)P.ha.size();
+  std::vector<double> x(6 * (size_t)nv);
+  for (int u = 0; u < nv; ++u)
+    for (int k = 0; k < 3; ++k) { x[6 * u + k] = xp[3 * u + k]; x[6 * u + 3 + k] = w[3 * u + k]; }
+  std::vector<double> d(nv), g(3 * (size_t)nv), er(6 * (size_t)nh), ej(72 * (size_t)nh);
+  rot_jacobian(P, x.data(), d, g, er, ej);
+  for (int c = 0; c < 6 * nv; ++c) grad[c] = 0.0;
+  for (int u = 0; u < nv; ++u)
+    for (int k = 0; k < 3; ++k) grad[6 * u + k] += d[u] * g[3 * u + k];
+  for (int h = 0; h < nh; ++h)
+    for (int c = 0; c < 12; ++c) {
+      int v, l;
+      rot_col(c, P.ha[h], P.hb[h], &v, &l);
+      for (int r = 0; r < 6; ++r) grad[6 * v + l] += ej[72 * h + 12 * r + c] * er[6 * h + r];
+    }
+}
+
+// DeformWithRot's final cost terms at (x, w) with rest offsets from x0.
+OR_API double mo_oracle_lm_rot_cost(const double* grid, int N, const double* xp, const double* w, const double* x0,
+                                    int nv, const int* F, int nf, double lambda, double* vcost, double* ecost) {
+  RotProblem P;
+  P.grid = grid; P.N = N; P.nv = nv; P.lambda = lambda;
+  for (int f = 0; f < nf; ++f)
+    for (int j = 0; j < 3; ++j) {
+      const int a = F[3 * f + j], b = F[3 * f + (j + 1) % 3];
+      if (a == b) continue;
+      P.ha.push_back(a);
+      P.hb.push_back(b);
+      for (int k = 0; k < 3; ++k) P.rest.push_back(x0[3 * a + k] - x0[3 * b + k]);
+    }
+  std::vector<double> x(6 * (size_t)nv);
+  for (int u = 0; u < nv; ++u)
+    for (int k = 0; k < 3; ++k) { x[6 * u + k] = xp[3 * u + k]; x[6 * u + 3 + k] = w[3 * u + k]; }
+  return rot_cost(P, x.data(), vcost, ecost);
 }
 
 // ---------------------------------------------------------------------------

@@ -65,6 +65,8 @@ def lib():
         _lib.mo_oracle_lm_cost.restype = C.c_double
         _lib.mo_oracle_lm_cost_ex.restype = C.c_double
         _lib.mo_oracle_lm_solve_ex.restype = C.c_int
+        _lib.mo_oracle_lm_rot_solve.restype = C.c_int
+        _lib.mo_oracle_lm_rot_cost.restype = C.c_double
     return _lib
 
 
@@ -287,6 +289,39 @@ def lm_cost_ex(grid, N, x, x0, E, lam_e=None, lam=1.0):
     tot = lib().mo_oracle_lm_cost_ex(_p(grid), C.c_int(N), _p(x), _p(x0), C.c_int(len(x)), _p(E), C.c_int(len(E)),
                                      _p(le) if le is not None else None, C.c_double(lam), C.byref(vc), C.byref(ec))
     return tot, vc.value, ec.value
+
+
+def lm_rot_solve(grid, N, V, F, lam=1.0, W=None, options: LMOptions | None = None):
+    """Deformer::DeformWithRot (deformer.cc:94-171): positions V [n,3] FP64
+    in the normalized frame (start and rest shape; not modified), rotations
+    W [n,3] (zeros by default, as the reference), faces F.
+    Returns (V_out, W_out, stats dict)."""
+    grid = _f64(grid); x = _f64(V).copy(); F = _i32(F)
+    w = np.zeros_like(x) if W is None else _f64(W).copy()
+    opt = options or default_lm_options()
+    st = LMStats()
+    lib().mo_oracle_lm_rot_solve(_p(grid), C.c_int(N), _p(x), _p(w), C.c_int(len(x)), _p(F), C.c_int(len(F)),
+                                 C.c_double(lam), C.byref(opt), C.byref(st))
+    return x, w, {f: getattr(st, f) for f, _ in LMStats._fields_}
+
+
+def lm_rot_cost(grid, N, x, w, x0, F, lam=1.0):
+    """Vertex / rigidity / total cost of DeformWithRot (deformer.cc:152-170)
+    at positions x and rotations w with rest offsets from x0."""
+    grid = _f64(grid); x = _f64(x); w = _f64(w); x0 = _f64(x0); F = _i32(F)
+    vc = C.c_double(); ec = C.c_double()
+    tot = lib().mo_oracle_lm_rot_cost(_p(grid), C.c_int(N), _p(x), _p(w), _p(x0), C.c_int(len(x)), _p(F),
+                                      C.c_int(len(F)), C.c_double(lam), C.byref(vc), C.byref(ec))
+    return tot, vc.value, ec.value
+
+
+def lm_rot_gradient(grid, N, x, w, x0, F, lam=1.0):
+    """J^T f of DeformWithRot at (x, w), [n,6]: d/dp then d/dw."""
+    grid = _f64(grid); x = _f64(x); w = _f64(w); x0 = _f64(x0); F = _i32(F)
+    g = np.zeros((len(x), 6))
+    lib().mo_oracle_lm_rot_gradient(_p(grid), C.c_int(N), _p(x), _p(w), _p(x0), C.c_int(len(x)), _p(F),
+                                    C.c_int(len(F)), C.c_double(lam), _p(g))
+    return g
 
 
 def adaptive_lambda(V, E, lam=1.0):
