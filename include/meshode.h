@@ -286,7 +286,10 @@ int mo_linear_estimation(double* V, int64_t nv, const int32_t* F, int64_t nf, co
  * EdgeLossWithRot per face half-edge (a, b) = (F[f][j], F[f][j+1]),
  *   lambda (rotate(w_a, p_a - p_b) - (V_a - V_b)),  w_a - w_b
  * (edgeloss.h:64-98; half-edges with a == b are skipped), by the LM loop of
- * mo_deform_batch (Ceres' defaults, max 100 iterations).  V [nv,3] FP64
+ * mo_deform_batch (Ceres' defaults, max 100 iterations; opt = null takes
+ * the defaults with pcg_relative_tolerance 1e-6, tighter than Deform's
+ * 1e-5: with 1e-5 the inexact steps can settle in a neighbouring minimum,
+ * 1e-3 above the exact-step solve's energy).  V [nv,3] FP64
  * device array in the template's normalized frame: the rest shape in, the
  * solution out.  rot [nv,3] FP64 device array or null: the start rotations
  * (zeros when null, as the reference) in, the final ones out.  F [nf,3]

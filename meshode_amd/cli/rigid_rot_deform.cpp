@@ -86,10 +86,8 @@ int main(int argc, char** argv) {
     return die("reference grid");
   if (mo_apply_transform_f64(dV, (int64_t)src.nv(), pid, 0, nullptr) != MO_OK) return die("transform");
 
-  mo_lm_options opt;
-  mo_lm_default_options(&opt);
   mo_lm_stats st;
-  if (mo_deform_rot(pid, dV, (int64_t)src.nv(), dF, (int64_t)src.nf(), lambda, &opt, nullptr, &st, nullptr) !=
+  if (mo_deform_rot(pid, dV, (int64_t)src.nv(), dF, (int64_t)src.nf(), lambda, nullptr, nullptr, &st, nullptr) !=
       MO_OK)
     return die("DeformWithRot");
   std::cout << "LM: " << st.iterations << " iterations (" << st.successful_steps << " successful), "

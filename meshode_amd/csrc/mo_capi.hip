@@ -959,7 +959,14 @@ MO_EXPORT int mo_deform_rot(int32_t param_id, double* V, int64_t nv, const int32
   MO_CHECK(V && nv > 0 && nv < (1LL << 24) && nf >= 0 && nf < (1LL << 28) && (F || nf == 0), MO_ERR_ARG,
            "mo_deform_rot: bad arguments (nv %lld, nf %lld)", (long long)nv, (long long)nf);
   mo_lm_options o;
-  if (opt) o = *opt; else mo_lm_default_options(&o);
+  if (opt) {
+    o = *opt;
+  } else {
+    // the rotation unknowns make the path sensitive to inexact steps: a
+    // tighter inner tolerance than Deform's keeps it on the exact-step path
+    mo_lm_default_options(&o);
+    o.pcg_relative_tolerance = 1e-6f;
+  }
   MO_CHECK(o.block_threads >= 64 && o.block_threads <= 1024 && o.block_threads % 64 == 0, MO_ERR_ARG,
            "block_threads must be a multiple of 64 in [64, 1024]");
   hipStream_t s = as_stream(stream);

@@ -87,9 +87,10 @@ def deform_rot(param_id, V, F, lam=1.0, W=None, options=None):
     elif W.dtype != torch.float64 or not W.is_cuda or not W.is_contiguous() or W.shape != V.shape:
         raise TypeError("deform_rot: W must be a contiguous float64 device tensor shaped like V")
     st = _lib.LMStats()
-    opt = options or default_options()
+    # options None: the library's DeformWithRot defaults (inner PCG tolerance 1e-6)
     _lib.call("mo_deform_rot", int(param_id), C.c_void_p(V.data_ptr()), V.shape[0], C.c_void_p(Fd.data_ptr()),
-              Fd.shape[0], float(lam), C.byref(opt), C.c_void_p(W.data_ptr()), C.byref(st), _stream())
+              Fd.shape[0], float(lam), C.byref(options) if options is not None else None, C.c_void_p(W.data_ptr()),
+              C.byref(st), _stream())
     return st.as_dict(), W
 
 
