@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--block-threads", type=int, default=1024, help="threads per pair of the global-memory kernel")
     ap.add_argument("--kernel-variant", type=int, default=0, help="0: register/LDS kernel when a pair fits; 1: global")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "neural2"],
+    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "neural2", "rot"],
                     help="batch: the headline line; serve / adam / ode: the serving, pyDeform-Adam and "
                          "NeuralODE workloads (their own JSON line)")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of Poisson arrivals")
@@ -107,7 +107,7 @@ def main():
         del a, b
     if args.mode != "batch":
         print(json.dumps({"serve": bench_serve, "adam": bench_adam, "ode": bench_ode,
-                          "neural2": bench_neural2}[args.mode](args)), flush=True)
+                          "neural2": bench_neural2, "rot": bench_rot}[args.mode](args)), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -353,6 +353,85 @@ def bench_ode(args):
             "approx_mlp_tflops": flops * steps / dt / 1e12,
             "config": {"workload": "config 3: velocity MLP 3-256-256-256-3 softplus, RK4x16, adjoint", "points":
                        int(P.shape[0]), "grid": 128}}
+
+
+def bench_rot(args):
+    """rigid_rot_deform's solve (Deformer::DeformWithRot) over a batch: the
+    headline's pairs (args.pairs, args.distinct distinct seeded 12-box unions
+    of args.verts vertices vs jittered targets, 64^3, lambda 1), templates
+    built once outside the timed region; solves/s of one
+    mo_deform_rot_batch call from the rest shapes.  Also the one-pair
+    latency (mo_deform_rot).  The oracle's exact-step LM on pair 0 gives the
+    energy check and, with its grid build, the CPU baseline (one thread)."""
+    import numpy as np
+    import torch
+    from meshode_amd import deform, pyDeform, synth
+    distinct = [synth.box_union_pair(s, n_verts=args.verts) for s in range(min(args.distinct, args.pairs))]
+    pids, rest, faces = [], [], []
+    for sV, sF, tV, tF in distinct:
+        pid = deform.template_f64(tV, tF, args.grid)
+        _, scale, trans = pyDeform.TemplateInfo(pid)
+        pids.append(pid)
+        rest.append(torch.from_numpy((sV - trans) / scale).cuda().contiguous())
+        faces.append(torch.from_numpy(sF).cuda())
+    idx = [i % len(distinct) for i in range(args.pairs)]
+    Vs = [rest[i].clone() for i in idx]
+    # one-pair latency
+    V1 = rest[0].clone()
+    deform.deform_rot(pids[0], V1, faces[0], args.lam)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    V1.copy_(rest[0])
+    st1, W1 = deform.deform_rot(pids[0], V1, faces[0], args.lam)
+    torch.cuda.synchronize()
+    single_ms = 1e3 * (time.perf_counter() - t0)
+    print(f"[bench] one pair: {single_ms:.1f} ms", file=sys.stderr, flush=True)
+    for _ in range(max(args.warmup, 0)):
+        deform.deform_rot_batch([pids[i] for i in idx], Vs, [faces[i] for i in idx], [args.lam] * args.pairs)
+        for V, i in zip(Vs, idx):
+            V.copy_(rest[i])
+    torch.cuda.synchronize()
+    steps = max(args.steps, 1)
+    elapsed = 0.0
+    for k in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stb, _ = deform.deform_rot_batch([pids[i] for i in idx], Vs, [faces[i] for i in idx], [args.lam] * args.pairs)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        elapsed += dt
+        print(f"[bench] step {k + 1}/{steps}: {1e3 * dt:.1f} ms", file=sys.stderr, flush=True)
+        for V, i in zip(Vs, idx):
+            V.copy_(rest[i])
+    value = args.pairs * steps / elapsed
+    out = {"metric": "rigid_rot_deform solves/sec (5k verts, 64^3, energy tol 1e-3)", "value": value,
+           "unit": "solves/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True,
+           "dtype": "f32 (LM vectors), f64 (reductions)", "data": "synthetic",
+           "single_pair_ms": single_ms,
+           "gpu_stats_pair0": {k: st1[k] for k in ("final_cost", "iterations", "pcg_iterations", "termination")},
+           "max_iterations_in_batch": int(max(s["iterations"] for s in stb)),
+           "config": {"workload": "rigid_rot_deform: %d pairs (%d distinct 12-box unions of %d verts vs jittered "
+                                  "targets)" % (args.pairs, len(distinct), args.verts),
+                      "grid": args.grid, "lambda": args.lam}}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        O.build()
+        sV, sF, tV, tF = distinct[0]
+        t0 = time.perf_counter()
+        tn, sc, pos = O.normalize(tV)
+        grid, _, _ = O.build_grid_bvh(tn, tF, args.grid)
+        sno = O.apply_transform(sV, sc, pos)
+        _, _, so = O.lm_rot_solve(grid, args.grid, sno, sF, args.lam)
+        tc = time.perf_counter() - t0
+        g64 = pyDeform.TemplateGrid(pids[0])[0].cpu().numpy().reshape(-1)
+        eg = O.lm_rot_cost(g64, args.grid, V1.cpu().numpy(), W1.cpu().numpy(), rest[0].cpu().numpy(), sF,
+                           args.lam)[0]
+        out["cpu_baseline"] = {"value": 1.0 / tc, "unit": "solves/s", "cores": 1, "kind": "port",
+                               "sample": "one full solve of pair 0 (grid build + exact-step LM, FP64)"}
+        out["energy_check_pair0"] = {"gpu": eg, "oracle": so["final_cost"],
+                                     "rel_diff": abs(eg - so["final_cost"]) / so["final_cost"]}
+    return out
 
 
 def _neural2_meshes():

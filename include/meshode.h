@@ -298,6 +298,15 @@ int mo_linear_estimation(double* V, int64_t nv, const int32_t* F, int64_t nf, co
 int mo_deform_rot(int32_t param_id, double* V, int64_t nv, const int32_t* F, int64_t nf, double lambda,
                   const mo_lm_options* opt, double* rot, mo_lm_stats* stats, void* stream);
 
+/* mo_deform_rot for a batch of independent problems in one launch (one
+ * workgroup each): pair p deforms V[p] ([nv[p],3] FP64 device, normalized
+ * frame of template param_ids[p]) with faces F[p] ([nf[p],3]) and weight
+ * lambda[p]; rot (nullable array, entries nullable) as in mo_deform_rot;
+ * stats[p] per pair.  Each pair's result does not depend on the others. */
+int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, double* const* V, const int64_t* nv,
+                        const int32_t* const* F, const int64_t* nf, const double* lambda, const mo_lm_options* opt,
+                        double* const* rot, mo_lm_stats* stats, void* stream);
+
 /* LinearEstimationWithRot (linear.cc:114-230; linear.h:14-15; pyDeform
  * SolveLinear with_rot = 1, linear_layer.cc:76-78): V [nv,3] FP64 rest
  * positions in, solution out; F [nf,3] int32; TV [nv,3] FP64 target

@@ -25,7 +25,7 @@ SYMBOLS = [
     "mo_lm_default_options", "mo_deform_batch", "mo_rigid_deform_batch", "mo_last_timings",
     "mo_cad_prepare", "mo_cad_sizes", "mo_cad_export", "mo_cad_destroy", "mo_delaunay2", "mo_delaunay3",
     "mo_apply_transform_f64", "mo_linear_estimation", "mo_deform_subdivision",
-    "mo_linear_estimation_rot", "mo_nearest_f32", "mo_deform_rot",
+    "mo_linear_estimation_rot", "mo_nearest_f32", "mo_deform_rot", "mo_deform_rot_batch",
 ]
 
 
@@ -118,6 +118,8 @@ _SIGS = {
     "mo_nearest_f32": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _vp, _vp]),
     "mo_deform_rot": (C.c_int, [_i32, _vp, _i64, _vp, _i64, _f64, C.POINTER(LMOptions), _vp,
                                 C.POINTER(LMStats), _vp]),
+    "mo_deform_rot_batch": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(LMOptions), _vp,
+                                      _vp, _vp]),
     "mo_deform_subdivision": (C.c_int, [_i32, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _f64, C.POINTER(LMOptions), _i32,
                                         _vp, _vp, _vp, C.POINTER(_i32), _vp]),
 }

@@ -952,12 +952,12 @@ __global__ void k_rot_finish(double* __restrict__ V, double* __restrict__ rot, i
 
 }  // namespace
 
-MO_EXPORT int mo_deform_rot(int32_t param_id, double* V, int64_t nv, const int32_t* F, int64_t nf, double lambda,
-                            const mo_lm_options* opt, double* rot, mo_lm_stats* stats, void* stream) {
-  Template* t = get_template(param_id);
-  MO_CHECK(t, MO_ERR_ARG, "mo_deform_rot: unknown param_id %d", param_id);
-  MO_CHECK(V && nv > 0 && nv < (1LL << 24) && nf >= 0 && nf < (1LL << 28) && (F || nf == 0), MO_ERR_ARG,
-           "mo_deform_rot: bad arguments (nv %lld, nf %lld)", (long long)nv, (long long)nf);
+MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, double* const* V, const int64_t* nv,
+                                  const int32_t* const* F, const int64_t* nf, const double* lambda,
+                                  const mo_lm_options* opt, double* const* rot, mo_lm_stats* stats, void* stream) {
+  MO_CHECK(npairs >= 0, MO_ERR_ARG, "mo_deform_rot_batch: npairs < 0");
+  if (npairs == 0) return MO_OK;
+  MO_CHECK(param_ids && V && nv && F && nf && lambda, MO_ERR_ARG, "mo_deform_rot_batch: null per-pair array");
   mo_lm_options o;
   if (opt) {
     o = *opt;
@@ -970,80 +970,122 @@ MO_EXPORT int mo_deform_rot(int32_t param_id, double* V, int64_t nv, const int32
   MO_CHECK(o.block_threads >= 64 && o.block_threads <= 1024 && o.block_threads % 64 == 0, MO_ERR_ARG,
            "block_threads must be a multiple of 64 in [64, 1024]");
   hipStream_t s = as_stream(stream);
-  // half-edges in residual order (face, j), skipping a == b (Ceres rejects
-  // a residual block that names one parameter block twice)
-  std::vector<int32_t> hF((size_t)nf * 3);
-  if (nf) MO_HIP(hipMemcpyAsync(hF.data(), F, hF.size() * 4, hipMemcpyDeviceToHost, s));
+  std::vector<RotPairDesc> pd(npairs);
+  std::vector<std::vector<int32_t>> hF(npairs);
+  for (int p = 0; p < npairs; ++p) {
+    Template* t = get_template(param_ids[p]);
+    MO_CHECK(t, MO_ERR_ARG, "mo_deform_rot: pair %d: unknown param_id %d", p, param_ids[p]);
+    MO_CHECK(V[p] && nv[p] > 0 && nv[p] < (1LL << 24) && nf[p] >= 0 && nf[p] < (1LL << 28) && (F[p] || nf[p] == 0),
+             MO_ERR_ARG, "mo_deform_rot: pair %d: bad arguments (nv %lld, nf %lld)", p, (long long)nv[p],
+             (long long)nf[p]);
+    pd[p].grid = t->grid32.as<float>();
+    pd[p].N = t->N;
+    pd[p].lambda = (float)lambda[p];
+    hF[p].resize((size_t)nf[p] * 3);
+    if (nf[p]) MO_HIP(hipMemcpyAsync(hF[p].data(), F[p], hF[p].size() * 4, hipMemcpyDeviceToHost, s));
+  }
   MO_HIP(hipStreamSynchronize(s));
-  for (size_t k = 0; k < hF.size(); ++k)
-    MO_CHECK(hF[k] >= 0 && hF[k] < nv, MO_ERR_ARG, "mo_deform_rot: face index %d out of range", hF[k]);
+  // per pair: half-edges in residual order (face, j), skipping a == b (Ceres
+  // rejects a residual block that names one parameter block twice), and
+  // each vertex's outgoing then incoming half-edges
   std::vector<int2> hab;
-  hab.reserve(hF.size());
-  for (int64_t f = 0; f < nf; ++f)
-    for (int j = 0; j < 3; ++j) {
-      const int a = hF[3 * f + j], b = hF[3 * f + (j + 1) % 3];
-      if (a != b) hab.push_back(make_int2(a, b));
-    }
-  const int n = (int)nv, nh = (int)hab.size();
-  std::vector<int> nout(n, 0), nin(n, 0);
-  for (const int2& e : hab) { nout[e.x]++; nin[e.y]++; }
-  std::vector<int4> vadj(n);
-  std::vector<int> vlist(2 * (size_t)nh);
-  {
+  std::vector<int4> vadj;
+  std::vector<int> vlist;
+  long long vtot = 0, htot = 0;
+  for (int p = 0; p < npairs; ++p) {
+    const int n = (int)nv[p];
+    for (size_t k = 0; k < hF[p].size(); ++k)
+      MO_CHECK(hF[p][k] >= 0 && hF[p][k] < n, MO_ERR_ARG, "mo_deform_rot: pair %d: face index %d out of range", p,
+               hF[p][k]);
+    const size_t h0 = hab.size();
+    for (int64_t f = 0; f < nf[p]; ++f)
+      for (int j = 0; j < 3; ++j) {
+        const int a = hF[p][3 * f + j], b = hF[p][3 * f + (j + 1) % 3];
+        if (a != b) hab.push_back(make_int2(a, b));
+      }
+    const int nh = (int)(hab.size() - h0);
+    std::vector<int> nout(n, 0), nin(n, 0);
+    for (size_t h = h0; h < hab.size(); ++h) { nout[hab[h].x]++; nin[hab[h].y]++; }
+    const size_t v0 = vadj.size(), l0 = vlist.size();
     int pos = 0;
     for (int i = 0; i < n; ++i) {
-      vadj[i] = make_int4(pos, nout[i], nin[i], 0);
+      vadj.push_back(make_int4(pos, nout[i], nin[i], 0));
       pos += nout[i] + nin[i];
     }
+    vlist.resize(l0 + (size_t)pos);
     std::vector<int> fo(n), fi(n);
-    for (int i = 0; i < n; ++i) { fo[i] = vadj[i].x; fi[i] = vadj[i].x + vadj[i].y; }
+    for (int i = 0; i < n; ++i) { fo[i] = vadj[v0 + i].x; fi[i] = vadj[v0 + i].x + vadj[v0 + i].y; }
     for (int h = 0; h < nh; ++h) {
-      vlist[fo[hab[h].x]++] = h;
-      vlist[fi[hab[h].y]++] = h;
+      vlist[l0 + fo[hab[h0 + h].x]++] = h;
+      vlist[l0 + fi[hab[h0 + h].y]++] = h;
     }
+    pd[p].nv = n;
+    pd[p].nh = nh;
+    pd[p].vbase = vtot;
+    pd[p].hbase = htot;
+    pd[p].lbase = (long long)l0;
+    vtot += n;
+    htot += nh;
   }
+  const long long hslots = htot ? htot : 1;
   DevPtr d_x0, d_u, d_w, d_adj, d_list, d_hab, d_vw, d_hw, d_pair, d_res;
-  if (int e_ = d_x0.alloc((size_t)n * 16)) return e_;
-  if (int e_ = d_u.alloc((size_t)n * 16)) return e_;
-  if (int e_ = d_w.alloc((size_t)n * 16)) return e_;
-  if (int e_ = d_adj.alloc((size_t)n * 16)) return e_;
+  if (int e_ = d_x0.alloc((size_t)vtot * 16)) return e_;
+  if (int e_ = d_u.alloc((size_t)vtot * 16)) return e_;
+  if (int e_ = d_w.alloc((size_t)vtot * 16)) return e_;
+  if (int e_ = d_adj.alloc((size_t)vtot * 16)) return e_;
   if (int e_ = d_list.alloc(vlist.size() * 4)) return e_;
-  if (int e_ = d_hab.alloc((size_t)nh * 8)) return e_;
-  if (int e_ = d_vw.alloc((size_t)n * 16 * kRotVSlots)) return e_;
-  if (int e_ = d_hw.alloc((size_t)nh * 16 * kRotHSlots)) return e_;
-  if (int e_ = d_pair.alloc(sizeof(RotPairDesc))) return e_;
-  if (int e_ = d_res.alloc(sizeof(LMResultDev))) return e_;
-  MO_HIP(hipMemcpyAsync(d_adj.p, vadj.data(), (size_t)n * 16, hipMemcpyHostToDevice, s));
+  if (int e_ = d_hab.alloc((size_t)htot * 8)) return e_;
+  if (int e_ = d_vw.alloc((size_t)vtot * 16 * kRotVSlots)) return e_;
+  if (int e_ = d_hw.alloc((size_t)hslots * 16 * kRotHSlots)) return e_;
+  if (int e_ = d_pair.alloc(npairs * sizeof(RotPairDesc))) return e_;
+  if (int e_ = d_res.alloc(npairs * sizeof(LMResultDev))) return e_;
+  MO_HIP(hipMemcpyAsync(d_adj.p, vadj.data(), (size_t)vtot * 16, hipMemcpyHostToDevice, s));
   if (!vlist.empty()) MO_HIP(hipMemcpyAsync(d_list.p, vlist.data(), vlist.size() * 4, hipMemcpyHostToDevice, s));
-  if (nh) MO_HIP(hipMemcpyAsync(d_hab.p, hab.data(), (size_t)nh * 8, hipMemcpyHostToDevice, s));
-  RotPairDesc pd{t->grid32.as<float>(), t->N, n, nh, 0, 0, 0, (float)lambda};
-  MO_HIP(hipMemcpyAsync(d_pair.p, &pd, sizeof(pd), hipMemcpyHostToDevice, s));
-  const unsigned blocks = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_rot_init, dim3(blocks), dim3(256), 0, s, V, rot, n, d_x0.as<float4>(), d_u.as<float4>(),
-                     d_w.as<float4>());
+  if (htot) MO_HIP(hipMemcpyAsync(d_hab.p, hab.data(), (size_t)htot * 8, hipMemcpyHostToDevice, s));
+  MO_HIP(hipMemcpyAsync(d_pair.p, pd.data(), npairs * sizeof(RotPairDesc), hipMemcpyHostToDevice, s));
+  for (int p = 0; p < npairs; ++p) {
+    const unsigned blocks = (unsigned)((nv[p] + 255) / 256);
+    hipLaunchKernelGGL(k_rot_init, dim3(blocks), dim3(256), 0, s, V[p], rot ? rot[p] : nullptr, (int)nv[p],
+                       d_x0.as<float4>() + pd[p].vbase, d_u.as<float4>() + pd[p].vbase,
+                       d_w.as<float4>() + pd[p].vbase);
+  }
   MO_HIP(hipGetLastError());
   RotArrays A{d_x0.as<float4>(), d_u.as<float4>(), d_w.as<float4>(), d_adj.as<int4>(), d_list.as<int>(),
               d_hab.as<int2>(), d_vw.as<float4>(), d_hw.as<float4>()};
-  int rc = lm_rot_launch(d_pair.as<RotPairDesc>(), 1, A, to_dev_options(o), d_res.as<LMResultDev>(), n,
-                         nh ? nh : 1, std::min(o.block_threads, 256), s);
+  int rc = lm_rot_launch(d_pair.as<RotPairDesc>(), npairs, A, to_dev_options(o), d_res.as<LMResultDev>(), vtot,
+                         hslots, std::min(o.block_threads, 256), s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_rot_finish, dim3(blocks), dim3(256), 0, s, V, rot, n, d_u.as<float4>(), d_w.as<float4>());
-  MO_HIP(hipGetLastError());
-  LMResultDev r;
-  MO_HIP(hipMemcpyAsync(&r, d_res.p, sizeof(r), hipMemcpyDeviceToHost, s));
-  MO_HIP(hipStreamSynchronize(s));
-  if (stats) {
-    stats->initial_cost = r.initial_cost;
-    stats->final_cost = r.final_cost;
-    stats->vertex_cost = r.vertex_cost;
-    stats->rigidity_cost = r.rigidity_cost;
-    stats->iterations = r.iterations;
-    stats->successful_steps = r.successful_steps;
-    stats->unsuccessful_steps = r.unsuccessful_steps;
-    stats->termination = r.termination;
-    stats->pcg_iterations = r.pcg_iterations;
-    stats->clock_start = r.clock_start;
-    stats->clock_end = r.clock_end;
+  for (int p = 0; p < npairs; ++p) {
+    const unsigned blocks = (unsigned)((nv[p] + 255) / 256);
+    hipLaunchKernelGGL(k_rot_finish, dim3(blocks), dim3(256), 0, s, V[p], rot ? rot[p] : nullptr, (int)nv[p],
+                       d_u.as<float4>() + pd[p].vbase, d_w.as<float4>() + pd[p].vbase);
   }
+  MO_HIP(hipGetLastError());
+  std::vector<LMResultDev> r(npairs);
+  MO_HIP(hipMemcpyAsync(r.data(), d_res.p, npairs * sizeof(LMResultDev), hipMemcpyDeviceToHost, s));
+  MO_HIP(hipStreamSynchronize(s));
+  if (stats)
+    for (int p = 0; p < npairs; ++p) {
+      mo_lm_stats& o2 = stats[p];
+      o2.initial_cost = r[p].initial_cost;
+      o2.final_cost = r[p].final_cost;
+      o2.vertex_cost = r[p].vertex_cost;
+      o2.rigidity_cost = r[p].rigidity_cost;
+      o2.iterations = r[p].iterations;
+      o2.successful_steps = r[p].successful_steps;
+      o2.unsuccessful_steps = r[p].unsuccessful_steps;
+      o2.termination = r[p].termination;
+      o2.pcg_iterations = r[p].pcg_iterations;
+      o2.clock_start = r[p].clock_start;
+      o2.clock_end = r[p].clock_end;
+    }
   return MO_OK;
+}
+
+MO_EXPORT int mo_deform_rot(int32_t param_id, double* V, int64_t nv, const int32_t* F, int64_t nf, double lambda,
+                            const mo_lm_options* opt, double* rot, mo_lm_stats* stats, void* stream) {
+  double* Vp[1] = {V};
+  double* Rp[1] = {rot};
+  const int32_t* Fp[1] = {F};
+  return mo_deform_rot_batch(1, &param_id, Vp, &nv, Fp, &nf, &lambda, opt, rot ? Rp : nullptr, stats, stream);
 }

@@ -94,6 +94,30 @@ def deform_rot(param_id, V, F, lam=1.0, W=None, options=None):
     return st.as_dict(), W
 
 
+def deform_rot_batch(param_ids, Vs, Fs, lambdas, options=None):
+    """deform_rot for many independent problems in one launch.  Vs: FP64
+    device tensors [nv,3] (normalized frames, updated in place); Fs: int32
+    faces.  Returns (list of stats dicts, list of rotation tensors)."""
+    n = len(param_ids)
+    Vs = list(Vs)
+    for V in Vs:
+        if V.dtype != torch.float64 or not V.is_cuda or not V.is_contiguous() or V.dim() != 2 or V.shape[1] != 3:
+            raise TypeError("deform_rot_batch expects contiguous [n,3] float64 device tensors")
+    Fd = [_as_dev(f, torch.int32).reshape(-1, 3) for f in Fs]
+    Ws = [torch.zeros_like(V) for V in Vs]
+    stats = (_lib.LMStats * n)()
+    _lib.call("mo_deform_rot_batch", n,
+              _arr(C.c_int32, [int(p) for p in param_ids]),
+              _arr(C.c_void_p, [V.data_ptr() for V in Vs]),
+              _arr(C.c_int64, [V.shape[0] for V in Vs]),
+              _arr(C.c_void_p, [f.data_ptr() for f in Fd]),
+              _arr(C.c_int64, [f.shape[0] for f in Fd]),
+              _arr(C.c_double, [float(l) for l in lambdas]),
+              C.byref(options) if options is not None else None,
+              _arr(C.c_void_p, [W.data_ptr() for W in Ws]), stats, _stream())
+    return [s.as_dict() for s in stats], Ws
+
+
 class PairBatch:
     """Device-resident inputs of a rigid_deform batch (FP64 vertices, int32 faces)."""
 

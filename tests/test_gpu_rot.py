@@ -114,6 +114,26 @@ def test_deform_rot_start_rotations_and_errors(gpu):
         deform.deform_rot(pid, V.float(), sF, lam)
 
 
+def test_deform_rot_batch_equals_alone(gpu):
+    """mo_deform_rot_batch: each problem's result is bit-identical to the
+    same problem solved alone."""
+    _say("[rot] batch of 5 vs alone")
+    probs = []
+    for k, name in enumerate(["shrunk_sphere_n16", "rotated_copy_n32", "low_rigidity_n16", "shrunk_sphere_n16",
+                              "rotated_copy_n32"]):
+        sV, sF, tV, tF, N, lam = _case(name)
+        pid = deform.template_f64(tV, tF, N)
+        _, scale, trans = pyDeform.TemplateInfo(pid)
+        probs.append((pid, (sV - trans) / scale, sF, lam + 0.1 * k))
+    Vb = [torch.from_numpy(p[1]).to(gpu).contiguous() for p in probs]
+    stb, Wb = deform.deform_rot_batch([p[0] for p in probs], Vb, [p[2] for p in probs], [p[3] for p in probs])
+    for p, V, W, st in zip(probs, Vb, Wb, stb):
+        V1 = torch.from_numpy(p[1]).to(gpu).contiguous()
+        st1, W1 = deform.deform_rot(p[0], V1, p[2], p[3])
+        assert torch.equal(V, V1) and torch.equal(W, W1)
+        assert st["final_cost"] == st1["final_cost"] and st["iterations"] == st1["iterations"]
+
+
 def test_rigid_rot_deform_cli(gpu, oracle, tmp_path):
     _say("[rot] rigid_rot_deform CLI vs library")
     sV, sF, tV, tF, N, lam = _case("rotated_copy_n32")
