@@ -61,6 +61,21 @@ def cpu_baseline(pair, N, lam):
     return dt, st, d, sn, sc, pos
 
 
+def cpu_baseline_all_cores(pairs, N, lam):
+    """The same oracle pipeline on one pair per host core at once (threads:
+    the oracle's C entry points release the GIL), aggregate pairs/s."""
+    from concurrent.futures import ThreadPoolExecutor
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    work = [pairs[i % len(pairs)] for i in range(cores)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        list(ex.map(lambda p: cpu_baseline(p, N, lam), work))
+    dt = time.perf_counter() - t0
+    return {"value": len(work) / dt, "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": "%d pairs of the workload, one per core in parallel (same oracle pipeline)" % len(work),
+            "seconds": dt}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,8 +89,9 @@ def main():
     ap.add_argument("--plain-pairs", type=int, default=8, help="pairs run one at a time for the plain GPU rate")
     ap.add_argument("--block-threads", type=int, default=1024, help="threads per pair of the global-memory kernel")
     ap.add_argument("--kernel-variant", type=int, default=0, help="0: register/LDS kernel when a pair fits; 1: global")
+    ap.add_argument("--pcg-tol", type=float, default=None, help="inner PCG relative tolerance (library default if unset)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "neural2", "rot"],
+    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "neural2", "rot", "cfg1"],
                     help="batch: the headline line; serve / adam / ode: the serving, pyDeform-Adam and "
                          "NeuralODE workloads (their own JSON line)")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of Poisson arrivals")
@@ -107,7 +123,8 @@ def main():
         del a, b
     if args.mode != "batch":
         print(json.dumps({"serve": bench_serve, "adam": bench_adam, "ode": bench_ode,
-                          "neural2": bench_neural2, "rot": bench_rot}[args.mode](args)), flush=True)
+                          "neural2": bench_neural2, "rot": bench_rot, "cfg1": bench_cfg1}[args.mode](args)),
+              flush=True)
         return
 
     t0 = time.perf_counter()
@@ -119,6 +136,8 @@ def main():
 
     batch = deform.PairBatch(pairs, grid_resolution=args.grid, lambdas=args.lam)
     opt = deform.default_options(block_threads=args.block_threads, timing=1, kernel_variant=args.kernel_variant)
+    if args.pcg_tol is not None:
+        opt.pcg_relative_tolerance = args.pcg_tol
     torch.cuda.synchronize()
 
     for w in range(args.warmup):
@@ -232,11 +251,53 @@ def main():
             "host_cpu": _cpu_model(),
             "host_nproc": os.cpu_count(),
         }
+        result["cpu_baseline_all_cores"] = cpu_baseline_all_cores(distinct, args.grid, args.lam)
         result["energy_check_pair0"] = {"oracle": so["final_cost"], "gpu": eg,
                                         "rel_diff": (eg - so["final_cost"]) / so["final_cost"],
                                         "oracle_iterations": so["iterations"],
                                         "gpu_iterations": stats_all[-1][0]["iterations"]}
     print(json.dumps(result), flush=True)
+
+
+def bench_cfg1(args):
+    """Config 1: icosphere L4 sources (2562 vertices) against displaced L5
+    icospheres scaled (1, 0.7, 1.3) (10242 vertices), 64^3, lambda 1; the
+    rigid_deform pipeline over a batch of args.pairs pairs (args.distinct
+    distinct seeds), pairs/s; the oracle on pair 0 for the energy check and
+    the CPU baseline."""
+    import numpy as np
+    import torch
+    from meshode_amd import _lib, deform, synth
+    distinct = [synth.icosphere_pair(s, 4) for s in range(min(args.distinct, args.pairs))]
+    pairs = [distinct[i % len(distinct)] for i in range(args.pairs)]
+    batch = deform.PairBatch(pairs, grid_resolution=args.grid, lambdas=args.lam)
+    opt = deform.default_options(timing=1)
+    for _ in range(max(args.warmup, 0)):
+        batch.run(opt)
+    torch.cuda.synchronize()
+    steps = max(args.steps, 1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = batch.run(opt)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    tm = _lib.last_timings()
+    out = {"metric": "deformation pairs/sec (config 1: 2562 -> 10242 verts, 64^3)", "value": args.pairs * steps / elapsed,
+           "unit": "pairs/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True, "dtype": "f32 (LM), f64 (grid)",
+           "data": "synthetic", "phase_ms_last": {k: tm[k] for k in ("grid_ms", "setup_ms", "lm_ms", "total_ms")},
+           "config": {"workload": "config 1: icosphere L4 vs displaced icosphere L5 (x 1, 0.7, 1.3)",
+                      "pairs": args.pairs, "distinct": len(distinct), "grid": args.grid, "lambda": args.lam}}
+    if not args.no_cpu_baseline:
+        dt, so, grid, sn, sc, pos = cpu_baseline(distinct[0], args.grid, args.lam)
+        from oracle import oracle as O
+        eg, _, _ = O.lm_cost(grid, args.grid, (batch.out[0].cpu().numpy() - pos) / sc, sn,
+                             O.face_edges(distinct[0][1]), args.lam)
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+                               "sample": "pair 0 (oracle pipeline, FP64, one thread)", "seconds": dt}
+        out["energy_check_pair0"] = {"oracle": so["final_cost"], "gpu": eg,
+                                     "rel_diff": (eg - so["final_cost"]) / so["final_cost"]}
+    return out
 
 
 def bench_serve(args):

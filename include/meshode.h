@@ -71,7 +71,11 @@ int mo_template_info(int32_t param_id, int32_t* grid_resolution, double* scale, 
  * dist64 = UniformGrid voxels (uniformgrid.h:33) as [z][y][x] FP64,
  * dist32 = the same rounded to float, nearest = nearest triangle index
  * (igl::point_mesh_squared_distance's I, discarded by mesh.cc:137-150;
- * lowest index among equal distances). */
+ * lowest index among equal distances).  A template grid is exact up to the
+ * band b = 0.2 + sqrt(3)/N + 0.01 and holds b (nearest -1) beyond it: the
+ * lookups truncate interpolated values above 0.2 (uniformgrid.cc:80-82) and
+ * the field is 1-Lipschitz, so no lookup result, value or gradient, depends
+ * on a value beyond b.  mo_grid_build computes the exact field everywhere. */
 int mo_template_grid(int32_t param_id, double* dist64, float* dist32, int32_t* nearest, void* stream);
 
 /* ------------------------------------------------------------------------

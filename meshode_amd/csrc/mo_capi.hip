@@ -164,7 +164,8 @@ int template_create(const T* V, int64_t nv, const int32_t* F, int64_t nf, int32_
   if ((rc = t->grid64.alloc(t->cells * sizeof(double)))) return rc;
   if ((rc = t->grid32.alloc(t->cells * sizeof(float)))) return rc;
   if ((rc = t->nearest.alloc(t->cells * sizeof(int)))) return rc;
-  GridSpec gs{nullptr, nv, F, nf, t->N, t->grid64.as<double>(), t->grid32.as<float>(), t->nearest.as<int>()};
+  GridSpec gs{nullptr, nv, F, nf, t->N, t->grid64.as<double>(), t->grid32.as<float>(), t->nearest.as<int>(),
+              lookup_band2(t->N)};
   const size_t gws = grid_batch_workspace_bytes(&gs, 1);
   Arena ar;
   if ((rc = ar.reserve(al(nv * 3 * sizeof(double)) + al(8 * sizeof(double)) + al(xform_workspace_bytes(1)) + al(gws) +
@@ -806,7 +807,8 @@ MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, co
     long long N = grid_resolution[p];
     bytes += al(tgt_nv[p] * 3 * sizeof(double)) + al(8 * sizeof(double)) +
              al(std::max<long long>(src_nv[p], 1) * 3 * sizeof(double)) + al(N * N * N * sizeof(float));
-    gspecs[p] = GridSpec{nullptr, tgt_nv[p], tgt_faces[p], tgt_nf[p], (int)N, nullptr, nullptr, nullptr};
+    gspecs[p] = GridSpec{nullptr, tgt_nv[p], tgt_faces[p], tgt_nf[p], (int)N, nullptr, nullptr, nullptr,
+                         lookup_band2((int)N)};
   }
   const size_t gws = grid_batch_workspace_bytes(gspecs.data(), npairs);
   bytes += al(gws) + 2 * al(xform_workspace_bytes(npairs));

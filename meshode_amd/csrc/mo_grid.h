@@ -24,6 +24,9 @@ struct GCull {
 
 // One distance grid to build: normalized FP64 target vertices [nv,3] and
 // int32 faces [nf,3] (device); any of dist64 / dist32 / nearest may be null.
+// band2: squared distance beyond which a point only needs to be known to be
+// farther: it stores sqrt(min(d^2, band2)) and nearest -1 when no face is
+// within the band (INFINITY: the exact field everywhere).
 struct GridSpec {
   const double* Vn;
   long long nv;
@@ -33,7 +36,20 @@ struct GridSpec {
   double* dist64;
   float* dist32;
   int* nearest;
+  double band2 = INFINITY;
 };
+
+// The band a template grid needs: the lookups (uniformgrid.cc:18-150)
+// interpolate the 8 corners of a cell and return 0 when the result exceeds
+// 0.2.  The field is 1-Lipschitz, so a cell with a corner at d <= 0.2 has all
+// corners within 0.2 + sqrt(3)/N; a cell with every corner beyond that is
+// truncated, and stays truncated when its corners hold any value above
+// 0.2 + 0.01.  Values clamped at this band therefore give every lookup the
+// same result, value and gradient, as the exact field.
+inline double lookup_band2(int N) {
+  const double b = 0.2 + 1.7320508075688772 / N + 0.01;
+  return b * b;
+}
 
 // Device-side job record (workspace slices included).
 struct GridJob {
@@ -49,6 +65,7 @@ struct GridJob {
   double* dist64;
   float* dist32;
   int* nearest;
+  double band2;
 };
 
 size_t grid_batch_workspace_bytes(const GridSpec* specs, int n);

@@ -288,7 +288,8 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
   const double lx = (double)(bx * 4) / N, ly = (double)(by * 4) / N, lz = (double)(bz * 4) / N;
   const double hx = (double)k1 / N, hy = (double)j1 / N, hz = (double)i1 / N;
 
-  double best = INFINITY;
+  // points farther than the band only need to be shown farther (GridSpec)
+  double best = JJ.band2;
   int bidx = INT_MAX;
 
   auto process_tile = [&](int t) {
@@ -395,7 +396,7 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
     double s = sqrt(best);
     if (dist64) dist64[cell] = s;
     if (dist32) dist32[cell] = (float)s;
-    if (nearest) nearest[cell] = bidx;
+    if (nearest) nearest[cell] = bidx == INT_MAX ? -1 : bidx;
   }
 }
 
@@ -482,6 +483,8 @@ int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipSt
     J.perm = (int*)w; w += al256((size_t)sp.nf * sizeof(int));
     J.tb = (double*)w; w += al256((size_t)J.ntiles * 6 * sizeof(double));
     J.dist64 = sp.dist64; J.dist32 = sp.dist32; J.nearest = sp.nearest;
+    MO_CHECK(sp.band2 > 0.0, MO_ERR_ARG, "grid band must be positive");
+    J.band2 = sp.band2;
     const long long bpa = (sp.N + 3) / 4;
     max_nf = std::max(max_nf, sp.nf);
     max_tiles = std::max(max_tiles, J.ntiles);

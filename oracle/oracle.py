@@ -153,6 +153,32 @@ def build_grid(V, F, N, cells=None):
     return d, d32, idx
 
 
+def lookup_band2(N):
+    """Squared band of a template grid: 0.2 + sqrt(3)/N + 0.01, squared (the
+    lookups truncate interpolated distances above 0.2, uniformgrid.cc:80-82,
+    and the field is 1-Lipschitz, so values beyond the band never reach a
+    lookup result)."""
+    b = 0.2 + 1.7320508075688772 / N + 0.01
+    return b * b
+
+
+def build_grid_banded(V, F, N, cells=None):
+    """The template grid: per grid point sqrt(min(d^2, band^2)) and the
+    nearest face (-1 when none is within the band).  Returns (dist64, dist32,
+    nearest) over (z, y, x) or over ``cells``."""
+    if cells is None:
+        cells = np.arange(N ** 3)
+    cells = np.asarray(cells, np.int64)
+    i, rem = np.divmod(cells, N * N)
+    j, k = np.divmod(rem, N)
+    Q = np.stack([k / N, j / N, i / N], 1).astype(np.float64)
+    d2, idx = point_mesh_d2(Q, V, F)
+    b2 = lookup_band2(N)
+    inside = d2 <= b2
+    d = np.where(inside, np.sqrt(np.where(inside, d2, b2)), np.sqrt(b2))
+    return d, d.astype(np.float32), np.where(inside, idx, -1).astype(np.int32)
+
+
 def lookup_f32(grid, N, P, want_grad=True):
     """UniformGrid::DistanceFloat<float / Jet<float,3>> (uniformgrid.cc:85-150)."""
     grid = _f64(grid); P = _f32(P)

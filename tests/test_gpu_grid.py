@@ -1,7 +1,9 @@
 """GPU parity: template normalization and the distance-grid build
 (Mesh::Normalize mesh.cc:66-96, Mesh::ConstructDistanceField mesh.cc:106-152)
 against the oracle.  Bar: bit-exact distances (FP64 and the FP32 rounding)
-and bit-exact nearest-triangle indices (lowest index on ties)."""
+and bit-exact nearest-triangle indices (lowest index on ties); template
+grids are banded at the lookups' reach (values clamped at 0.2 + sqrt(3)/N +
+0.01, which no lookup result can see), mo_grid_build is exact everywhere."""
 import numpy as np
 import pytest
 import torch
@@ -13,6 +15,22 @@ pytestmark = pytest.mark.gpu
 
 def _say(msg):
     print(msg, flush=True)
+
+
+def _grid_build(Vn, F, N):
+    """mo_grid_build (exact field) on normalized FP64 vertices."""
+    import ctypes as C
+    from meshode_amd import _lib
+    Vd = torch.from_numpy(np.ascontiguousarray(Vn, np.float64)).cuda()
+    Fd = torch.from_numpy(np.ascontiguousarray(F, np.int32)).cuda()
+    d64 = torch.empty(N ** 3, dtype=torch.float64, device="cuda")
+    d32 = torch.empty(N ** 3, dtype=torch.float32, device="cuda")
+    nn = torch.empty(N ** 3, dtype=torch.int32, device="cuda")
+    _lib.call("mo_grid_build", C.c_void_p(Vd.data_ptr()), len(Vn), C.c_void_p(Fd.data_ptr()), len(F), N,
+              C.c_void_p(d64.data_ptr()), C.c_void_p(d32.data_ptr()), C.c_void_p(nn.data_ptr()),
+              C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    return d64.cpu().numpy(), d32.cpu().numpy(), nn.cpu().numpy()
 
 
 def _template(V, F, N):
@@ -39,10 +57,15 @@ def test_grid_bit_exact_small(gpu, oracle, case):
     assert Nn == N
     assert scale == sc_o and np.array_equal(trans, pos_o)
     d64, d32, nn = [t.cpu().numpy().reshape(-1) for t in pyDeform.TemplateGrid(pid)]
-    od, od32, oi = oracle.build_grid(Vn, F, N)
+    # template grids are banded at the lookups' reach (oracle.build_grid_banded)
+    od, od32, oi = oracle.build_grid_banded(Vn, F, N)
     assert np.array_equal(d64, od), np.max(np.abs(d64 - od))
     assert np.array_equal(d32, od32)
     assert np.array_equal(nn, oi), np.flatnonzero(nn != oi)[:10]
+    # the exact field everywhere through mo_grid_build
+    fd, fd32, fi = _grid_build(Vn, F, N)
+    ed, ed32, ei = oracle.build_grid(Vn, F, N)
+    assert np.array_equal(fd, ed) and np.array_equal(fd32, ed32) and np.array_equal(fi, ei)
 
 
 def test_grid_full_size_sampled(gpu, oracle):
@@ -57,9 +80,10 @@ def test_grid_full_size_sampled(gpu, oracle):
     Vn, _, _ = oracle.normalize(V32)
     rng = np.random.default_rng(0)
     cells = rng.choice(N ** 3, 4096, replace=False)
-    od, od32, oi = oracle.build_grid(Vn, F, N, cells=cells)
+    od, od32, oi = oracle.build_grid_banded(Vn, F, N, cells=cells)
     assert np.array_equal(d64.reshape(-1)[cells], od)
     assert np.array_equal(nn.reshape(-1)[cells], oi)
+    assert (oi >= 0).mean() > 0.3                          # plenty of cells inside the band
     h = 1.0 / N
     for ax in range(3):
         diff = np.abs(np.diff(d64, axis=ax))

@@ -109,3 +109,31 @@ def test_distance_loss_layer(oracle):
     assert np.array_equal(fwd, d * d)
     # 0.5 * d(d^2)/dp = d * grad d, rounded once to float
     assert np.array_equal(bwd, (d[:, None] * gr).astype(np.float32))
+
+
+def test_banded_grid_gives_identical_lookups(oracle):
+    """Template grids hold sqrt(min(d^2, band^2)), band = 0.2 + sqrt(3)/N +
+    0.01: every lookup (FP32 and FP64 paths, value and gradient) equals the
+    lookup on the exact field, on probes spread over the box and
+    concentrated where the field crosses 0.2 and the band."""
+    from meshode_amd import synth
+    _, _, V, F = synth.displaced_sphere_pair(4, src_level=1, tgt_level=2)
+    for N in (12, 24):
+        Vn, _, _ = oracle.normalize(V)
+        full, _, idx = oracle.build_grid(Vn, F, N)
+        band, band32, bidx = oracle.build_grid_banded(Vn, F, N)
+        b = np.sqrt(oracle.lookup_band2(N))
+        inside = full < b
+        assert np.array_equal(band[inside], full[inside]) and np.array_equal(bidx[inside], idx[inside])
+        assert np.all(band[~inside] == b) and np.all(bidx[~inside] == -1) and (~inside).any()
+        rng = np.random.default_rng(N)
+        P = rng.uniform(-0.05, 1.05, (40000, 3))
+        # probes where the interpolated field is near 0.2 or the band
+        d = oracle.lookup_f64(full, N, P, want_grad=False)
+        near = P[(np.abs(d - 0.2) < 0.03) | (d == 0)][:20000]
+        P = np.concatenate([P, near + rng.normal(0, 1e-3, near.shape)])
+        for fn in (oracle.lookup_f32, oracle.lookup_f64):
+            a_d, a_g = fn(full, N, P)
+            b_d, b_g = fn(band, N, P)
+            assert np.array_equal(a_d, b_d) and np.array_equal(a_g, b_g)
+        assert (oracle.lookup_f64(full, N, P, want_grad=False) == 0).sum() > 1000   # the truncation is exercised
