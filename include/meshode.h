@@ -335,6 +335,33 @@ int mo_linear_estimation_rot(double* V, int64_t nv, const int32_t* F, int64_t nf
 int mo_nearest_f32(const float* src, int64_t ns, const float* tar, int64_t nt, int32_t* index, double* dist2,
                    void* stream);
 
+/* ---- NeuralODE flow: one fixed-grid RK4 step of ODEFunc -----------------
+ * Replaces the torch evaluation of odeint(ODEFunc, y0, [t0, t1],
+ * method="rk4") (neuralode_fast.py:7-33, 42-61; torchdiffeq's 3/8-rule
+ * step) and its autograd backward.  ODEFunc is 4 -> 50 -> 50 -> 50 -> 3,
+ * LeakyReLU 0.01, input cat(y, t) - 0.5.  params: the module's 5503
+ * parameters, flat in its parameter order (W0 [50,4], b0, W1 [50,50], b1,
+ * W2 [50,50], b2, W3 [3,50], b3), FP32 device.  tau4 (host): the network's
+ * time input at the four stages; sign: +1, or -1 for a decreasing grid
+ * (integrated as t -> -t with f -> -f(-t, y), so tau = -stage time); dt:
+ * the step in the (possibly negated) grid.  y0, y1 [n,3] FP32 device.
+ *
+ * mo_ode_rk4_forward: y1 = y0 + (k1 + 3 (k2 + k3) + k4) dt / 8.  zsave
+ * [n,9] (optional, device) receives the stage inputs z2, z3, z4 for the
+ * backward.
+ * mo_ode_rk4_backward: from gy1 = dL/dy1 writes gy0 = dL/dy0 (optional)
+ * and gparams [5503] = dL/dparams (overwritten).  work: device workspace
+ * of mo_ode_rk4_workspace_bytes(n) bytes (8-byte aligned), or null (then
+ * stream-ordered allocation).  Sums over points are formed per 64-point
+ * chunk (FP32) and added in a fixed order in FP64: repeated calls give
+ * identical results. */
+int64_t mo_ode_rk4_workspace_bytes(int64_t n);
+int mo_ode_rk4_forward(const float* y0, int64_t n, const float* params, const float* tau4, float sign, float dt,
+                       float* y1, float* zsave, void* stream);
+int mo_ode_rk4_backward(const float* y0, int64_t n, const float* params, const float* tau4, float sign, float dt,
+                        const float* zsave, const float* gy1, float* gy0, float* gparams, void* work,
+                        int64_t work_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,7 @@ SYMBOLS = [
     "mo_cad_prepare", "mo_cad_sizes", "mo_cad_export", "mo_cad_destroy", "mo_delaunay2", "mo_delaunay3",
     "mo_apply_transform_f64", "mo_linear_estimation", "mo_deform_subdivision",
     "mo_linear_estimation_rot", "mo_nearest_f32", "mo_deform_rot", "mo_deform_rot_batch",
+    "mo_ode_rk4_workspace_bytes", "mo_ode_rk4_forward", "mo_ode_rk4_backward",
 ]
 
 
@@ -116,6 +117,10 @@ _SIGS = {
     "mo_linear_estimation": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _f64, C.POINTER(_i32), _vp]),
     "mo_linear_estimation_rot": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _f64, C.POINTER(_i32), _vp]),
     "mo_nearest_f32": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _vp, _vp]),
+    "mo_ode_rk4_workspace_bytes": (_i64, [_i64]),
+    "mo_ode_rk4_forward": (C.c_int, [_vp, _i64, _vp, _vp, C.c_float, C.c_float, _vp, _vp, _vp]),
+    "mo_ode_rk4_backward": (C.c_int, [_vp, _i64, _vp, _vp, C.c_float, C.c_float, _vp, _vp, _vp, _vp, _vp, _i64,
+                                      _vp]),
     "mo_deform_rot": (C.c_int, [_i32, _vp, _i64, _vp, _i64, _f64, C.POINTER(LMOptions), _vp,
                                 C.POINTER(LMStats), _vp]),
     "mo_deform_rot_batch": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(LMOptions), _vp,

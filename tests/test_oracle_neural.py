@@ -231,3 +231,34 @@ def test_rot_solve_coincident_vertices_limit(tmp_path):
     np.add.at(diag, cls[pairs[0]], pairs[2])
     np.add.at(diag, cls[pairs[1]], pairs[2])
     assert (np.abs(res).max(axis=1) / diag).max() < 1e-15
+
+
+@pytest.mark.parametrize("grid", [(0.0, 1.0), (1.0, 0.0), (0.25, 0.8), (0.9, 0.1)])
+def test_fused_step_times_match_the_torch_step(grid):
+    """The (tau, sign, dt) handed to the fused HIP step are the network time
+    inputs the float32 torch step feeds ODEFunc, and the step's sign and
+    size: recorded from the torch path on the CPU."""
+    torch.manual_seed(2)
+    func = nf.ODEFunc()
+    seen = []
+
+    class Rec(torch.nn.Module):
+        def forward(self, t, y):
+            seen.append(float(t))
+            return func(t, y)
+
+    y = torch.rand(5, 3)
+    t = torch.tensor(grid, dtype=torch.float32)
+    y1 = nf.odeint(Rec(), y, t)[1]
+    tau, sign, dt = nf.step_times(*grid)
+    assert list(tau) == seen
+    assert sign == (1.0 if grid[1] > grid[0] else -1.0)
+    assert dt == pytest.approx(abs(grid[1] - grid[0]), rel=1e-6)
+    # the same step written out with the recorded stage values
+    k = []
+    for s, tt in enumerate(tau):
+        third = np.float32(1 / 3)
+        z = (y if s == 0 else y + dt * k[0] * third if s == 1 else y + dt * (k[1] - k[0] * third) if s == 2
+             else y + dt * (k[0] - k[1] + k[2]))
+        k.append(sign * func(torch.tensor(tt), z))
+    assert torch.equal(y1, y + (k[0] + 3 * (k[1] + k[2]) + k[3]) * dt * 0.125)
