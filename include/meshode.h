@@ -105,6 +105,13 @@ int mo_distance_loss_forward(const float* V, int64_t nv, int32_t param_id, float
  * out[i] = d(V_i) * grad d(V_i), out [nv,3]. */
 int mo_distance_loss_backward(const float* V, int64_t nv, int32_t param_id, float* out, void* stream);
 
+/* Edge losses (rigid and graph, forward and backward): a template checks an
+ * index buffer (F or E) on the host the first time it sees it (MO_ERR_RANGE
+ * for an index outside [0, nv)) and keeps the buffer's incidence lists by
+ * its device address; later calls with that address neither copy nor wait
+ * on the stream.  A buffer whose contents change must come at a new address
+ * (or be stored again). */
+
 /* StoreRigidityInformation (rigid_layer.h:9-12, rigid_layer.cc:11-46). */
 int mo_rigidity_store(const float* V, int64_t nv, const int32_t* F, int64_t nf, int32_t param_id, void* stream);
 

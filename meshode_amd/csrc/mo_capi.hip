@@ -289,13 +289,12 @@ int edge_forward(const float* V, int64_t nv, const int32_t* idx, int64_t n, int3
            "edge loss called with %lld edges but %lld offsets are stored (call the Store function first)", nh,
            t->edge_count);
   hipStream_t s = as_stream(stream);
-  Arena ar;
+  // The index buffer is checked on the host once (with the incidence lists
+  // the backward needs, kept for this buffer): later calls with the same
+  // buffer launch without a device error flag or a wait on the stream.
   int rc;
-  if ((rc = ar.reserve(256, s))) return rc;
-  int* err = ar.take<int>(1);
-  if ((rc = zero_fill(err, sizeof(int), s))) return rc;
-  if ((rc = edge_fwd_launch(V, nv, idx, kind, nh, t->edge_off.as<float>(), out, err, s))) return rc;
-  return check_err_flag(err, s, "edge loss forward");
+  if ((rc = ensure_incidence(t, idx, n, nv, kind, s))) return rc;
+  return edge_fwd_launch(V, nv, idx, kind, nh, t->edge_off.as<float>(), out, nullptr, s);
 }
 
 int edge_backward(const float* V, int64_t nv, const int32_t* idx, int64_t n, int32_t pid, float* out, int kind,

@@ -95,8 +95,8 @@ __global__ void k_edge_fwd(const float* __restrict__ V, long long nv, const int*
   if (h >= nh) return;
   int a, b;
   edge_ends(idx, kind, h, &a, &b);
-  if (a < 0 || b < 0 || a >= nv || b >= nv) {
-    atomicOr(err, 1);
+  if (a < 0 || b < 0 || a >= nv || b >= nv) {   // err may be null: the indices were checked on the host
+    if (err) atomicOr(err, 1);
     out[h * 3] = out[h * 3 + 1] = out[h * 3 + 2] = 0.0f;
     return;
   }

@@ -8,7 +8,10 @@ against the oracle.
   errors.
 * mo_nearest_f32 (ReverseLossLayer's k = 1 query) bit-exact (index and FP64
   squared distance) against the oracle's brute force, with duplicates and
-  exact ties, over launch shapes with one and many source chunks.
+  exact ties, over launch shapes with one and many source chunks, and on
+  the uniform-grid search (4096 sources or more): box faces with near, far
+  and inside targets, zero extent along an axis or all axes, duplicated
+  sources and lattice ties.
 * ReverseLossLayer loss and source gradient; GraphLoss2Layer forward and
   backward against the oracle's distance and graph-edge losses (bit-exact
   gradient) with each graph normalized by its own template.
@@ -114,6 +117,51 @@ def test_nearest_bit_exact(gpu, ns, nt):
     assert np.array_equal(d2.cpu().numpy(), od2)
     if ns > 10 and nt > 4:
         assert k[0] == 3 and k[2] == ns - 1 and od2[2] == 0.0
+
+
+def _nn_case(name, rng):
+    """Source / target sets for the grid search (4096 sources or more)."""
+    if name == "box_surface":      # a box's faces, targets near and far, one inside
+        n = 6000
+        u = rng.random((n, 2)).astype(np.float32)
+        face = rng.integers(0, 6, n)
+        P = np.zeros((n, 3), np.float32)
+        for f in range(6):
+            m = face == f
+            ax = f // 2
+            oth = [a for a in range(3) if a != ax]
+            P[m, ax] = 0.2 + 0.6 * (f % 2)
+            P[m, oth[0]] = 0.2 + 0.6 * u[m, 0]
+            P[m, oth[1]] = 0.2 + 0.6 * u[m, 1]
+        tar = (P[rng.integers(0, n, 3000)] + rng.normal(0, 0.02, (3000, 3))).astype(np.float32)
+        return P, np.vstack([tar, np.array([[5, 5, 5], [-3, 0.5, 0.5], [0.5, 0.5, 0.5]], np.float32)])
+    if name == "flat_z":           # zero extent along z
+        P = rng.random((4500, 3)).astype(np.float32)
+        P[:, 2] = 0.25
+        return P, rng.random((500, 3)).astype(np.float32)
+    if name == "all_equal":        # zero extent: one cell
+        return np.tile(np.array([[0.3, 0.4, 0.5]], np.float32), (4200, 1)), rng.random((60, 3)).astype(np.float32)
+    if name == "line_dups":        # duplicated sources: the lower index wins
+        line = np.zeros((4100, 3), np.float32)
+        line[:, 0] = np.linspace(0, 1, 4100)
+        line = np.vstack([line, line[:50]])
+        return line, np.vstack([line[:20] + 1e-7, rng.random((40, 3)).astype(np.float32)])
+    if name == "lattice_ties":     # targets at cell centres: up to 8 sources equally near
+        g = np.stack(np.meshgrid(*[np.arange(17)] * 3, indexing="ij"), -1).reshape(-1, 3).astype(np.float32) / 16
+        return g, (g[rng.integers(0, len(g), 2000)] + 1 / 32).astype(np.float32)
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", ["box_surface", "flat_z", "all_equal", "line_dups", "lattice_ties"])
+def test_nearest_grid_search_bit_exact(gpu, name):
+    """The uniform-grid search (4096 sources or more) returns the brute
+    force's index and FP64 distance bits, ties to the lowest index."""
+    _say(f"[neural] nearest grid search {name}")
+    src, tar = _nn_case(name, np.random.default_rng(11))
+    idx, d2 = reverse_loss_layer.nearest(torch.from_numpy(src).to(gpu), torch.from_numpy(tar).to(gpu), True)
+    k, od2 = neural_oracle.nearest(src, tar)
+    assert np.array_equal(idx.cpu().numpy(), k)
+    assert np.array_equal(d2.cpu().numpy(), od2)
 
 
 def test_reverse_loss_layer(gpu):
