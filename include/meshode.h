@@ -369,6 +369,21 @@ int mo_ode_rk4_backward(const float* y0, int64_t n, const float* params, const f
                         const float* zsave, const float* gy1, float* gy0, float* gparams, void* work,
                         int64_t work_bytes, void* stream);
 
+/* ---- config-3 velocity field: one RK4 step on FP32 MFMA -------------------
+ * The field v(y) = MLP(y - 0.5), 3 -> 256 -> 256 -> 256 -> 3 with softplus
+ * (torch's threshold 20), the widened ODEFunc of neuralode_fast.py:7-33,
+ * and the 3/8-rule step of odeint(method="rk4") (neuralode_fast.py:55-60):
+ * y1 = y0 + h (k1 + 3 (k2 + k3) + k4) / 8.  Replaces the torch evaluation of
+ * a forward step (meshode_amd/neuralode.py's rk4_step).
+ * mo_flow256_pack: the module's tensors (W0 [256,3], b0 [256], W1 [256,256],
+ * b1, W2 [256,256], b2, W3 [3,256], b3 [3]; FP32 device, row-major) into
+ * `packed` (mo_flow256_packed_floats() floats, device, 16-byte aligned).
+ * mo_flow256_step: y0, y1 [n,3] FP32 device (distinct buffers). */
+int64_t mo_flow256_packed_floats(void);
+int mo_flow256_pack(const float* W0, const float* b0, const float* W1, const float* b1, const float* W2,
+                    const float* b2, const float* W3, const float* b3, float* packed, void* stream);
+int mo_flow256_step(const float* y0, int64_t n, const float* packed, float h, float* y1, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

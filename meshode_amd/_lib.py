@@ -27,6 +27,7 @@ SYMBOLS = [
     "mo_apply_transform_f64", "mo_linear_estimation", "mo_deform_subdivision",
     "mo_linear_estimation_rot", "mo_nearest_f32", "mo_deform_rot", "mo_deform_rot_batch",
     "mo_ode_rk4_workspace_bytes", "mo_ode_rk4_forward", "mo_ode_rk4_backward",
+    "mo_flow256_packed_floats", "mo_flow256_pack", "mo_flow256_step",
 ]
 
 
@@ -118,6 +119,9 @@ _SIGS = {
     "mo_linear_estimation_rot": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _f64, C.POINTER(_i32), _vp]),
     "mo_nearest_f32": (C.c_int, [_vp, _i64, _vp, _i64, _vp, _vp, _vp]),
     "mo_ode_rk4_workspace_bytes": (_i64, [_i64]),
+    "mo_flow256_packed_floats": (_i64, []),
+    "mo_flow256_pack": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mo_flow256_step": (C.c_int, [_vp, _i64, _vp, C.c_float, _vp, _vp]),
     "mo_ode_rk4_forward": (C.c_int, [_vp, _i64, _vp, _vp, C.c_float, C.c_float, _vp, _vp, _vp]),
     "mo_ode_rk4_backward": (C.c_int, [_vp, _i64, _vp, _vp, C.c_float, C.c_float, _vp, _vp, _vp, _vp, _vp, _i64,
                                       _vp]),

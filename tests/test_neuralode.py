@@ -132,3 +132,49 @@ def test_gpu_flow_and_loss_against_fp64(gpu, oracle):
     for p32, p64 in zip(model.field.parameters(), m64.parameters()):
         a, b = p32.grad.double().cpu(), p64.grad
         assert torch.linalg.norm(a - b) <= 2e-3 * torch.linalg.norm(b) + 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 64, 100, 55000])
+def test_gpu_fused_flow256_step_against_fp64(gpu, n):
+    """mo_flow256_step (the four RK4 stages of the 256-wide field on FP32
+    MFMA) against the FP64 torch step with the same weights: y1 within 1e-5
+    of the FP64 step's increment scale, or within 1.5 times the float32 torch
+    step's own error where that is larger; points inside and outside the
+    unit box."""
+    print(f"[ode] fused flow256 step n={n}", flush=True)
+    torch.manual_seed(n)
+    m = NO.VelocityMLP(hidden=256, seed=3).to(gpu)
+    with torch.no_grad():                      # non-zero biases exercise the epilogue
+        for b in m.biases:
+            b.normal_(0, 0.3)
+    y = (torch.rand(n, 3) * 1.6 - 0.3).to(gpu)
+    h = 1.0 / 16
+    got = NO.flow256_step(NO.flow256_pack(m), y, h)
+    m64 = NO.VelocityMLP(hidden=256, seed=3, dtype=torch.float64)
+    m64.load_state_dict({k: v.double().cpu() for k, v in m.state_dict().items()})
+    with torch.no_grad():
+        want = NO.rk4_step(m64, y.double().cpu(), h)
+        f32 = NO.rk4_step(m.cpu(), y.cpu(), h).double()
+    m.to(gpu)
+    scale = float((want - y.double().cpu()).abs().max())
+    err = float((got.double().cpu() - want).abs().max())
+    ref = float((f32 - want).abs().max())
+    print(f"[ode]   max |diff| {err:.3e} (increment scale {scale:.3e}; torch f32 {ref:.3e})", flush=True)
+    assert err <= max(1e-5 * scale, 1.5 * ref)
+
+
+@pytest.mark.gpu
+def test_gpu_odeint_uses_fused_step(gpu, monkeypatch):
+    """odeint_rk4's forward takes the fused step for a 256-wide FP32 field on
+    the GPU (one kernel per step) and the torch step otherwise."""
+    m = NO.VelocityMLP(hidden=256, seed=1).to(gpu)
+    calls = []
+    orig = NO.flow256_step
+    monkeypatch.setattr(NO, "flow256_step", lambda *a: calls.append(1) or orig(*a))
+    y = torch.rand(300, 3, device=gpu)
+    out = NO.odeint_rk4(m, y, steps=4)
+    assert len(calls) == 4 and out.shape == y.shape
+    small = NO.VelocityMLP(hidden=8, seed=1).to(gpu)
+    NO.odeint_rk4(small, y, steps=4)
+    assert len(calls) == 4
