@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--kernel-variant", type=int, default=0, help="0: register/LDS kernel when a pair fits; 1: global")
     ap.add_argument("--pcg-tol", type=float, default=None, help="inner PCG relative tolerance (library default if unset)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "neural2", "rot", "cfg1"],
+    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "flow256", "neural2", "rot", "cfg1"],
                     help="batch: the headline line; serve / adam / ode: the serving, pyDeform-Adam and "
                          "NeuralODE workloads (their own JSON line)")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of Poisson arrivals")
@@ -122,7 +122,7 @@ def main():
         torch.cuda.synchronize()
         del a, b
     if args.mode != "batch":
-        print(json.dumps({"serve": bench_serve, "adam": bench_adam, "ode": bench_ode,
+        print(json.dumps({"serve": bench_serve, "adam": bench_adam, "ode": bench_ode, "flow256": bench_flow256,
                           "neural2": bench_neural2, "rot": bench_rot, "cfg1": bench_cfg1}[args.mode](args)),
               flush=True)
         return
@@ -414,6 +414,47 @@ def bench_ode(args):
             "approx_mlp_tflops": flops * steps / dt / 1e12,
             "config": {"workload": "config 3: velocity MLP 3-256-256-256-3 softplus, RK4x16, adjoint", "points":
                        int(P.shape[0]), "grid": 128}}
+
+
+def bench_flow256(args):
+    """Config 3's forward integration alone: 16 fused RK4 steps
+    (mo_flow256_step, FP32 MFMA) of the 3-256-256-256-3 softplus field over
+    55k points; kernel time from HIP events on the launch stream, MFMA
+    roofline at the FP32 dense peak."""
+    import torch
+    from meshode_amd import neuralode
+    n, steps = 55000, 16
+    torch.manual_seed(0)
+    y = torch.rand(n, 3, device="cuda")
+    field = neuralode.VelocityMLP(256, seed=0).cuda()
+    packed = neuralode.flow256_pack(field)
+    for _ in range(2):
+        yy = y
+        for _ in range(steps):
+            yy = neuralode.flow256_step(packed, yy, 1.0 / steps)
+    torch.cuda.synchronize()
+    reps = max(args.steps, 1) * 5
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(reps):
+        yy = y
+        for _ in range(steps):
+            yy = neuralode.flow256_step(packed, yy, 1.0 / steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = e0.elapsed_time(e1) / (reps * steps)
+    flops = 2.0 * n * (3 * 256 + 2 * 256 * 256 + 256 * 3) * 4      # four field evaluations per step
+    tflops = flops / (ms_step * 1e-3) / 1e12
+    return {"metric": "config-3 forward RK4 steps/sec (55k points, fused FP32 MFMA step)",
+            "value": reps * steps / wall, "unit": "steps/s", "steps": reps * steps, "ms_per_step": ms_step,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "roofline": {"bound": "mfma", "achieved": tflops, "peak": 157.3, "unit": "TFLOP/s",
+                         "frac": tflops / 157.3, "traffic": None},
+            "config": {"workload": "velocity MLP 3-256-256-256-3 softplus, one RK4 step per launch", "points": n,
+                       "flops_per_step": flops}}
 
 
 def bench_rot(args):

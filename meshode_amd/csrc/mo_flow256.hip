@@ -56,16 +56,13 @@ __device__ __forceinline__ int tidx(int u, int p) {
   return kk * (kPts * 2) + 2 * (p ^ (kk & 31)) + (u & 1);
 }
 
-// softplus with torch's threshold: x > 20 ? x : log1p(exp(x)).  exp and log
-// from the hardware's exp2 / log2; log1p(e) = log(u) e / (u - 1), u = 1 + e
-// (Kahan's correction of the rounding of 1 + e).
+// softplus = max(x, 0) + log1p(exp(-|x|)), from the hardware's exp2 and
+// log2: the rounding of 1 + exp(-|x|) leaves an absolute error below 1e-7
+// (torch computes x > 20 ? x : log1p(exp(x)); beyond 20 both give x to
+// within exp(-20)).
 __device__ __forceinline__ float softplus(float x) {
-  if (x > 20.f) return x;
-  const float e = __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
-  const float u = 1.f + e;
-  const float d = u - 1.f;
-  const float lg = __builtin_amdgcn_logf(u) * 0.6931471805599453f;
-  return d == 0.f ? e : (d == e ? lg : lg * e * __builtin_amdgcn_rcpf(d));
+  const float e = __builtin_amdgcn_exp2f(-fabsf(x) * 1.4426950408889634f);
+  return fmaxf(x, 0.f) + __builtin_amdgcn_logf(1.f + e) * 0.6931471805599453f;
 }
 
 // Packs the module's tensors: fragment (cb, g, l, i) of a 256 x 256 layer =
@@ -97,6 +94,7 @@ __global__ void k_flow256_pack(const float* __restrict__ W0, const float* __rest
 
 // Tout = softplus(Tin W^T + b) for this wave's 32 units [32 w, 32 w + 32)
 // and all 64 points (two 32 x 32 MFMA blocks, 128 steps of k = 2)
+template <bool kTwo>
 __device__ __forceinline__ void hidden_layer(const float* __restrict__ Wf, const float* __restrict__ bias,
                                              const float* __restrict__ Tin, float* __restrict__ Tout, int w,
                                              int l) {
@@ -114,13 +112,13 @@ __device__ __forceinline__ void hidden_layer(const float* __restrict__ Wf, const
       const int kk = 4 * g + i;
       const int row = kk * (kPts * 2) + hi, sw = kk & 31;
       a0[i] = Tin[row + 2 * (pl ^ sw)];
-      a1[i] = Tin[row + 2 * ((32 + pl) ^ sw)];
+      a1[i] = kTwo ? Tin[row + 2 * ((32 + pl) ^ sw)] : 0.f;
     }
     const float bb[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[i], bb[i], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[i], bb[i], acc1, 0, 0, 0);
+      if (kTwo) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[i], bb[i], acc1, 0, 0, 0);
     }
   }
   // D layout: unit 32 w + l % 32, point 32 pb + 8 (r / 4) + 4 (l / 32) + r % 4
@@ -130,10 +128,13 @@ __device__ __forceinline__ void hidden_layer(const float* __restrict__ Wf, const
   for (int r = 0; r < 16; ++r) {
     const int p = 8 * (r >> 2) + 4 * hi + (r & 3);
     Tout[tidx(u, p)] = softplus(acc0[r] + bu);
-    Tout[tidx(u, 32 + p)] = softplus(acc1[r] + bu);
+    if (kTwo) Tout[tidx(u, 32 + p)] = softplus(acc1[r] + bu);
   }
 }
 
+// Workgroup b of G takes points [n b / G, n (b + 1) / G) in chunks of 64
+// (G = one per CU: the shares differ by at most one point, and a chunk of
+// at most 32 points skips the second 32-point MFMA block).
 __global__ __launch_bounds__(kThreads) void k_flow256_step(const float* __restrict__ y0, int n,
                                                            const float* __restrict__ P, float h,
                                                            float* __restrict__ y1) {
@@ -143,67 +144,86 @@ __global__ __launch_bounds__(kThreads) void k_flow256_step(const float* __restri
   __shared__ float part[kWaves][kPts][3];
   const int tid = threadIdx.x, l = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int base = blockIdx.x * kPts;
-  if (tid < kPts) {
-    const int i = base + tid;
-    for (int c = 0; c < 3; ++c) Y[tid][c] = i < n ? y0[3 * (size_t)i + c] : 0.5f;
-  }
-  __syncthreads();
+  const int p0 = (int)((long long)n * blockIdx.x / gridDim.x);
+  const int p1 = (int)((long long)n * (blockIdx.x + 1) / gridDim.x);
 #pragma unroll 1
-  for (int s = 0; s < 4; ++s) {
+  for (int base = p0; base < p1; base += kPts) {
+    const int nvalid = min(kPts, p1 - base);
     if (tid < kPts) {
-      for (int c = 0; c < 3; ++c) {
-        const float y = Y[tid][c];
-        Z[tid][c] = s == 0 ? y
-                  : s == 1 ? y + h * K[0][tid][c] / 3.0f
-                  : s == 2 ? y + h * (K[1][tid][c] - K[0][tid][c] / 3.0f)
-                           : y + h * (K[0][tid][c] - K[1][tid][c] + K[2][tid][c]);
-      }
+      const int i = base + tid;
+      for (int c = 0; c < 3; ++c) Y[tid][c] = tid < nvalid ? y0[3 * (size_t)i + c] : 0.5f;
     }
     __syncthreads();
-    // layer 0 (VALU): lane l = point, wave w computes units w + 8 m
-    {
-      const float x0 = Z[l][0] - 0.5f, x1 = Z[l][1] - 0.5f, x2 = Z[l][2] - 0.5f;
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+      if (tid < kPts) {
+        for (int c = 0; c < 3; ++c) {
+          const float y = Y[tid][c];
+          Z[tid][c] = s == 0 ? y
+                    : s == 1 ? y + h * K[0][tid][c] / 3.0f
+                    : s == 2 ? y + h * (K[1][tid][c] - K[0][tid][c] / 3.0f)
+                             : y + h * (K[0][tid][c] - K[1][tid][c] + K[2][tid][c]);
+        }
+      }
+      __syncthreads();
+      // layer 0 (VALU): lane l = point, wave w computes units w + 8 m
+      {
+        const float x0 = Z[l][0] - 0.5f, x1 = Z[l][1] - 0.5f, x2 = Z[l][2] - 0.5f;
 #pragma unroll 4
-      for (int u = w; u < kH; u += kWaves) {
-        const float4 wr = *reinterpret_cast<const float4*>(P + kPW0 + 4 * u);
-        TA[tidx(u, l)] = softplus(P[kPB0 + u] + wr.x * x0 + wr.y * x1 + wr.z * x2);
+        for (int u = w; u < kH; u += kWaves) {
+          const float4 wr = *reinterpret_cast<const float4*>(P + kPW0 + 4 * u);
+          TA[tidx(u, l)] = softplus(P[kPB0 + u] + wr.x * x0 + wr.y * x1 + wr.z * x2);
+        }
       }
-    }
-    __syncthreads();
-    hidden_layer(P + kPW1, P + kPB1, TA, TB, w, l);
-    __syncthreads();
-    hidden_layer(P + kPW2, P + kPB2, TB, TA, w, l);
-    __syncthreads();
-    // layer 3 (VALU): wave w sums units [32 w, 32 w + 32) for point l
-    {
-      float o0 = 0.f, o1 = 0.f, o2 = 0.f;
+      __syncthreads();
+      if (nvalid > 32) hidden_layer<true>(P + kPW1, P + kPB1, TA, TB, w, l);
+      else hidden_layer<false>(P + kPW1, P + kPB1, TA, TB, w, l);
+      __syncthreads();
+      if (nvalid > 32) hidden_layer<true>(P + kPW2, P + kPB2, TB, TA, w, l);
+      else hidden_layer<false>(P + kPW2, P + kPB2, TB, TA, w, l);
+      __syncthreads();
+      // layer 3 (VALU): wave w sums units [32 w, 32 w + 32) for point l
+      {
+        float o0 = 0.f, o1 = 0.f, o2 = 0.f;
 #pragma unroll 4
-      for (int u = 32 * w; u < 32 * w + 32; ++u) {
-        const float hv = TA[tidx(u, l)];
-        o0 += P[kPW3 + u] * hv;
-        o1 += P[kPW3 + kH + u] * hv;
-        o2 += P[kPW3 + 2 * kH + u] * hv;
+        for (int u = 32 * w; u < 32 * w + 32; ++u) {
+          const float hv = TA[tidx(u, l)];
+          o0 += P[kPW3 + u] * hv;
+          o1 += P[kPW3 + kH + u] * hv;
+          o2 += P[kPW3 + 2 * kH + u] * hv;
+        }
+        part[w][l][0] = o0;
+        part[w][l][1] = o1;
+        part[w][l][2] = o2;
       }
-      part[w][l][0] = o0;
-      part[w][l][1] = o1;
-      part[w][l][2] = o2;
+      __syncthreads();
+      if (tid < kPts)
+        for (int c = 0; c < 3; ++c) {
+          float o = 0.f;
+          for (int q = 0; q < kWaves; ++q) o += part[q][tid][c];
+          K[s][tid][c] = P[kPB3 + c] + o;
+        }
+      __syncthreads();
     }
-    __syncthreads();
-    if (tid < kPts)
-      for (int c = 0; c < 3; ++c) {
-        float o = 0.f;
-        for (int q = 0; q < kWaves; ++q) o += part[q][tid][c];
-        K[s][tid][c] = P[kPB3 + c] + o;
-      }
+    if (tid < nvalid) {
+      const int i = base + tid;
+      for (int c = 0; c < 3; ++c)
+        y1[3 * (size_t)i + c] =
+            Y[tid][c] + h * (K[0][tid][c] + 3.0f * (K[1][tid][c] + K[2][tid][c]) + K[3][tid][c]) * 0.125f;
+    }
     __syncthreads();
   }
-  if (tid < kPts && base + tid < n) {
-    const int i = base + tid;
-    for (int c = 0; c < 3; ++c)
-      y1[3 * (size_t)i + c] =
-          Y[tid][c] + h * (K[0][tid][c] + 3.0f * (K[1][tid][c] + K[2][tid][c]) + K[3][tid][c]) * 0.125f;
+}
+
+int cu_count() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess || cus <= 0)
+      cus = 256;
   }
+  return cus;
 }
 
 }  // namespace
@@ -226,8 +246,8 @@ MO_EXPORT int mo_flow256_step(const float* y0, int64_t n, const float* packed, f
   MO_CHECK(packed && n >= 0 && n <= (1LL << 30) && (n == 0 || (y0 && y1)), MO_ERR_ARG,
            "mo_flow256_step: bad arguments");
   if (n == 0) return MO_OK;
-  hipLaunchKernelGGL(k_flow256_step, dim3((unsigned)((n + kPts - 1) / kPts)), dim3(kThreads), 0, (hipStream_t)stream,
-                     y0, (int)n, packed, h, y1);
+  const int grid = (int)std::min<int64_t>((n + kPts - 1) / kPts, cu_count());
+  hipLaunchKernelGGL(k_flow256_step, dim3(grid), dim3(kThreads), 0, (hipStream_t)stream, y0, (int)n, packed, h, y1);
   MO_HIP(hipGetLastError());
   return MO_OK;
 }
