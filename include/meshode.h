@@ -383,6 +383,26 @@ int64_t mo_flow256_packed_floats(void);
 int mo_flow256_pack(const float* W0, const float* b0, const float* W1, const float* b1, const float* W2,
                     const float* b2, const float* W3, const float* b3, float* packed, void* stream);
 int mo_flow256_step(const float* y0, int64_t n, const float* packed, float h, float* y1, void* stream);
+/* The adjoint of a step (the reference differentiates odeint by autograd).
+ * mo_flow256_step_save: mo_flow256_step that also writes the stage inputs
+ * x = z - 0.5 to Xs [4][n][3] and the activations H1, H2, H3 to
+ * Hs [3][4][n][256] (layer, stage, point).  mo_flow256_step_backward: from
+ * those and gy1 = dL/dy1 writes gy0 = dL/dy0, the pre-activation gradients
+ * dA1, dA2, dA3 to dAs [3][4][n][256] and dL/dk_s to Gs [4][n][3]; the
+ * parameter gradients are then dW0 = dA1^T X, dW1 = dA2^T H1,
+ * dW2 = dA3^T H2, dW3 = G^T H3 and the column sums of dA1, dA2, dA3, G for
+ * the biases (rows = the 4n stage points).  Hs and dAs 16-byte aligned. */
+int mo_flow256_step_save(const float* y0, int64_t n, const float* packed, float h, float* y1, float* Hs, float* Xs,
+                         void* stream);
+int mo_flow256_step_backward(int64_t n, const float* packed, float h, const float* Hs, const float* gy1, float* gy0,
+                             float* dAs, float* Gs, void* stream);
+/* The small parameter gradients of a step from mo_flow256_step_backward's
+ * outputs: out [mo_flow256_small_grad_floats() = 2307] = dW0 [256][3], db0,
+ * db1, db2 [256 each], dW3 [3][256], db3 [3]; work: 256 x 2307 floats
+ * (device).  Sums in a fixed order (FP64 across workgroups). */
+int64_t mo_flow256_small_grad_floats(void);
+int mo_flow256_small_grads(int64_t n, const float* dAs, const float* Hs, const float* Xs, const float* Gs, float* out,
+                           float* work, void* stream);
 
 #ifdef __cplusplus
 }

@@ -27,7 +27,8 @@ SYMBOLS = [
     "mo_apply_transform_f64", "mo_linear_estimation", "mo_deform_subdivision",
     "mo_linear_estimation_rot", "mo_nearest_f32", "mo_deform_rot", "mo_deform_rot_batch",
     "mo_ode_rk4_workspace_bytes", "mo_ode_rk4_forward", "mo_ode_rk4_backward",
-    "mo_flow256_packed_floats", "mo_flow256_pack", "mo_flow256_step",
+    "mo_flow256_packed_floats", "mo_flow256_pack", "mo_flow256_step", "mo_flow256_step_save",
+    "mo_flow256_step_backward", "mo_flow256_small_grad_floats", "mo_flow256_small_grads",
 ]
 
 
@@ -122,6 +123,10 @@ _SIGS = {
     "mo_flow256_packed_floats": (_i64, []),
     "mo_flow256_pack": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mo_flow256_step": (C.c_int, [_vp, _i64, _vp, C.c_float, _vp, _vp]),
+    "mo_flow256_step_save": (C.c_int, [_vp, _i64, _vp, C.c_float, _vp, _vp, _vp, _vp]),
+    "mo_flow256_step_backward": (C.c_int, [_i64, _vp, C.c_float, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mo_flow256_small_grad_floats": (_i64, []),
+    "mo_flow256_small_grads": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mo_ode_rk4_forward": (C.c_int, [_vp, _i64, _vp, _vp, C.c_float, C.c_float, _vp, _vp, _vp]),
     "mo_ode_rk4_backward": (C.c_int, [_vp, _i64, _vp, _vp, C.c_float, C.c_float, _vp, _vp, _vp, _vp, _vp, _i64,
                                       _vp]),

@@ -44,7 +44,9 @@ constexpr int kPW2 = kPB1 + kH;
 constexpr int kPB2 = kPW2 + kH * kH;
 constexpr int kPW3 = kPB2 + kH;         // W3 [3][256]
 constexpr int kPB3 = kPW3 + 3 * kH;     // b3 [3] (+1 pad)
-constexpr int kPacked = kPB3 + 4;
+constexpr int kPW1T = kPB3 + 4;         // W1^T fragments (backward: dH1 = dA2 W1)
+constexpr int kPW2T = kPW1T + kH * kH;  // W2^T fragments (backward: dH2 = dA3 W2)
+constexpr int kPacked = kPW2T + kH * kH;
 
 // Activation tile: [128 unit pairs][64 points][2], the point index XOR-ed
 // with the unit pair (mod 32) so that the epilogue's writes (32 units, 2
@@ -66,7 +68,8 @@ __device__ __forceinline__ float softplus(float x) {
 }
 
 // Packs the module's tensors: fragment (cb, g, l, i) of a 256 x 256 layer =
-// W[32 cb + l % 32][8 g + 2 i + l / 32].
+// W[32 cb + l % 32][8 g + 2 i + l / 32] (the B operand of out = in W^T), and
+// of its transpose W[8 g + 2 i + l / 32][32 cb + l % 32] (of d_in = d_out W).
 __global__ void k_flow256_pack(const float* __restrict__ W0, const float* __restrict__ b0,
                                const float* __restrict__ W1, const float* __restrict__ b1,
                                const float* __restrict__ W2, const float* __restrict__ b2,
@@ -87,18 +90,22 @@ __global__ void k_flow256_pack(const float* __restrict__ W0, const float* __rest
     const int row = 32 * cb + (l & 31), col = 8 * g + 2 * i + (l >> 5);
     P[kPW1 + e] = W1[row * kH + col];
     P[kPW2 + e] = W2[row * kH + col];
+    P[kPW1T + e] = W1[col * kH + row];
+    P[kPW2T + e] = W2[col * kH + row];
   }
   if (e < 3 * kH) P[kPW3 + e] = W3[e];
   if (e < 4) P[kPB3 + e] = e < 3 ? b3[e] : 0.f;
 }
 
-// Tout = softplus(Tin W^T + b) for this wave's 32 units [32 w, 32 w + 32)
-// and all 64 points (two 32 x 32 MFMA blocks, 128 steps of k = 2)
+// acc = Tin W^T for this wave's 32 units [32 w, 32 w + 32) and the 64 (or,
+// without kTwo, the first 32) points: 32 x 32 MFMA blocks, 128 steps of
+// k = 2.  Result layout: unit 32 w + l % 32, point 32 pb + 8 (r / 4) +
+// 4 (l / 32) + r % 4.
 template <bool kTwo>
-__device__ __forceinline__ void hidden_layer(const float* __restrict__ Wf, const float* __restrict__ bias,
-                                             const float* __restrict__ Tin, float* __restrict__ Tout, int w,
-                                             int l) {
-  v16f acc0 = v16f{0}, acc1 = v16f{0};
+__device__ __forceinline__ void mfma_layer(const float* __restrict__ Wf, const float* __restrict__ Tin, int w, int l,
+                                           v16f& acc0, v16f& acc1) {
+  acc0 = v16f{0};
+  acc1 = v16f{0};
   const float4* B = reinterpret_cast<const float4*>(Wf) + (size_t)w * 32 * 64 + l;
   const int hi = l >> 5, pl = l & 31;
   float4 nb[2] = {B[0], B[64]};
@@ -121,8 +128,16 @@ __device__ __forceinline__ void hidden_layer(const float* __restrict__ Wf, const
       if (kTwo) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[i], bb[i], acc1, 0, 0, 0);
     }
   }
-  // D layout: unit 32 w + l % 32, point 32 pb + 8 (r / 4) + 4 (l / 32) + r % 4
-  const int u = 32 * w + pl;
+}
+
+// Tout = softplus(Tin W^T + b) (forward hidden layer)
+template <bool kTwo>
+__device__ __forceinline__ void hidden_layer(const float* __restrict__ Wf, const float* __restrict__ bias,
+                                             const float* __restrict__ Tin, float* __restrict__ Tout, int w,
+                                             int l) {
+  v16f acc0, acc1;
+  mfma_layer<kTwo>(Wf, Tin, w, l, acc0, acc1);
+  const int u = 32 * w + (l & 31), hi = l >> 5;
   const float bu = bias[u];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -132,12 +147,56 @@ __device__ __forceinline__ void hidden_layer(const float* __restrict__ Wf, const
   }
 }
 
+// softplus'(a) from h = softplus(a): sigmoid(a) = 1 - exp(-h)
+__device__ __forceinline__ float dsoftplus_h(float h) { return 1.f - __builtin_amdgcn_exp2f(-h * 1.4426950408889634f); }
+
+// Tout = (Tin W) * softplus'(A) (backward hidden layer; Wf = W^T fragments):
+// H = the layer's saved activations, rows of 256 (the chunk's first row)
+template <bool kTwo>
+__device__ __forceinline__ void back_layer(const float* __restrict__ WTf, const float* __restrict__ H, int nvalid,
+                                           const float* __restrict__ Tin, float* __restrict__ Tout, int w, int l) {
+  v16f acc0, acc1;
+  mfma_layer<kTwo>(WTf, Tin, w, l, acc0, acc1);
+  const int j = 32 * w + (l & 31), hi = l >> 5;
+  // one 32-point block at a time (the compiler barrier keeps the second
+  // block's loads of H from being hoisted next to the first's)
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int p = 8 * (r >> 2) + 4 * hi + (r & 3);
+    Tout[tidx(j, p)] = p < nvalid ? acc0[r] * dsoftplus_h(H[(size_t)p * kH + j]) : 0.f;
+  }
+  if (kTwo) {
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int p = 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+      Tout[tidx(j, p)] = p < nvalid ? acc1[r] * dsoftplus_h(H[(size_t)p * kH + j]) : 0.f;
+    }
+  }
+}
+
+// dst rows [0, nvalid) of 256 floats <- the tile (coalesced float4 stores)
+__device__ __forceinline__ void store_tile(const float* __restrict__ T, float* __restrict__ dst, int nvalid) {
+  const int p = threadIdx.x >> 3, part = threadIdx.x & 7;
+  if (p >= nvalid) return;
+  float4* d = reinterpret_cast<float4*>(dst + (size_t)p * kH + 32 * part);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int u = 32 * part + 4 * q;
+    d[q] = make_float4(T[tidx(u, p)], T[tidx(u + 1, p)], T[tidx(u + 2, p)], T[tidx(u + 3, p)]);
+  }
+}
+
 // Workgroup b of G takes points [n b / G, n (b + 1) / G) in chunks of 64
 // (G = one per CU: the shares differ by at most one point, and a chunk of
-// at most 32 points skips the second 32-point MFMA block).
+// at most 32 points skips the second 32-point MFMA block).  With kSave the
+// stage inputs x = z - 0.5 go to Xs [4][n][3] and the activations H1, H2, H3
+// to Hs [3][4][n][256] (layer, stage, point) for the backward.
+template <bool kSave>
 __global__ __launch_bounds__(kThreads) void k_flow256_step(const float* __restrict__ y0, int n,
                                                            const float* __restrict__ P, float h,
-                                                           float* __restrict__ y1) {
+                                                           float* __restrict__ y1, float* __restrict__ Hs,
+                                                           float* __restrict__ Xs) {
   __shared__ __align__(16) float TA[kTile];
   __shared__ __align__(16) float TB[kTile];
   __shared__ float Y[kPts][3], Z[kPts][3], K[4][kPts][3];
@@ -146,6 +205,7 @@ __global__ __launch_bounds__(kThreads) void k_flow256_step(const float* __restri
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int p0 = (int)((long long)n * blockIdx.x / gridDim.x);
   const int p1 = (int)((long long)n * (blockIdx.x + 1) / gridDim.x);
+  const size_t plane = (size_t)n * kH;    // one (layer, stage) plane of Hs
 #pragma unroll 1
   for (int base = p0; base < p1; base += kPts) {
     const int nvalid = min(kPts, p1 - base);
@@ -163,6 +223,7 @@ __global__ __launch_bounds__(kThreads) void k_flow256_step(const float* __restri
                     : s == 1 ? y + h * K[0][tid][c] / 3.0f
                     : s == 2 ? y + h * (K[1][tid][c] - K[0][tid][c] / 3.0f)
                              : y + h * (K[0][tid][c] - K[1][tid][c] + K[2][tid][c]);
+          if (kSave && tid < nvalid) Xs[((size_t)s * n + base + tid) * 3 + c] = Z[tid][c] - 0.5f;
         }
       }
       __syncthreads();
@@ -176,12 +237,15 @@ __global__ __launch_bounds__(kThreads) void k_flow256_step(const float* __restri
         }
       }
       __syncthreads();
+      if (kSave) store_tile(TA, Hs + (0 * 4 + s) * plane + (size_t)base * kH, nvalid);
       if (nvalid > 32) hidden_layer<true>(P + kPW1, P + kPB1, TA, TB, w, l);
       else hidden_layer<false>(P + kPW1, P + kPB1, TA, TB, w, l);
       __syncthreads();
+      if (kSave) store_tile(TB, Hs + (1 * 4 + s) * plane + (size_t)base * kH, nvalid);
       if (nvalid > 32) hidden_layer<true>(P + kPW2, P + kPB2, TB, TA, w, l);
       else hidden_layer<false>(P + kPW2, P + kPB2, TB, TA, w, l);
       __syncthreads();
+      if (kSave) store_tile(TA, Hs + (2 * 4 + s) * plane + (size_t)base * kH, nvalid);
       // layer 3 (VALU): wave w sums units [32 w, 32 w + 32) for point l
       {
         float o0 = 0.f, o1 = 0.f, o2 = 0.f;
@@ -213,6 +277,178 @@ __global__ __launch_bounds__(kThreads) void k_flow256_step(const float* __restri
     }
     __syncthreads();
   }
+}
+
+// Backward of one step from the saved activations (Hs [3][4][n][256]):
+// gy1 = dL/dy1 -> gy0 = dL/dy0; the pre-activation gradients dA1..dA3 go
+// to dAs [3][4][n][256] (layer, stage, point) and the output gradients
+// dL/dk_s to Gs [4][n][3], from which the caller forms the parameter
+// gradients (sums over stages and points).  Per stage, last first:
+// dA3 = (W3^T g) * softplus'(A3) (VALU), dA2 = (dA3 W2) * softplus'(A2),
+// dA1 = (dA2 W1) * softplus'(A1) (MFMA), dz = W0^T dA1 (VALU), and the
+// RK4 chain: y1 = y + h (k1 + 3 k2 + 3 k3 + k4) / 8, z2 = y + h k1 / 3,
+// z3 = y + h (k2 - k1 / 3), z4 = y + h (k1 - k2 + k3).
+__global__ __launch_bounds__(kThreads) void k_flow256_step_bwd(int n, const float* __restrict__ P, float h,
+                                                               const float* __restrict__ Hs,
+                                                               const float* __restrict__ gy1,
+                                                               float* __restrict__ gy0, float* __restrict__ dAs,
+                                                               float* __restrict__ Gs) {
+  __shared__ __align__(16) float TA[kTile];
+  __shared__ __align__(16) float TB[kTile];
+  __shared__ float DK[4][kPts][3], DY[kPts][3];
+  __shared__ float part[kWaves][kPts][3];
+  const int tid = threadIdx.x, l = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int p0 = (int)((long long)n * blockIdx.x / gridDim.x);
+  const int p1 = (int)((long long)n * (blockIdx.x + 1) / gridDim.x);
+  const size_t plane = (size_t)n * kH;
+  const float third = h / 3.0f;
+#pragma unroll 1
+  for (int base = p0; base < p1; base += kPts) {
+    const int nvalid = min(kPts, p1 - base);
+    if (tid < kPts)
+      for (int c = 0; c < 3; ++c) {
+        const float g = tid < nvalid ? gy1[3 * (size_t)(base + tid) + c] : 0.f;
+        DY[tid][c] = g;
+        const float a = h * g * 0.125f;
+        DK[0][tid][c] = a;
+        DK[1][tid][c] = 3.0f * a;
+        DK[2][tid][c] = 3.0f * a;
+        DK[3][tid][c] = a;
+      }
+    __syncthreads();
+#pragma unroll 1
+    for (int s = 3; s >= 0; --s) {
+      const float* H1 = Hs + (0 * 4 + s) * plane + (size_t)base * kH;
+      const float* H2 = Hs + (1 * 4 + s) * plane + (size_t)base * kH;
+      const float* H3 = Hs + (2 * 4 + s) * plane + (size_t)base * kH;
+      if (tid < nvalid)
+        for (int c = 0; c < 3; ++c) Gs[((size_t)s * n + base + tid) * 3 + c] = DK[s][tid][c];
+      // dA3 (VALU): lane l = point, wave w = units [32 w, 32 w + 32)
+      {
+        const float g0 = DK[s][l][0], g1 = DK[s][l][1], g2 = DK[s][l][2];
+        const bool ok = l < nvalid;
+#pragma unroll 4
+        for (int q = 0; q < 8; ++q) {
+          const int u = 32 * w + 4 * q;
+          const float4 hv = ok ? *reinterpret_cast<const float4*>(H3 + (size_t)l * kH + u) : make_float4(0, 0, 0, 0);
+          const float hs[4] = {hv.x, hv.y, hv.z, hv.w};
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const float dh = P[kPW3 + u + e] * g0 + P[kPW3 + kH + u + e] * g1 + P[kPW3 + 2 * kH + u + e] * g2;
+            TA[tidx(u + e, l)] = ok ? dh * dsoftplus_h(hs[e]) : 0.f;
+          }
+        }
+      }
+      __syncthreads();
+      store_tile(TA, dAs + (2 * 4 + s) * plane + (size_t)base * kH, nvalid);
+      if (nvalid > 32) back_layer<true>(P + kPW2T, H2, nvalid, TA, TB, w, l);
+      else back_layer<false>(P + kPW2T, H2, nvalid, TA, TB, w, l);
+      __syncthreads();
+      store_tile(TB, dAs + (1 * 4 + s) * plane + (size_t)base * kH, nvalid);
+      if (nvalid > 32) back_layer<true>(P + kPW1T, H1, nvalid, TB, TA, w, l);
+      else back_layer<false>(P + kPW1T, H1, nvalid, TB, TA, w, l);
+      __syncthreads();
+      store_tile(TA, dAs + (0 * 4 + s) * plane + (size_t)base * kH, nvalid);
+      // dz = W0^T dA1 (VALU): wave w sums units [32 w, 32 w + 32) for point l
+      {
+        float o0 = 0.f, o1 = 0.f, o2 = 0.f;
+#pragma unroll 4
+        for (int u = 32 * w; u < 32 * w + 32; ++u) {
+          const float d = TA[tidx(u, l)];
+          const float4 wr = *reinterpret_cast<const float4*>(P + kPW0 + 4 * u);
+          o0 += wr.x * d;
+          o1 += wr.y * d;
+          o2 += wr.z * d;
+        }
+        part[w][l][0] = o0;
+        part[w][l][1] = o1;
+        part[w][l][2] = o2;
+      }
+      __syncthreads();
+      if (tid < kPts)
+        for (int c = 0; c < 3; ++c) {
+          float dz = 0.f;
+          for (int q = 0; q < kWaves; ++q) dz += part[q][tid][c];
+          DY[tid][c] += dz;
+          if (s == 3) {          // z4 = y + h (k1 - k2 + k3)
+            DK[0][tid][c] += h * dz;
+            DK[1][tid][c] -= h * dz;
+            DK[2][tid][c] += h * dz;
+          } else if (s == 2) {   // z3 = y + h (k2 - k1 / 3)
+            DK[1][tid][c] += h * dz;
+            DK[0][tid][c] -= third * dz;
+          } else if (s == 1) {   // z2 = y + h k1 / 3
+            DK[0][tid][c] += third * dz;
+          }
+        }
+      __syncthreads();
+    }
+    if (tid < nvalid)
+      for (int c = 0; c < 3; ++c) gy0[3 * (size_t)(base + tid) + c] = DY[tid][c];
+    __syncthreads();
+  }
+}
+
+// The small parameter gradients of a step from the saved tensors (rows =
+// the 4n stage points): dW0 = dA1^T X [256][3], db0, db1, db2 (column sums of
+// dA1, dA2, dA3), dW3 = G^T H3 [3][256], db3 (column sums of G).  Thread u of
+// workgroup b sums rows [R b / G, R (b + 1) / G) for unit u (coalesced rows
+// of dA1, dA2, dA3, H3) and writes the workgroup's partial row;
+// k_flow256_grad_reduce adds the rows in workgroup order (FP64).
+constexpr int kSmallGrad = 3 * kH + 3 * kH + 3 * kH + 3;   // 2307
+__global__ __launch_bounds__(kH) void k_flow256_grad_small(int rows, const float* __restrict__ dA1,
+                                                           const float* __restrict__ dA2,
+                                                           const float* __restrict__ dA3,
+                                                           const float* __restrict__ H3, const float* __restrict__ X,
+                                                           const float* __restrict__ G, float* __restrict__ partial) {
+  const int u = threadIdx.x;
+  const int r0 = (int)((long long)rows * blockIdx.x / gridDim.x);
+  const int r1 = (int)((long long)rows * (blockIdx.x + 1) / gridDim.x);
+  float w00 = 0.f, w01 = 0.f, w02 = 0.f, b0 = 0.f, b1 = 0.f, b2 = 0.f, w30 = 0.f, w31 = 0.f, w32 = 0.f;
+  float g0s = 0.f, g1s = 0.f, g2s = 0.f;
+#pragma unroll 4
+  for (int r = r0; r < r1; ++r) {
+    const size_t o = (size_t)r * kH + u;
+    const float a1 = dA1[o], a2 = dA2[o], a3 = dA3[o], h3 = H3[o];
+    const float x0 = X[3 * (size_t)r], x1 = X[3 * (size_t)r + 1], x2 = X[3 * (size_t)r + 2];
+    const float g0 = G[3 * (size_t)r], g1 = G[3 * (size_t)r + 1], g2 = G[3 * (size_t)r + 2];
+    w00 += a1 * x0;
+    w01 += a1 * x1;
+    w02 += a1 * x2;
+    b0 += a1;
+    b1 += a2;
+    b2 += a3;
+    w30 += g0 * h3;
+    w31 += g1 * h3;
+    w32 += g2 * h3;
+    g0s += g0;
+    g1s += g1;
+    g2s += g2;
+  }
+  float* row = partial + (size_t)blockIdx.x * kSmallGrad;
+  row[3 * u] = w00;
+  row[3 * u + 1] = w01;
+  row[3 * u + 2] = w02;
+  row[3 * kH + u] = b0;
+  row[4 * kH + u] = b1;
+  row[5 * kH + u] = b2;
+  row[6 * kH + u] = w30;
+  row[7 * kH + u] = w31;
+  row[8 * kH + u] = w32;
+  if (u == 0) {
+    row[9 * kH] = g0s;
+    row[9 * kH + 1] = g1s;
+    row[9 * kH + 2] = g2s;
+  }
+}
+
+__global__ void k_flow256_grad_reduce(const float* __restrict__ partial, int nblocks, float* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kSmallGrad) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += partial[(size_t)b * kSmallGrad + e];
+  out[e] = (float)s;
 }
 
 int cu_count() {
@@ -247,7 +483,53 @@ MO_EXPORT int mo_flow256_step(const float* y0, int64_t n, const float* packed, f
            "mo_flow256_step: bad arguments");
   if (n == 0) return MO_OK;
   const int grid = (int)std::min<int64_t>((n + kPts - 1) / kPts, cu_count());
-  hipLaunchKernelGGL(k_flow256_step, dim3(grid), dim3(kThreads), 0, (hipStream_t)stream, y0, (int)n, packed, h, y1);
+  hipLaunchKernelGGL(k_flow256_step<false>, dim3(grid), dim3(kThreads), 0, (hipStream_t)stream, y0, (int)n, packed, h,
+                     y1, nullptr, nullptr);
+  MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+MO_EXPORT int mo_flow256_step_save(const float* y0, int64_t n, const float* packed, float h, float* y1, float* Hs,
+                                   float* Xs, void* stream) {
+  MO_CHECK(packed && n >= 0 && n <= (1LL << 30) / 12 && (n == 0 || (y0 && y1 && Hs && Xs)), MO_ERR_ARG,
+           "mo_flow256_step_save: bad arguments");
+  MO_CHECK(((uintptr_t)Hs & 15) == 0, MO_ERR_ARG, "mo_flow256_step_save: Hs not 16-byte aligned");
+  if (n == 0) return MO_OK;
+  const int grid = (int)std::min<int64_t>((n + kPts - 1) / kPts, cu_count());
+  hipLaunchKernelGGL(k_flow256_step<true>, dim3(grid), dim3(kThreads), 0, (hipStream_t)stream, y0, (int)n, packed, h,
+                     y1, Hs, Xs);
+  MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+MO_EXPORT int mo_flow256_step_backward(int64_t n, const float* packed, float h, const float* Hs, const float* gy1,
+                                       float* gy0, float* dAs, float* Gs, void* stream) {
+  MO_CHECK(packed && n >= 0 && n <= (1LL << 30) / 12 && (n == 0 || (Hs && gy1 && gy0 && dAs && Gs)), MO_ERR_ARG,
+           "mo_flow256_step_backward: bad arguments");
+  MO_CHECK(((uintptr_t)Hs & 15) == 0 && ((uintptr_t)dAs & 15) == 0, MO_ERR_ARG,
+           "mo_flow256_step_backward: Hs / dAs not 16-byte aligned");
+  if (n == 0) return MO_OK;
+  const int grid = (int)std::min<int64_t>((n + kPts - 1) / kPts, cu_count());
+  hipLaunchKernelGGL(k_flow256_step_bwd, dim3(grid), dim3(kThreads), 0, (hipStream_t)stream, (int)n, packed, h, Hs,
+                     gy1, gy0, dAs, Gs);
+  MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+MO_EXPORT int64_t mo_flow256_small_grad_floats(void) { return kSmallGrad; }
+
+// out [2307]: dW0 [256][3], db0 [256], db1 [256], db2 [256], dW3 [3][256], db3 [3]
+MO_EXPORT int mo_flow256_small_grads(int64_t n, const float* dAs, const float* Hs, const float* Xs, const float* Gs,
+                                     float* out, float* work, void* stream) {
+  MO_CHECK(n > 0 && n <= (1LL << 30) / 12 && dAs && Hs && Xs && Gs && out && work, MO_ERR_ARG,
+           "mo_flow256_small_grads: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t plane4 = (size_t)4 * n * kH;          // one layer, all four stages
+  const int rows = (int)(4 * n);
+  const int nb = 256;
+  hipLaunchKernelGGL(k_flow256_grad_small, dim3(nb), dim3(kH), 0, s, rows, dAs, dAs + plane4, dAs + 2 * plane4,
+                     Hs + 2 * plane4, Xs, Gs, work);
+  hipLaunchKernelGGL(k_flow256_grad_reduce, dim3((kSmallGrad + 255) / 256), dim3(256), 0, s, work, nb, out);
   MO_HIP(hipGetLastError());
   return MO_OK;
 }

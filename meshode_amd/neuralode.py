@@ -14,7 +14,12 @@ cad_neural_deform2.py:40-108, in the configuration of the config-3 workload).
 * On the GPU, with a 256-wide field in FP32, each forward step of
   ``odeint_rk4`` is one fused HIP kernel (``mo_flow256_step``: the four
   stages, the 256 x 256 layers on FP32 MFMA, softplus in their epilogue);
-  the parameters are packed once per call (``mo_flow256_pack``).
+  the parameters are packed once per call (``mo_flow256_pack``).  The
+  adjoint's recomputation saves each step's activations
+  (``mo_flow256_step_save``), a second kernel runs the backward through the
+  four stages (``mo_flow256_step_backward``, the transposed 256 x 256
+  products on MFMA), and the parameter gradients are GEMMs over the saved
+  tensors.
 * ``RK4Adjoint``: the gradient of the discretized flow (discrete adjoint).
   The forward keeps only the state at each step boundary (one checkpoint
   per step); the backward recomputes one step at a time from its
@@ -110,13 +115,62 @@ class RK4Adjoint(Function):
             for _ in range(steps):
                 y = flow256_step(packed, y, h) if packed is not None else rk4_step(func, y, h)
                 ys.append(y)
-        ctx.func, ctx.steps, ctx.h = func, steps, h
+        ctx.func, ctx.steps, ctx.h, ctx.packed = func, steps, h, packed
         ctx.save_for_backward(*ys[:-1])
         ctx.n_params = len(params)
         return y
 
     @staticmethod
+    def _fused_backward(ctx, gy):
+        """Per step, last first: the fused step again, saving its stage
+        inputs and activations (mo_flow256_step_save), the fused backward
+        through its four stages (mo_flow256_step_backward: dL/dy and the
+        pre-activation gradients), and the parameter gradients as sums over
+        the step's 4 n stage points: a GEMM each for W1 and W2, one
+        reduction kernel for W0, W3 and the biases
+        (mo_flow256_small_grads)."""
+        import ctypes as C
+        from . import _lib
+        ys = ctx.saved_tensors
+        n, dev, h = ys[0].shape[0], ys[0].device, ctx.h
+        params = list(ctx.func.parameters())
+        gp = [torch.zeros_like(p) for p in params]
+        Hs = torch.empty(3 * 4 * n * 256, dtype=torch.float32, device=dev)
+        dAs = torch.empty_like(Hs)
+        Xs = torch.empty(4 * n * 3, dtype=torch.float32, device=dev)
+        Gs = torch.empty_like(Xs)
+        y1 = torch.empty(n, 3, dtype=torch.float32, device=dev)
+        nsmall = int(_lib.load().mo_flow256_small_grad_floats())
+        small = torch.empty(nsmall, dtype=torch.float32, device=dev)
+        work = torch.empty(256 * nsmall, dtype=torch.float32, device=dev)
+        H, dA = Hs.view(3, 4 * n, 256), dAs.view(3, 4 * n, 256)
+        X, G = Xs.view(4 * n, 3), Gs.view(4 * n, 3)
+        g = gy.contiguous().to(torch.float32)
+        ptr = lambda t: C.c_void_p(t.data_ptr())      # noqa: E731
+        for k in reversed(range(ctx.steps)):
+            _lib.call("mo_flow256_step_save", ptr(ys[k].contiguous()), n, ptr(ctx.packed), float(h), ptr(y1),
+                      ptr(Hs), ptr(Xs), pyDeform._stream())
+            gnew = torch.empty(n, 3, dtype=torch.float32, device=dev)
+            _lib.call("mo_flow256_step_backward", n, ptr(ctx.packed), float(h), ptr(Hs), ptr(g), ptr(gnew),
+                      ptr(dAs), ptr(Gs), pyDeform._stream())
+            # weights[0..3], biases[0..3]: W0 [256,3], W1, W2 [256,256], W3 [3,256]
+            gp[1].addmm_(dA[1].t(), H[0])
+            gp[2].addmm_(dA[2].t(), H[1])
+            _lib.call("mo_flow256_small_grads", n, ptr(dAs), ptr(Hs), ptr(Xs), ptr(Gs), ptr(small), ptr(work),
+                      pyDeform._stream())
+            gp[0].add_(small[0:768].view(256, 3))
+            gp[4].add_(small[768:1024])
+            gp[5].add_(small[1024:1280])
+            gp[6].add_(small[1280:1536])
+            gp[3].add_(small[1536:2304].view(3, 256))
+            gp[7].add_(small[2304:2307])
+            g = gnew
+        return (g, None, None, *gp)
+
+    @staticmethod
     def backward(ctx, gy):
+        if ctx.packed is not None:
+            return RK4Adjoint._fused_backward(ctx, gy)
         ys = ctx.saved_tensors
         params = [p for p in ctx.func.parameters()]
         gp = [torch.zeros_like(p) for p in params]

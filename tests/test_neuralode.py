@@ -178,3 +178,36 @@ def test_gpu_odeint_uses_fused_step(gpu, monkeypatch):
     small = NO.VelocityMLP(hidden=8, seed=1).to(gpu)
     NO.odeint_rk4(small, y, steps=4)
     assert len(calls) == 4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [40, 1000])
+def test_gpu_fused_adjoint_against_fp64(gpu, n):
+    """The fused adjoint (mo_flow256_step_save / _backward + GEMMs) of a
+    4-step integration against FP64 autograd through the same steps: the
+    parameter gradients and dL/dy0 within 1e-4 (relative, in norm); 40
+    points exercise the one-block (at most 32 points) path."""
+    print(f"[ode] fused adjoint n={n}", flush=True)
+    torch.manual_seed(n)
+    m = NO.VelocityMLP(hidden=256, seed=7).to(gpu)
+    with torch.no_grad():
+        for b in m.biases:
+            b.normal_(0, 0.2)
+    y0 = (torch.rand(n, 3) * 1.2 - 0.1).to(gpu).requires_grad_(True)
+    w = torch.randn(n, 3).to(gpu)
+    out = NO.odeint_rk4(m, y0, steps=4)
+    (out * w).sum().backward()
+    m64 = NO.VelocityMLP(hidden=256, seed=7, dtype=torch.float64)
+    m64.load_state_dict({k: v.double().cpu() for k, v in m.state_dict().items()})
+    y64 = y0.detach().double().cpu().requires_grad_(True)
+    y = y64
+    for _ in range(4):
+        y = NO.rk4_step(m64, y, 0.25)
+    (y * w.double().cpu()).sum().backward()
+    assert float((out.detach().double().cpu() - y.detach()).abs().max()) <= 1e-5 * float(y.detach().abs().max())
+    pairs = [(y0.grad, y64.grad, "dL/dy0")] + [(a.grad, b.grad, name) for (name, a), b in
+                                            zip(m.named_parameters(), m64.parameters())]
+    for got, want, name in pairs:
+        rel = float(torch.linalg.norm(got.double().cpu() - want) / torch.linalg.norm(want))
+        print(f"[ode]   {name}: relative error {rel:.2e}", flush=True)
+        assert rel <= 1e-4, name
