@@ -403,6 +403,11 @@ int mo_flow256_step_backward(int64_t n, const float* packed, float h, const floa
 int64_t mo_flow256_small_grad_floats(void);
 int mo_flow256_small_grads(int64_t n, const float* dAs, const float* Hs, const float* Xs, const float* Gs, float* out,
                            float* work, void* stream);
+/* dW1 = dA2^T H1 and dW2 = dA3^T H2 of a step (sums over its 4n stage
+ * points; split-K on MFMA, workgroup partials added in a fixed order):
+ * out [2][256][256]; work: mo_flow256_dw_work_floats() floats (device). */
+int64_t mo_flow256_dw_work_floats(void);
+int mo_flow256_dw_grads(int64_t n, const float* dAs, const float* Hs, float* out, float* work, void* stream);
 
 #ifdef __cplusplus
 }

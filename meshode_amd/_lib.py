@@ -29,6 +29,7 @@ SYMBOLS = [
     "mo_ode_rk4_workspace_bytes", "mo_ode_rk4_forward", "mo_ode_rk4_backward",
     "mo_flow256_packed_floats", "mo_flow256_pack", "mo_flow256_step", "mo_flow256_step_save",
     "mo_flow256_step_backward", "mo_flow256_small_grad_floats", "mo_flow256_small_grads",
+    "mo_flow256_dw_work_floats", "mo_flow256_dw_grads",
 ]
 
 
@@ -127,6 +128,8 @@ _SIGS = {
     "mo_flow256_step_backward": (C.c_int, [_i64, _vp, C.c_float, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mo_flow256_small_grad_floats": (_i64, []),
     "mo_flow256_small_grads": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mo_flow256_dw_work_floats": (_i64, []),
+    "mo_flow256_dw_grads": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "mo_ode_rk4_forward": (C.c_int, [_vp, _i64, _vp, _vp, C.c_float, C.c_float, _vp, _vp, _vp]),
     "mo_ode_rk4_backward": (C.c_int, [_vp, _i64, _vp, _vp, C.c_float, C.c_float, _vp, _vp, _vp, _vp, _vp, _i64,
                                       _vp]),

@@ -395,7 +395,7 @@ __global__ __launch_bounds__(kThreads) void k_flow256_step_bwd(int n, const floa
 // dA1, dA2, dA3), dW3 = G^T H3 [3][256], db3 (column sums of G).  Thread u of
 // workgroup b sums rows [R b / G, R (b + 1) / G) for unit u (coalesced rows
 // of dA1, dA2, dA3, H3) and writes the workgroup's partial row;
-// k_flow256_grad_reduce adds the rows in workgroup order (FP64).
+// k_flow256_rows_reduce adds the rows in a fixed order (FP64).
 constexpr int kSmallGrad = 3 * kH + 3 * kH + 3 * kH + 3;   // 2307
 __global__ __launch_bounds__(kH) void k_flow256_grad_small(int rows, const float* __restrict__ dA1,
                                                            const float* __restrict__ dA2,
@@ -443,12 +443,91 @@ __global__ __launch_bounds__(kH) void k_flow256_grad_small(int rows, const float
   }
 }
 
-__global__ void k_flow256_grad_reduce(const float* __restrict__ partial, int nblocks, float* __restrict__ out) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= kSmallGrad) return;
+// out[e] = sum over the nblocks rows (stride `len`) of partial[b][e], in a
+// fixed order (FP64): a block covers 64 elements, wave w adds rows w, w + 8,
+// ..., and the eight wave sums are added in wave order
+__global__ __launch_bounds__(512) void k_flow256_rows_reduce(const float* __restrict__ partial, int nblocks, int len,
+                                                             float* __restrict__ out) {
+  __shared__ double S[8][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + lane;
+  const float* p = partial + (size_t)blockIdx.y * nblocks * len;
   double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += partial[(size_t)b * kSmallGrad + e];
-  out[e] = (float)s;
+  if (e < len) {
+#pragma unroll 4
+    for (int b = w; b < nblocks; b += 8) s += p[(size_t)b * len + e];
+  }
+  S[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && e < len) {
+    double t = S[0][lane];
+    for (int k = 1; k < 8; ++k) t += S[k][lane];
+    out[(size_t)blockIdx.y * len + e] = (float)t;
+  }
+}
+
+// dW1 = dA2^T H1 and dW2 = dA3^T H2 over the 4n stage points of a step
+// (blockIdx.y picks the pair): split over the rows, workgroup b of G takes
+// rows [R b / G, R (b + 1) / G) in chunks of 32, staged through LDS (rows
+// padded to 288 floats: the two half-waves' reads fall in disjoint banks)
+// with the next chunk prefetched into registers.  Wave w accumulates
+// dW[32 w + i][j] for all 256 j: eight 32 x 32 MFMA blocks (A = dA rows
+// transposed, B = H rows, k = 2 rows per step).  Partials per workgroup;
+// k_flow256_rows_reduce adds them in a fixed order (FP64).
+constexpr int kDwRows = 32;
+constexpr int kDwLd = 288;
+__global__ __launch_bounds__(kThreads) void k_flow256_dw(int rows, const float* __restrict__ dA2,
+                                                         const float* __restrict__ H1, const float* __restrict__ dA3,
+                                                         const float* __restrict__ H2, float* __restrict__ partial) {
+  __shared__ __align__(16) float SD[kDwRows * kDwLd];
+  __shared__ __align__(16) float SH[kDwRows * kDwLd];
+  const float* D = blockIdx.y ? dA3 : dA2;
+  const float* Hm = blockIdx.y ? H2 : H1;
+  const int tid = threadIdx.x, l = tid & 63, hi = l >> 5, pl = l & 31;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int r0 = (int)((long long)rows * blockIdx.x / gridDim.x);
+  const int r1 = (int)((long long)rows * (blockIdx.x + 1) / gridDim.x);
+  v16f acc[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) acc[c] = v16f{0};
+  float4 pd[4], ph[4];
+  auto fetch = [&](int base) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = tid + kThreads * q, row = idx >> 6, c4 = idx & 63;
+      const bool ok = base + row < r1;
+      pd[q] = ok ? *reinterpret_cast<const float4*>(D + (size_t)(base + row) * kH + 4 * c4) : make_float4(0, 0, 0, 0);
+      ph[q] = ok ? *reinterpret_cast<const float4*>(Hm + (size_t)(base + row) * kH + 4 * c4) : make_float4(0, 0, 0, 0);
+    }
+  };
+  if (r0 < r1) fetch(r0);
+#pragma unroll 1
+  for (int base = r0; base < r1; base += kDwRows) {
+    __syncthreads();                       // the previous chunk's reads are done
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = tid + kThreads * q, row = idx >> 6, c4 = idx & 63;
+      *reinterpret_cast<float4*>(SD + row * kDwLd + 4 * c4) = pd[q];
+      *reinterpret_cast<float4*>(SH + row * kDwLd + 4 * c4) = ph[q];
+    }
+    __syncthreads();
+    if (base + kDwRows < r1) fetch(base + kDwRows);
+#pragma unroll 4
+    for (int kk = 0; kk < kDwRows / 2; ++kk) {
+      const int row = (2 * kk + hi) * kDwLd;
+      const float a = SD[row + 32 * w + pl];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, SH[row + 32 * c + pl], acc[c], 0, 0, 0);
+    }
+  }
+  float* out = partial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (kH * kH);
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int u = 32 * w + 8 * (r >> 2) + 4 * hi + (r & 3);
+      out[u * kH + 32 * c + pl] = acc[c][r];
+    }
 }
 
 int cu_count() {
@@ -529,7 +608,28 @@ MO_EXPORT int mo_flow256_small_grads(int64_t n, const float* dAs, const float* H
   const int nb = 256;
   hipLaunchKernelGGL(k_flow256_grad_small, dim3(nb), dim3(kH), 0, s, rows, dAs, dAs + plane4, dAs + 2 * plane4,
                      Hs + 2 * plane4, Xs, Gs, work);
-  hipLaunchKernelGGL(k_flow256_grad_reduce, dim3((kSmallGrad + 255) / 256), dim3(256), 0, s, work, nb, out);
+  hipLaunchKernelGGL(k_flow256_rows_reduce, dim3((kSmallGrad + 63) / 64, 1), dim3(512), 0, s, work, nb, kSmallGrad,
+                     out);
+  MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+MO_EXPORT int64_t mo_flow256_dw_work_floats(void) { return (int64_t)2 * cu_count() * kH * kH; }
+
+// out [2][256][256] = dW1, dW2 of a step (rows = the 4n stage points of
+// mo_flow256_step_backward's dAs and mo_flow256_step_save's Hs); work:
+// mo_flow256_dw_work_floats() floats (device)
+MO_EXPORT int mo_flow256_dw_grads(int64_t n, const float* dAs, const float* Hs, float* out, float* work,
+                                  void* stream) {
+  MO_CHECK(n > 0 && n <= (1LL << 30) / 12 && dAs && Hs && out && work, MO_ERR_ARG, "mo_flow256_dw_grads: bad arguments");
+  MO_CHECK(((uintptr_t)dAs & 15) == 0 && ((uintptr_t)Hs & 15) == 0, MO_ERR_ARG,
+           "mo_flow256_dw_grads: dAs / Hs not 16-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t plane4 = (size_t)4 * n * kH;
+  const int G = cu_count();
+  hipLaunchKernelGGL(k_flow256_dw, dim3(G, 2), dim3(kThreads), 0, s, (int)(4 * n), dAs + plane4, Hs,
+                     dAs + 2 * plane4, Hs + plane4, work);
+  hipLaunchKernelGGL(k_flow256_rows_reduce, dim3(kH * kH / 64, 2), dim3(512), 0, s, work, G, kH * kH, out);
   MO_HIP(hipGetLastError());
   return MO_OK;
 }

@@ -126,9 +126,9 @@ class RK4Adjoint(Function):
         inputs and activations (mo_flow256_step_save), the fused backward
         through its four stages (mo_flow256_step_backward: dL/dy and the
         pre-activation gradients), and the parameter gradients as sums over
-        the step's 4 n stage points: a GEMM each for W1 and W2, one
-        reduction kernel for W0, W3 and the biases
-        (mo_flow256_small_grads)."""
+        the step's 4 n stage points: a split-K MFMA kernel for W1 and W2
+        (mo_flow256_dw_grads), one reduction kernel for W0, W3 and the
+        biases (mo_flow256_small_grads)."""
         import ctypes as C
         from . import _lib
         ys = ctx.saved_tensors
@@ -143,6 +143,8 @@ class RK4Adjoint(Function):
         nsmall = int(_lib.load().mo_flow256_small_grad_floats())
         small = torch.empty(nsmall, dtype=torch.float32, device=dev)
         work = torch.empty(256 * nsmall, dtype=torch.float32, device=dev)
+        dw = torch.empty(2, 256, 256, dtype=torch.float32, device=dev)
+        dwork = torch.empty(int(_lib.load().mo_flow256_dw_work_floats()), dtype=torch.float32, device=dev)
         H, dA = Hs.view(3, 4 * n, 256), dAs.view(3, 4 * n, 256)
         X, G = Xs.view(4 * n, 3), Gs.view(4 * n, 3)
         g = gy.contiguous().to(torch.float32)
@@ -154,8 +156,9 @@ class RK4Adjoint(Function):
             _lib.call("mo_flow256_step_backward", n, ptr(ctx.packed), float(h), ptr(Hs), ptr(g), ptr(gnew),
                       ptr(dAs), ptr(Gs), pyDeform._stream())
             # weights[0..3], biases[0..3]: W0 [256,3], W1, W2 [256,256], W3 [3,256]
-            gp[1].addmm_(dA[1].t(), H[0])
-            gp[2].addmm_(dA[2].t(), H[1])
+            _lib.call("mo_flow256_dw_grads", n, ptr(dAs), ptr(Hs), ptr(dw), ptr(dwork), pyDeform._stream())
+            gp[1].add_(dw[0])
+            gp[2].add_(dw[1])
             _lib.call("mo_flow256_small_grads", n, ptr(dAs), ptr(Hs), ptr(Xs), ptr(Gs), ptr(small), ptr(work),
                       pyDeform._stream())
             gp[0].add_(small[0:768].view(256, 3))
