@@ -101,9 +101,13 @@ __global__ void k_flow256_pack(const float* __restrict__ W0, const float* __rest
 // without kTwo, the first 32) points: 32 x 32 MFMA blocks, 128 steps of
 // k = 2.  Result layout: unit 32 w + l % 32, point 32 pb + 8 (r / 4) +
 // 4 (l / 32) + r % 4.
-template <bool kTwo>
+// With kStore, the loop also copies Tin to dst (rows [0, nvalid) of 256
+// floats, the layout of store_tile), one float4 per thread every fourth
+// k-group, so that the copy runs in the shadow of the MFMAs.
+template <bool kTwo, bool kStore = false>
 __device__ __forceinline__ void mfma_layer(const float* __restrict__ Wf, const float* __restrict__ Tin, int w, int l,
-                                           v16f& acc0, v16f& acc1) {
+                                           v16f& acc0, v16f& acc1, float* __restrict__ dst = nullptr,
+                                           int nvalid = 0) {
   acc0 = v16f{0};
   acc1 = v16f{0};
   const float4* B = reinterpret_cast<const float4*>(Wf) + (size_t)w * 32 * 64 + l;
@@ -127,16 +131,23 @@ __device__ __forceinline__ void mfma_layer(const float* __restrict__ Wf, const f
       acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[i], bb[i], acc0, 0, 0, 0);
       if (kTwo) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[i], bb[i], acc1, 0, 0, 0);
     }
+    if (kStore && (g & 3) == 3) {
+      const int p = threadIdx.x >> 3, u = 32 * (threadIdx.x & 7) + 4 * (g >> 2);
+      if (p < nvalid)
+        *reinterpret_cast<float4*>(dst + (size_t)p * kH + u) =
+            make_float4(Tin[tidx(u, p)], Tin[tidx(u + 1, p)], Tin[tidx(u + 2, p)], Tin[tidx(u + 3, p)]);
+    }
   }
 }
 
-// Tout = softplus(Tin W^T + b) (forward hidden layer)
-template <bool kTwo>
+// Tout = softplus(Tin W^T + b) (forward hidden layer); with kStore Tin is
+// also copied to dst (see mfma_layer)
+template <bool kTwo, bool kStore = false>
 __device__ __forceinline__ void hidden_layer(const float* __restrict__ Wf, const float* __restrict__ bias,
                                              const float* __restrict__ Tin, float* __restrict__ Tout, int w,
-                                             int l) {
+                                             int l, float* __restrict__ dst = nullptr, int nvalid = 0) {
   v16f acc0, acc1;
-  mfma_layer<kTwo>(Wf, Tin, w, l, acc0, acc1);
+  mfma_layer<kTwo, kStore>(Wf, Tin, w, l, acc0, acc1, dst, nvalid);
   const int u = 32 * w + (l & 31), hi = l >> 5;
   const float bu = bias[u];
 #pragma unroll
@@ -154,9 +165,10 @@ __device__ __forceinline__ float dsoftplus_h(float h) { return 1.f - __builtin_a
 // H = the layer's saved activations, rows of 256 (the chunk's first row)
 template <bool kTwo>
 __device__ __forceinline__ void back_layer(const float* __restrict__ WTf, const float* __restrict__ H, int nvalid,
-                                           const float* __restrict__ Tin, float* __restrict__ Tout, int w, int l) {
+                                           const float* __restrict__ Tin, float* __restrict__ Tout, int w, int l,
+                                           float* __restrict__ dst) {
   v16f acc0, acc1;
-  mfma_layer<kTwo>(WTf, Tin, w, l, acc0, acc1);
+  mfma_layer<kTwo, true>(WTf, Tin, w, l, acc0, acc1, dst, nvalid);   // also copies Tin to dst
   const int j = 32 * w + (l & 31), hi = l >> 5;
   // one 32-point block at a time (the compiler barrier keeps the second
   // block's loads of H from being hoisted next to the first's)
@@ -237,13 +249,17 @@ __global__ __launch_bounds__(kThreads) void k_flow256_step(const float* __restri
         }
       }
       __syncthreads();
-      if (kSave) store_tile(TA, Hs + (0 * 4 + s) * plane + (size_t)base * kH, nvalid);
-      if (nvalid > 32) hidden_layer<true>(P + kPW1, P + kPB1, TA, TB, w, l);
-      else hidden_layer<false>(P + kPW1, P + kPB1, TA, TB, w, l);
+      {
+        float* d1 = kSave ? Hs + (0 * 4 + s) * plane + (size_t)base * kH : nullptr;
+        if (nvalid > 32) hidden_layer<true, kSave>(P + kPW1, P + kPB1, TA, TB, w, l, d1, nvalid);
+        else hidden_layer<false, kSave>(P + kPW1, P + kPB1, TA, TB, w, l, d1, nvalid);
+      }
       __syncthreads();
-      if (kSave) store_tile(TB, Hs + (1 * 4 + s) * plane + (size_t)base * kH, nvalid);
-      if (nvalid > 32) hidden_layer<true>(P + kPW2, P + kPB2, TB, TA, w, l);
-      else hidden_layer<false>(P + kPW2, P + kPB2, TB, TA, w, l);
+      {
+        float* d2 = kSave ? Hs + (1 * 4 + s) * plane + (size_t)base * kH : nullptr;
+        if (nvalid > 32) hidden_layer<true, kSave>(P + kPW2, P + kPB2, TB, TA, w, l, d2, nvalid);
+        else hidden_layer<false, kSave>(P + kPW2, P + kPB2, TB, TA, w, l, d2, nvalid);
+      }
       __syncthreads();
       if (kSave) store_tile(TA, Hs + (2 * 4 + s) * plane + (size_t)base * kH, nvalid);
       // layer 3 (VALU): wave w sums units [32 w, 32 w + 32) for point l
@@ -341,13 +357,17 @@ __global__ __launch_bounds__(kThreads) void k_flow256_step_bwd(int n, const floa
         }
       }
       __syncthreads();
-      store_tile(TA, dAs + (2 * 4 + s) * plane + (size_t)base * kH, nvalid);
-      if (nvalid > 32) back_layer<true>(P + kPW2T, H2, nvalid, TA, TB, w, l);
-      else back_layer<false>(P + kPW2T, H2, nvalid, TA, TB, w, l);
+      {
+        float* d3 = dAs + (2 * 4 + s) * plane + (size_t)base * kH;
+        if (nvalid > 32) back_layer<true>(P + kPW2T, H2, nvalid, TA, TB, w, l, d3);
+        else back_layer<false>(P + kPW2T, H2, nvalid, TA, TB, w, l, d3);
+      }
       __syncthreads();
-      store_tile(TB, dAs + (1 * 4 + s) * plane + (size_t)base * kH, nvalid);
-      if (nvalid > 32) back_layer<true>(P + kPW1T, H1, nvalid, TB, TA, w, l);
-      else back_layer<false>(P + kPW1T, H1, nvalid, TB, TA, w, l);
+      {
+        float* d2 = dAs + (1 * 4 + s) * plane + (size_t)base * kH;
+        if (nvalid > 32) back_layer<true>(P + kPW1T, H1, nvalid, TB, TA, w, l, d2);
+        else back_layer<false>(P + kPW1T, H1, nvalid, TB, TA, w, l, d2);
+      }
       __syncthreads();
       store_tile(TA, dAs + (0 * 4 + s) * plane + (size_t)base * kH, nvalid);
       // dz = W0^T dA1 (VALU): wave w sums units [32 w, 32 w + 32) for point l
