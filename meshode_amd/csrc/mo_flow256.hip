@@ -104,10 +104,14 @@ __global__ void k_flow256_pack(const float* __restrict__ W0, const float* __rest
 // With kStore, the loop also copies Tin to dst (rows [0, nvalid) of 256
 // floats, the layout of store_tile), one float4 per thread every fourth
 // k-group, so that the copy runs in the shadow of the MFMAs.
-template <bool kTwo, bool kStore = false>
+// With kStage, it also loads rows [0, nvalid) of src (256 floats each) into
+// the tile Tst (one float4 per thread every fourth k-group; Tst must be idle
+// during the loop, and a barrier must separate the loop from its readers).
+template <bool kTwo, bool kStore = false, bool kStage = false>
 __device__ __forceinline__ void mfma_layer(const float* __restrict__ Wf, const float* __restrict__ Tin, int w, int l,
                                            v16f& acc0, v16f& acc1, float* __restrict__ dst = nullptr,
-                                           int nvalid = 0) {
+                                           int nvalid = 0, const float* __restrict__ src = nullptr,
+                                           float* __restrict__ Tst = nullptr) {
   acc0 = v16f{0};
   acc1 = v16f{0};
   const float4* B = reinterpret_cast<const float4*>(Wf) + (size_t)w * 32 * 64 + l;
@@ -136,6 +140,16 @@ __device__ __forceinline__ void mfma_layer(const float* __restrict__ Wf, const f
       if (p < nvalid)
         *reinterpret_cast<float4*>(dst + (size_t)p * kH + u) =
             make_float4(Tin[tidx(u, p)], Tin[tidx(u + 1, p)], Tin[tidx(u + 2, p)], Tin[tidx(u + 3, p)]);
+    }
+    if (kStage && (g & 3) == 1) {
+      const int p = threadIdx.x >> 3, u = 32 * (threadIdx.x & 7) + 4 * (g >> 2);
+      if (p < nvalid) {
+        const float4 v = *reinterpret_cast<const float4*>(src + (size_t)p * kH + u);
+        Tst[tidx(u, p)] = v.x;
+        Tst[tidx(u + 1, p)] = v.y;
+        Tst[tidx(u + 2, p)] = v.z;
+        Tst[tidx(u + 3, p)] = v.w;
+      }
     }
   }
 }
@@ -167,23 +181,18 @@ template <bool kTwo>
 __device__ __forceinline__ void back_layer(const float* __restrict__ WTf, const float* __restrict__ H, int nvalid,
                                            const float* __restrict__ Tin, float* __restrict__ Tout, int w, int l,
                                            float* __restrict__ dst) {
+  // the loop also copies Tin to dst and stages the chunk's rows of H into
+  // Tout (idle until this epilogue, which reads each value at the position
+  // it then overwrites)
   v16f acc0, acc1;
-  mfma_layer<kTwo, true>(WTf, Tin, w, l, acc0, acc1, dst, nvalid);   // also copies Tin to dst
+  mfma_layer<kTwo, true, true>(WTf, Tin, w, l, acc0, acc1, dst, nvalid, H, Tout);
+  __syncthreads();
   const int j = 32 * w + (l & 31), hi = l >> 5;
-  // one 32-point block at a time (the compiler barrier keeps the second
-  // block's loads of H from being hoisted next to the first's)
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int p = 8 * (r >> 2) + 4 * hi + (r & 3);
-    Tout[tidx(j, p)] = p < nvalid ? acc0[r] * dsoftplus_h(H[(size_t)p * kH + j]) : 0.f;
-  }
-  if (kTwo) {
-    asm volatile("" ::: "memory");
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int p = 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
-      Tout[tidx(j, p)] = p < nvalid ? acc1[r] * dsoftplus_h(H[(size_t)p * kH + j]) : 0.f;
-    }
+    Tout[tidx(j, p)] = p < nvalid ? acc0[r] * dsoftplus_h(Tout[tidx(j, p)]) : 0.f;
+    if (kTwo) Tout[tidx(j, 32 + p)] = 32 + p < nvalid ? acc1[r] * dsoftplus_h(Tout[tidx(j, 32 + p)]) : 0.f;
   }
 }
 
