@@ -64,3 +64,20 @@ def test_cli_matches_library_and_oracle(gpu, oracle, tmp_path):
     _, so, _ = oracle.lm_solve(grid, 32, sn, E, 1.0)
     eg, _, _ = oracle.lm_cost(grid, 32, (outs[0].cpu().numpy() - pos) / sc, sn, E, 1.0)
     assert abs(eg - so["final_cost"]) <= 1e-3 * so["final_cost"]
+
+
+def test_load_mesh_returns_raw_coordinates(tmp_path):
+    """pyDeform.LoadMesh returns the OBJ's coordinates as they are (the
+    reference's src.Normalize() there is dead code, mesh_tensor.cc:95-97)
+    as float32 and 0-based int32 faces; SaveMesh writes them back."""
+    from meshode_amd import pyDeform
+    p = tmp_path / "raw.obj"
+    p.write_text("v 120.5 -3.25 7\nv 121 -3 8\nv 119 -4 7.5\nf 1 2 3\n")
+    V, F = pyDeform.LoadMesh(str(p))
+    assert V.dtype.is_floating_point and V.dtype.itemsize == 4 and F.dtype.itemsize == 4
+    assert V.numpy().tolist() == [[120.5, -3.25, 7.0], [121.0, -3.0, 8.0], [119.0, -4.0, 7.5]]
+    assert F.numpy().tolist() == [[0, 1, 2]]
+    q = tmp_path / "back.obj"
+    pyDeform.SaveMesh(str(q), V, F)
+    V2, F2 = pyDeform.LoadMesh(str(q))
+    assert V2.numpy().tolist() == V.numpy().tolist() and F2.numpy().tolist() == [[0, 1, 2]]

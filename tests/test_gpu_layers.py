@@ -136,3 +136,52 @@ def test_layer_errors(sphere_template):
     pid2 = pyDeform.InitializeDeformTemplate(Vt, Ft, 0, 8)
     with pytest.raises(_lib.MoError):
         pyDeform.RigidEdgeLoss_forward(Vt, Ft, pid2)
+
+
+def test_graph_loss_layer_masks_distance_gradient(gpu, oracle):
+    """GraphLossLayer (graph_loss_layer.py:9-57): the loss is 0.5 sum D +
+    0.5 rigidity^2 sum R with nothing masked, while the backward drops the
+    distance gradient of vertices with 0.5 d^2 >= 0.5 * 0.03^2
+    (graph_loss_layer.py:18, 40; the reference quirk kept).  Value within
+    float32 summation order of the oracle's; gradient bit-exact."""
+    from meshode_amd.layers import GraphLossLayer
+    _say("[layers] GraphLossLayer: unmasked forward, masked distance gradient")
+    sV, sF, tV, tF = synth.displaced_sphere_pair(7, src_level=2, tgt_level=3)
+    src = torch.from_numpy((sV * 0.9).astype(np.float32))
+    E = torch.from_numpy(np.concatenate([sF[:, [0, 1]], sF[:, [1, 2]]]).astype(np.int32))
+    rig = 0.7
+    layer = GraphLossLayer(src, E, torch.from_numpy(tV.astype(np.float32)), torch.from_numpy(tF.astype(np.int32)),
+                           rig, grid_resolution=32)
+    pid, N = int(layer.param_id), 32
+    grid = pyDeform.TemplateGrid(pid)[0].cpu().numpy().reshape(-1)
+    rest = src.numpy().copy()                     # normalized in place by the layer
+    X0 = rest + np.random.default_rng(3).normal(0, 0.01, rest.shape).astype(np.float32)
+    X = torch.from_numpy(X0).to(gpu).requires_grad_(True)
+    loss = layer(X, E.to(gpu))
+    loss.backward()
+    D = oracle.distance_loss_fwd(grid, N, X0)
+    off = oracle.store_graph(rest, E.numpy())
+    R = oracle.graph_fwd(X0, E.numpy(), off)
+    r2 = np.float32(torch.tensor(rig * rig).item())
+    want = 0.5 * D.astype(np.float64).sum() + 0.5 * R.astype(np.float64).sum() * float(r2)
+    assert float(loss) == pytest.approx(want, rel=1e-5)
+    keep = (D * np.float32(0.5)) < np.float32(0.5 * 0.03 * 0.03)
+    assert keep.any() and (~keep).any()           # both sides of the mask occur
+    gD = oracle.distance_loss_bwd(grid, N, X0) * keep[:, None].astype(np.float32)
+    gR = oracle.graph_bwd(X0, E.numpy(), off)
+    assert np.array_equal(X.grad.cpu().numpy(), gD + gR * r2)
+
+
+def test_template_symmetry_flag_is_a_no_op(gpu):
+    """InitializeDeformTemplate's symmetry argument symmetrizes an empty mesh
+    in the reference (deform_params.cc:28-33): 0 and 1 give the same
+    template (the CLI's symmetry is the real reflection, tested there)."""
+    _say("[layers] InitializeDeformTemplate symmetry = 0 vs 1")
+    _, _, V, F = synth.displaced_sphere_pair(9, src_level=1, tgt_level=2)
+    Vt, Ft = torch.from_numpy(V.astype(np.float32)), torch.from_numpy(F.astype(np.int32))
+    a = pyDeform.InitializeDeformTemplate(Vt, Ft, 0, 16)
+    b = pyDeform.InitializeDeformTemplate(Vt, Ft, 1, 16)
+    ia, ib = pyDeform.TemplateInfo(a), pyDeform.TemplateInfo(b)
+    assert ia[0] == ib[0] and ia[1] == ib[1] and np.array_equal(ia[2], ib[2])
+    for x, y in zip(pyDeform.TemplateGrid(a), pyDeform.TemplateGrid(b)):
+        assert torch.equal(x, y)
