@@ -310,7 +310,7 @@ def bench_serve(args):
     for i in range(24):
         nv = int(np.exp(rng.uniform(np.log(2000), np.log(args.max_verts))))
         if i % 2 == 0:
-            p = synth.box_union_pair(1000 + i, n_verts=max(nv, 1000))
+            p = synth.box_union_pair(1000 + i, n_verts=max(nv - nv % 2, 1000))   # box unions have even counts
         else:
             lvl = 3 if nv < 4000 else 4
             p = synth.displaced_sphere_pair(1000 + i, src_level=lvl, tgt_level=lvl + 1)
