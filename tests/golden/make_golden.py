@@ -53,6 +53,17 @@ def main():
     x, st, _ = O.lm_solve(g, M, x0, O.face_edges(pF), 1.0)
     np.savez_compressed(os.path.join(HERE, "lm_sphere_n24.npz"), src=pV, src_f=pF, tgt=qV, tgt_f=qF, N=M,
                         x0=x0, x=x, final_cost=st["final_cost"], iterations=st["iterations"])
+    # a pair above the register/LDS solver's 8192 vertices (icosphere level 5
+    # source, level 4 target, 48^3): only the converged energy is kept, since
+    # the oracle's exact-step LM needs about a minute for it
+    pV, pF, qV, qF = synth.displaced_sphere_pair(31, src_level=5, tgt_level=4)
+    qn, qs, qp = O.normalize(qV)
+    M = 48
+    g, _, _ = O.build_grid_bvh(qn, qF, M)
+    x0 = O.apply_transform(pV, qs, qp)
+    _, st, _ = O.lm_solve(g, M, x0, O.face_edges(pF), 1.0)
+    np.savez_compressed(os.path.join(HERE, "lm_sphere_l5_n48.npz"), seed=31, src_level=5, tgt_level=4, N=M, lam=1.0,
+                        final_cost=st["final_cost"], iterations=st["iterations"])
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -114,3 +114,33 @@ def test_grid_build_primitive_and_errors(gpu, oracle):
     with pytest.raises(_lib.MoError):
         _lib.call("mo_grid_build", C.c_void_p(Vd.data_ptr()), len(Vn), C.c_void_p(Fbad.data_ptr()), len(F), N,
                   C.c_void_p(d64.data_ptr()), None, None, s)
+
+
+@pytest.mark.parametrize("N", [2, 7, 13])
+def test_grid_ragged_resolutions_degenerate_and_duplicate_faces(gpu, oracle, N):
+    """Resolutions that leave partial 4x4x4 bricks (N = 2 is the smallest
+    accepted: a lookup's cell index runs over [0, N - 2]); a target with a
+    repeated face and a reversed copy (equal distances: the lowest index
+    wins), a zero-area face with a repeated corner (a segment) and a
+    collinear face.  Both the banded template grid and the exact field
+    (mo_grid_build) are bit-exact."""
+    _say(f"[grid] case=degenerate N={N}")
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0.5, 0, 0], [0.25, 0.25, 0.9]], float)
+    F = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3], [0, 1, 3], [3, 1, 0], [5, 5, 2], [0, 4, 1]],
+                 np.int32)
+    V32 = np.asarray(V, np.float32).astype(np.float64)
+    pid = _template(V32, F, N)
+    d64, d32, nn = [t.cpu().numpy().reshape(-1) for t in pyDeform.TemplateGrid(pid)]
+    Vn, _, _ = oracle.normalize(V32)
+    od, od32, oi = oracle.build_grid_banded(Vn, F, N)
+    assert np.array_equal(d64, od) and np.array_equal(d32, od32)
+    assert np.array_equal(nn, oi), np.flatnonzero(nn != oi)[:10]
+    fd, fd32, fi = _grid_build(Vn, F, N)
+    ed, ed32, ei = oracle.build_grid(Vn, F, N)
+    assert np.array_equal(fd, ed) and np.array_equal(fd32, ed32)
+    assert np.array_equal(fi, ei), np.flatnonzero(fi != ei)[:10]
+    assert not (fi == 4).any()                # the exact copy of face 1 never wins its ties
+    if N == 2:
+        from meshode_amd import _lib
+        with pytest.raises(_lib.MoError):
+            _template(V32, F, 1)

@@ -82,3 +82,81 @@ def test_alone_equals_in_the_mix(gpu):
         alone, st = deform.rigid_deform_batch([p], grid_resolution=grids[i], lambdas=lams[i])
         assert torch.equal(alone[0], mix[i]), i
         assert st[0]["final_cost"] == st_mix[i]["final_cost"]
+
+
+def _parity(oracle, pair, N, lam=1.0, max_iterations=None, **gpu_opts):
+    """Solve one pair on the GPU (deform_batch) and with the oracle's
+    exact-step LM on the same grid and source; return (oracle stats, GPU
+    energy under the oracle's FP64 cost, GPU stats)."""
+    pid, grid, sn, scale, pos = _setup(oracle, pair, N)
+    E = oracle.face_edges(pair[1])
+    oo = oracle.default_lm_options()
+    if max_iterations is not None:
+        oo.max_num_iterations = max_iterations
+        gpu_opts["max_num_iterations"] = max_iterations
+    xo, so, _ = oracle.lm_solve(grid, N, sn, E, lam, options=oo)
+    V = torch.from_numpy(sn.copy()).cuda()
+    st = deform.deform_batch([pid], [V], [pair[1]], [0], [lam], options=deform.default_options(**gpu_opts))[0]
+    eg, _, _ = oracle.lm_cost(grid, N, V.cpu().numpy(), sn, E, lam)
+    return so, eg, st
+
+
+@pytest.mark.parametrize("case", ["sphere_l2_n32", "box1000_n32"])
+def test_lm_global_kernel_energy_parity(gpu, oracle, case):
+    """The global-memory kernel (k_lm_solve, kernel_variant 1) on the small
+    cases: same energy bar as the register/LDS kernel."""
+    _say(f"[lm] case={case} kernel_variant=1")
+    if case == "sphere_l2_n32":
+        pair = synth.displaced_sphere_pair(11, src_level=2, tgt_level=3)
+    else:
+        pair = synth.box_union_pair(3, n_boxes=4, n_verts=1000)
+    so, eg, st = _parity(oracle, pair, 32, kernel_variant=1)
+    _say(f"[lm] {case} global: oracle E={so['final_cost']:.8g}  gpu E={eg:.8g} it={st['iterations']}")
+    assert st["final_cost"] < st["initial_cost"]
+    assert abs(eg - so["final_cost"]) <= ENERGY_RTOL * so["final_cost"]
+    assert abs(st["final_cost"] - eg) <= 1e-4 * eg
+
+
+def test_lm_nonmanifold_edge_energy_parity(gpu, oracle):
+    """A fin triangle on an existing edge gives that vertex pair three
+    directed edges (the register/LDS kernel stores multiplicities 1 and 2
+    only), so the pair goes to the global-memory kernel on its own."""
+    _say("[lm] case=fin_l2_n32")
+    sV, sF, tV, tF = synth.displaced_sphere_pair(12, src_level=2, tgt_level=3)
+    a, b = int(sF[0, 0]), int(sF[0, 1])
+    mid = 0.5 * (sV[a] + sV[b])
+    fin = mid + 0.3 * mid / np.linalg.norm(mid)
+    sV2 = np.vstack([sV, fin[None]])
+    sF2 = np.vstack([sF, np.array([[a, b, len(sV)]], sF.dtype)])
+    E = oracle.face_edges(sF2)
+    und = np.sort(E, 1)
+    _, counts = np.unique(und, axis=0, return_counts=True)
+    assert counts.max() == 3
+    so, eg, st = _parity(oracle, (sV2, sF2, tV, tF), 32)
+    _say(f"[lm] fin: oracle E={so['final_cost']:.8g}  gpu E={eg:.8g} it={st['iterations']}")
+    assert st["final_cost"] < st["initial_cost"]
+    assert abs(eg - so["final_cost"]) <= ENERGY_RTOL * so["final_cost"]
+
+
+def test_lm_large_pair_energy_parity(gpu, oracle):
+    """A 10242-vertex source (icosphere level 5, above the register/LDS
+    kernel's 8192, so the global-memory kernel solves it) against a level-4
+    target at 48^3.  The oracle's converged energy for this seeded pair is
+    the golden fixture lm_sphere_l5_n48.npz (tests/golden/make_golden.py);
+    the GPU result is evaluated with the oracle's FP64 cost on the GPU's
+    template grid, which is bit-exact to the oracle's."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lm_sphere_l5_n48.npz"))
+    N = int(z["N"])
+    _say(f"[lm] case=sphere_l5_n{N} (global kernel)")
+    pair = synth.displaced_sphere_pair(int(z["seed"]), src_level=int(z["src_level"]), tgt_level=int(z["tgt_level"]))
+    assert len(pair[0]) > 8192
+    pid, grid, sn, scale, pos = _setup(oracle, pair, N)
+    E = oracle.face_edges(pair[1])
+    V = torch.from_numpy(sn.copy()).cuda()
+    st = deform.deform_batch([pid], [V], [pair[1]], [0], [float(z["lam"])])[0]
+    eg, _, _ = oracle.lm_cost(grid, N, V.cpu().numpy(), sn, E, float(z["lam"]))
+    ref = float(z["final_cost"])
+    _say(f"[lm] large: oracle E={ref:.8g} it={int(z['iterations'])}  gpu E={eg:.8g} it={st['iterations']}")
+    assert abs(eg - ref) <= ENERGY_RTOL * ref
+    assert abs(st["final_cost"] - eg) <= 1e-4 * eg
