@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--kernel-variant", type=int, default=0, help="0: register/LDS kernel when a pair fits; 1: global")
     ap.add_argument("--pcg-tol", type=float, default=None, help="inner PCG relative tolerance (library default if unset)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "flow256", "neural2", "rot", "cfg1"],
+    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "flow256", "neural2", "rot", "cfg1", "cfg4"],
                     help="batch: the headline line; serve / adam / ode: the serving, pyDeform-Adam and "
                          "NeuralODE workloads (their own JSON line)")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of Poisson arrivals")
@@ -123,7 +123,8 @@ def main():
         del a, b
     if args.mode != "batch":
         print(json.dumps({"serve": bench_serve, "adam": bench_adam, "ode": bench_ode, "flow256": bench_flow256,
-                          "neural2": bench_neural2, "rot": bench_rot, "cfg1": bench_cfg1}[args.mode](args)),
+                          "neural2": bench_neural2, "rot": bench_rot, "cfg1": bench_cfg1,
+                          "cfg4": bench_cfg4}[args.mode](args)),
               flush=True)
         return
 
@@ -298,6 +299,61 @@ def bench_cfg1(args):
         out["energy_check_pair0"] = {"oracle": so["final_cost"], "gpu": eg,
                                      "rel_diff": (eg - so["final_cost"]) / so["final_cost"]}
     return out
+
+
+def bench_cfg4(args):
+    """Config 4: 256 pairs in one rigid_deform batch, each built like config 1
+    (displaced-icosphere target one level finer, capped at 6) with the source
+    level drawn from {3, 4, 5, 6} (642 to 40962 vertices), the grid from
+    {64^3, 128^3} and lambda ~ U(0.5, 2) (numpy PCG64 seed 4); fixed work of
+    50 LM iterations with 30 PCG iterations each (tolerances zeroed).  Pairs
+    of at most 8192 vertices run on the register/LDS kernel, larger ones on
+    the global-memory kernel, in the same call.  8 distinct seeds per level
+    are repeated to fill the batch (the level-6 generator takes ~1 s a pair
+    on the host)."""
+    import numpy as np
+    import torch
+    from meshode_amd import _lib, deform, synth
+    n = 256
+    rng = np.random.Generator(np.random.PCG64(4))
+    levels = rng.choice([3, 4, 5, 6], n)
+    grids = [int(g) for g in rng.choice([64, 128], n)]
+    lams = [float(x) for x in rng.uniform(0.5, 2.0, n)]
+    cache = {}
+    pairs = []
+    for i in range(n):
+        key = (int(levels[i]), i % 8)
+        if key not in cache:
+            cache[key] = synth.icosphere_pair(1000 * key[0] + key[1], key[0])
+        pairs.append(cache[key])
+    batch = deform.PairBatch(pairs, grid_resolution=grids, lambdas=lams)
+    opt = deform.default_options(timing=1, max_num_iterations=50, pcg_max_iterations=30,
+                                 function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0,
+                                 pcg_relative_tolerance=0.0, block_threads=args.block_threads)
+    for _ in range(max(args.warmup, 0)):
+        batch.run(opt)
+    torch.cuda.synchronize()
+    steps = max(args.steps, 1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = batch.run(opt)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    tm = _lib.last_timings()
+    nv = np.array([len(p[0]) for p in pairs])
+    it = np.array([s["iterations"] for s in st])
+    pcg = np.array([s["pcg_iterations"] for s in st])
+    return {"metric": "deformation pairs/sec (config 4: 256 mixed pairs, L3-L6, 64^3/128^3)",
+            "value": n * steps / elapsed, "unit": "pairs/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True, "dtype": "f32 (LM), f64 (grid)",
+            "data": "synthetic", "phase_ms_last": {k: tm[k] for k in ("grid_ms", "setup_ms", "lm_ms", "total_ms")},
+            "lm_iterations": {"min": int(it.min()), "max": int(it.max())},
+            "pcg_per_lm_iteration": float(pcg.sum() / max(it.sum(), 1)),
+            "vertices_per_step": int(nv.sum()),
+            "config": {"workload": "config 4: icosphere L3-L6 sources vs displaced icospheres one level finer",
+                       "pairs": n, "levels": {str(L): int((levels == L).sum()) for L in (3, 4, 5, 6)},
+                       "grids": {str(g): grids.count(g) for g in (64, 128)}, "lambda": "U(0.5, 2)",
+                       "lm_iterations": 50, "pcg_iterations": 30, "seed": 4}}
 
 
 def bench_serve(args):
