@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--kernel-variant", type=int, default=0, help="0: register/LDS kernel when a pair fits; 1: global")
     ap.add_argument("--pcg-tol", type=float, default=None, help="inner PCG relative tolerance (library default if unset)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the launch-shape autotuner (untimed, before warm-up) and use the built-in order")
     ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "flow256", "neural2", "rot", "cfg1", "cfg4"],
                     help="batch: the headline line; serve / adam / ode: the serving, pyDeform-Adam and "
                          "NeuralODE workloads (their own JSON line)")
@@ -139,6 +141,7 @@ def main():
     opt = deform.default_options(block_threads=args.block_threads, timing=1, kernel_variant=args.kernel_variant)
     if args.pcg_tol is not None:
         opt.pcg_relative_tolerance = args.pcg_tol
+    shapes_used = _autotune(args, distinct, args.grid, args.lam, opt)
     torch.cuda.synchronize()
 
     for w in range(args.warmup):
@@ -211,9 +214,9 @@ def main():
             "grid": args.grid,
             "lambda": args.lam,
             "lm_max_iterations": 100,
-            "block_threads": args.block_threads,
+            "block_threads": opt.block_threads,
             "kernel_variant": args.kernel_variant,
-            "lm_shapes": os.environ.get("MO_LM_SHAPES", "default"),
+            "lm_shapes": shapes_used,
         },
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "max": float(lat.max())},
@@ -301,6 +304,24 @@ def bench_cfg1(args):
     return out
 
 
+def _autotune(args, pairs, grid, lam, opt, repeats=2):
+    """Run the launch-shape autotuner on the workload's pairs before the
+    warm-up (untimed) and install its table; returns the shape summary for
+    the JSON line.  Sets opt.block_threads when the workload has pairs for
+    the global-memory kernel."""
+    if args.no_autotune or args.kernel_variant != 0 or "MO_LM_SHAPES" in os.environ:
+        return os.environ.get("MO_LM_SHAPES", "built-in order")
+    from meshode_amd import autotune
+    t0 = time.perf_counter()
+    res = autotune.tune(pairs, grid_resolution=grid, lambdas=lam, options=opt, repeats=repeats,
+                        max_pairs=len(pairs))
+    if res["block_threads"]:
+        opt.block_threads = res["block_threads"]
+    desc = autotune.describe(res)
+    print(f"[bench] autotuned launch shapes in {time.perf_counter() - t0:.1f}s: {desc}", file=sys.stderr, flush=True)
+    return "autotuned: " + desc
+
+
 def bench_cfg4(args):
     """Config 4: 256 pairs in one rigid_deform batch, each built like config 1
     (displaced-icosphere target one level finer, capped at 6) with the source
@@ -310,7 +331,7 @@ def bench_cfg4(args):
     of at most 8192 vertices run on the register/LDS kernel, larger ones on
     the global-memory kernel, in the same call.  8 distinct seeds per level
     are repeated to fill the batch (the level-6 generator takes ~1 s a pair
-    on the host)."""
+    on the host).  Launch shapes come from the autotuner (untimed)."""
     import numpy as np
     import torch
     from meshode_amd import _lib, deform, synth
@@ -330,6 +351,7 @@ def bench_cfg4(args):
     opt = deform.default_options(timing=1, max_num_iterations=50, pcg_max_iterations=30,
                                  function_tolerance=0.0, gradient_tolerance=0.0, parameter_tolerance=0.0,
                                  pcg_relative_tolerance=0.0, block_threads=args.block_threads)
+    shapes_used = _autotune(args, pairs, grids, lams, opt, repeats=1)
     for _ in range(max(args.warmup, 0)):
         batch.run(opt)
     torch.cuda.synchronize()
@@ -353,7 +375,8 @@ def bench_cfg4(args):
             "config": {"workload": "config 4: icosphere L3-L6 sources vs displaced icospheres one level finer",
                        "pairs": n, "levels": {str(L): int((levels == L).sum()) for L in (3, 4, 5, 6)},
                        "grids": {str(g): grids.count(g) for g in (64, 128)}, "lambda": "U(0.5, 2)",
-                       "lm_iterations": 50, "pcg_iterations": 30, "seed": 4}}
+                       "lm_iterations": 50, "pcg_iterations": 30, "seed": 4, "lm_shapes": shapes_used,
+                       "block_threads": opt.block_threads}}
 
 
 def bench_serve(args):

@@ -176,6 +176,23 @@ typedef struct mo_lm_stats {
 /* Fills the defaults listed above. */
 void mo_lm_default_options(mo_lm_options* opt);
 
+/* Launch shapes of the register/LDS LM kernel (no counterpart in the
+ * reference, whose Ceres solve is single-threaded, deformer.cc:55-59: the
+ * tuning interface of this library's batched solver, for the autotuner in
+ * meshode_amd/autotune.py).  Shape i runs one pair on threads[i] threads
+ * holding vertices_per_thread[i] vertices each, so it takes pairs of at most
+ * threads * vertices_per_thread vertices (and multiplicities 1 and 2).
+ * mo_lm_set_shape_table installs a size-class table: a pair of nv vertices
+ * uses shape[r] for the first r (max_vertices ascending) with
+ * nv <= max_vertices[r]; a larger pair, or every pair when n = 0 (the
+ * default), takes the first fitting shape of the built-in order
+ * (MO_LM_SHAPES="TxVPT,..." overrides that order).  Each row's shape must
+ * hold max_vertices[r] vertices.  The table is process-wide and read under a
+ * lock by every later mo_deform_batch / mo_rigid_deform_batch call. */
+int32_t mo_lm_shape_count(void);
+int mo_lm_shape_info(int32_t index, int32_t* threads, int32_t* vertices_per_thread);
+int mo_lm_set_shape_table(const int32_t* max_vertices, const int32_t* shape, int32_t n);
+
 /* Deformer::Deform (deformer.cc:18-92; kind 0, idx = faces [n,3]) or
  * Deformer::DeformGraph (deformer.cc:370-442; kind 1, idx = edges [n,2]) or
  * the same with AdaptiveEdgeLoss residuals (edgeloss.h:35-61; kind 2, idx =

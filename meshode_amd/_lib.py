@@ -23,6 +23,7 @@ SYMBOLS = [
     "mo_rigidity_store", "mo_rigid_loss_forward", "mo_rigid_loss_backward",
     "mo_graph_store", "mo_graph_loss_forward", "mo_graph_loss_backward",
     "mo_lm_default_options", "mo_deform_batch", "mo_rigid_deform_batch", "mo_last_timings",
+    "mo_lm_shape_count", "mo_lm_shape_info", "mo_lm_set_shape_table",
     "mo_cad_prepare", "mo_cad_sizes", "mo_cad_export", "mo_cad_destroy", "mo_delaunay2", "mo_delaunay3",
     "mo_apply_transform_f64", "mo_linear_estimation", "mo_deform_subdivision",
     "mo_linear_estimation_rot", "mo_nearest_f32", "mo_deform_rot", "mo_deform_rot_batch",
@@ -110,6 +111,9 @@ _SIGS = {
     "mo_rigid_deform_batch": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                         C.POINTER(LMOptions), _vp, _vp, _vp]),
     "mo_last_timings": (C.c_int, [_vp]),
+    "mo_lm_shape_count": (_i32, []),
+    "mo_lm_shape_info": (C.c_int, [_i32, C.POINTER(_i32), C.POINTER(_i32)]),
+    "mo_lm_set_shape_table": (C.c_int, [_vp, _vp, _i32]),
     "mo_cad_prepare": (C.c_int, [_vp, _i64, _vp, _i64, _f64, _f64, _f64, _i32, C.POINTER(_i64)]),
     "mo_cad_sizes": (C.c_int, [_i64, _vp]),
     "mo_cad_export": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -752,6 +752,21 @@ MO_EXPORT void mo_lm_default_options(mo_lm_options* o) {
   o->max_successful_steps = 0;
 }
 
+MO_EXPORT int32_t mo_lm_shape_count(void) { return lm_lds_buckets(); }
+
+MO_EXPORT int mo_lm_shape_info(int32_t index, int32_t* threads, int32_t* vertices_per_thread) {
+  int t = 0, v = 0;
+  const int rc = lm_lds_shape(index, &t, &v);
+  if (rc) return rc;
+  if (threads) *threads = t;
+  if (vertices_per_thread) *vertices_per_thread = v;
+  return MO_OK;
+}
+
+MO_EXPORT int mo_lm_set_shape_table(const int32_t* max_vertices, const int32_t* shape, int32_t n) {
+  return lm_lds_set_table(max_vertices, shape, n);
+}
+
 MO_EXPORT int mo_last_timings(double* out6) {
   MO_CHECK(out6, MO_ERR_ARG, "null output");
   for (int k = 0; k < 6; ++k) out6[k] = g_timings[k];

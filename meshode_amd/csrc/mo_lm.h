@@ -72,6 +72,8 @@ int lm_solve_launch(const LMPairDesc* pairs_dev, const int* plist, int npairs, c
 // vertices whose adjacency multiplicities are 1 or 2, in size buckets.
 int lm_lds_buckets();
 int lm_lds_bucket_for(int nv);                       // -1: too large
+int lm_lds_shape(int bucket, int* threads, int* vertices_per_thread);
+int lm_lds_set_table(const int* max_nv, const int* shape, int n);   // n = 0: built-in order
 size_t lm_lds_bytes(int nv, long long entries);      // dynamic LDS for one pair
 int lm_lds_launch(int bucket, const LMPairDesc* pairs_dev, const int* plist, int count, const LMArrays& arrays,
                   const LMOptionsDev& opt, LMResultDev* results_dev, long long total_vertices, size_t lds_bytes,

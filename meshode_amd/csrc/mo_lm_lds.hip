@@ -28,6 +28,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -371,18 +372,22 @@ int launch_bucket(const LMPairDesc* pairs, const int* plist, int count, const LM
 }  // namespace
 
 // Shapes of the register/LDS kernel: (threads, vertices per thread).  A pair
-// of nv vertices can use any shape with T * VPT >= nv; the preference order
-// below is the tuned default (MO_LM_SHAPES="TxVPT,TxVPT,..." overrides it,
-// which is how the shape sweep in the benchmark tooling explores it).
+// of nv vertices can use any shape with T * VPT >= nv.  Which one it gets:
+//   1. the size-class table, when one is set (lm_lds_set_table: pairs with
+//      nv <= max_nv[r], first r in ascending order, use shape[r]; this is
+//      what the autotuner installs after timing the candidates);
+//   2. else the first fitting shape in the preference order: the built-in
+//      order below, or MO_LM_SHAPES="TxVPT,TxVPT,..." read once.
 static const int kShapeT[] = {1024, 1024, 1024, 1024, 512, 512, 512, 512, 256, 256, 256};
 static const int kShapeVPT[] = {1, 2, 4, 5, 8, 10, 12, 16, 20, 24, 32};
 constexpr int kNumShapes = 11;
 
-static std::vector<int> shape_preference() {
-  static std::vector<int> pref;
-  static bool init = false;
-  if (!init) {
-    init = true;
+static std::mutex g_shape_mu;
+static std::vector<int> g_table_nv, g_table_shape;   // guarded by g_shape_mu
+
+static const std::vector<int>& shape_preference() {
+  static const std::vector<int> pref = [] {
+    std::vector<int> p;
     const char* env = getenv("MO_LM_SHAPES");
     if (env) {
       std::string e(env);
@@ -393,20 +398,50 @@ static std::vector<int> shape_preference() {
         int t = 0, v = 0;
         if (sscanf(tok.c_str(), "%dx%d", &t, &v) == 2)
           for (int b = 0; b < kNumShapes; ++b)
-            if (kShapeT[b] == t && kShapeVPT[b] == v) pref.push_back(b);
+            if (kShapeT[b] == t && kShapeVPT[b] == v) p.push_back(b);
         if (comma == std::string::npos) break;
         pos = comma + 1;
       }
     }
-    if (pref.empty())
-      for (int b = 0; b < kNumShapes; ++b) pref.push_back(b);
-  }
+    if (p.empty())
+      for (int b = 0; b < kNumShapes; ++b) p.push_back(b);
+    return p;
+  }();
   return pref;
 }
 
 int lm_lds_buckets() { return kNumShapes; }
 
+int lm_lds_shape(int b, int* threads, int* vpt) {
+  if (b < 0 || b >= kNumShapes) return fail(MO_ERR_ARG, "shape index %d out of range [0, %d)", b, kNumShapes);
+  if (threads) *threads = kShapeT[b];
+  if (vpt) *vpt = kShapeVPT[b];
+  return MO_OK;
+}
+
+int lm_lds_set_table(const int* max_nv, const int* shape, int n) {
+  MO_CHECK(n >= 0 && (n == 0 || (max_nv && shape)), MO_ERR_ARG, "shape table: bad arguments");
+  for (int r = 0; r < n; ++r) {
+    MO_CHECK(shape[r] >= 0 && shape[r] < kNumShapes, MO_ERR_ARG, "shape table row %d: shape %d out of range", r,
+             shape[r]);
+    MO_CHECK(max_nv[r] >= 1 && (r == 0 || max_nv[r] > max_nv[r - 1]), MO_ERR_ARG,
+             "shape table row %d: max_nv must be positive and ascending", r);
+    MO_CHECK((long long)kShapeT[shape[r]] * kShapeVPT[shape[r]] >= max_nv[r], MO_ERR_ARG,
+             "shape table row %d: shape %dx%d holds fewer than %d vertices", r, kShapeT[shape[r]],
+             kShapeVPT[shape[r]], max_nv[r]);
+  }
+  std::lock_guard<std::mutex> lk(g_shape_mu);
+  g_table_nv.assign(max_nv, max_nv + n);
+  g_table_shape.assign(shape, shape + n);
+  return MO_OK;
+}
+
 int lm_lds_bucket_for(int nv) {
+  {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    for (size_t r = 0; r < g_table_nv.size(); ++r)
+      if (nv <= g_table_nv[r]) return g_table_shape[r];
+  }
   for (int b : shape_preference())
     if (nv <= kShapeT[b] * kShapeVPT[b]) return b;
   return -1;

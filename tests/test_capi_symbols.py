@@ -41,3 +41,25 @@ def test_host_only_calls():
     N = C.c_int32()
     rc = lib.mo_template_info(12345, C.byref(N), None, None)
     assert rc == -1 and b"param_id" in lib.mo_last_error()
+
+
+def test_lm_shape_table_host_calls():
+    """The LM launch-shape table is host state: shapes are listed, a fitting
+    ascending table installs, and bad tables are rejected (no device work)."""
+    import pytest
+    from meshode_amd import autotune
+    shp = autotune.shapes()
+    assert len(shp) == 11 and shp[0] == (1024, 1) and max(t * v for t, v in shp) == autotune.LDS_MAX_VERTICES
+    assert autotune.size_classes() == [1024, 2048, 4096, 5120, 6144, 8192]
+    try:
+        autotune.set_table([(1024, 4), (5120, 3)])          # 512x8 for <= 1024, 1024x5 for <= 5120
+        with pytest.raises(_lib.MoError):
+            autotune.set_table([(2048, 0)])                 # 1024x1 holds 1024 vertices only
+        with pytest.raises(_lib.MoError):
+            autotune.set_table([(4096, 2), (2048, 1)])      # not ascending
+        with pytest.raises(_lib.MoError):
+            autotune.set_table([(100, 11)])                 # no such shape
+        with pytest.raises(_lib.MoError):
+            _lib.call("mo_lm_shape_info", 11, None, None)
+    finally:
+        autotune.set_table([])
