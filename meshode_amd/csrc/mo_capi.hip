@@ -445,6 +445,8 @@ struct PhaseTimer {
   }
 };
 
+std::mutex g_team_mu;   // one multi-workgroup LM launch at a time (lm_core)
+
 int device_lds_limit() {
   static int cached = -1;
   if (cached < 0) {
@@ -571,10 +573,81 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
       if ((rc = lm_lds_launch(b, d_pairs, d_plist + start[b], (int)bucket[b].size(), A, dopt, d_res, nvalloc,
                               bucket_lds[b], s)))
         return rc;
-  if (!bucket[nb].empty())
-    if ((rc = lm_solve_launch(d_pairs, d_plist + start[nb], (int)bucket[nb].size(), A, dopt, d_res, nvalloc,
-                              opt.block_threads, s)))
-      return rc;
+  // Pairs the register/LDS kernel does not take.  A pair of nv vertices
+  // gets a team of ceil(nv / (4 block_threads)) co-resident workgroups
+  // (at most lm_team_max(); kernel_variant 2: ceil(nv / block_threads), and
+  // every pair goes here), a function of the pair alone so its result does
+  // not depend on the rest of the batch.  Teams of one run on k_lm_solve
+  // (kernel_variant 2 keeps them on the team kernel); the others are packed,
+  // largest first, into cooperative launches of at most lm_team_capacity()
+  // workgroups.  One team launch at a time in the process (g_team_mu, held
+  // until the final stream synchronization): two cooperative grids from
+  // different streams must not wait on each other's residency.
+  std::vector<int> solo, teams, team_c;
+  std::vector<int4> wg;
+  std::unique_lock<std::mutex> team_lock(g_team_mu, std::defer_lock);
+  int* d_abort = nullptr;
+  Arena tar;
+  {
+    const int bt = opt.block_threads;
+    const long long vpw = opt.kernel_variant == 2 ? bt : 4LL * bt;
+    const int cap = opt.kernel_variant == 1 ? 0 : lm_team_capacity(bt);
+    for (int p : bucket[nb]) {
+      const long long c = std::min<long long>(std::max<long long>((pairs[p].nv + vpw - 1) / vpw, 1), lm_team_max());
+      if (cap >= lm_team_max() && (c > 1 || opt.kernel_variant == 2)) {
+        teams.push_back(p);
+        team_c.push_back((int)c);
+      } else {
+        solo.push_back(p);
+      }
+    }
+    if (!solo.empty()) {
+      MO_HIP(hipMemcpyAsync(d_plist + start[nb], solo.data(), solo.size() * sizeof(int), hipMemcpyHostToDevice, s));
+      if ((rc = lm_solve_launch(d_pairs, d_plist + start[nb], (int)solo.size(), A, dopt, d_res, nvalloc, bt, s)))
+        return rc;
+    }
+    if (!teams.empty()) {
+      const int nt = (int)teams.size();
+      const int per = 2 * lm_team_max() * 8;
+      long long nwg_all = 0;
+      for (int c : team_c) nwg_all += c;
+      if ((rc = tar.reserve(al(nt * sizeof(int)) + al(nwg_all * sizeof(int4)) + al((size_t)nt * 32 * sizeof(unsigned) +
+                                                                                       sizeof(int)) +
+                                al((size_t)nt * per * sizeof(double)) + 512,
+                            s)))
+        return rc;
+      int* d_tp = tar.take<int>(nt);
+      int4* d_wg = tar.take<int4>(nwg_all);
+      unsigned* d_cnt = tar.take<unsigned>((size_t)nt * 32 + 1);
+      double* d_part = tar.take<double>((size_t)nt * per);
+      d_abort = reinterpret_cast<int*>(d_cnt + (size_t)nt * 32);
+      if ((rc = zero_fill(d_cnt, ((size_t)nt * 32 + 1) * sizeof(unsigned), s))) return rc;
+      // chunks: consecutive teams (largest first) whose workgroups fit one grid
+      std::vector<int> chunk_team{0}, chunk_wg{0};
+      int used = 0;
+      for (int t = 0; t < nt; ++t) {
+        if (used + team_c[t] > cap) {
+          chunk_team.push_back(t);
+          chunk_wg.push_back((int)wg.size());
+          used = 0;
+        }
+        const int rel = t - chunk_team.back();
+        for (int r = 0; r < team_c[t]; ++r) wg.push_back(make_int4(rel, r, team_c[t], 0));
+        used += team_c[t];
+      }
+      chunk_team.push_back(nt);
+      chunk_wg.push_back((int)wg.size());
+      MO_HIP(hipMemcpyAsync(d_tp, teams.data(), nt * sizeof(int), hipMemcpyHostToDevice, s));
+      MO_HIP(hipMemcpyAsync(d_wg, wg.data(), wg.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+      team_lock.lock();
+      for (size_t k = 0; k + 1 < chunk_team.size(); ++k) {
+        const int t0 = chunk_team[k];
+        if ((rc = lm_team_launch(d_pairs, d_tp + t0, d_wg + chunk_wg[k], chunk_wg[k + 1] - chunk_wg[k], A, dopt, d_res,
+                                 nvalloc, bt, d_cnt + (size_t)t0 * 32, d_part + (size_t)t0 * per, d_abort, s)))
+          return rc;
+      }
+    }
+  }
   if ((rc = timer.mark(3))) return rc;
   for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
@@ -584,8 +657,11 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   }
   MO_HIP(hipGetLastError());
   std::vector<LMResultDev> h_res(np);
+  int h_abort = 0;
   MO_HIP(hipMemcpyAsync(h_res.data(), d_res, np * sizeof(LMResultDev), hipMemcpyDeviceToHost, s));
+  if (d_abort) MO_HIP(hipMemcpyAsync(&h_abort, d_abort, sizeof(int), hipMemcpyDeviceToHost, s));
   MO_HIP(hipStreamSynchronize(s));
+  MO_CHECK(h_abort == 0, MO_ERR_HIP, "LM team solve: a barrier wait exceeded its 20 s bound (results invalid)");
   if (stats) {
     for (int p = 0; p < np; ++p) {
       const LMResultDev& r = h_res[p];

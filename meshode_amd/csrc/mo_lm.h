@@ -68,6 +68,19 @@ int lm_solve_launch(const LMPairDesc* pairs_dev, const int* plist, int npairs, c
                     const LMOptionsDev& opt, LMResultDev* results_dev, long long total_vertices, int block_threads,
                     hipStream_t s);
 
+// Multi-workgroup kernel (k_lm_solve_team): team t of wg_dev's entries
+// (team, rank, size, -) solves pair plist[t] on `size` co-resident
+// workgroups (cooperative launch, nwg <= lm_team_capacity).  cnt: one
+// zeroed counter per team at a 128-byte stride (32 unsigned); part: 2 *
+// lm_team_max() * 8 doubles per team; abort_flag: one zeroed int, set when a
+// barrier wait exceeded its bound.
+int lm_team_max();
+int lm_team_capacity(int block_threads);
+int lm_team_launch(const LMPairDesc* pairs_dev, const int* plist, const int4* wg_dev, int nwg,
+                   const LMArrays& arrays, const LMOptionsDev& opt, LMResultDev* results_dev,
+                   long long total_vertices, int block_threads, unsigned* cnt, double* part, int* abort_flag,
+                   hipStream_t s);
+
 // Register/LDS-resident kernel (mo_lm_lds.hip) for pairs of at most 8192
 // vertices whose adjacency multiplicities are 1 or 2, in size buckets.
 int lm_lds_buckets();

@@ -31,6 +31,7 @@
 #include "mo_lm_util.h"
 #include "mo_lookup.h"
 
+#include <algorithm>
 #include <cfloat>
 
 namespace mo {
@@ -41,13 +42,87 @@ namespace {
 enum Slot { kG = 0, kGR, kS, kDG, kDT, kMI0, kMI1, kDL, kR, kZ, kP0, kP1, kQ, kNumSlots };
 static_assert(kNumSlots == kLMWorkSlots, "work slot count is part of the host contract");
 
-__global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict__ pairs,
-                                                   const int* __restrict__ plist, LMArrays A, LMOptionsDev opt,
-                                                   LMResultDev* __restrict__ res, long long totalV) {
-  __shared__ double red[2 * 8 * kMaxWaves];
-  const int pid = plist[blockIdx.x];
-  const LMPairDesc P = pairs[pid];
+// One pair's workgroup(s).  SoloTeam: the whole pair on one workgroup.
+struct SoloTeam {
+  __device__ int rank() const { return 0; }
+  __device__ int size() const { return 1; }
+  __device__ bool leader() const { return true; }
+  __device__ void barrier() { __syncthreads(); }
+  template <int K, int M>
+  __device__ void reduce(double (&v)[K + M], double* red, int& sel) { block_reduce<K, M>(v, red, sel); }
+};
+
+// GridTeam: the pair's vertices spread over `c` co-resident workgroups
+// (vertex i belongs to workgroup (i / T) % c).  A reduction is the
+// workgroup's own block_reduce, written to a per-pair slot in global memory,
+// a team barrier, then the c partials summed in rank order by every
+// workgroup, so all of them hold the same bits.  The barrier is an arrival
+// counter per pair (monotonic; workgroup w waits for (n + 1) c arrivals at
+// its n-th barrier) with agent-scope release/acquire fences around it: the
+// team's workgroups may sit on different XCDs, whose L2 caches are not
+// coherent with each other.  Every wait is bounded by `limit` device clock
+// ticks; a wait that runs out sets the shared abort flag, after which no
+// barrier waits (the solve finishes in bounded time and the host reports
+// the error).
+constexpr int kMaxTeam = 16;
+
+struct GridTeam {
+  int r, c;
+  unsigned* cnt;        // this pair's arrival counter
+  double* part;         // this pair's partials [2][kMaxTeam][8]
+  int* abort_flag;
+  long long limit;
+  unsigned arrivals;
+  int gsel;
+  __device__ int rank() const { return r; }
+  __device__ int size() const { return c; }
+  __device__ bool leader() const { return r == 0; }
+  __device__ void barrier() {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned target = (arrivals + 1u) * (unsigned)c;
+      const long long t0 = wall_clock64();
+      while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (wall_clock64() - t0 > limit) {
+          __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __threadfence();
+    }
+    ++arrivals;
+    __syncthreads();
+  }
+  template <int K, int M>
+  __device__ void reduce(double (&v)[K + M], double* red, int& sel) {
+    block_reduce<K, M>(v, red, sel);
+    double* slot = part + gsel * (kMaxTeam * 8);
+    gsel ^= 1;
+    if (threadIdx.x == 0)
+      for (int k = 0; k < K + M; ++k) slot[r * 8 + k] = v[k];
+    barrier();
+    for (int k = 0; k < K; ++k) {
+      double t = 0.0;
+      for (int q = 0; q < c; ++q) t += slot[q * 8 + k];
+      v[k] = t;
+    }
+    for (int k = K; k < K + M; ++k) {
+      double t = slot[k];
+      for (int q = 1; q < c; ++q) t = slot[q * 8 + k] > t ? slot[q * 8 + k] : t;
+      v[k] = t;
+    }
+  }
+};
+
+template <class Team>
+__device__ inline void lm_solve_pair(const LMPairDesc P, LMResultDev* out, const LMArrays& A,
+                                     const LMOptionsDev& opt, long long totalV, Team& team, double* red) {
   const int T = blockDim.x, tid = threadIdx.x;
+  const int i0 = team.rank() * T + tid, stride = team.size() * T;
   const int nv = P.nv;
   const long long vb = P.vbase;
   const float l2 = P.lambda * P.lambda;
@@ -79,7 +154,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
   // ---- evaluate residuals, gradient J^T f, cost at the current u --------
   auto eval_jacobian = [&](bool first, double& cv, double& ce, double& gmax) {
     double v[3] = {0.0, 0.0, 0.0};
-    for (int i = tid; i < nv; i += T) {
+    for (int i = i0; i < nv; i += stride) {
       const float4 x0 = X0[i];
       const float4 ui = U[i];
       const JetF J = lookup_jet(grid, N, x0.x + ui.x, x0.y + ui.y, x0.z + ui.z);
@@ -109,7 +184,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
         S[i] = make_float4(1.0f / (1.0f + sqrtf(c0)), 1.0f / (1.0f + sqrtf(c1)), 1.0f / (1.0f + sqrtf(c2)), 0.f);
       }
     }
-    block_reduce<2, 1>(v, red, sel);
+    team.template reduce<2, 1>(v, red, sel);
     cv = v[0]; ce = v[1]; gmax = v[2];
   };
 
@@ -117,7 +192,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
   auto setup_linear = [&](bool reuse, double radius, double& rz_out, double& rr_out) {
     double v[2] = {0.0, 0.0};
     const double inv_radius = 1.0 / radius;
-    for (int i = tid; i < nv; i += T) {
+    for (int i = i0; i < nv; i += stride) {
       const float4 g = G[i];
       const float deg = X0[i].w;
       const float4 s = S[i];
@@ -172,7 +247,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
       v[0] += (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
       v[1] += (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
     }
-    block_reduce<2, 0>(v, red, sel);
+    team.template reduce<2, 0>(v, red, sel);
     rz_out = v[0];
     rr_out = v[1];
   };
@@ -192,7 +267,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
       const float fb = (float)beta;
       const bool first = (k == 0);
       double v1[1] = {0.0};
-      for (int i = tid; i < nv; i += T) {
+      for (int i = i0; i < nv; i += stride) {
         const float4 zi = Z[i];
         float4 pi = zi;
         if (!first) { const float4 po = Pold[i]; pi.x += fb * po.x; pi.y += fb * po.y; pi.z += fb * po.z; }
@@ -216,13 +291,13 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
         Q[i] = make_float4(qx, qy, qz, 0.f);
         v1[0] += (double)pi.x * qx + (double)pi.y * qy + (double)pi.z * qz;
       }
-      block_reduce<1, 0>(v1, red, sel);
+      team.template reduce<1, 0>(v1, red, sel);
       const double pq = v1[0];
       if (!(pq > 0.0)) break;
       const double alpha = rz / pq;
       const float fa = (float)alpha;
       double v2[2] = {0.0, 0.0};
-      for (int i = tid; i < nv; i += T) {
+      for (int i = i0; i < nv; i += stride) {
         const float4 pi = Pnew[i];
         const float4 qi = Q[i];
         float4 d = DL[i];
@@ -239,7 +314,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
         v2[0] += (double)r.x * z0 + (double)r.y * z1 + (double)r.z * z2;
         v2[1] += (double)r.x * r.x + (double)r.y * r.y + (double)r.z * r.z;
       }
-      block_reduce<2, 0>(v2, red, sel);
+      team.template reduce<2, 0>(v2, red, sel);
       beta = v2[0] / rz;
       rz = v2[0];
       if (v2[1] <= tol2) { ++k; break; }
@@ -250,7 +325,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
   // ---- model cost change, candidate cost, step and parameter norms ------
   auto eval_candidate = [&]() -> LMCandidate {
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int i = tid; i < nv; i += T) {
+    for (int i = i0; i < nv; i += stride) {
       const float4 x0 = X0[i];
       const float4 ui = U[i];
       const float4 di = DL[i];
@@ -281,7 +356,7 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
       const float px = x0.x + ui.x, py = x0.y + ui.y, pz = x0.z + ui.z;
       v[4] += (double)px * px + (double)py * py + (double)pz * pz;
     }
-    block_reduce<5, 0>(v, red, sel);
+    team.template reduce<5, 0>(v, red, sel);
     return LMCandidate{-v[0], v[1] + v[2], sqrt(v[3]), sqrt(v[4])};
   };
 
@@ -292,15 +367,37 @@ __global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict_
     return pcg(rz, rr);
   };
   auto accept = [&]() {
-    for (int i = tid; i < nv; i += T) {
+    for (int i = i0; i < nv; i += stride) {
       float4 ui = U[i];
       const float4 di = DL[i];
       ui.x += di.x; ui.y += di.y; ui.z += di.z;
       U[i] = ui;
     }
-    __syncthreads();
+    team.barrier();
   };
-  lm_trust_region_loop(opt, eval, solve, eval_candidate, accept, res + pid, clock_start);
+  lm_trust_region_loop(opt, eval, solve, eval_candidate, accept, team.leader() ? out : nullptr, clock_start);
+}
+
+__global__ void __launch_bounds__(1024) k_lm_solve(const LMPairDesc* __restrict__ pairs,
+                                                   const int* __restrict__ plist, LMArrays A, LMOptionsDev opt,
+                                                   LMResultDev* __restrict__ res, long long totalV) {
+  __shared__ double red[2 * 8 * kMaxWaves];
+  const int pid = plist[blockIdx.x];
+  SoloTeam team;
+  lm_solve_pair(pairs[pid], res + pid, A, opt, totalV, team, red);
+}
+
+// wg[b] = (team, rank, team size, -): team t solves pair plist[t].
+__global__ void __launch_bounds__(1024) k_lm_solve_team(const LMPairDesc* __restrict__ pairs,
+                                                        const int* __restrict__ plist, const int4* __restrict__ wg,
+                                                        LMArrays A, LMOptionsDev opt, LMResultDev* __restrict__ res,
+                                                        long long totalV, unsigned* cnt, double* part,
+                                                        int* abort_flag, long long limit) {
+  __shared__ double red[2 * 8 * kMaxWaves];
+  const int4 m = wg[blockIdx.x];
+  const int pid = plist[m.x];
+  GridTeam team{m.y, m.z, cnt + (size_t)m.x * 32, part + (size_t)m.x * 2 * kMaxTeam * 8, abort_flag, limit, 0u, 0};
+  lm_solve_pair(pairs[pid], res + pid, A, opt, totalV, team, red);
 }
 
 }  // namespace
@@ -314,6 +411,39 @@ int lm_solve_launch(const LMPairDesc* pairs_dev, const int* plist, int npairs, c
   hipLaunchKernelGGL(k_lm_solve, dim3(npairs), dim3(block_threads), 0, s, pairs_dev, plist, arrays, opt,
                      results_dev, total_vertices);
   MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+int lm_team_max() { return kMaxTeam; }
+
+int lm_team_capacity(int block_threads) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_lm_solve_team, block_threads, 0) !=
+          hipSuccess)
+    return 0;
+  return per * cus;
+}
+
+int lm_team_launch(const LMPairDesc* pairs_dev, const int* plist, const int4* wg_dev, int nwg,
+                   const LMArrays& arrays, const LMOptionsDev& opt, LMResultDev* results_dev,
+                   long long total_vertices, int block_threads, unsigned* cnt, double* part, int* abort_flag,
+                   hipStream_t s) {
+  if (nwg <= 0) return MO_OK;
+  MO_CHECK(block_threads >= 64 && block_threads <= 1024 && block_threads % 64 == 0, MO_ERR_ARG,
+           "block_threads must be a multiple of 64 in [64, 1024] (got %d)", block_threads);
+  int dev = 0, khz = 0;
+  MO_HIP(hipGetDevice(&dev));
+  MO_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  long long limit = (long long)std::max(khz, 1) * 1000LL * 20;   // a 20 s bound on any one wait
+  LMArrays a = arrays;
+  LMOptionsDev o = opt;
+  LMResultDev* r = results_dev;
+  long long tv = total_vertices;
+  void* args[] = {(void*)&pairs_dev, (void*)&plist, (void*)&wg_dev, (void*)&a, (void*)&o, (void*)&r, (void*)&tv,
+                  (void*)&cnt, (void*)&part, (void*)&abort_flag, (void*)&limit};
+  MO_HIP(hipLaunchCooperativeKernel((const void*)k_lm_solve_team, dim3(nwg), dim3(block_threads), args, 0, s));
   return MO_OK;
 }
 

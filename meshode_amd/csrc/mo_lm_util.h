@@ -59,7 +59,8 @@ struct LMCandidate {
 //   solve(reuse_diagonal, radius)    LM step delta (returns PCG iterations)
 //   candidate()                      -> LMCandidate
 //   accept()                         x <- x + delta
-// Writes the per-pair result from thread 0.
+// Writes the per-pair result from thread 0 when out is not null (a
+// multi-workgroup solve passes it to its first workgroup only).
 template <class Eval, class Solve, class Cand, class Accept>
 __device__ inline void lm_trust_region_loop(const LMOptionsDev opt, Eval eval, Solve solve, Cand candidate,
                                             Accept accept, LMResultDev* out, long long clock_start) {
@@ -112,7 +113,7 @@ __device__ inline void lm_trust_region_loop(const LMOptionsDev opt, Eval eval, S
       ++nfail;
     }
   }
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && out) {
     LMResultDev r;
     r.initial_cost = initial_cost;
     r.final_cost = cv + ce;

@@ -160,3 +160,42 @@ def test_lm_large_pair_energy_parity(gpu, oracle):
     _say(f"[lm] large: oracle E={ref:.8g} it={int(z['iterations'])}  gpu E={eg:.8g} it={st['iterations']}")
     assert abs(eg - ref) <= ENERGY_RTOL * ref
     assert abs(st["final_cost"] - eg) <= 1e-4 * eg
+
+
+@pytest.mark.parametrize("case", ["sphere_l2_n32", "box1000_n32", "ico4_n32"])
+def test_lm_team_kernel_energy_parity(gpu, oracle, case):
+    """The multi-workgroup kernel (k_lm_solve_team) forced with 256-thread
+    workgroups (kernel_variant 2: a team of ceil(nv / 256)): a team of one
+    for the 162-vertex sphere, 4 workgroups for the box union, 11 for the
+    2562-vertex icosphere.  Same energy bar as the other kernels."""
+    _say(f"[lm] case={case} kernel_variant=2 block_threads=256")
+    if case == "sphere_l2_n32":
+        pair = synth.displaced_sphere_pair(11, src_level=2, tgt_level=3)
+    elif case == "box1000_n32":
+        pair = synth.box_union_pair(3, n_boxes=4, n_verts=1000)
+    else:
+        pair = synth.displaced_sphere_pair(13, src_level=4, tgt_level=4)
+    so, eg, st = _parity(oracle, pair, 32, kernel_variant=2, block_threads=256)
+    _say(f"[lm] {case} team: oracle E={so['final_cost']:.8g}  gpu E={eg:.8g} it={st['iterations']}")
+    assert st["final_cost"] < st["initial_cost"]
+    assert abs(eg - so["final_cost"]) <= ENERGY_RTOL * so["final_cost"]
+    assert abs(st["final_cost"] - eg) <= 1e-4 * eg
+
+
+def test_team_kernel_alone_equals_in_the_mix(gpu):
+    """A team's size depends on its pair alone, so each result is
+    bit-identical alone, inside a mixed batch, and on a second run."""
+    _say("[lm] case=team alone_vs_mix")
+    pairs = [synth.box_union_pair(21, n_boxes=6, n_verts=2000),
+             synth.displaced_sphere_pair(22, src_level=3, tgt_level=4),
+             synth.displaced_sphere_pair(24, src_level=2, tgt_level=3)]
+    opt = deform.default_options(kernel_variant=2, block_threads=256)
+    mix, st_mix = deform.rigid_deform_batch(pairs, grid_resolution=[48, 32, 32], lambdas=[1.0, 0.5, 2.0],
+                                            options=opt)
+    again, _ = deform.rigid_deform_batch(pairs, grid_resolution=[48, 32, 32], lambdas=[1.0, 0.5, 2.0], options=opt)
+    for i, p in enumerate(pairs):
+        alone, st = deform.rigid_deform_batch([p], grid_resolution=[48, 32, 32][i], lambdas=[1.0, 0.5, 2.0][i],
+                                              options=opt)
+        assert torch.equal(alone[0], mix[i]), i
+        assert torch.equal(again[i], mix[i]), i
+        assert st[0]["final_cost"] == st_mix[i]["final_cost"]
