@@ -308,7 +308,7 @@ def _autotune(args, pairs, grid, lam, opt, repeats=2):
     """Run the launch-shape autotuner on the workload's pairs before the
     warm-up (untimed) and install its table; returns the shape summary for
     the JSON line.  Sets opt.block_threads when the workload has pairs for
-    the global-memory kernel."""
+    the global-memory kernels (and opt.team_vertices)."""
     if args.no_autotune or args.kernel_variant != 0 or "MO_LM_SHAPES" in os.environ:
         return os.environ.get("MO_LM_SHAPES", "built-in order")
     from meshode_amd import autotune
@@ -317,6 +317,7 @@ def _autotune(args, pairs, grid, lam, opt, repeats=2):
                         max_pairs=len(pairs))
     if res["block_threads"]:
         opt.block_threads = res["block_threads"]
+        opt.team_vertices = res["team_vertices"]
     desc = autotune.describe(res)
     print(f"[bench] autotuned launch shapes in {time.perf_counter() - t0:.1f}s: {desc}", file=sys.stderr, flush=True)
     return "autotuned: " + desc
@@ -376,7 +377,7 @@ def bench_cfg4(args):
                        "pairs": n, "levels": {str(L): int((levels == L).sum()) for L in (3, 4, 5, 6)},
                        "grids": {str(g): grids.count(g) for g in (64, 128)}, "lambda": "U(0.5, 2)",
                        "lm_iterations": 50, "pcg_iterations": 30, "seed": 4, "lm_shapes": shapes_used,
-                       "block_threads": opt.block_threads}}
+                       "block_threads": opt.block_threads, "team_vertices": opt.team_vertices}}
 
 
 def bench_serve(args):

@@ -157,6 +157,8 @@ typedef struct mo_lm_options {
   int32_t kernel_variant;                /* 0: register/LDS kernel when a pair fits, else the
                                             global-memory kernel; 1: global-memory kernel only */
   int32_t max_successful_steps;          /* > 0: stop after that many accepted steps (0) */
+  int32_t team_vertices;                 /* pairs on the multi-workgroup kernel: vertices per
+                                            workgroup of a pair's team (0: 4 * block_threads) */
 } mo_lm_options;
 
 typedef struct mo_lm_stats {

@@ -61,6 +61,7 @@ class LMOptions(C.Structure):
         ("timing", C.c_int32),
         ("kernel_variant", C.c_int32),
         ("max_successful_steps", C.c_int32),
+        ("team_vertices", C.c_int32),
     ]
 
 

@@ -10,9 +10,10 @@ caller's own pairs, one size class at a time, with the library's phase
 timer (HIP events around the LM launch), and installs the fastest shape of
 each class as the library's size-class table (``mo_lm_set_shape_table``).
 Pairs that the register/LDS kernel cannot take (more than 8192 vertices, or
-edge multiplicities above 2) run on the global-memory kernel, whose block
-size (``block_threads``, threads per pair) is tuned the same way and
-returned for the caller's options.
+edge multiplicities above 2) run on the global-memory kernels, a team of
+ceil(nv / team_vertices) co-resident workgroups of ``block_threads``
+threads per pair; those two options are tuned the same way and returned
+for the caller's options.
 
 The reference has no counterpart (its Ceres solve is single-threaded,
 deformer.cc:55-59).  A shape changes only how a pair's vertices are spread
@@ -66,7 +67,7 @@ def _lm_ms(batch, options, repeats):
 
 
 def tune(pairs, grid_resolution=64, lambdas=1.0, options=None, repeats=3, max_pairs=256,
-         block_threads=(256, 512, 1024), install=True):
+         block_threads=(512, 1024), team_factors=(2, 4, 8), install=True):
     """Time the launch shapes on ``pairs`` [(src_V, src_F, tgt_V, tgt_F)] as
     the rigid_deform pipeline runs them (``deform.PairBatch``) and pick the
     fastest per size class.
@@ -76,9 +77,11 @@ def tune(pairs, grid_resolution=64, lambdas=1.0, options=None, repeats=3, max_pa
     the work, so tune with the ones you will use); at most ``max_pairs``
     pairs per class are timed.  With ``install`` the table is installed for
     every later call in this process (else the built-in order is restored).
+    Pairs above 8192 vertices are timed at every block_threads and
+    team_vertices = factor * block_threads.
     Returns {"table": [(max_vertices, threads, vpt, lm_ms)], "candidates":
-    {max_vertices: {"TxVPT": lm_ms}}, "block_threads": best or None,
-    "block_threads_ms": {bt: lm_ms}}."""
+    {max_vertices: {"TxVPT": lm_ms}}, "block_threads" and "team_vertices":
+    the best (None without large pairs), "large_ms": {"bt/tv": lm_ms}}."""
     n = len(pairs)
     gr = list(grid_resolution) if isinstance(grid_resolution, (list, tuple)) else [grid_resolution] * n
     lam = list(lambdas) if isinstance(lambdas, (list, tuple)) else [lambdas] * n
@@ -119,22 +122,25 @@ def tune(pairs, grid_resolution=64, lambdas=1.0, options=None, repeats=3, max_pa
             del batch
     finally:
         set_table(table if install else [])
-    bt_ms, best_bt = {}, None
+    large_ms, best = {}, (None, None)
     if large:
         batch = sub(large)
         for bt in block_threads:
-            opt.block_threads = int(bt)
-            bt_ms[int(bt)] = _lm_ms(batch, opt, repeats)
-        best_bt = min(bt_ms, key=bt_ms.get)
+            for f in team_factors:
+                opt.block_threads, opt.team_vertices = int(bt), int(f * bt)
+                large_ms[(int(bt), int(f * bt))] = _lm_ms(batch, opt, repeats)
+        best = min(large_ms, key=large_ms.get)
         del batch
     torch.cuda.synchronize()
     return {"table": [(ub, shp[b][0], shp[b][1], cand_ms[ub][f"{shp[b][0]}x{shp[b][1]}"]) for ub, b in table],
-            "candidates": cand_ms, "block_threads": best_bt, "block_threads_ms": bt_ms}
+            "candidates": cand_ms, "block_threads": best[0], "team_vertices": best[1],
+            "large_ms": {f"{bt}/{tv}": m for (bt, tv), m in large_ms.items()}}
 
 
 def describe(result):
-    """One-line summary of a tune() result, e.g. '<=5120: 1024x5; global: 1024 threads'."""
+    """One-line summary of a tune() result, e.g.
+    '<=5120: 1024x5; teams: 1024 threads, 4096 vertices per workgroup'."""
     parts = [f"<={ub}: {t}x{v}" for ub, t, v, _ in result["table"]]
     if result["block_threads"]:
-        parts.append(f"global: {result['block_threads']} threads")
+        parts.append(f"teams: {result['block_threads']} threads, {result['team_vertices']} vertices per workgroup")
     return "; ".join(parts) if parts else "built-in"

@@ -574,10 +574,11 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
                               bucket_lds[b], s)))
         return rc;
   // Pairs the register/LDS kernel does not take.  A pair of nv vertices
-  // gets a team of ceil(nv / (4 block_threads)) co-resident workgroups
-  // (at most lm_team_max(); kernel_variant 2: ceil(nv / block_threads), and
-  // every pair goes here), a function of the pair alone so its result does
-  // not depend on the rest of the batch.  Teams of one run on k_lm_solve
+  // gets a team of ceil(nv / team_vertices) co-resident workgroups (at most
+  // lm_team_max(); team_vertices 0 means 4 block_threads, or block_threads
+  // under kernel_variant 2, which sends every pair here), a function of the
+  // pair and the options alone so its result does not depend on the rest of
+  // the batch.  Teams of one run on k_lm_solve
   // (kernel_variant 2 keeps them on the team kernel); the others are packed,
   // largest first, into cooperative launches of at most lm_team_capacity()
   // workgroups.  One team launch at a time in the process (g_team_mu, held
@@ -590,7 +591,8 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   Arena tar;
   {
     const int bt = opt.block_threads;
-    const long long vpw = opt.kernel_variant == 2 ? bt : 4LL * bt;
+    MO_CHECK(opt.team_vertices >= 0, MO_ERR_ARG, "team_vertices must be >= 0");
+    const long long vpw = opt.team_vertices > 0 ? opt.team_vertices : (opt.kernel_variant == 2 ? bt : 4LL * bt);
     const int cap = opt.kernel_variant == 1 ? 0 : lm_team_capacity(bt);
     for (int p : bucket[nb]) {
       const long long c = std::min<long long>(std::max<long long>((pairs[p].nv + vpw - 1) / vpw, 1), lm_team_max());
@@ -826,6 +828,7 @@ MO_EXPORT void mo_lm_default_options(mo_lm_options* o) {
   o->timing = 0;
   o->kernel_variant = 0;
   o->max_successful_steps = 0;
+  o->team_vertices = 0;
 }
 
 MO_EXPORT int32_t mo_lm_shape_count(void) { return lm_lds_buckets(); }

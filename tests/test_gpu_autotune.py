@@ -64,14 +64,16 @@ def test_tuner_installs_the_fastest_fitting_shapes(gpu):
 
 
 def test_tuner_block_threads_for_large_pairs(gpu):
-    """A pair above 8192 vertices is timed on the global-memory kernel at
-    each block size; the table stays empty."""
+    """A pair above 8192 vertices is timed on the multi-workgroup kernel at
+    each block size and team size; the table stays empty."""
     _say("[autotune] case=tune L5 (global kernel)")
     pairs = [synth.icosphere_pair(30, 5)]
     opt = deform.default_options(max_num_iterations=3)
     try:
-        res = autotune.tune(pairs, grid_resolution=32, options=opt, repeats=1, block_threads=(512, 1024))
-        _say(f"[autotune] {autotune.describe(res)} {res['block_threads_ms']}")
+        res = autotune.tune(pairs, grid_resolution=32, options=opt, repeats=1, block_threads=(512, 1024),
+                            team_factors=(2, 4))
+        _say(f"[autotune] {autotune.describe(res)} {res['large_ms']}")
         assert res["table"] == [] and res["block_threads"] in (512, 1024)
+        assert res["team_vertices"] in (2 * res["block_threads"], 4 * res["block_threads"])
     finally:
         autotune.set_table([])
