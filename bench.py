@@ -392,8 +392,8 @@ def bench_serve(args):
         if i % 2 == 0:
             p = synth.box_union_pair(1000 + i, n_verts=max(nv - nv % 2, 1000))   # box unions have even counts
         else:
-            lvl = 3 if nv < 4000 else 4
-            p = synth.displaced_sphere_pair(1000 + i, src_level=lvl, tgt_level=lvl + 1)
+            lvl = 3 if nv < 4000 else 4 if nv < 15000 else 5 if nv < 60000 else 6
+            p = synth.displaced_sphere_pair(1000 + i, src_level=lvl, tgt_level=min(lvl + 1, 6))
         grid = int(rng.choice([64, 128, 256]))
         pool.append((p, grid))
     n_req = int(args.rate * args.duration)
