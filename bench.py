@@ -88,7 +88,8 @@ def main():
     ap.add_argument("--lam", type=float, default=1.0)
     ap.add_argument("--plain-pairs", type=int, default=8, help="pairs run one at a time for the plain GPU rate")
     ap.add_argument("--block-threads", type=int, default=1024, help="threads per pair of the global-memory kernel")
-    ap.add_argument("--kernel-variant", type=int, default=0, help="0: register/LDS kernel when a pair fits; 1: global")
+    ap.add_argument("--kernel-variant", type=int, default=0,
+                    help="0: register/LDS kernel when a pair fits, else teams; 1: one-workgroup global; 2: teams")
     ap.add_argument("--pcg-tol", type=float, default=None, help="inner PCG relative tolerance (library default if unset)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",

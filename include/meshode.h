@@ -155,7 +155,9 @@ typedef struct mo_lm_options {
   int32_t block_threads;                 /* threads per pair of the global-memory kernel (1024) */
   int32_t timing;                        /* nonzero: record phase times (mo_last_timings) */
   int32_t kernel_variant;                /* 0: register/LDS kernel when a pair fits, else the
-                                            global-memory kernel; 1: global-memory kernel only */
+                                            global-memory kernels (a team of workgroups per pair
+                                            above team_vertices); 1: one-workgroup global-memory
+                                            kernel only; 2: multi-workgroup kernel for every pair */
   int32_t max_successful_steps;          /* > 0: stop after that many accepted steps (0) */
   int32_t team_vertices;                 /* pairs on the multi-workgroup kernel: vertices per
                                             workgroup of a pair's team (0: 4 * block_threads) */
