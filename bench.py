@@ -99,7 +99,7 @@ def main():
                          "NeuralODE workloads (their own JSON line)")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of Poisson arrivals")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: requests per second")
-    ap.add_argument("--max-verts", type=int, default=8000, help="serve: largest source size")
+    ap.add_argument("--max-verts", type=int, default=100000, help="serve: largest source size")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one plain 256 MiB device copy (known bytes) to scale HBM counters")
     args = ap.parse_args()
