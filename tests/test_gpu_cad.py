@@ -130,11 +130,15 @@ def test_cad_deform_cli(gpu, tmp_path):
 
 
 @pytest.mark.gpu
-def test_adaptive_edge_lm_against_oracle(gpu, oracle):
+@pytest.mark.parametrize("team", [False, True])
+def test_adaptive_edge_lm_against_oracle(gpu, oracle, team):
     """kind 2 (AdaptiveEdgeLoss weights) through mo_deform_batch: full solve
-    and a solve capped at two successful steps, against the oracle."""
+    and a solve capped at two successful steps, against the oracle; with
+    `team`, on the multi-workgroup kernel (kernel_variant 2, 64-thread
+    workgroups: a team of 3 for the 162 vertices)."""
     from meshode_amd import deform, pyDeform
-    print("[cad] case=adaptive-weight LM sphere", flush=True)
+    print(f"[cad] case=adaptive-weight LM sphere team={team}", flush=True)
+    kopt = {"kernel_variant": 2, "block_threads": 64} if team else {}
     sV, sF, tV, tF = synth.displaced_sphere_pair(5, src_level=2, tgt_level=3)
     N = 32
     pid = deform.template_f64(tV, tF, N)
@@ -147,7 +151,7 @@ def test_adaptive_edge_lm_against_oracle(gpu, oracle):
         for cap in (0, 2):
             V = torch.from_numpy(sn.copy()).cuda()
             st = deform.deform_batch([pid], [V], [torch.from_numpy(E).cuda()], [2], [1.0],
-                                     options=deform.default_options(max_successful_steps=cap))[0]
+                                     options=deform.default_options(max_successful_steps=cap, **kopt))[0]
             xo, so = oracle.lm_solve_ex(grid, N, sn, sn, E, lam_e=lam, max_successful=cap)
             eg, _, _ = oracle.lm_cost_ex(grid, N, V.cpu().numpy(), sn, E, lam)
             assert st["final_cost"] == pytest.approx(eg, rel=1e-5)

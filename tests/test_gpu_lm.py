@@ -199,3 +199,22 @@ def test_team_kernel_alone_equals_in_the_mix(gpu):
         assert torch.equal(alone[0], mix[i]), i
         assert torch.equal(again[i], mix[i]), i
         assert st[0]["final_cost"] == st_mix[i]["final_cost"]
+
+
+def test_graph_kind_on_team_kernel(gpu, oracle):
+    """DeformGraph's input (kind 1: an explicit directed edge list) on the
+    multi-workgroup kernel (kernel_variant 2, 128-thread workgroups: a team
+    of 8 for the 1000-vertex box union), against the oracle's LM on the same
+    edges."""
+    _say("[lm] case=graph kind 1 on a team")
+    sV, sF, tV, tF = synth.box_union_pair(4, n_boxes=4, n_verts=1000)
+    N = 32
+    pid, grid, sn, scale, pos = _setup(oracle, (sV, sF, tV, tF), N)
+    E = oracle.face_edges(sF)
+    _, so, _ = oracle.lm_solve(grid, N, sn, E, 0.7)
+    V = torch.from_numpy(sn.copy()).cuda()
+    st = deform.deform_batch([pid], [V], [torch.from_numpy(E).cuda()], [1], [0.7],
+                             options=deform.default_options(kernel_variant=2, block_threads=128))[0]
+    eg, _, _ = oracle.lm_cost(grid, N, V.cpu().numpy(), sn, E, 0.7)
+    _say(f"[lm] graph team: oracle E={so['final_cost']:.8g}  gpu E={eg:.8g} it={st['iterations']}")
+    assert abs(eg - so["final_cost"]) <= ENERGY_RTOL * so["final_cost"]
