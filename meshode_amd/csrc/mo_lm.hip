@@ -64,7 +64,7 @@ struct SoloTeam {
 // ticks; a wait that runs out sets the shared abort flag, after which no
 // barrier waits (the solve finishes in bounded time and the host reports
 // the error).
-constexpr int kMaxTeam = 16;
+constexpr int kMaxTeam = 32;
 
 struct GridTeam {
   int r, c;
