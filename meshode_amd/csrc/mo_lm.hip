@@ -33,6 +33,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstdlib>
 
 namespace mo {
 
@@ -437,6 +438,9 @@ int lm_team_launch(const LMPairDesc* pairs_dev, const int* plist, const int4* wg
   MO_HIP(hipGetDevice(&dev));
   MO_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
   long long limit = (long long)std::max(khz, 1) * 1000LL * 20;   // a 20 s bound on any one wait
+  // MO_LM_TEAM_WAIT_US overrides the bound (microseconds; a negative value
+  // makes the first wait give up at once: the failure path's test hook)
+  if (const char* w = getenv("MO_LM_TEAM_WAIT_US")) limit = atoll(w) * (long long)std::max(khz, 1) / 1000LL;
   LMArrays a = arrays;
   LMOptionsDev o = opt;
   LMResultDev* r = results_dev;

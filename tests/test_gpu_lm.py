@@ -218,3 +218,21 @@ def test_graph_kind_on_team_kernel(gpu, oracle):
     eg, _, _ = oracle.lm_cost(grid, N, V.cpu().numpy(), sn, E, 0.7)
     _say(f"[lm] graph team: oracle E={so['final_cost']:.8g}  gpu E={eg:.8g} it={st['iterations']}")
     assert abs(eg - so["final_cost"]) <= ENERGY_RTOL * so["final_cost"]
+
+
+def test_team_barrier_timeout_is_reported(gpu, monkeypatch):
+    """A team barrier wait that exceeds its bound (forced from the host:
+    MO_LM_TEAM_WAIT_US=-1 makes every wait give up at once) stops all
+    waiting, the kernel still finishes, and the call raises; the next call
+    with the normal bound solves as before."""
+    from meshode_amd import _lib
+    _say("[lm] case=team barrier bound forced")
+    pair = synth.box_union_pair(3, n_boxes=4, n_verts=1000)
+    opt = deform.default_options(kernel_variant=2, block_threads=64, max_num_iterations=5)
+    ref, _ = deform.rigid_deform_batch([pair], grid_resolution=32, options=opt)
+    monkeypatch.setenv("MO_LM_TEAM_WAIT_US", "-1")
+    with pytest.raises(_lib.MoError, match="barrier wait"):
+        deform.rigid_deform_batch([pair], grid_resolution=32, options=opt)
+    monkeypatch.delenv("MO_LM_TEAM_WAIT_US")
+    again, _ = deform.rigid_deform_batch([pair], grid_resolution=32, options=opt)
+    assert torch.equal(ref[0], again[0])
