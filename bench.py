@@ -569,6 +569,17 @@ def bench_rot(args):
     torch.cuda.synchronize()
     single_ms = 1e3 * (time.perf_counter() - t0)
     print(f"[bench] one pair: {single_ms:.1f} ms", file=sys.stderr, flush=True)
+    # the same pair on a team of workgroups (64-thread workgroups, 512 vertices each)
+    topt = deform.default_options(pcg_relative_tolerance=1e-6, block_threads=64, team_vertices=512)
+    V1.copy_(rest[0])
+    deform.deform_rot(pids[0], V1, faces[0], args.lam, options=topt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    V1.copy_(rest[0])
+    stt, _ = deform.deform_rot(pids[0], V1, faces[0], args.lam, options=topt)
+    torch.cuda.synchronize()
+    team_ms = 1e3 * (time.perf_counter() - t0)
+    print(f"[bench] one pair on a team: {team_ms:.1f} ms", file=sys.stderr, flush=True)
     for _ in range(max(args.warmup, 0)):
         deform.deform_rot_batch([pids[i] for i in idx], Vs, [faces[i] for i in idx], [args.lam] * args.pairs)
         for V, i in zip(Vs, idx):
@@ -592,7 +603,9 @@ def bench_rot(args):
            "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True,
            "dtype": "f32 (LM vectors), f64 (reductions)", "data": "synthetic",
            "single_pair_ms": single_ms,
+           "single_pair_team_ms": team_ms,
            "gpu_stats_pair0": {k: st1[k] for k in ("final_cost", "iterations", "pcg_iterations", "termination")},
+           "gpu_stats_pair0_team": {k: stt[k] for k in ("final_cost", "iterations", "pcg_iterations", "termination")},
            "max_iterations_in_batch": int(max(s["iterations"] for s in stb)),
            "config": {"workload": "rigid_rot_deform: %d pairs (%d distinct 12-box unions of %d verts vs jittered "
                                   "targets)" % (args.pairs, len(distinct), args.verts),

@@ -1147,9 +1147,69 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
   MO_HIP(hipGetLastError());
   RotArrays A{d_x0.as<float4>(), d_u.as<float4>(), d_w.as<float4>(), d_adj.as<int4>(), d_list.as<int>(),
               d_hab.as<int2>(), d_vw.as<float4>(), d_hw.as<float4>()};
-  int rc = lm_rot_launch(d_pair.as<RotPairDesc>(), npairs, A, to_dev_options(o), d_res.as<LMResultDev>(), vtot,
-                         hslots, std::min(o.block_threads, 256), s);
-  if (rc) return rc;
+  // team_vertices > 0: each problem on a team of ceil(nv / team_vertices)
+  // co-resident workgroups (k_lm_rot_team; at most lm_team_max()), packed
+  // largest first into cooperative launches, one such launch at a time in
+  // the process (g_team_mu); otherwise one workgroup per problem.
+  const int rbt = std::min(o.block_threads, 256);
+  std::unique_lock<std::mutex> team_lock(g_team_mu, std::defer_lock);
+  DevPtr d_team;
+  int* d_abort = nullptr;
+  int rc;
+  MO_CHECK(o.team_vertices >= 0, MO_ERR_ARG, "team_vertices must be >= 0");
+  const int rcap = o.team_vertices > 0 ? lm_rot_team_capacity(rbt) : 0;
+  if (o.team_vertices > 0 && rcap >= lm_team_max()) {
+    std::vector<int> order(npairs);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nv[a] > nv[b]; });
+    std::vector<int> tc(npairs);
+    long long nwg_all = 0;
+    for (int t = 0; t < npairs; ++t) {
+      tc[t] = (int)std::min<long long>(std::max<long long>((nv[order[t]] + o.team_vertices - 1) / o.team_vertices, 1),
+                                       lm_team_max());
+      nwg_all += tc[t];
+    }
+    std::vector<int4> wg;
+    std::vector<int> chunk_team{0}, chunk_wg{0};
+    int used = 0;
+    for (int t = 0; t < npairs; ++t) {
+      if (used + tc[t] > rcap) {
+        chunk_team.push_back(t);
+        chunk_wg.push_back((int)wg.size());
+        used = 0;
+      }
+      for (int r = 0; r < tc[t]; ++r) wg.push_back(make_int4(t - chunk_team.back(), r, tc[t], 0));
+      used += tc[t];
+    }
+    chunk_team.push_back(npairs);
+    chunk_wg.push_back((int)wg.size());
+    const int per = 2 * lm_team_max() * 8;
+    const size_t b_tp = al(npairs * sizeof(int)), b_wg = al(nwg_all * sizeof(int4)),
+                 b_cnt = al(((size_t)npairs * 32 + 1) * sizeof(unsigned)), b_part = al((size_t)npairs * per * 8);
+    if (int e_ = d_team.alloc(b_tp + b_wg + b_cnt + b_part)) return e_;
+    char* base = d_team.as<char>();
+    int* d_tp = (int*)base;
+    int4* d_wg = (int4*)(base + b_tp);
+    unsigned* d_cnt = (unsigned*)(base + b_tp + b_wg);
+    double* d_part = (double*)(base + b_tp + b_wg + b_cnt);
+    d_abort = (int*)(d_cnt + (size_t)npairs * 32);
+    if ((rc = zero_fill(d_cnt, ((size_t)npairs * 32 + 1) * sizeof(unsigned), s))) return rc;
+    MO_HIP(hipMemcpyAsync(d_tp, order.data(), npairs * sizeof(int), hipMemcpyHostToDevice, s));
+    MO_HIP(hipMemcpyAsync(d_wg, wg.data(), wg.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+    MO_HIP(hipStreamSynchronize(s));   // host vectors above go out of scope
+    team_lock.lock();
+    for (size_t k = 0; k + 1 < chunk_team.size(); ++k) {
+      const int t0 = chunk_team[k];
+      if ((rc = lm_rot_team_launch(d_pair.as<RotPairDesc>(), d_tp + t0, d_wg + chunk_wg[k],
+                                   chunk_wg[k + 1] - chunk_wg[k], A, to_dev_options(o), d_res.as<LMResultDev>(), vtot,
+                                   hslots, rbt, d_cnt + (size_t)t0 * 32, d_part + (size_t)t0 * per, d_abort, s)))
+        return rc;
+    }
+  } else {
+    rc = lm_rot_launch(d_pair.as<RotPairDesc>(), npairs, A, to_dev_options(o), d_res.as<LMResultDev>(), vtot, hslots,
+                       rbt, s);
+    if (rc) return rc;
+  }
   for (int p = 0; p < npairs; ++p) {
     const unsigned blocks = (unsigned)((nv[p] + 255) / 256);
     hipLaunchKernelGGL(k_rot_finish, dim3(blocks), dim3(256), 0, s, V[p], rot ? rot[p] : nullptr, (int)nv[p],
@@ -1157,8 +1217,11 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
   }
   MO_HIP(hipGetLastError());
   std::vector<LMResultDev> r(npairs);
+  int h_abort = 0;
   MO_HIP(hipMemcpyAsync(r.data(), d_res.p, npairs * sizeof(LMResultDev), hipMemcpyDeviceToHost, s));
+  if (d_abort) MO_HIP(hipMemcpyAsync(&h_abort, d_abort, sizeof(int), hipMemcpyDeviceToHost, s));
   MO_HIP(hipStreamSynchronize(s));
+  MO_CHECK(h_abort == 0, MO_ERR_HIP, "DeformWithRot team solve: a barrier wait exceeded its bound (results invalid)");
   if (stats)
     for (int p = 0; p < npairs; ++p) {
       mo_lm_stats& o2 = stats[p];

@@ -36,4 +36,13 @@ int lm_rot_launch(const RotPairDesc* pairs_dev, int npairs, const RotArrays& arr
                   LMResultDev* results_dev, long long total_vertices, long long total_half_edges, int block_threads,
                   hipStream_t s);
 
+// Multi-workgroup variant (k_lm_rot_team), laid out as lm_team_launch
+// (mo_lm.h): wg_dev entries (team, rank, size, -), team t solves problem
+// plist[t]; cnt / part / abort_flag as there.
+int lm_rot_team_capacity(int block_threads);
+int lm_rot_team_launch(const RotPairDesc* pairs_dev, const int* plist, const int4* wg_dev, int nwg,
+                       const RotArrays& arrays, const LMOptionsDev& opt, LMResultDev* results_dev,
+                       long long total_vertices, long long total_half_edges, int block_threads, unsigned* cnt,
+                       double* part, int* abort_flag, hipStream_t s);
+
 }  // namespace mo

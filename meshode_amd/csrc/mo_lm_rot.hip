@@ -20,6 +20,8 @@
 // its outgoing and incoming half-edges), preconditioned by the inverse of
 // each vertex's 6x6 diagonal block of J^T J + D^2.  Jacobi scaling, LM
 // diagonal and trust-region decisions are those of the shared loop.
+#include <algorithm>
+#include <cstdlib>
 #include "mo_common.h"
 #include "mo_lm.h"
 #include "mo_lm_rot.h"
@@ -96,13 +98,12 @@ __device__ inline bool spd6_inverse(double (&M)[6][6], float (&out)[21]) {
   return true;
 }
 
-__global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ pairs, RotArrays A,
-                                                 LMOptionsDev opt, LMResultDev* __restrict__ res, long long totalV,
-                                                 long long totalH) {
-  __shared__ double red[2 * 8 * kMaxWaves];
-  const int pid = blockIdx.x;
-  const RotPairDesc P = pairs[pid];
+template <class Team>
+__device__ inline void lm_rot_pair(const RotPairDesc P, LMResultDev* out, const RotArrays& A,
+                                   const LMOptionsDev& opt, long long totalV, long long totalH, Team& team,
+                                   double* red) {
   const int T = blockDim.x, tid = threadIdx.x;
+  const int i0 = team.rank() * T + tid, stride = team.size() * T;
   const int nv = P.nv, nh = P.nh;
   const float lam = P.lambda, l2 = lam * lam;
   const float* __restrict__ grid = P.grid;
@@ -187,7 +188,7 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
   // ---- residuals, Jacobian blocks, gradient J^T f, cost -----------------
   auto eval_jacobian = [&](bool first, double& cv, double& ce, double& gmax) {
     double v[3] = {0.0, 0.0, 0.0};
-    for (int i = tid; i < nv; i += T) {
+    for (int i = i0; i < nv; i += stride) {
       const float4 x0 = X0[i], ui = U[i], wi = WR[i];
       const JetF J = lookup_jet(grid, N, x0.x + ui.x, x0.y + ui.y, x0.z + ui.z);
       V[vG][i] = make_float4(J.v0, J.v1, J.v2, J.a);
@@ -199,8 +200,8 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
       V[vA2][i] = make_float4(Am[6], Am[7], Am[8], 0.f);
       v[0] += 0.5 * (double)J.a * (double)J.a;
     }
-    __syncthreads();
-    for (int h = tid; h < nh; h += T) {
+    team.barrier();
+    for (int h = i0; h < nh; h += stride) {
       const int2 ab = HAB[h];
       const float4 xa = X0[ab.x], xb = X0[ab.y], ua = U[ab.x], ub = U[ab.y], wa = WR[ab.x], wb = WR[ab.y];
       const float3 vh = make_float3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
@@ -222,8 +223,8 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
       H[hD][h] = f4(d);
       v[1] += 0.5 * (dot3(r, r) + dot3(sg, sg));
     }
-    __syncthreads();
-    for (int i = tid; i < nv; i += T) {
+    team.barrier();
+    for (int i = i0; i < nv; i += stride) {
       const float4 g = V[vG][i];
       const int4 adj = ADJ[i];
       float3 gp = make_float3(g.w * g.x, g.w * g.y, g.w * g.z), gw = make_float3(0.f, 0.f, 0.f);
@@ -251,7 +252,7 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
         V[vSw][i] = make_float4(s[3], s[4], s[5], 0.f);
       }
     }
-    block_reduce<2, 1>(v, red, sel);
+    team.template reduce<2, 1>(v, red, sel);
     cv = v[0]; ce = v[1]; gmax = v[2];
   };
 
@@ -259,7 +260,7 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
   auto setup_linear = [&](bool reuse, double radius, double& rz_out, double& rr_out) {
     double v[2] = {0.0, 0.0};
     const double inv_radius = 1.0 / radius;
-    for (int i = tid; i < nv; i += T) {
+    for (int i = i0; i < nv; i += stride) {
       double M[6][6];
       vertex_block(i, M);
       const float4 sp = V[vSp][i], sw = V[vSw][i];
@@ -307,7 +308,7 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
       v[0] += dot3(rp, zp) + dot3(rw, zw);
       v[1] += dot3(rp, rp) + dot3(rw, rw);
     }
-    block_reduce<2, 0>(v, red, sel);
+    team.template reduce<2, 0>(v, red, sel);
     rz_out = v[0];
     rr_out = v[1];
   };
@@ -318,7 +319,7 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
     if (!(rr0 > 0.0)) return 0;
     int k = 0;
     for (; k < opt.pcg_max_iterations; ++k) {
-      for (int h = tid; h < nh; h += T) {
+      for (int h = i0; h < nh; h += stride) {
         const int2 ab = HAB[h];
         const float3 pa = f3(V[vPp][ab.x]), pb = f3(V[vPp][ab.y]);
         const float3 wa = f3(V[vPw][ab.x]), wb = f3(V[vPw][ab.y]);
@@ -330,9 +331,9 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
         H[hD][h] = f4(add(mul(mtv(g0, g1, g2, t), lam), s));
         H[hE][h] = f4(s);
       }
-      __syncthreads();
+      team.barrier();
       double v1[1] = {0.0};
-      for (int i = tid; i < nv; i += T) {
+      for (int i = i0; i < nv; i += stride) {
         const float3 pp = f3(V[vPp][i]), pw = f3(V[vPw][i]);
         const float4 g = V[vG][i];
         const float gd = g.x * pp.x + g.y * pp.y + g.z * pp.z;
@@ -354,12 +355,12 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
         V[vQw][i] = f4(qw);
         v1[0] += dot3(pp, qp) + dot3(pw, qw);
       }
-      block_reduce<1, 0>(v1, red, sel);
+      team.template reduce<1, 0>(v1, red, sel);
       const double pq = v1[0];
       if (!(pq > 0.0)) break;
       const float fa = (float)(rz / pq);
       double v2[2] = {0.0, 0.0};
-      for (int i = tid; i < nv; i += T) {
+      for (int i = i0; i < nv; i += stride) {
         const float3 pp = f3(V[vPp][i]), pw = f3(V[vPw][i]);
         V[vDLp][i] = f4(add(f3(V[vDLp][i]), mul(pp, fa)));
         V[vDLw][i] = f4(add(f3(V[vDLw][i]), mul(pw, fa)));
@@ -374,15 +375,15 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
         v2[0] += dot3(rp, zp) + dot3(rw, zw);
         v2[1] += dot3(rp, rp) + dot3(rw, rw);
       }
-      block_reduce<2, 0>(v2, red, sel);
+      team.template reduce<2, 0>(v2, red, sel);
       const float fb = (float)(v2[0] / rz);
       rz = v2[0];
       if (v2[1] <= tol2) { ++k; break; }
-      for (int i = tid; i < nv; i += T) {
+      for (int i = i0; i < nv; i += stride) {
         V[vPp][i] = f4(add(f3(V[vZp][i]), mul(f3(V[vPp][i]), fb)));
         V[vPw][i] = f4(add(f3(V[vZw][i]), mul(f3(V[vPw][i]), fb)));
       }
-      __syncthreads();
+      team.barrier();
     }
     return k;
   };
@@ -390,7 +391,7 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
   // ---- model cost change, candidate cost, step and parameter norms ------
   auto eval_candidate = [&]() -> LMCandidate {
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int i = tid; i < nv; i += T) {
+    for (int i = i0; i < nv; i += stride) {
       const float4 x0 = X0[i], ui = U[i], wi = WR[i];
       const float3 dp = f3(V[vDLp][i]), dw = f3(V[vDLw][i]);
       const float4 g = V[vG][i];
@@ -402,7 +403,7 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
       const float3 x = make_float3(x0.x + ui.x, x0.y + ui.y, x0.z + ui.z);
       v[4] += dot3(x, x) + dot3(f3(wi), f3(wi));
     }
-    for (int h = tid; h < nh; h += T) {
+    for (int h = i0; h < nh; h += stride) {
       const int2 ab = HAB[h];
       const float3 dpa = f3(V[vDLp][ab.x]), dpb = f3(V[vDLp][ab.y]);
       const float3 dwa = f3(V[vDLw][ab.x]), dwb = f3(V[vDLw][ab.y]);
@@ -424,7 +425,7 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
       const float3 sc = make_float3(w[0] - (wb.x + dwb.x), w[1] - (wb.y + dwb.y), w[2] - (wb.z + dwb.z));
       v[2] += 0.5 * (dot3(rc, rc) + dot3(sc, sc));
     }
-    block_reduce<5, 0>(v, red, sel);
+    team.template reduce<5, 0>(v, red, sel);
     return LMCandidate{-v[0], v[1] + v[2], sqrt(v[3]), sqrt(v[4])};
   };
 
@@ -435,13 +436,35 @@ __global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ 
     return pcg(rz, rr);
   };
   auto accept = [&]() {
-    for (int i = tid; i < nv; i += T) {
+    for (int i = i0; i < nv; i += stride) {
       U[i] = f4(add(f3(U[i]), f3(V[vDLp][i])));
       WR[i] = f4(add(f3(WR[i]), f3(V[vDLw][i])));
     }
-    __syncthreads();
+    team.barrier();
   };
-  lm_trust_region_loop(opt, eval, solve, eval_candidate, accept, res + pid, clock_start);
+  lm_trust_region_loop(opt, eval, solve, eval_candidate, accept, team.leader() ? out : nullptr, clock_start);
+}
+
+__global__ void __launch_bounds__(256) k_lm_rot(const RotPairDesc* __restrict__ pairs, RotArrays A,
+                                                 LMOptionsDev opt, LMResultDev* __restrict__ res, long long totalV,
+                                                 long long totalH) {
+  __shared__ double red[2 * 8 * kMaxWaves];
+  const int pid = blockIdx.x;
+  SoloTeam team;
+  lm_rot_pair(pairs[pid], res + pid, A, opt, totalV, totalH, team, red);
+}
+
+// wg[b] = (team, rank, team size, -): team t solves problem plist[t].
+__global__ void __launch_bounds__(256) k_lm_rot_team(const RotPairDesc* __restrict__ pairs,
+                                                      const int* __restrict__ plist, const int4* __restrict__ wg,
+                                                      RotArrays A, LMOptionsDev opt, LMResultDev* __restrict__ res,
+                                                      long long totalV, long long totalH, unsigned* cnt, double* part,
+                                                      int* abort_flag, long long limit) {
+  __shared__ double red[2 * 8 * kMaxWaves];
+  const int4 m = wg[blockIdx.x];
+  const int pid = plist[m.x];
+  GridTeam team{m.y, m.z, cnt + (size_t)m.x * 32, part + (size_t)m.x * 2 * kMaxTeam * 8, abort_flag, limit, 0u, 0};
+  lm_rot_pair(pairs[pid], res + pid, A, opt, totalV, totalH, team, red);
 }
 
 }  // namespace
@@ -455,6 +478,37 @@ int lm_rot_launch(const RotPairDesc* pairs_dev, int npairs, const RotArrays& arr
   hipLaunchKernelGGL(k_lm_rot, dim3(npairs), dim3(block_threads), 0, s, pairs_dev, arrays, opt, results_dev,
                      total_vertices, total_half_edges);
   MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+int lm_rot_team_capacity(int block_threads) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_lm_rot_team, block_threads, 0) != hipSuccess)
+    return 0;
+  return per * cus;
+}
+
+int lm_rot_team_launch(const RotPairDesc* pairs_dev, const int* plist, const int4* wg_dev, int nwg,
+                       const RotArrays& arrays, const LMOptionsDev& opt, LMResultDev* results_dev,
+                       long long total_vertices, long long total_half_edges, int block_threads, unsigned* cnt,
+                       double* part, int* abort_flag, hipStream_t s) {
+  if (nwg <= 0) return MO_OK;
+  MO_CHECK(block_threads >= 64 && block_threads <= 256 && block_threads % 64 == 0, MO_ERR_ARG,
+           "block_threads must be a multiple of 64 in [64, 256] (got %d)", block_threads);
+  int dev = 0, khz = 0;
+  MO_HIP(hipGetDevice(&dev));
+  MO_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  long long limit = (long long)std::max(khz, 1) * 1000LL * 20;   // a 20 s bound on any one wait
+  if (const char* w = getenv("MO_LM_TEAM_WAIT_US")) limit = atoll(w) * (long long)std::max(khz, 1) / 1000LL;
+  RotArrays a = arrays;
+  LMOptionsDev o = opt;
+  LMResultDev* r = results_dev;
+  long long tv = total_vertices, th = total_half_edges;
+  void* args[] = {(void*)&pairs_dev, (void*)&plist, (void*)&wg_dev, (void*)&a, (void*)&o, (void*)&r, (void*)&tv,
+                  (void*)&th, (void*)&cnt, (void*)&part, (void*)&abort_flag, (void*)&limit};
+  MO_HIP(hipLaunchCooperativeKernel((const void*)k_lm_rot_team, dim3(nwg), dim3(block_threads), args, 0, s));
   return MO_OK;
 }
 

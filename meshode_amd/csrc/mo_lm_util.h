@@ -44,6 +44,82 @@ __device__ inline void block_reduce(double (&v)[K + M], double* red, int& sel) {
 __device__ inline unsigned nb_index(unsigned e) { return e & 0xFFFFFFu; }
 __device__ inline float nb_mult(unsigned e) { return (float)(e >> 24); }
 
+// The workgroup(s) that solve one problem.  SoloTeam: one workgroup.
+struct SoloTeam {
+  __device__ int rank() const { return 0; }
+  __device__ int size() const { return 1; }
+  __device__ bool leader() const { return true; }
+  __device__ void barrier() { __syncthreads(); }
+  template <int K, int M>
+  __device__ void reduce(double (&v)[K + M], double* red, int& sel) { block_reduce<K, M>(v, red, sel); }
+};
+
+// GridTeam: the problem's vertices (and half-edges) spread over `c`
+// co-resident workgroups (item i belongs to workgroup (i / T) % c).  A
+// reduction is the workgroup's own block_reduce, written to a per-problem
+// slot in global memory, a team barrier, then the c partials summed in rank
+// order by every workgroup, so all of them hold the same bits.  The barrier
+// is an arrival counter per problem (monotonic; workgroup w waits for
+// (n + 1) c arrivals at its n-th barrier) with agent-scope release/acquire
+// fences around it: the team's workgroups may sit on different XCDs, whose
+// L2 caches are not coherent with each other.  Every wait is bounded by
+// `limit` device clock ticks; a wait that runs out sets the shared abort
+// flag, after which no barrier waits (the solve finishes in bounded time
+// and the host reports the error).
+constexpr int kMaxTeam = 32;
+
+struct GridTeam {
+  int r, c;
+  unsigned* cnt;        // this pair's arrival counter
+  double* part;         // this pair's partials [2][kMaxTeam][8]
+  int* abort_flag;
+  long long limit;
+  unsigned arrivals;
+  int gsel;
+  __device__ int rank() const { return r; }
+  __device__ int size() const { return c; }
+  __device__ bool leader() const { return r == 0; }
+  __device__ void barrier() {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned target = (arrivals + 1u) * (unsigned)c;
+      const long long t0 = wall_clock64();
+      while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (wall_clock64() - t0 > limit) {
+          __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __threadfence();
+    }
+    ++arrivals;
+    __syncthreads();
+  }
+  template <int K, int M>
+  __device__ void reduce(double (&v)[K + M], double* red, int& sel) {
+    block_reduce<K, M>(v, red, sel);
+    double* slot = part + gsel * (kMaxTeam * 8);
+    gsel ^= 1;
+    if (threadIdx.x == 0)
+      for (int k = 0; k < K + M; ++k) slot[r * 8 + k] = v[k];
+    barrier();
+    for (int k = 0; k < K; ++k) {
+      double t = 0.0;
+      for (int q = 0; q < c; ++q) t += slot[q * 8 + k];
+      v[k] = t;
+    }
+    for (int k = K; k < K + M; ++k) {
+      double t = slot[k];
+      for (int q = 1; q < c; ++q) t = slot[q * 8 + k] > t ? slot[q * 8 + k] : t;
+      v[k] = t;
+    }
+  }
+};
+
 // Candidate evaluation of one trust-region step.
 struct LMCandidate {
   double model_change;   // model_cost_change = -(J delta).(f + J delta / 2)

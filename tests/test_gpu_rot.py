@@ -53,16 +53,23 @@ def _case(name):
     raise KeyError(name)
 
 
+# the multi-workgroup variant: 64-thread workgroups, 16 vertices each (teams
+# of 3 for the 42-vertex sphere, 11 for the 162-vertex ones), the library's
+# DeformWithRot PCG tolerance
+TEAM = dict(pcg_relative_tolerance=1e-6, team_vertices=16, block_threads=64)
+
+
+@pytest.mark.parametrize("team", [False, True])
 @pytest.mark.parametrize("name", ["shrunk_sphere_n16", "rotated_copy_n32", "low_rigidity_n16"])
-def test_deform_rot_energy_parity(gpu, oracle, name):
-    _say(f"[rot] case={name}")
+def test_deform_rot_energy_parity(gpu, oracle, name, team):
+    _say(f"[rot] case={name} team={team}")
     sV, sF, tV, tF, N, lam = _case(name)
     pid = deform.template_f64(tV, tF, N)
     _, scale, trans = pyDeform.TemplateInfo(pid)
     grid = pyDeform.TemplateGrid(pid)[0].cpu().numpy().reshape(-1)
     sn = (sV - trans) / scale                          # Mesh::ApplyTransform
     V = torch.from_numpy(sn).to(gpu).contiguous()
-    st, W = deform.deform_rot(pid, V, sF, lam)
+    st, W = deform.deform_rot(pid, V, sF, lam, options=deform.default_options(**TEAM) if team else None)
     x, w = V.cpu().numpy(), W.cpu().numpy()
     _, _, so = oracle.lm_rot_solve(grid, N, sn, sF, lam)
     eg, vg, rg = oracle.lm_rot_cost(grid, N, x, w, sn, sF, lam)
@@ -114,10 +121,12 @@ def test_deform_rot_start_rotations_and_errors(gpu):
         deform.deform_rot(pid, V.float(), sF, lam)
 
 
-def test_deform_rot_batch_equals_alone(gpu):
+@pytest.mark.parametrize("team", [False, True])
+def test_deform_rot_batch_equals_alone(gpu, team):
     """mo_deform_rot_batch: each problem's result is bit-identical to the
-    same problem solved alone."""
-    _say("[rot] batch of 5 vs alone")
+    same problem solved alone (one workgroup per problem, or teams)."""
+    _say(f"[rot] batch of 5 vs alone team={team}")
+    opt = deform.default_options(**TEAM) if team else None
     probs = []
     for k, name in enumerate(["shrunk_sphere_n16", "rotated_copy_n32", "low_rigidity_n16", "shrunk_sphere_n16",
                               "rotated_copy_n32"]):
@@ -126,10 +135,11 @@ def test_deform_rot_batch_equals_alone(gpu):
         _, scale, trans = pyDeform.TemplateInfo(pid)
         probs.append((pid, (sV - trans) / scale, sF, lam + 0.1 * k))
     Vb = [torch.from_numpy(p[1]).to(gpu).contiguous() for p in probs]
-    stb, Wb = deform.deform_rot_batch([p[0] for p in probs], Vb, [p[2] for p in probs], [p[3] for p in probs])
+    stb, Wb = deform.deform_rot_batch([p[0] for p in probs], Vb, [p[2] for p in probs], [p[3] for p in probs],
+                                      options=opt)
     for p, V, W, st in zip(probs, Vb, Wb, stb):
         V1 = torch.from_numpy(p[1]).to(gpu).contiguous()
-        st1, W1 = deform.deform_rot(p[0], V1, p[2], p[3])
+        st1, W1 = deform.deform_rot(p[0], V1, p[2], p[3], options=opt)
         assert torch.equal(V, V1) and torch.equal(W, W1)
         assert st["final_cost"] == st1["final_cost"] and st["iterations"] == st1["iterations"]
 
