@@ -160,7 +160,10 @@ typedef struct mo_lm_options {
                                             kernel only; 2: multi-workgroup kernel for every pair */
   int32_t max_successful_steps;          /* > 0: stop after that many accepted steps (0) */
   int32_t team_vertices;                 /* pairs on the multi-workgroup kernel: vertices per
-                                            workgroup of a pair's team (0: 4 * block_threads) */
+                                            workgroup of a pair's team (0: 4 * block_threads);
+                                            mo_deform_rot(_batch): > 0 puts each problem on a
+                                            team of that many vertices per workgroup (0: one
+                                            workgroup per problem) */
 } mo_lm_options;
 
 typedef struct mo_lm_stats {
