@@ -445,8 +445,6 @@ struct PhaseTimer {
   }
 };
 
-std::mutex g_team_mu;   // one multi-workgroup LM launch at a time (lm_core)
-
 int device_lds_limit() {
   static int cached = -1;
   if (cached < 0) {
@@ -581,12 +579,12 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   // the batch.  Teams of one run on k_lm_solve
   // (kernel_variant 2 keeps them on the team kernel); the others are packed,
   // largest first, into cooperative launches of at most lm_team_capacity()
-  // workgroups.  One team launch at a time in the process (g_team_mu, held
+  // workgroups.  One team launch at a time in the process (team_launch_mutex, held
   // until the final stream synchronization): two cooperative grids from
   // different streams must not wait on each other's residency.
   std::vector<int> solo, teams, team_c;
   std::vector<int4> wg;
-  std::unique_lock<std::mutex> team_lock(g_team_mu, std::defer_lock);
+  std::unique_lock<std::mutex> team_lock(team_launch_mutex(), std::defer_lock);
   int* d_abort = nullptr;
   Arena tar;
   {
@@ -1150,9 +1148,9 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
   // team_vertices > 0: each problem on a team of ceil(nv / team_vertices)
   // co-resident workgroups (k_lm_rot_team; at most lm_team_max()), packed
   // largest first into cooperative launches, one such launch at a time in
-  // the process (g_team_mu); otherwise one workgroup per problem.
+  // the process (team_launch_mutex); otherwise one workgroup per problem.
   const int rbt = std::min(o.block_threads, 256);
-  std::unique_lock<std::mutex> team_lock(g_team_mu, std::defer_lock);
+  std::unique_lock<std::mutex> team_lock(team_launch_mutex(), std::defer_lock);
   DevPtr d_team;
   int* d_abort = nullptr;
   int rc;

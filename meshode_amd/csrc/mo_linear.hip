@@ -37,15 +37,19 @@
 #include <unordered_map>
 #include <vector>
 
+#include <cstdlib>
+#include <mutex>
 #include "mo_common.h"
 #include "mo_linear.h"
 #include "mo_polar.h"
+#include "mo_lm.h"
 #include "mo_lm_util.h"
 
 namespace mo {
 namespace {
 
 constexpr int kCGThreads = 1024;
+constexpr int kTeamRows = 4096;   // systems this large run on a team of workgroups
 constexpr int kMaxBlock = 64;
 
 struct CGArgs {
@@ -76,8 +80,8 @@ struct CGArgs {
   int* info;
 };
 
-__device__ inline void cluster_sums(const CGArgs& a, const double* __restrict__ Y) {
-  for (int g = threadIdx.x; g < a.ng; g += blockDim.x) {
+__device__ inline void cluster_sums(const CGArgs& a, const double* __restrict__ Y, int g0, int gstride) {
+  for (int g = g0; g < a.ng; g += gstride) {
     double s0 = 0, s1 = 0, s2 = 0;
     for (int k = a.cp[g]; k < a.cp[g + 1]; ++k) {
       const int m = a.cm[k];
@@ -120,15 +124,19 @@ __device__ inline void precond_row(const CGArgs& a, const double* __restrict__ R
   for (int c = 0; c < 3; ++c) out[c] = z[c];
 }
 
-__global__ __launch_bounds__(kCGThreads) void k_cg3(CGArgs a) {
-  __shared__ double red[2 * 8 * kMaxWaves];
+// Preconditioned CG on the three right-hand sides, by one workgroup
+// (SoloTeam) or by all workgroups of a cooperative grid (GridTeam: rows
+// split as in the LM team solver, FP64 partials summed in rank order).
+template <class Team>
+__device__ inline void cg3_body(const CGArgs& a, Team& team, double* red) {
   int sel = 0;
-  const int tid = threadIdx.x, nt = blockDim.x, n = a.n;
+  const int tid = threadIdx.x, n = a.n;
+  const int i0 = team.rank() * blockDim.x + tid, nt = team.size() * blockDim.x;
   // r = b - A x
-  cluster_sums(a, a.X);
-  __syncthreads();
+  cluster_sums(a, a.X, i0, nt);
+  team.barrier();
   double v[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int i = tid; i < n; i += nt) {
+  for (int i = i0; i < n; i += nt) {
     double ax[3];
     apply_row(a, a.X, i, ax);
     for (int c = 0; c < 3; ++c) {
@@ -139,9 +147,9 @@ __global__ __launch_bounds__(kCGThreads) void k_cg3(CGArgs a) {
       v[6] = fmax(v[6], fabs(r));
     }
   }
-  __syncthreads();
+  team.barrier();
   // z = M^-1 r, p = z
-  for (int i = tid; i < n; i += nt) {
+  for (int i = i0; i < n; i += nt) {
     double z[3];
     precond_row(a, a.R, i, z);
     for (int c = 0; c < 3; ++c) {
@@ -150,17 +158,17 @@ __global__ __launch_bounds__(kCGThreads) void k_cg3(CGArgs a) {
       v[c] += a.R[3 * i + c] * z[c];
     }
   }
-  block_reduce<6, 1>(v, red, sel);
+  team.template reduce<6, 1>(v, red, sel);
   double rz[3] = {v[0], v[1], v[2]};
   // one tolerance for the three columns, relative to the largest |b|
   const double tol2 = a.rtol * a.rtol * fmax(v[3], fmax(v[4], v[5]));
   bool done = v[6] == 0.0;
   int it = 0;
   while (!done && it < a.max_iter) {
-    cluster_sums(a, a.P);
-    __syncthreads();
+    cluster_sums(a, a.P, i0, nt);
+    team.barrier();
     double pq[3] = {0, 0, 0};
-    for (int i = tid; i < n; i += nt) {
+    for (int i = i0; i < n; i += nt) {
       double q[3];
       apply_row(a, a.P, i, q);
       for (int c = 0; c < 3; ++c) {
@@ -168,19 +176,19 @@ __global__ __launch_bounds__(kCGThreads) void k_cg3(CGArgs a) {
         pq[c] += a.P[3 * i + c] * q[c];
       }
     }
-    block_reduce<3, 0>(pq, red, sel);
+    team.template reduce<3, 0>(pq, red, sel);
     double alpha[3];
     for (int c = 0; c < 3; ++c) alpha[c] = pq[c] > 0 ? rz[c] / pq[c] : 0.0;
     double w[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = tid; i < n; i += nt)
+    for (int i = i0; i < n; i += nt)
       for (int c = 0; c < 3; ++c) {
         a.X[3 * i + c] += alpha[c] * a.P[3 * i + c];
         const double r = a.R[3 * i + c] - alpha[c] * a.Q[3 * i + c];
         a.R[3 * i + c] = r;
         w[3 + c] += r * r;
       }
-    __syncthreads();
-    for (int i = tid; i < n; i += nt) {
+    team.barrier();
+    for (int i = i0; i < n; i += nt) {
       double z[3];
       precond_row(a, a.R, i, z);
       for (int c = 0; c < 3; ++c) {
@@ -188,7 +196,7 @@ __global__ __launch_bounds__(kCGThreads) void k_cg3(CGArgs a) {
         w[c] += a.R[3 * i + c] * z[c];
       }
     }
-    block_reduce<6, 0>(w, red, sel);
+    team.template reduce<6, 0>(w, red, sel);
     ++it;
     done = w[3] <= tol2 && w[4] <= tol2 && w[5] <= tol2;
     if (done) break;
@@ -197,14 +205,31 @@ __global__ __launch_bounds__(kCGThreads) void k_cg3(CGArgs a) {
       beta[c] = rz[c] > 0 ? w[c] / rz[c] : 0.0;
       rz[c] = w[c];
     }
-    for (int i = tid; i < n; i += nt)
+    for (int i = i0; i < n; i += nt)
       for (int c = 0; c < 3; ++c) a.P[3 * i + c] = a.Z[3 * i + c] + beta[c] * a.P[3 * i + c];
-    __syncthreads();
+    team.barrier();
   }
-  if (tid == 0) {
+  if (tid == 0 && team.leader()) {
     a.info[0] = it;
     a.info[1] = done ? 1 : 0;
   }
+}
+
+__global__ void k_zero_words(int* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(kCGThreads) void k_cg3(CGArgs a) {
+  __shared__ double red[2 * 8 * kMaxWaves];
+  SoloTeam team;
+  cg3_body(a, team, red);
+}
+
+__global__ __launch_bounds__(kCGThreads) void k_cg3_team(CGArgs a, unsigned* cnt, double* part, int* abort_flag,
+                                                          long long limit) {
+  __shared__ double red[2 * 8 * kMaxWaves];
+  GridTeam team{(int)blockIdx.x, (int)gridDim.x, cnt, part, abort_flag, limit, 0u, 0};
+  cg3_body(a, team, red);
 }
 
 struct System {
@@ -485,7 +510,7 @@ int run_cg(const System& S, double* X, int max_iter, double rtol, int* iteration
   const size_t nn = (size_t)S.n, ng = (size_t)S.ng, nnz = S.ci.size(), nb = S.bmem.size(), ni = S.binv.size();
   const size_t bytes = al((nn + 1) * 4) + al(nnz * 4) + al(nnz * 8) + al(nn * 4) + al((ng + 1) * 4) + al(nn * 4) +
                        al(ng * 8) + 3 * al(nn * 4) + al(nb * 4) + al(ni * 8) + 5 * al(nn * 24) + al(ng * 24) +
-                       al(16) + al(nn * 8);
+                       al(32) + al(nn * 8) + al(2 * kMaxTeam * 8 * 8);
   char* ws = nullptr;
   MO_HIP(hipMallocAsync((void**)&ws, bytes, s));
   char* w = ws;
@@ -516,8 +541,9 @@ int run_cg(const System& S, double* X, int max_iter, double rtol, int* iteration
   a.P = (double*)take(nn * 24);
   a.Q = (double*)take(nn * 24);
   a.S = (double*)take(ng * 24);
-  a.info = (int*)take(16);
+  a.info = (int*)take(32);   // info[0..1]; team counter [4], abort flag [5]
   double* d_dg = (double*)take(S.dg.size() * 8);
+  double* d_part = (double*)take(2 * kMaxTeam * 8 * 8);   // team partials [2][kMaxTeam][8]
   a.dg = S.dg.empty() ? nullptr : d_dg;
   a.rp = d_rp; a.ci = d_ci; a.w = d_w; a.cid = d_cid; a.cp = d_cp; a.cm = d_cm; a.cw2 = d_cw2;
   a.pstart = d_ps; a.psize = d_pz; a.prow = d_pr; a.bmem = d_bm; a.binv = d_bi; a.B = d_B;
@@ -543,15 +569,58 @@ int run_cg(const System& S, double* X, int max_iter, double rtol, int* iteration
       rc = fail(MO_ERR_HIP, "%s: set-up kernels failed", who);
       break;
     }
-    hipLaunchKernelGGL(k_cg3, dim3(1), dim3(kCGThreads), 0, s, a);
-    if (hipGetLastError() != hipSuccess) {
-      rc = fail(MO_ERR_HIP, "%s: launch failed", who);
-      break;
+    // Systems of at least kTeamRows rows run on a cooperative grid of
+    // ceil(n / kCGThreads) workgroups (at most lm_team_max(), at most what is
+    // co-resident), one such launch at a time in the process.
+    int team = 1;
+    if (S.n >= kTeamRows) {
+      int dev = 0, cus = 0, per = 0;
+      if (hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_cg3_team, kCGThreads, 0) == hipSuccess)
+        team = std::min({(S.n + kCGThreads - 1) / kCGThreads, lm_team_max(), per * cus});
+      team = std::max(team, 1);
     }
-    int info[2] = {0, 0};
-    if (hipMemcpyAsync(info, a.info, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+    std::unique_lock<std::mutex> team_lock(team_launch_mutex(), std::defer_lock);
+    if (team > 1) {
+      int dev = 0, khz = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess) {
+        rc = fail(MO_ERR_HIP, "%s: device query failed", who);
+        break;
+      }
+      long long limit = (long long)std::max(khz, 1) * 1000LL * 20;   // a 20 s bound on any one wait
+      if (const char* wv = getenv("MO_LM_TEAM_WAIT_US")) limit = atoll(wv) * (long long)std::max(khz, 1) / 1000LL;
+      unsigned* cnt = (unsigned*)a.info + 4;                  // counter + abort flag after info[0..1]
+      int* abort_flag = (int*)a.info + 5;
+      double* part = d_part;
+      hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, a.info, 8);
+      if (hipGetLastError() != hipSuccess) {
+        rc = fail(MO_ERR_HIP, "%s: clear failed", who);
+        break;
+      }
+      void* args[] = {(void*)&a, (void*)&cnt, (void*)&part, (void*)&abort_flag, (void*)&limit};
+      team_lock.lock();
+      if (hipLaunchCooperativeKernel((const void*)k_cg3_team, dim3(team), dim3(kCGThreads), args, 0, s) !=
+          hipSuccess) {
+        rc = fail(MO_ERR_HIP, "%s: cooperative launch of %d workgroups failed", who, team);
+        break;
+      }
+    } else {
+      hipLaunchKernelGGL(k_cg3, dim3(1), dim3(kCGThreads), 0, s, a);
+      if (hipGetLastError() != hipSuccess) {
+        rc = fail(MO_ERR_HIP, "%s: launch failed", who);
+        break;
+      }
+    }
+    int info[6] = {0, 0, 0, 0, 0, 0};
+    if (hipMemcpyAsync(info, a.info, 24, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
       rc = fail(MO_ERR_HIP, "%s: solve failed", who);
+      break;
+    }
+    if (team > 1 && info[5]) {
+      rc = fail(MO_ERR_HIP, "%s: a team barrier wait exceeded its bound (result invalid)", who);
       break;
     }
     if (iterations) *iterations = info[0];

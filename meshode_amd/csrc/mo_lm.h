@@ -2,6 +2,8 @@
 // Deformer::DeformGraph, deformer.cc:18-92 and 370-442) on gfx950.
 #pragma once
 
+#include <mutex>
+
 #include "mo_common.h"
 
 namespace mo {
@@ -76,6 +78,10 @@ int lm_solve_launch(const LMPairDesc* pairs_dev, const int* plist, int npairs, c
 // barrier wait exceeded its bound.
 int lm_team_max();
 int lm_team_capacity(int block_threads);
+// Held from a multi-workgroup (cooperative) launch until its stream is
+// synchronized: two cooperative grids on different streams must not wait on
+// each other's residency.
+std::mutex& team_launch_mutex();
 int lm_team_launch(const LMPairDesc* pairs_dev, const int* plist, const int4* wg_dev, int nwg,
                    const LMArrays& arrays, const LMOptionsDev& opt, LMResultDev* results_dev,
                    long long total_vertices, int block_threads, unsigned* cnt, double* part, int* abort_flag,

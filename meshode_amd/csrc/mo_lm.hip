@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstdlib>
+#include <mutex>
 
 namespace mo {
 
@@ -340,6 +341,11 @@ int lm_solve_launch(const LMPairDesc* pairs_dev, const int* plist, int npairs, c
 }
 
 int lm_team_max() { return kMaxTeam; }
+
+std::mutex& team_launch_mutex() {
+  static std::mutex m;
+  return m;
+}
 
 int lm_team_capacity(int block_threads) {
   int dev = 0, cus = 0, per = 0;
