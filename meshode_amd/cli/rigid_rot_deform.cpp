@@ -86,9 +86,16 @@ int main(int argc, char** argv) {
     return die("reference grid");
   if (mo_apply_transform_f64(dV, (int64_t)src.nv(), pid, 0, nullptr) != MO_OK) return die("transform");
 
+  // DeformWithRot's defaults (inner PCG tolerance 1e-6) with the one problem
+  // spread over a team of 64-thread workgroups, 512 vertices each (a
+  // 5000-vertex solve: 276 ms against 747 ms on one workgroup)
+  mo_lm_options opt;
+  mo_lm_default_options(&opt);
+  opt.pcg_relative_tolerance = 1e-6f;
+  opt.block_threads = 64;
+  opt.team_vertices = 512;
   mo_lm_stats st;
-  if (mo_deform_rot(pid, dV, (int64_t)src.nv(), dF, (int64_t)src.nf(), lambda, nullptr, nullptr, &st, nullptr) !=
-      MO_OK)
+  if (mo_deform_rot(pid, dV, (int64_t)src.nv(), dF, (int64_t)src.nf(), lambda, &opt, nullptr, &st, nullptr) != MO_OK)
     return die("DeformWithRot");
   std::cout << "LM: " << st.iterations << " iterations (" << st.successful_steps << " successful), "
             << st.pcg_iterations << " PCG iterations, " << termination_name(st.termination) << std::endl;

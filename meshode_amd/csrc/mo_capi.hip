@@ -642,9 +642,15 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
       team_lock.lock();
       for (size_t k = 0; k + 1 < chunk_team.size(); ++k) {
         const int t0 = chunk_team[k];
-        if ((rc = lm_team_launch(d_pairs, d_tp + t0, d_wg + chunk_wg[k], chunk_wg[k + 1] - chunk_wg[k], A, dopt, d_res,
-                                 nvalloc, bt, d_cnt + (size_t)t0 * 32, d_part + (size_t)t0 * per, d_abort, s)))
-          return rc;
+        rc = lm_team_launch(d_pairs, d_tp + t0, d_wg + chunk_wg[k], chunk_wg[k + 1] - chunk_wg[k], A, dopt, d_res,
+                            nvalloc, bt, d_cnt + (size_t)t0 * 32, d_part + (size_t)t0 * per, d_abort, s);
+        if (rc == kCoopRefused) {
+          // the runtime refused the cooperative grid: this chunk's and the
+          // remaining teams' pairs run one workgroup each (d_tp lists them)
+          if ((rc = lm_solve_launch(d_pairs, d_tp + t0, nt - t0, A, dopt, d_res, nvalloc, bt, s))) return rc;
+          break;
+        }
+        if (rc) return rc;
       }
     }
   }
@@ -1198,10 +1204,19 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
     team_lock.lock();
     for (size_t k = 0; k + 1 < chunk_team.size(); ++k) {
       const int t0 = chunk_team[k];
-      if ((rc = lm_rot_team_launch(d_pair.as<RotPairDesc>(), d_tp + t0, d_wg + chunk_wg[k],
-                                   chunk_wg[k + 1] - chunk_wg[k], A, to_dev_options(o), d_res.as<LMResultDev>(), vtot,
-                                   hslots, rbt, d_cnt + (size_t)t0 * 32, d_part + (size_t)t0 * per, d_abort, s)))
-        return rc;
+      rc = lm_rot_team_launch(d_pair.as<RotPairDesc>(), d_tp + t0, d_wg + chunk_wg[k], chunk_wg[k + 1] - chunk_wg[k],
+                              A, to_dev_options(o), d_res.as<LMResultDev>(), vtot, hslots, rbt,
+                              d_cnt + (size_t)t0 * 32, d_part + (size_t)t0 * per, d_abort, s);
+      if (rc == kCoopRefused && k == 0) {
+        // the runtime refused the cooperative grid before any problem ran:
+        // one workgroup per problem instead
+        if ((rc = lm_rot_launch(d_pair.as<RotPairDesc>(), npairs, A, to_dev_options(o), d_res.as<LMResultDev>(), vtot,
+                                hslots, rbt, s)))
+          return rc;
+        break;
+      }
+      if (rc == kCoopRefused) return fail(MO_ERR_HIP, "DeformWithRot: cooperative launch refused after %zu chunks", k);
+      if (rc) return rc;
     }
   } else {
     rc = lm_rot_launch(d_pair.as<RotPairDesc>(), npairs, A, to_dev_options(o), d_res.as<LMResultDev>(), vtot, hslots,

@@ -603,10 +603,11 @@ int run_cg(const System& S, double* X, int max_iter, double rtol, int* iteration
       team_lock.lock();
       if (hipLaunchCooperativeKernel((const void*)k_cg3_team, dim3(team), dim3(kCGThreads), args, 0, s) !=
           hipSuccess) {
-        rc = fail(MO_ERR_HIP, "%s: cooperative launch of %d workgroups failed", who, team);
-        break;
+        (void)hipGetLastError();   // refused: nothing was launched; solve on one workgroup
+        team = 1;
       }
-    } else {
+    }
+    if (team == 1) {
       hipLaunchKernelGGL(k_cg3, dim3(1), dim3(kCGThreads), 0, s, a);
       if (hipGetLastError() != hipSuccess) {
         rc = fail(MO_ERR_HIP, "%s: launch failed", who);

@@ -76,6 +76,10 @@ int lm_solve_launch(const LMPairDesc* pairs_dev, const int* plist, int npairs, c
 // zeroed counter per team at a 128-byte stride (32 unsigned); part: 2 *
 // lm_team_max() * 8 doubles per team; abort_flag: one zeroed int, set when a
 // barrier wait exceeded its bound.
+// lm_team_launch / lm_rot_team_launch return this (> 0, not an error) when
+// the runtime refuses the cooperative launch; nothing was launched and the
+// caller runs the one-workgroup kernel instead.
+constexpr int kCoopRefused = 1;
 int lm_team_max();
 int lm_team_capacity(int block_threads);
 // Held from a multi-workgroup (cooperative) launch until its stream is

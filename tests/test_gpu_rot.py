@@ -163,7 +163,10 @@ def test_rigid_rot_deform_cli(gpu, oracle, tmp_path):
     pid = deform.template_f64(tV2, tF2, N)
     _, scale, trans = pyDeform.TemplateInfo(pid)
     V = torch.from_numpy((sV2 - trans) / scale).to(gpu).contiguous()
-    st, _ = deform.deform_rot(pid, V, sF2, 1.0)
+    # the CLI solves its one problem on a team (64-thread workgroups, 512 vertices each)
+    st, _ = deform.deform_rot(pid, V, sF2, 1.0,
+                              options=deform.default_options(pcg_relative_tolerance=1e-6, block_threads=64,
+                                                             team_vertices=512))
     Vo, Fo = objio.read_obj(str(out))
     assert np.array_equal(Fo, sF2)
     np.testing.assert_allclose(Vo, V.cpu().numpy() * scale + trans, rtol=1e-5, atol=1e-6)
