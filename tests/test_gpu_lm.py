@@ -236,3 +236,14 @@ def test_team_barrier_timeout_is_reported(gpu, monkeypatch):
     monkeypatch.delenv("MO_LM_TEAM_WAIT_US")
     again, _ = deform.rigid_deform_batch([pair], grid_resolution=32, options=opt)
     assert torch.equal(ref[0], again[0])
+
+
+def test_team_with_idle_workgroups(gpu, oracle):
+    """team_vertices 1 gives the 162-vertex sphere a team of 32 workgroups of
+    64 threads, of which only the first 3 own vertices; the idle members
+    still take part in every reduction and barrier."""
+    _say("[lm] case=team of 32 with 29 idle workgroups")
+    pair = synth.displaced_sphere_pair(11, src_level=2, tgt_level=3)
+    so, eg, st = _parity(oracle, pair, 32, kernel_variant=2, block_threads=64, team_vertices=1)
+    _say(f"[lm] idle team: oracle E={so['final_cost']:.8g}  gpu E={eg:.8g} it={st['iterations']}")
+    assert abs(eg - so["final_cost"]) <= ENERGY_RTOL * so["final_cost"]
