@@ -277,6 +277,7 @@ def bench_cfg1(args):
     pairs = [distinct[i % len(distinct)] for i in range(args.pairs)]
     batch = deform.PairBatch(pairs, grid_resolution=args.grid, lambdas=args.lam)
     opt = deform.default_options(timing=1)
+    shapes_used = _autotune(args, distinct, args.grid, args.lam, opt)
     for _ in range(max(args.warmup, 0)):
         batch.run(opt)
     torch.cuda.synchronize()
@@ -292,7 +293,8 @@ def bench_cfg1(args):
            "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True, "dtype": "f32 (LM), f64 (grid)",
            "data": "synthetic", "phase_ms_last": {k: tm[k] for k in ("grid_ms", "setup_ms", "lm_ms", "total_ms")},
            "config": {"workload": "config 1: icosphere L4 vs displaced icosphere L5 (x 1, 0.7, 1.3)",
-                      "pairs": args.pairs, "distinct": len(distinct), "grid": args.grid, "lambda": args.lam}}
+                      "pairs": args.pairs, "distinct": len(distinct), "grid": args.grid, "lambda": args.lam,
+                      "lm_shapes": shapes_used}}
     if not args.no_cpu_baseline:
         dt, so, grid, sn, sc, pos = cpu_baseline(distinct[0], args.grid, args.lam)
         from oracle import oracle as O
