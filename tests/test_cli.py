@@ -66,6 +66,41 @@ def test_cli_matches_library_and_oracle(gpu, oracle, tmp_path):
     assert abs(eg - so["final_cost"]) <= 1e-3 * so["final_cost"]
 
 
+@pytest.mark.gpu
+def test_cli_large_source_matches_library(gpu, oracle, tmp_path):
+    """A 10242-vertex source through the CLI: above the register/LDS
+    kernel's reach, so its LM runs as a team of workgroups, or, when the
+    runtime refuses cooperative launches in the CLI's process (as it does
+    while this test process holds the GPU), on one workgroup per pair.  The
+    CLI's final energy agrees within 1e-3 with the library's in-process team
+    solve of the same OBJ-parsed inputs, and the oracle's cost of the CLI's
+    output agrees with the energy it prints."""
+    print("[cli] case=sphere_l5_n48 (large source)", flush=True)
+    N = 48
+    sV, sF, tV, tF = synth.displaced_sphere_pair(31, src_level=5, tgt_level=4)
+    src, tgt, out = tmp_path / "s.obj", tmp_path / "t.obj", tmp_path / "o.obj"
+    objio.write_obj(str(src), sV, sF)
+    objio.write_obj(str(tgt), tV, tF)
+    r = subprocess.run([CLI, str(src), str(tgt), str(out), str(N), "5000", "1.0"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.splitlines()[-1] == "Deformed"
+    final = float(re.search(r"Final cost: (\S+)", r.stdout).group(1))
+    from meshode_amd import deform
+    sV2, sF2 = objio.read_obj(str(src))
+    tV2, tF2 = objio.read_obj(str(tgt))
+    outs, st = deform.rigid_deform_batch([(sV2, sF2, tV2, tF2)], grid_resolution=N, lambdas=1.0)
+    Vo, _ = objio.read_obj(str(out))
+    tn, sc, pos = oracle.normalize(tV2)
+    grid, _, _ = oracle.build_grid_bvh(tn, tF2, N)
+    sn = oracle.apply_transform(sV2, sc, pos)
+    eg, _, _ = oracle.lm_cost(grid, N, (Vo - pos) / sc, sn, oracle.face_edges(sF2), 1.0)
+    print(f"[cli] large: CLI E={final:.8g} (oracle cost of its output {eg:.8g}), library team E="
+          f"{st[0]['final_cost']:.8g}", flush=True)
+    assert abs(final - st[0]["final_cost"]) <= 1e-3 * st[0]["final_cost"]
+    assert abs(eg - final) <= 1e-4 * final
+
+
 def test_load_mesh_returns_raw_coordinates(tmp_path):
     """pyDeform.LoadMesh returns the OBJ's coordinates as they are (the
     reference's src.Normalize() there is dead code, mesh_tensor.cc:95-97)
