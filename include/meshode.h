@@ -178,7 +178,19 @@ typedef struct mo_lm_stats {
   int64_t pcg_iterations;
   int64_t clock_start;   /* device wall clock (ticks) when the pair's solve began */
   int64_t clock_end;     /* ... and ended; see mo_last_timings for the rate */
+  int32_t solver_path;   /* MO_PATH_*: which kernel solved the pair */
+  int32_t reserved;
 } mo_lm_stats;
+
+/* mo_lm_stats.solver_path.  MO_PATH_COOP_REFUSED marks a pair meant for a
+ * multi-workgroup team that ran on one workgroup because the runtime refused
+ * the cooperative launch (e.g. while another process held the GPU): same
+ * algorithm, different reduction order (results differ in low bits) and a
+ * longer solve. */
+#define MO_PATH_LDS 0            /* register/LDS kernel, one workgroup */
+#define MO_PATH_ONE_WORKGROUP 1  /* global-memory kernel, one workgroup */
+#define MO_PATH_TEAM 2           /* multi-workgroup (cooperative) kernel */
+#define MO_PATH_COOP_REFUSED 3   /* one workgroup after a refused cooperative launch */
 
 /* Fills the defaults listed above. */
 void mo_lm_default_options(mo_lm_options* opt);

@@ -78,10 +78,18 @@ class LMStats(C.Structure):
         ("pcg_iterations", C.c_int64),
         ("clock_start", C.c_int64),
         ("clock_end", C.c_int64),
+        ("solver_path", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        d["solver_path_name"] = SOLVER_PATHS.get(self.solver_path, "unknown")
+        return d
+
+
+# mo_lm_stats.solver_path (MO_PATH_* in include/meshode.h)
+SOLVER_PATHS = {0: "lds", 1: "one_workgroup", 2: "team", 3: "one_workgroup_coop_refused"}
 
 
 _lib = None

@@ -166,7 +166,9 @@ int main(int argc, char** argv) {
     if (mo_deform_batch(1, &pid, gvs, &n[3], ges, &n[4], &kind, &lambda, &opt, &st, nullptr) != MO_OK)
       return die("DeformGraph");
     std::cout << "LM: " << st.iterations << " iterations (" << st.successful_steps << " successful), "
-              << st.pcg_iterations << " PCG iterations, " << termination_name(st.termination) << std::endl;
+              << st.pcg_iterations << " PCG iterations, " << termination_name(st.termination) << ", solver "
+              << mo_cli::solver_path_name(st.solver_path) << std::endl;
+    mo_cli::warn_if_degraded(st.solver_path);
     std::cout << "Vertices cost: " << st.vertex_cost << std::endl;
     std::cout << "Rigidity cost: " << st.rigidity_cost << std::endl;
     std::cout << "Final cost: " << st.vertex_cost + st.rigidity_cost << std::endl;

@@ -10,6 +10,25 @@
 
 namespace mo_cli {
 
+// mo_lm_stats.solver_path, spelled for the "LM:" line; a solve that fell
+// back from a refused cooperative launch is also reported on stderr.
+inline const char* solver_path_name(int p) {
+  switch (p) {
+    case 0: return "lds";
+    case 1: return "one-workgroup";
+    case 2: return "team";
+    case 3: return "one-workgroup (cooperative launch refused)";
+    default: return "unknown";
+  }
+}
+
+inline void warn_if_degraded(int p) {
+  if (p == 3)
+    std::cerr << "warning: the runtime refused the multi-workgroup (cooperative) launch; the solve ran on one "
+                 "workgroup per problem (slower; low-bit differences from the team result)"
+              << std::endl;
+}
+
 struct Mesh {
   std::vector<double> V;   // [n,3]
   std::vector<int> F;      // [m,3]

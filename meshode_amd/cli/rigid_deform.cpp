@@ -101,7 +101,8 @@ int main(int argc, char** argv) {
   std::cout << "LM: " << st.iterations << " iterations (" << st.successful_steps << " successful), "
             << st.pcg_iterations << " PCG iterations, "
             << (st.termination == 0 ? "CONVERGENCE" : st.termination == 1 ? "NO_CONVERGENCE" : "FAILURE")
-            << std::endl;
+            << ", solver " << mo_cli::solver_path_name(st.solver_path) << std::endl;
+  mo_cli::warn_if_degraded(st.solver_path);
   std::cout << "Vertices cost: " << st.vertex_cost << std::endl;
   std::cout << "Rigidity cost: " << st.rigidity_cost << std::endl;
   std::cout << "Final cost: " << st.vertex_cost + st.rigidity_cost << std::endl;

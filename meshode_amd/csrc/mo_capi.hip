@@ -109,15 +109,14 @@ struct Template {
   DevPtr edge_off;
   long long edge_count = 0;
   int edge_kind = -1;
-  // incidence lists for the backward gather, keyed by the index buffer
-  DevPtr inc_rp, inc;
-  const void* inc_key = nullptr;
-  long long inc_n = -1, inc_nv = -1;
-  int inc_kind = -1;
 };
 
 std::mutex g_mu;
-std::vector<std::unique_ptr<Template>> g_templates;
+// Never destroyed: templates left at process exit are not freed by a static
+// destructor (hipFree during static destruction may run after the HIP
+// runtime or a profiling tool has shut down); the process's exit releases
+// their device memory.  mo_template_destroy frees a template explicitly.
+std::vector<std::unique_ptr<Template>>& g_templates = *new std::vector<std::unique_ptr<Template>>();
 
 Template* get_template(int id) {
   std::lock_guard<std::mutex> lk(g_mu);
@@ -196,62 +195,6 @@ int template_create(const T* V, int64_t nv, const int32_t* F, int64_t nf, int32_
   return MO_OK;
 }
 
-// Incidence lists for the backward gather: vertex v's entries are
-// (h << 1) | role in increasing h, role 0 (tail, subtract) before role 1
-// (head, add) — the order in which the reference's sequential loop updates
-// that vertex.  Built on the host from the index buffer.
-int ensure_incidence(Template* t, const int32_t* idx, long long nidx, long long nv, int kind, hipStream_t s) {
-  if (t->inc_key == (const void*)idx && t->inc_n == nidx && t->inc_nv == nv && t->inc_kind == kind) return MO_OK;
-  const int w = kind == 0 ? 3 : 2;
-  std::vector<int32_t> h_idx((size_t)nidx * w);
-  if (nidx > 0) {
-    MO_HIP(hipMemcpyAsync(h_idx.data(), idx, h_idx.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    MO_HIP(hipStreamSynchronize(s));
-  }
-  const long long nh = kind == 0 ? 3 * nidx : nidx;
-  std::vector<long long> rp((size_t)nv + 1, 0);
-  auto ends = [&](long long h, int* a, int* b) {
-    if (kind == 0) {
-      long long f = h / 3;
-      int j = (int)(h - f * 3);
-      *a = h_idx[(size_t)f * 3 + j];
-      *b = h_idx[(size_t)f * 3 + (j + 1) % 3];
-    } else {
-      *a = h_idx[(size_t)h * 2];
-      *b = h_idx[(size_t)h * 2 + 1];
-    }
-  };
-  for (long long h = 0; h < nh; ++h) {
-    int a, b;
-    ends(h, &a, &b);
-    MO_CHECK(a >= 0 && b >= 0 && a < nv && b < nv, MO_ERR_RANGE,
-             "edge %lld references vertex out of range [0, %lld)", h, nv);
-    rp[a + 1]++;
-    rp[b + 1]++;
-  }
-  for (long long v = 0; v < nv; ++v) rp[v + 1] += rp[v];
-  std::vector<long long> inc((size_t)rp[nv]);
-  std::vector<long long> cur(rp.begin(), rp.end() - 1);
-  for (long long h = 0; h < nh; ++h) {
-    int a, b;
-    ends(h, &a, &b);
-    inc[cur[a]++] = (h << 1) | 0;
-    inc[cur[b]++] = (h << 1) | 1;
-  }
-  int rc;
-  if ((rc = t->inc_rp.alloc(rp.size() * sizeof(long long)))) return rc;
-  if ((rc = t->inc.alloc(std::max<size_t>(inc.size(), 1) * sizeof(long long)))) return rc;
-  MO_HIP(hipMemcpyAsync(t->inc_rp.p, rp.data(), rp.size() * sizeof(long long), hipMemcpyHostToDevice, s));
-  if (!inc.empty())
-    MO_HIP(hipMemcpyAsync(t->inc.p, inc.data(), inc.size() * sizeof(long long), hipMemcpyHostToDevice, s));
-  MO_HIP(hipStreamSynchronize(s));
-  t->inc_key = idx;
-  t->inc_n = nidx;
-  t->inc_nv = nv;
-  t->inc_kind = kind;
-  return MO_OK;
-}
-
 int check_err_flag(int* err_dev, hipStream_t s, const char* what) {
   int h = 0;
   MO_HIP(hipMemcpyAsync(&h, err_dev, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -276,7 +219,7 @@ int edge_store(const float* V, int64_t nv, const int32_t* idx, int64_t n, int32_
   if ((rc = check_err_flag(err, s, kind == 0 ? "StoreRigidityInformation" : "StoreGraphInformation"))) return rc;
   t->edge_count = nh;
   t->edge_kind = kind;
-  return ensure_incidence(t, idx, n, nv, kind, s);
+  return MO_OK;
 }
 
 int edge_forward(const float* V, int64_t nv, const int32_t* idx, int64_t n, int32_t pid, float* out, int kind,
@@ -289,12 +232,15 @@ int edge_forward(const float* V, int64_t nv, const int32_t* idx, int64_t n, int3
            "edge loss called with %lld edges but %lld offsets are stored (call the Store function first)", nh,
            t->edge_count);
   hipStream_t s = as_stream(stream);
-  // The index buffer is checked on the host once (with the incidence lists
-  // the backward needs, kept for this buffer): later calls with the same
-  // buffer launch without a device error flag or a wait on the stream.
+  // Like the reference (rigid_layer.cc:48-89), every call reads the index
+  // buffer as it is now; ends outside [0, nv) raise.
+  Arena ar;
   int rc;
-  if ((rc = ensure_incidence(t, idx, n, nv, kind, s))) return rc;
-  return edge_fwd_launch(V, nv, idx, kind, nh, t->edge_off.as<float>(), out, nullptr, s);
+  if ((rc = ar.reserve(256, s))) return rc;
+  int* err = ar.take<int>(1);
+  if ((rc = zero_fill(err, sizeof(int), s))) return rc;
+  if ((rc = edge_fwd_launch(V, nv, idx, kind, nh, t->edge_off.as<float>(), out, err, s))) return rc;
+  return check_err_flag(err, s, kind == 0 ? "RigidEdgeLoss_forward" : "GraphEdgeLoss_forward");
 }
 
 int edge_backward(const float* V, int64_t nv, const int32_t* idx, int64_t n, int32_t pid, float* out, int kind,
@@ -306,10 +252,20 @@ int edge_backward(const float* V, int64_t nv, const int32_t* idx, int64_t n, int
   MO_CHECK(t->edge_off.p && nh <= t->edge_count, MO_ERR_STATE,
            "edge loss backward called with %lld edges but %lld offsets are stored", nh, t->edge_count);
   hipStream_t s = as_stream(stream);
+  // The incidence lists are rebuilt from the index buffer on every call
+  // (the reference re-reads tensorF each time, rigid_layer.cc:91-133), so a
+  // buffer whose contents changed at the same address is never stale.
+  const size_t wsb = edge_incidence_workspace_bytes(nh, nv);
+  Arena ar;
   int rc;
-  if ((rc = ensure_incidence(t, idx, n, nv, kind, s))) return rc;
-  return edge_bwd_launch(V, nv, idx, kind, t->edge_off.as<float>(), t->inc_rp.as<long long>(),
-                         t->inc.as<long long>(), out, s);
+  if ((rc = ar.reserve(al(wsb) + 256, s))) return rc;
+  int* err = ar.take<int>(1);
+  void* ws = ar.take<char>(wsb);
+  if ((rc = zero_fill(err, sizeof(int), s))) return rc;
+  long long *rp = nullptr, *inc = nullptr;
+  if ((rc = edge_incidence_build(idx, kind, nh, nv, ws, &rp, &inc, err, s))) return rc;
+  if ((rc = edge_bwd_launch(V, nv, idx, kind, t->edge_off.as<float>(), rp, inc, out, s))) return rc;
+  return check_err_flag(err, s, kind == 0 ? "RigidEdgeLoss_backward" : "GraphEdgeLoss_backward");
 }
 
 // ---------------------------------------------------------------------------
@@ -547,7 +503,11 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   for (int p : order) {
     const CorePair& c = pairs[p];
     int b = -1;
-    if (opt.kernel_variant == 0 && c.kind != 2 && c.nv > 0 && c.nv <= 32767 && pstat[2 * p + 1] <= 2) {
+    // the register/LDS kernel packs a vertex's LDS adjacency offset and count
+    // into 16 bits each and a neighbour index into 15: a pair qualifies only
+    // when all of them fit
+    if (opt.kernel_variant == 0 && c.kind != 2 && c.nv > 0 && c.nv <= 32767 && pstat[2 * p + 1] <= 2 &&
+        pstat[2 * p] < 65536) {
       b = lm_lds_bucket_for((int)c.nv);
       if (b >= 0 && lm_lds_bytes((int)c.nv, pstat[2 * p]) > lds_cap) b = -1;
     }
@@ -584,7 +544,18 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   // different streams must not wait on each other's residency.
   std::vector<int> solo, teams, team_c;
   std::vector<int4> wg;
+  std::vector<int> path(np, MO_PATH_LDS);
+  for (int p : bucket[nb]) path[p] = MO_PATH_ONE_WORKGROUP;
   std::unique_lock<std::mutex> team_lock(team_launch_mutex(), std::defer_lock);
+  // leaving with the lock held (an error after a cooperative launch): wait
+  // for the grid first, so the next team launch cannot overlap it
+  struct TeamDrain {
+    std::unique_lock<std::mutex>& lk;
+    hipStream_t s;
+    ~TeamDrain() {
+      if (lk.owns_lock()) (void)hipStreamSynchronize(s);
+    }
+  } team_drain{team_lock, s};
   int* d_abort = nullptr;
   Arena tar;
   {
@@ -597,6 +568,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
       if (cap >= lm_team_max() && (c > 1 || opt.kernel_variant == 2)) {
         teams.push_back(p);
         team_c.push_back((int)c);
+        path[p] = MO_PATH_TEAM;
       } else {
         solo.push_back(p);
       }
@@ -648,6 +620,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
           // the runtime refused the cooperative grid: this chunk's and the
           // remaining teams' pairs run one workgroup each (d_tp lists them)
           if ((rc = lm_solve_launch(d_pairs, d_tp + t0, nt - t0, A, dopt, d_res, nvalloc, bt, s))) return rc;
+          for (int t = t0; t < nt; ++t) path[teams[t]] = MO_PATH_COOP_REFUSED;
           break;
         }
         if (rc) return rc;
@@ -655,6 +628,15 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
     }
   }
   if ((rc = timer.mark(3))) return rc;
+  if (d_abort) {
+    // a team whose barrier wait ran out leaves invalid solutions: check
+    // before anything is written to the caller's buffers
+    int h_abort = 0;
+    MO_HIP(hipMemcpyAsync(&h_abort, d_abort, sizeof(int), hipMemcpyDeviceToHost, s));
+    MO_HIP(hipStreamSynchronize(s));
+    MO_CHECK(h_abort == 0, MO_ERR_HIP,
+             "LM team solve: a barrier wait exceeded its bound (results invalid; outputs not written)");
+  }
   for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
     if (c.nv == 0) continue;
@@ -663,11 +645,8 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   }
   MO_HIP(hipGetLastError());
   std::vector<LMResultDev> h_res(np);
-  int h_abort = 0;
   MO_HIP(hipMemcpyAsync(h_res.data(), d_res, np * sizeof(LMResultDev), hipMemcpyDeviceToHost, s));
-  if (d_abort) MO_HIP(hipMemcpyAsync(&h_abort, d_abort, sizeof(int), hipMemcpyDeviceToHost, s));
   MO_HIP(hipStreamSynchronize(s));
-  MO_CHECK(h_abort == 0, MO_ERR_HIP, "LM team solve: a barrier wait exceeded its 20 s bound (results invalid)");
   if (stats) {
     for (int p = 0; p < np; ++p) {
       const LMResultDev& r = h_res[p];
@@ -683,6 +662,8 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
       o.pcg_iterations = r.pcg_iterations;
       o.clock_start = r.clock_start;
       o.clock_end = r.clock_end;
+      o.solver_path = path[p];
+      o.reserved = 0;
     }
   }
   return MO_OK;
@@ -1157,6 +1138,14 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
   // the process (team_launch_mutex); otherwise one workgroup per problem.
   const int rbt = std::min(o.block_threads, 256);
   std::unique_lock<std::mutex> team_lock(team_launch_mutex(), std::defer_lock);
+  struct TeamDrain {
+    std::unique_lock<std::mutex>& lk;
+    hipStream_t s;
+    ~TeamDrain() {
+      if (lk.owns_lock()) (void)hipStreamSynchronize(s);
+    }
+  } team_drain{team_lock, s};
+  int path = MO_PATH_ONE_WORKGROUP;
   DevPtr d_team;
   int* d_abort = nullptr;
   int rc;
@@ -1202,6 +1191,7 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
     MO_HIP(hipMemcpyAsync(d_wg, wg.data(), wg.size() * sizeof(int4), hipMemcpyHostToDevice, s));
     MO_HIP(hipStreamSynchronize(s));   // host vectors above go out of scope
     team_lock.lock();
+    path = MO_PATH_TEAM;
     for (size_t k = 0; k + 1 < chunk_team.size(); ++k) {
       const int t0 = chunk_team[k];
       rc = lm_rot_team_launch(d_pair.as<RotPairDesc>(), d_tp + t0, d_wg + chunk_wg[k], chunk_wg[k + 1] - chunk_wg[k],
@@ -1213,6 +1203,7 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
         if ((rc = lm_rot_launch(d_pair.as<RotPairDesc>(), npairs, A, to_dev_options(o), d_res.as<LMResultDev>(), vtot,
                                 hslots, rbt, s)))
           return rc;
+        path = MO_PATH_COOP_REFUSED;
         break;
       }
       if (rc == kCoopRefused) return fail(MO_ERR_HIP, "DeformWithRot: cooperative launch refused after %zu chunks", k);
@@ -1223,6 +1214,15 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
                        rbt, s);
     if (rc) return rc;
   }
+  if (d_abort) {
+    // invalid solutions after a barrier wait ran out: raise before the
+    // caller's V / rot are overwritten
+    int h_abort = 0;
+    MO_HIP(hipMemcpyAsync(&h_abort, d_abort, sizeof(int), hipMemcpyDeviceToHost, s));
+    MO_HIP(hipStreamSynchronize(s));
+    MO_CHECK(h_abort == 0, MO_ERR_HIP,
+             "DeformWithRot team solve: a barrier wait exceeded its bound (results invalid; outputs not written)");
+  }
   for (int p = 0; p < npairs; ++p) {
     const unsigned blocks = (unsigned)((nv[p] + 255) / 256);
     hipLaunchKernelGGL(k_rot_finish, dim3(blocks), dim3(256), 0, s, V[p], rot ? rot[p] : nullptr, (int)nv[p],
@@ -1230,11 +1230,8 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
   }
   MO_HIP(hipGetLastError());
   std::vector<LMResultDev> r(npairs);
-  int h_abort = 0;
   MO_HIP(hipMemcpyAsync(r.data(), d_res.p, npairs * sizeof(LMResultDev), hipMemcpyDeviceToHost, s));
-  if (d_abort) MO_HIP(hipMemcpyAsync(&h_abort, d_abort, sizeof(int), hipMemcpyDeviceToHost, s));
   MO_HIP(hipStreamSynchronize(s));
-  MO_CHECK(h_abort == 0, MO_ERR_HIP, "DeformWithRot team solve: a barrier wait exceeded its bound (results invalid)");
   if (stats)
     for (int p = 0; p < npairs; ++p) {
       mo_lm_stats& o2 = stats[p];
@@ -1249,6 +1246,8 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
       o2.pcg_iterations = r[p].pcg_iterations;
       o2.clock_start = r[p].clock_start;
       o2.clock_end = r[p].clock_end;
+      o2.solver_path = path;
+      o2.reserved = 0;
     }
   return MO_OK;
 }

@@ -19,6 +19,15 @@ int edge_store_launch(const float* V, long long nv, const int* idx, int kind, lo
                       hipStream_t s);
 int edge_fwd_launch(const float* V, long long nv, const int* idx, int kind, long long nh, const float* off,
                     float* out, int* err, hipStream_t s);
+// Incidence lists for the backward gather, built on the device from the
+// index buffer: vertex v's entries rp[v]..rp[v+1] are (h << 1) | role in
+// increasing h, role 0 (tail, subtract) before role 1 (head, add) -- the
+// order in which the reference's sequential loop updates that vertex.  Ends
+// outside [0, nv) set *err and are left out.  ws holds
+// edge_incidence_workspace_bytes(nh, nv) bytes; *rp and *inc point into it.
+size_t edge_incidence_workspace_bytes(long long nh, long long nv);
+int edge_incidence_build(const int* idx, int kind, long long nh, long long nv, void* ws, long long** rp,
+                         long long** inc, int* err, hipStream_t s);
 int edge_bwd_launch(const float* V, long long nv, const int* idx, int kind, const float* off, const long long* rp,
                     const long long* inc, float* out, hipStream_t s);
 

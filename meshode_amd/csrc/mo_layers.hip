@@ -16,6 +16,10 @@
 #include "mo_layers.h"
 #include "mo_lookup.h"
 
+#include <hipcub/device/device_radix_sort.hpp>
+
+#include <algorithm>
+
 namespace mo {
 
 namespace {
@@ -131,7 +135,44 @@ __global__ void k_edge_bwd(const float* __restrict__ V, long long nv, const int*
   out[u * 3 + 2] = g2;
 }
 
+// Incidence items of the 2 nh directed-edge ends: item q = 2 h + role is
+// keyed by its vertex (tail for role 0, head for role 1); an end outside
+// [0, nv) gets the key nv, which sorts after every vertex and is left out
+// of the lists (and flagged).
+__global__ void k_inc_items(const int* __restrict__ idx, int kind, long long nh, long long nv,
+                            unsigned* __restrict__ key, long long* __restrict__ val, int* __restrict__ err) {
+  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 2 * nh) return;
+  int a, b;
+  edge_ends(idx, kind, q >> 1, &a, &b);
+  const int v = (q & 1) ? b : a;
+  const bool ok = a >= 0 && b >= 0 && a < nv && b < nv;
+  if (!ok) atomicOr(err, 1);
+  key[q] = ok ? (unsigned)v : (unsigned)nv;
+  val[q] = q;
+}
+
+// rp[v] = first position of key v in the sorted keys (v = 0..nv).
+__global__ void k_inc_rows(const unsigned* __restrict__ key, long long n, long long nv, long long* __restrict__ rp) {
+  long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > nv) return;
+  long long lo = 0, hi = n;
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    if ((long long)key[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  rp[v] = lo;
+}
+
 inline unsigned blocks_for(long long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+inline int key_bits(long long nv) {
+  int b = 1;
+  while (b < 32 && (1LL << b) <= nv) ++b;
+  return b;
+}
+
+inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 }  // namespace
 
@@ -173,6 +214,41 @@ int edge_fwd_launch(const float* V, long long nv, const int* idx, int kind, long
   if (nh <= 0) return MO_OK;
   hipLaunchKernelGGL(k_edge_fwd, dim3(blocks_for(nh, 256)), dim3(256), 0, s, V, nv, idx, kind, nh, off, out, err);
   MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+size_t edge_incidence_workspace_bytes(long long nh, long long nv) {
+  const long long n = std::max<long long>(2 * nh, 1);
+  size_t temp = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                           (const long long*)nullptr, (long long*)nullptr, n, 0, key_bits(nv));
+  return 2 * al256((size_t)n * sizeof(unsigned)) + 2 * al256((size_t)n * sizeof(long long)) +
+         al256((size_t)(nv + 1) * sizeof(long long)) + al256(temp) + 256;
+}
+
+int edge_incidence_build(const int* idx, int kind, long long nh, long long nv, void* ws, long long** rp,
+                         long long** inc, int* err, hipStream_t s) {
+  const long long n = std::max<long long>(2 * nh, 1);
+  char* w = (char*)ws;
+  unsigned* k_in = (unsigned*)w; w += al256((size_t)n * sizeof(unsigned));
+  unsigned* k_out = (unsigned*)w; w += al256((size_t)n * sizeof(unsigned));
+  long long* v_in = (long long*)w; w += al256((size_t)n * sizeof(long long));
+  long long* v_out = (long long*)w; w += al256((size_t)n * sizeof(long long));
+  long long* r = (long long*)w; w += al256((size_t)(nv + 1) * sizeof(long long));
+  size_t temp = 0;
+  MO_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, k_in, k_out, v_in, v_out, n, 0, key_bits(nv), s));
+  if (nh > 0) {
+    hipLaunchKernelGGL(k_inc_items, dim3(blocks_for(2 * nh, 256)), dim3(256), 0, s, idx, kind, nh, nv, k_in, v_in,
+                       err);
+    MO_HIP(hipGetLastError());
+    // LSD radix sort is stable: each vertex's items stay in increasing q =
+    // 2 h + role, the order of the reference's sequential loop
+    MO_HIP(hipcub::DeviceRadixSort::SortPairs((void*)w, temp, k_in, k_out, v_in, v_out, 2 * nh, 0, key_bits(nv), s));
+  }
+  hipLaunchKernelGGL(k_inc_rows, dim3(blocks_for(nv + 1, 256)), dim3(256), 0, s, k_out, nh > 0 ? 2 * nh : 0, nv, r);
+  MO_HIP(hipGetLastError());
+  *rp = r;
+  *inc = v_out;
   return MO_OK;
 }
 

@@ -508,6 +508,11 @@ int lm_rot_team_launch(const RotPairDesc* pairs_dev, const int* plist, const int
   long long tv = total_vertices, th = total_half_edges;
   void* args[] = {(void*)&pairs_dev, (void*)&plist, (void*)&wg_dev, (void*)&a, (void*)&o, (void*)&r, (void*)&tv,
                   (void*)&th, (void*)&cnt, (void*)&part, (void*)&abort_flag, (void*)&limit};
+  // MO_LM_COOP_REFUSE=1 treats the launch as refused without trying it (the
+  // fallback path's test hook; the runtime refuses cooperative grids while
+  // another process holds the GPU)
+  if (const char* f = getenv("MO_LM_COOP_REFUSE"))
+    if (f[0] == '1') return kCoopRefused;
   if (hipLaunchCooperativeKernel((const void*)k_lm_rot_team, dim3(nwg), dim3(block_threads), args, 0, s) !=
       hipSuccess) {
     (void)hipGetLastError();   // the runtime refused the cooperative grid: nothing was launched

@@ -71,9 +71,11 @@ def _to_dev(t):
     return t.detach().contiguous()
 
 
-# Device copies of CPU index tensors, keyed by storage pointer, size and
-# version counter, so that repeated calls with the same F/E pass the same
-# device buffer (the library caches its incidence lists by that buffer).
+# Device copies of CPU index tensors, so that an Adam loop calling the edge
+# layers with the same F/E every step uploads it once.  An entry is keyed by
+# storage pointer, size, shape and version counter and holds the source
+# tensor itself: while the entry lives, that storage cannot be freed and its
+# address reused by another tensor, and an in-place change bumps _version.
 _index_cache: dict = {}
 
 
@@ -81,14 +83,14 @@ def _index_to_dev(t):
     dev = _device()
     if t.device == dev:
         return t.contiguous()
-    key = (t.data_ptr(), t.numel(), t._version, str(t.dtype))
+    key = (t.data_ptr(), t.numel(), tuple(t.shape), t._version, str(t.dtype))
     hit = _index_cache.get(key)
     if hit is not None:
-        return hit
+        return hit[1]
     d = t.detach().to(dev).contiguous()
     if len(_index_cache) > 32:
         _index_cache.clear()
-    _index_cache[key] = d
+    _index_cache[key] = (t, d)
     return d
 
 

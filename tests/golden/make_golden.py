@@ -69,5 +69,23 @@ def main():
             print(f, os.path.getsize(os.path.join(HERE, f)))
 
 
+def cfg4_l6():
+    """Config 4's largest pair: icosphere level 6 source (40962 vertices)
+    against a displaced level-6 target at 128^3, lambda 1.3, the oracle's
+    converged energy (its exact-step LM needs tens of minutes here)."""
+    seed, level, M, lam = 6000, 6, 128, 1.3
+    pV, pF, qV, qF = synth.icosphere_pair(seed, level)
+    qn, qs, qp = O.normalize(qV)
+    g, _, _ = O.build_grid_bvh(qn, qF, M)
+    x0 = O.apply_transform(pV, qs, qp)
+    _, st, _ = O.lm_solve(g, M, x0, O.face_edges(pF), lam)
+    np.savez_compressed(os.path.join(HERE, "lm_cfg4_l6_n128.npz"), seed=seed, level=level, N=M, lam=lam,
+                        final_cost=st["final_cost"], initial_cost=st["initial_cost"], iterations=st["iterations"])
+    print("lm_cfg4_l6_n128", st)
+
+
+FIXTURES = {"all": main, "cfg4_l6": cfg4_l6}
+
 if __name__ == "__main__":
-    main()
+    for name in sys.argv[1:] or ["all"]:
+        FIXTURES[name]()

@@ -185,3 +185,73 @@ def test_template_symmetry_flag_is_a_no_op(gpu):
     assert ia[0] == ib[0] and ia[1] == ib[1] and np.array_equal(ia[2], ib[2])
     for x, y in zip(pyDeform.TemplateGrid(a), pyDeform.TemplateGrid(b)):
         assert torch.equal(x, y)
+
+
+def test_edge_lists_follow_buffer_contents(sphere_template, oracle):
+    """The rigidity layers read the index buffer as it is at each call, like
+    the reference (rigid_layer.cc:48-133 re-reads tensorF every time): new
+    contents written into the same device tensor (same address) give the
+    new mesh's loss and gradient, with and without a second Store."""
+    pid, N, _ = sphere_template
+    _say("[layers] rigid loss after the face buffer is rewritten in place")
+    sV, sF, _, _ = synth.displaced_sphere_pair(8, src_level=3, tgt_level=3)
+    V0 = (sV * 0.3 + 0.5).astype(np.float32)
+    F1 = sF.astype(np.int32)
+    rng = np.random.default_rng(9)
+    F2 = F1[rng.permutation(len(F1))][:, [1, 2, 0]].copy()       # same faces, other order and rotation
+    F3 = np.ascontiguousarray(rng.integers(0, len(V0), F1.shape).astype(np.int32))
+    V1 = (V0 + rng.normal(0, 0.01, V0.shape)).astype(np.float32)
+    V0t, V1t = torch.from_numpy(V0).cuda(), torch.from_numpy(V1).cuda()
+    Fd = torch.from_numpy(F1).cuda()
+    addr = Fd.data_ptr()
+    pyDeform.StoreRigidityInformation(V0t, Fd, pid)
+    off1 = oracle.store_rigidity(V0, F1)
+    assert np.array_equal(pyDeform.RigidEdgeLoss_backward(V1t, Fd, pid).cpu().numpy(), oracle.rigid_bwd(V1, F1, off1))
+    Fd.copy_(torch.from_numpy(F2))
+    assert Fd.data_ptr() == addr
+    pyDeform.StoreRigidityInformation(V0t, Fd, pid)
+    off2 = oracle.store_rigidity(V0, F2)
+    assert np.array_equal(pyDeform.RigidEdgeLoss_forward(V1t, Fd, pid).cpu().numpy(), oracle.rigid_fwd(V1, F2, off2))
+    assert np.array_equal(pyDeform.RigidEdgeLoss_backward(V1t, Fd, pid).cpu().numpy(), oracle.rigid_bwd(V1, F2, off2))
+    # no Store: the stored offsets stay, the faces are read afresh
+    Fd.copy_(torch.from_numpy(F3))
+    assert np.array_equal(pyDeform.RigidEdgeLoss_forward(V1t, Fd, pid).cpu().numpy(), oracle.rigid_fwd(V1, F3, off2))
+    assert np.array_equal(pyDeform.RigidEdgeLoss_backward(V1t, Fd, pid).cpu().numpy(), oracle.rigid_bwd(V1, F3, off2))
+    # an out-of-range index written into the same buffer raises on both passes
+    from meshode_amd import _lib
+    Fd[5, 1] = len(V0) + 3
+    with pytest.raises(_lib.MoError, match="out of range"):
+        pyDeform.RigidEdgeLoss_forward(V1t, Fd, pid)
+    with pytest.raises(_lib.MoError, match="out of range"):
+        pyDeform.RigidEdgeLoss_backward(V1t, Fd, pid)
+
+
+def test_cpu_index_cache_holds_its_source(sphere_template, oracle):
+    """CPU index tensors are uploaded once per (storage, version); the cache
+    entry keeps the CPU tensor alive, so its address cannot be handed to a
+    new tensor with other contents while the entry exists, and a new tensor
+    of the same size with other contents gets its own upload."""
+    import gc
+    import weakref
+    pid, N, _ = sphere_template
+    _say("[layers] CPU index tensor cache")
+    sV, sF, _, _ = synth.displaced_sphere_pair(8, src_level=2, tgt_level=2)
+    V0 = (sV * 0.3 + 0.5).astype(np.float32)
+    rng = np.random.default_rng(10)
+    V1 = (V0 + rng.normal(0, 0.01, V0.shape)).astype(np.float32)
+    V0t, V1t = torch.from_numpy(V0), torch.from_numpy(V1)
+    Fa = torch.from_numpy(sF.astype(np.int32).copy())
+    pyDeform.StoreRigidityInformation(V0t, Fa, pid)
+    off = oracle.store_rigidity(V0, sF.astype(np.int32))
+    assert np.array_equal(pyDeform.RigidEdgeLoss_backward(V1t, Fa, pid).numpy(),
+                          oracle.rigid_bwd(V1, sF.astype(np.int32), off))
+    ref = weakref.ref(Fa)
+    del Fa
+    gc.collect()
+    assert ref() is not None, "the cache entry must keep its source tensor alive"
+    Fb_np = np.ascontiguousarray(sF.astype(np.int32)[::-1])
+    Fb = torch.from_numpy(Fb_np.copy())
+    assert np.array_equal(pyDeform.RigidEdgeLoss_backward(V1t, Fb, pid).numpy(), oracle.rigid_bwd(V1, Fb_np, off))
+    Fb[0, 0] = Fb[0, 1]          # in-place change: a new version, a new upload
+    assert np.array_equal(pyDeform.RigidEdgeLoss_forward(V1t, Fb, pid).numpy(),
+                          oracle.rigid_fwd(V1, Fb.numpy(), off))

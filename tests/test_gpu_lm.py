@@ -233,6 +233,14 @@ def test_team_barrier_timeout_is_reported(gpu, monkeypatch):
     monkeypatch.setenv("MO_LM_TEAM_WAIT_US", "-1")
     with pytest.raises(_lib.MoError, match="barrier wait"):
         deform.rigid_deform_batch([pair], grid_resolution=32, options=opt)
+    # the in-place entry point leaves the caller's vertices untouched
+    pid = deform.template_f64(pair[2], pair[3], 32)
+    _, scale, pos = pyDeform.TemplateInfo(pid)
+    V = torch.from_numpy((pair[0] - pos) / scale).cuda()
+    V_before = V.clone()
+    with pytest.raises(_lib.MoError, match="outputs not written"):
+        deform.deform_batch([pid], [V], [pair[1]], [0], [1.0], options=opt)
+    assert torch.equal(V, V_before)
     monkeypatch.delenv("MO_LM_TEAM_WAIT_US")
     again, _ = deform.rigid_deform_batch([pair], grid_resolution=32, options=opt)
     assert torch.equal(ref[0], again[0])
@@ -246,4 +254,53 @@ def test_team_with_idle_workgroups(gpu, oracle):
     pair = synth.displaced_sphere_pair(11, src_level=2, tgt_level=3)
     so, eg, st = _parity(oracle, pair, 32, kernel_variant=2, block_threads=64, team_vertices=1)
     _say(f"[lm] idle team: oracle E={so['final_cost']:.8g}  gpu E={eg:.8g} it={st['iterations']}")
+    assert abs(eg - so["final_cost"]) <= ENERGY_RTOL * so["final_cost"]
+
+
+def test_solver_path_is_reported(gpu, monkeypatch):
+    """mo_lm_stats.solver_path names the kernel that solved each pair; a
+    refused cooperative launch (forced from the host with
+    MO_LM_COOP_REFUSE=1) is reported as such and gives the one-workgroup
+    kernel's result bit for bit."""
+    _say("[lm] case=solver path reporting")
+    small = synth.box_union_pair(3, n_boxes=4, n_verts=1000)
+    _, st = deform.rigid_deform_batch([small], grid_resolution=32)
+    assert st[0]["solver_path_name"] == "lds"
+    one, st1 = deform.rigid_deform_batch([small], grid_resolution=32, options=deform.default_options(kernel_variant=1))
+    assert st1[0]["solver_path_name"] == "one_workgroup"
+    team_opt = deform.default_options(kernel_variant=2, block_threads=128)
+    _, st2 = deform.rigid_deform_batch([small], grid_resolution=32, options=team_opt)
+    assert st2[0]["solver_path_name"] == "team"
+    monkeypatch.setenv("MO_LM_COOP_REFUSE", "1")
+    fb, st3 = deform.rigid_deform_batch([small], grid_resolution=32, options=team_opt)
+    monkeypatch.delenv("MO_LM_COOP_REFUSE")
+    assert st3[0]["solver_path_name"] == "one_workgroup_coop_refused"
+    ref1, _ = deform.rigid_deform_batch([small], grid_resolution=32,
+                                        options=deform.default_options(kernel_variant=1, block_threads=128))
+    assert torch.equal(fb[0], ref1[0])
+
+
+def test_dense_graph_beyond_lds_offsets(gpu, oracle):
+    """A complete graph on 270 vertices (kind 1) has 72,630 adjacency
+    entries: it fits the LDS budget, but its LDS offsets do not fit the
+    register/LDS kernel's 16-bit packing, so it must go to the global
+    kernel, and still meet the energy bar."""
+    _say("[lm] case=K270 dense graph")
+    rng = np.random.default_rng(31)
+    _, _, tV, tF = synth.displaced_sphere_pair(12, src_level=2, tgt_level=3)
+    N = 32
+    pid = deform.template_f64(tV, tF, N)
+    grid = pyDeform.TemplateGrid(pid)[0].cpu().numpy().reshape(-1)
+    n = 270
+    sn = rng.uniform(0.2, 0.8, (n, 3))
+    ii, jj = np.triu_indices(n, 1)
+    E = np.stack([ii, jj], 1).astype(np.int32)
+    lam = 0.05
+    _, so, _ = oracle.lm_solve(grid, N, sn, E, lam)
+    V = torch.from_numpy(sn.copy()).cuda()
+    st = deform.deform_batch([pid], [V], [torch.from_numpy(E).cuda()], [1], [lam])[0]
+    eg, _, _ = oracle.lm_cost(grid, N, V.cpu().numpy(), sn, E, lam)
+    _say(f"[lm] K270: oracle E={so['final_cost']:.8g}  gpu E={eg:.8g} it={st['iterations']} "
+         f"path={st['solver_path_name']}")
+    assert st["solver_path_name"] == "one_workgroup"
     assert abs(eg - so["final_cost"]) <= ENERGY_RTOL * so["final_cost"]
