@@ -225,6 +225,23 @@ int mo_deform_batch(int32_t npairs, const int32_t* param_ids, double* const* V, 
                     const int32_t* const* idx, const int64_t* nidx, const int32_t* kind, const double* lambda,
                     const mo_lm_options* opt, mo_lm_stats* stats, void* stream);
 
+/* mo_deform_batch without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (tight serving
+ * loops, code that must not call hipMalloc/hipFree).  Same arguments,
+ * kernels and results as mo_deform_batch; the solver's scratch (per-vertex
+ * state, adjacency, team counters and partials, pair descriptors, results)
+ * lives in one library-owned device array of mo_deform_batch_noalloc_max_bytes()
+ * bytes (64 MiB) per GPU, kept until the process ends.  A batch that needs
+ * more returns MO_ERR_RANGE before anything is launched (about 1.8 MB per
+ * 5000-vertex, 30k-half-edge pair: ~35 such pairs).  options.timing must be 0.
+ * Calls on one GPU share its array: within a process they are serialized
+ * (each returns after its device work has finished); the array keeps the
+ * last call's contents, and whatever a call needs zeroed it clears itself. */
+int64_t mo_deform_batch_noalloc_max_bytes(void);
+int mo_deform_batch_noalloc(int32_t npairs, const int32_t* param_ids, double* const* V, const int64_t* nv,
+                            const int32_t* const* idx, const int64_t* nidx, const int32_t* kind,
+                            const double* lambda, const mo_lm_options* opt, mo_lm_stats* stats, void* stream);
+
 /* The whole rigid_deform pipeline (rigid_deform.cc:49-61) for a batch of
  * pairs, device-resident end to end: normalize target p
  * (Mesh::Normalize), build its grid_resolution[p]^3 distance field, bring

@@ -31,6 +31,7 @@ SYMBOLS = [
     "mo_flow256_packed_floats", "mo_flow256_pack", "mo_flow256_step", "mo_flow256_step_save",
     "mo_flow256_step_backward", "mo_flow256_small_grad_floats", "mo_flow256_small_grads",
     "mo_flow256_dw_work_floats", "mo_flow256_dw_grads",
+    "mo_deform_batch_noalloc_max_bytes", "mo_deform_batch_noalloc",
 ]
 
 
@@ -117,6 +118,9 @@ _SIGS = {
     "mo_graph_loss_backward": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _vp, _vp]),
     "mo_lm_default_options": (None, [C.POINTER(LMOptions)]),
     "mo_deform_batch": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(LMOptions), _vp, _vp]),
+    "mo_deform_batch_noalloc_max_bytes": (_i64, []),
+    "mo_deform_batch_noalloc": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(LMOptions), _vp,
+                                          _vp]),
     "mo_rigid_deform_batch": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                         C.POINTER(LMOptions), _vp, _vp, _vp]),
     "mo_last_timings": (C.c_int, [_vp]),

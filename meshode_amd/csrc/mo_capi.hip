@@ -67,13 +67,16 @@ struct DevPtr {
   template <class T> T* as() const { return (T*)p; }
 };
 
-// Stream-ordered scratch arena: one hipMallocAsync carved into aligned slices.
+// Stream-ordered scratch arena carved into aligned slices: one
+// hipMallocAsync (reserve), or a caller-provided region that is neither
+// allocated nor freed here (attach).
 struct Arena {
   char* base = nullptr;
   size_t size = 0, used = 0;
   hipStream_t stream = nullptr;
+  bool owned = false;
   ~Arena() {
-    if (base) (void)hipFreeAsync(base, stream);
+    if (base && owned) (void)hipFreeAsync(base, stream);
   }
   int reserve(size_t bytes, hipStream_t s) {
     stream = s;
@@ -83,8 +86,21 @@ struct Arena {
       base = nullptr;
       return fail(MO_ERR_NOMEM, "hipMallocAsync(%zu) failed: %s", size, hipGetErrorString(e));
     }
+    owned = true;
     return MO_OK;
   }
+  int attach(char* region, size_t cap, size_t bytes, hipStream_t s) {
+    MO_CHECK(bytes + 256 <= cap, MO_ERR_RANGE,
+             "batch needs %zu bytes of solver scratch; the no-allocation entry point holds at most %zu "
+             "(split the batch, or call mo_deform_batch)",
+             bytes + 256, cap);
+    stream = s;
+    base = region;
+    size = cap;
+    owned = false;
+    return MO_OK;
+  }
+  size_t end() const { return (used + 255) & ~(size_t)255; }
   template <class T> T* take(size_t count) {
     size_t off = (used + 255) & ~(size_t)255;
     used = off + count * sizeof(T);
@@ -414,8 +430,10 @@ int device_lds_limit() {
   return cached;
 }
 
+// ext / ext_cap: a caller-provided device region for all of the solve's
+// scratch (the no-allocation entry point); null: stream-ordered allocations.
 int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_stats* stats, hipStream_t s,
-            PhaseTimer& timer) {
+            PhaseTimer& timer, char* ext = nullptr, size_t ext_cap = 0) {
   const int np = (int)pairs.size();
   if (np == 0) return MO_OK;
   MO_CHECK(opt.block_threads >= 64 && opt.block_threads <= 1024 && opt.block_threads % 64 == 0, MO_ERR_ARG,
@@ -443,7 +461,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
                  al(2 * np * sizeof(int)) + al(np * sizeof(int)) + 512;
   Arena ar;
   int rc;
-  if ((rc = ar.reserve(bytes, s))) return rc;
+  if ((rc = ext ? ar.attach(ext, ext_cap, bytes, s) : ar.reserve(bytes, s))) return rc;
   float4* x0 = ar.take<float4>(nvalloc);
   float4* u = ar.take<float4>(nvalloc);
   int2* adj = ar.take<int2>(nvalloc);
@@ -583,10 +601,11 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
       const int per = 2 * lm_team_max() * 8;
       long long nwg_all = 0;
       for (int c : team_c) nwg_all += c;
-      if ((rc = tar.reserve(al(nt * sizeof(int)) + al(nwg_all * sizeof(int4)) + al((size_t)nt * 32 * sizeof(unsigned) +
-                                                                                       sizeof(int)) +
-                                al((size_t)nt * per * sizeof(double)) + 512,
-                            s)))
+      const size_t team_bytes = al(nt * sizeof(int)) + al(nwg_all * sizeof(int4)) +
+                                al((size_t)nt * 32 * sizeof(unsigned) + sizeof(int)) +
+                                al((size_t)nt * per * sizeof(double)) + 512;
+      if ((rc = ext ? tar.attach(ext + ar.end(), ext_cap - std::min(ext_cap, ar.end()), team_bytes, s)
+                    : tar.reserve(team_bytes, s)))
         return rc;
       int* d_tp = tar.take<int>(nt);
       int4* d_wg = tar.take<int4>(nwg_all);
@@ -837,14 +856,18 @@ MO_EXPORT int mo_last_timings(double* out6) {
   return MO_OK;
 }
 
-MO_EXPORT int mo_deform_batch(int32_t npairs, const int32_t* param_ids, double* const* V, const int64_t* nv,
-                              const int32_t* const* idx, const int64_t* nidx, const int32_t* kind,
-                              const double* lambda, const mo_lm_options* opt, mo_lm_stats* stats, void* stream) {
+namespace {
+
+int deform_batch_impl(int32_t npairs, const int32_t* param_ids, double* const* V, const int64_t* nv,
+                      const int32_t* const* idx, const int64_t* nidx, const int32_t* kind, const double* lambda,
+                      const mo_lm_options* opt, mo_lm_stats* stats, void* stream, char* ext, size_t ext_cap) {
   MO_CHECK(npairs >= 0, MO_ERR_ARG, "npairs < 0");
   if (npairs == 0) return MO_OK;
   MO_CHECK(param_ids && V && nv && idx && nidx && kind && lambda, MO_ERR_ARG, "null per-pair array");
   mo_lm_options o;
   if (opt) o = *opt; else mo_lm_default_options(&o);
+  MO_CHECK(!ext || o.timing == 0, MO_ERR_ARG,
+           "mo_deform_batch_noalloc: options.timing needs allocations; set it to 0");
   std::vector<CorePair> pairs(npairs);
   for (int p = 0; p < npairs; ++p) {
     Template* t = get_template(param_ids[p]);
@@ -858,8 +881,50 @@ MO_EXPORT int mo_deform_batch(int32_t npairs, const int32_t* param_ids, double* 
   int rc;
   if ((rc = timer.start(o.timing != 0, s))) return rc;
   if ((rc = timer.mark(1))) return rc;
-  if ((rc = lm_core(pairs, o, stats, s, timer))) return rc;
+  if ((rc = lm_core(pairs, o, stats, s, timer, ext, ext_cap))) return rc;
   return timer.finish();
+}
+
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu[kMaxDevices];           // one call at a time per device (they share its array)
+char* g_scratch_ptr[kMaxDevices] = {nullptr};   // set under g_scratch_mu[device]
+
+}  // namespace
+
+// Scratch of mo_deform_batch_noalloc: one array per GPU, held until the
+// process ends.  Host code never passes `scratch` itself (in host code the
+// name is a stand-in, not a device address): each device's address comes
+// from hipGetSymbolAddress, taken once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+MO_EXPORT int mo_deform_batch(int32_t npairs, const int32_t* param_ids, double* const* V, const int64_t* nv,
+                              const int32_t* const* idx, const int64_t* nidx, const int32_t* kind,
+                              const double* lambda, const mo_lm_options* opt, mo_lm_stats* stats, void* stream) {
+  return deform_batch_impl(npairs, param_ids, V, nv, idx, nidx, kind, lambda, opt, stats, stream, nullptr, 0);
+}
+
+MO_EXPORT int64_t mo_deform_batch_noalloc_max_bytes(void) { return (int64_t)kNoAllocScratchBytes; }
+
+MO_EXPORT int mo_deform_batch_noalloc(int32_t npairs, const int32_t* param_ids, double* const* V,
+                                      const int64_t* nv, const int32_t* const* idx, const int64_t* nidx,
+                                      const int32_t* kind, const double* lambda, const mo_lm_options* opt,
+                                      mo_lm_stats* stats, void* stream) {
+  int dev = 0;
+  MO_HIP(hipGetDevice(&dev));
+  MO_CHECK(dev >= 0 && dev < kMaxDevices, MO_ERR_ARG, "mo_deform_batch_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu[dev]);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    MO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MO_CHECK(p != nullptr, MO_ERR_HIP, "mo_deform_batch_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = (char*)p;
+  }
+  const int rc = deform_batch_impl(npairs, param_ids, V, nv, idx, nidx, kind, lambda, opt, stats, stream,
+                                   g_scratch_ptr[dev], kNoAllocScratchBytes);
+  // the array is reused by the next call: nothing of this call may still run
+  (void)hipStreamSynchronize(as_stream(stream));
+  return rc;
 }
 
 MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, const int64_t* src_nv,

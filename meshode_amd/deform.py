@@ -46,12 +46,14 @@ def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def deform_batch(param_ids, Vs, idxs, kinds, lambdas, options=None):
+def deform_batch(param_ids, Vs, idxs, kinds, lambdas, options=None, no_alloc=False):
     """Deformer::Deform / DeformGraph for each pair.
 
     Vs: FP64 device tensors [nv,3] in the template's normalized frame,
     updated in place.  idxs: int32 faces [n,3] (kind 0) or edges [n,2]
-    (kind 1).  Returns a list of stats dicts."""
+    (kind 1).  no_alloc: solve through mo_deform_batch_noalloc (no device
+    allocation; the batch's scratch must fit the library's fixed array).
+    Returns a list of stats dicts."""
     n = len(param_ids)
     Vs = list(Vs)
     for V in Vs:
@@ -60,7 +62,7 @@ def deform_batch(param_ids, Vs, idxs, kinds, lambdas, options=None):
     I = [_as_dev(i, torch.int32) for i in idxs]
     stats = (_lib.LMStats * n)()
     opt = options or default_options()
-    _lib.call("mo_deform_batch", n,
+    _lib.call("mo_deform_batch_noalloc" if no_alloc else "mo_deform_batch", n,
               _arr(C.c_int32, [int(p) for p in param_ids]),
               _arr(C.c_void_p, [V.data_ptr() for V in Vs]),
               _arr(C.c_int64, [V.shape[0] for V in Vs]),
