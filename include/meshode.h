@@ -63,6 +63,17 @@ int mo_template_create_f64(const double* V, int64_t nv, const int32_t* F, int64_
  * g_params slots, deform_params.cc:7-14).  Ids are not reused. */
 int mo_template_destroy(int32_t param_id);
 
+/* Rebuilds an existing template in place for a new FP64 target: the same
+ * normalization and grid build as mo_template_create_f64
+ * (deform_params.cc:16-40 repeated on an existing slot), into the
+ * template's buffers at its grid resolution, with no device allocation or
+ * free for the grid (hipFree synchronizes the whole device).  Stored edge
+ * offsets are dropped.  For a serving cache that recycles evicted targets'
+ * slots.  The caller must not have work using the template in flight.
+ * Synchronizes `stream`. */
+int mo_template_rebuild_f64(int32_t param_id, const double* V, int64_t nv, const int32_t* F, int64_t nf,
+                            void* stream);
+
 /* Grid resolution and the normalization of a template:
  * DeformParams::scale / trans (deform_params.h:15-16). */
 int mo_template_info(int32_t param_id, int32_t* grid_resolution, double* scale, double* trans3);

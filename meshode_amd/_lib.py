@@ -31,7 +31,7 @@ SYMBOLS = [
     "mo_flow256_packed_floats", "mo_flow256_pack", "mo_flow256_step", "mo_flow256_step_save",
     "mo_flow256_step_backward", "mo_flow256_small_grad_floats", "mo_flow256_small_grads",
     "mo_flow256_dw_work_floats", "mo_flow256_dw_grads",
-    "mo_deform_batch_noalloc_max_bytes", "mo_deform_batch_noalloc",
+    "mo_deform_batch_noalloc_max_bytes", "mo_deform_batch_noalloc", "mo_template_rebuild_f64",
 ]
 
 
@@ -103,6 +103,7 @@ _SIGS = {
     "mo_template_create_f32": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _i32, _vp, C.POINTER(_i32)]),
     "mo_template_create_f64": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _i32, _vp, C.POINTER(_i32)]),
     "mo_template_destroy": (C.c_int, [_i32]),
+    "mo_template_rebuild_f64": (C.c_int, [_i32, _vp, _i64, _vp, _i64, _vp]),
     "mo_template_info": (C.c_int, [_i32, C.POINTER(_i32), C.POINTER(_f64), _vp]),
     "mo_template_grid": (C.c_int, [_i32, _vp, _vp, _vp, _vp]),
     "mo_grid_build": (C.c_int, [_vp, _i64, _vp, _i64, _i32, _vp, _vp, _vp, _vp]),

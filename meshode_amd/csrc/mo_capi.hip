@@ -161,24 +161,14 @@ int zero_fill(void* p, size_t bytes, hipStream_t s) {
   return MO_OK;
 }
 
+// Normalize the target (Mesh::Normalize) and build the template's grid
+// (Mesh::ConstructDistanceField) into t's existing buffers.
 template <class T>
-int template_create(const T* V, int64_t nv, const int32_t* F, int64_t nf, int32_t grid_resolution, void* stream,
-                    int32_t* param_id) {
-  MO_CHECK(param_id, MO_ERR_ARG, "param_id output pointer is null");
+int template_fill(Template* t, const T* V, int64_t nv, const int32_t* F, int64_t nf, hipStream_t s) {
   MO_CHECK(V && F, MO_ERR_ARG, "template vertices/faces must be non-null device pointers");
   MO_CHECK(nv > 0 && nf > 0, MO_ERR_ARG, "template needs vertices and faces (nv=%lld nf=%lld)", (long long)nv,
            (long long)nf);
-  MO_CHECK(grid_resolution >= 2 && grid_resolution <= 1024, MO_ERR_ARG, "grid_resolution %d out of range [2, 1024]",
-           grid_resolution);
-  hipStream_t s = as_stream(stream);
-  auto t = std::make_unique<Template>();
-  MO_HIP(hipGetDevice(&t->device));
-  t->N = grid_resolution;
-  t->cells = (long long)grid_resolution * grid_resolution * grid_resolution;
   int rc;
-  if ((rc = t->grid64.alloc(t->cells * sizeof(double)))) return rc;
-  if ((rc = t->grid32.alloc(t->cells * sizeof(float)))) return rc;
-  if ((rc = t->nearest.alloc(t->cells * sizeof(int)))) return rc;
   GridSpec gs{nullptr, nv, F, nf, t->N, t->grid64.as<double>(), t->grid32.as<float>(), t->nearest.as<int>(),
               lookup_band2(t->N)};
   const size_t gws = grid_batch_workspace_bytes(&gs, 1);
@@ -207,6 +197,28 @@ int template_create(const T* V, int64_t nv, const int32_t* F, int64_t nf, int32_
   t->trans[0] = host[1];
   t->trans[1] = host[2];
   t->trans[2] = host[3];
+  return MO_OK;
+}
+
+template <class T>
+int template_create(const T* V, int64_t nv, const int32_t* F, int64_t nf, int32_t grid_resolution, void* stream,
+                    int32_t* param_id) {
+  MO_CHECK(param_id, MO_ERR_ARG, "param_id output pointer is null");
+  MO_CHECK(V && F, MO_ERR_ARG, "template vertices/faces must be non-null device pointers");
+  MO_CHECK(nv > 0 && nf > 0, MO_ERR_ARG, "template needs vertices and faces (nv=%lld nf=%lld)", (long long)nv,
+           (long long)nf);
+  MO_CHECK(grid_resolution >= 2 && grid_resolution <= 1024, MO_ERR_ARG, "grid_resolution %d out of range [2, 1024]",
+           grid_resolution);
+  hipStream_t s = as_stream(stream);
+  auto t = std::make_unique<Template>();
+  MO_HIP(hipGetDevice(&t->device));
+  t->N = grid_resolution;
+  t->cells = (long long)grid_resolution * grid_resolution * grid_resolution;
+  int rc;
+  if ((rc = t->grid64.alloc(t->cells * sizeof(double)))) return rc;
+  if ((rc = t->grid32.alloc(t->cells * sizeof(float)))) return rc;
+  if ((rc = t->nearest.alloc(t->cells * sizeof(int)))) return rc;
+  if ((rc = template_fill<T>(t.get(), V, nv, F, nf, s))) return rc;
   *param_id = register_template(std::move(t));
   return MO_OK;
 }
@@ -710,6 +722,22 @@ MO_EXPORT int mo_template_create_f64(const double* V, int64_t nv, const int32_t*
                                      int32_t grid_resolution, void* stream, int32_t* param_id) {
   (void)symmetry;
   return template_create<double>(V, nv, F, nf, grid_resolution, stream, param_id);
+}
+
+MO_EXPORT int mo_template_rebuild_f64(int32_t param_id, const double* V, int64_t nv, const int32_t* F, int64_t nf,
+                                      void* stream) {
+  Template* t = get_template(param_id);
+  MO_CHECK(t, MO_ERR_ARG, "unknown param_id %d", param_id);
+  int dev = 0;
+  MO_HIP(hipGetDevice(&dev));
+  MO_CHECK(dev == t->device, MO_ERR_ARG, "template %d lives on device %d, not the current device %d", param_id,
+           t->device, dev);
+  t->edge_off.reset();
+  t->edge_count = 0;
+  t->edge_kind = -1;
+  const int rc = template_fill<double>(t, V, nv, F, nf, as_stream(stream));
+  if (rc) t->scale = 0.0;   // a failed rebuild leaves no usable template behind
+  return rc;
 }
 
 MO_EXPORT int mo_template_destroy(int32_t param_id) {

@@ -172,3 +172,12 @@ def template_f64(V, F, grid_resolution):
     _lib.call("mo_template_create_f64", C.c_void_p(Vd.data_ptr()), Vd.shape[0], C.c_void_p(Fd.data_ptr()),
               Fd.shape[0], 0, int(grid_resolution), _stream(), C.byref(pid))
     return int(pid.value)
+
+
+def template_rebuild_f64(param_id, V, F):
+    """mo_template_rebuild_f64: rebuild an existing template in place for a
+    new FP64 target (same grid resolution; no grid allocation or free)."""
+    Vd = _as_dev(V, torch.float64)
+    Fd = _as_dev(F, torch.int32)
+    _lib.call("mo_template_rebuild_f64", int(param_id), C.c_void_p(Vd.data_ptr()), Vd.shape[0],
+              C.c_void_p(Fd.data_ptr()), Fd.shape[0], _stream())
