@@ -3,21 +3,23 @@
 // with Ceres' trust-region LM restated and a matrix-free block-Jacobi PCG).
 //
 // One workgroup of T threads solves one pair whose nv <= T * VPT vertices
-// are statically owned (vertex i by thread i % T, slot i / T), so the
-// per-vertex vectors the PCG iterates on (r, p, q/z and the distance
-// gradient g) live in VGPRs across the whole solve; the step delta and the
-// LM diagonal are kept per vertex in global memory (read/updated once per
-// PCG iteration, coalesced), which keeps the register footprint inside the
-// 128 (T = 1024) or 256 (T = 512) VGPRs a thread has.  The only vector neighbours read (u while
-// evaluating the Jacobian, p inside PCG, u + delta for the candidate) is
-// published to one float4-per-vertex LDS array, and the pair's adjacency is
-// copied once into LDS as uint16 entries (neighbour << 1 | multiplicity - 1),
-// so the PCG loop touches no global memory at all.
+// are statically owned (vertex i by thread i % T, slot i / T).  Everything
+// the PCG iterates on lives in VGPRs for the whole solve: the distance
+// gradient g, the residual r, the direction p, q = A p, the step delta and
+// d = lambda^2 deg + LM diagonal / radius (the diagonal of A apart from
+// g g^T, which is also the block-Jacobi preconditioner's diagonal).  The
+// one vector neighbours read (u while evaluating the Jacobian, z = M^-1 r
+// inside PCG, u + delta for the candidate) is published to a
+// float4-per-vertex LDS array, and the pair's adjacency is copied once into
+// LDS as uint16 entries (neighbour << 1 | multiplicity - 1), so the PCG loop
+// touches no global memory.  Reductions are FP64, through DPP lane moves
+// and one LDS exchange between waves.
 //
 // Differences from the global kernel that do not change the algorithm:
-//   * the block-Jacobi preconditioner M = g g^T + diag(lambda^2 deg + D^2)
-//     is applied with the Sherman-Morrison formula each iteration instead of
-//     a stored inverse (saves 6 VGPRs per vertex);
+//   * the preconditioner M = g g^T + diag(d) is applied with the
+//     Sherman-Morrison formula each iteration instead of a stored inverse;
+//   * A p is formed as A z + beta A p_prev (p = z + beta p_prev), so only z
+//     is published and an iteration needs two barriers;
 //   * the rigidity residuals are linear in x, so the rigidity part of the
 //     model cost change equals its actual change E_r(u + delta) - E_r(u);
 //     only u + delta has to be gathered for the candidate.
@@ -36,7 +38,7 @@ namespace mo {
 
 namespace {
 
-enum GSlot { gGR = 1, gS = 2, gDG = 3, gE = 4, gT = 5 };   // global slots of LMArrays::work used here
+enum GSlot { gGR = 1, gS = 2, gDG = 3 };   // global slots of LMArrays::work used here
 
 // Block-wide exclusive scan of one int per thread (T <= 1024).
 __device__ inline int block_exclusive_scan(int x, int* buf, int& total) {
@@ -92,8 +94,6 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
   float4* __restrict__ GR = A.work + (size_t)gGR * totalV + vb;
   float4* __restrict__ S = A.work + (size_t)gS * totalV + vb;
   float4* __restrict__ DG = A.work + (size_t)gDG * totalV + vb;
-  float4* __restrict__ EV = A.work + (size_t)gE * totalV + vb;
-  float4* __restrict__ TV = A.work + (size_t)gT * totalV + vb;
   float4* VEC = (float4*)smem;
   unsigned short* LCSR = (unsigned short*)(smem + (size_t)nv * sizeof(float4));
   int sel = 0;
@@ -129,21 +129,39 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
   }
 
   // ---- per-vertex register state ----------------------------------------
-  float gx[VPT], gy[VPT], gz[VPT];   // grad d
-  float rx[VPT], ry[VPT], rz_[VPT];     // PCG residual
-  float px[VPT], py[VPT], pz[VPT];      // PCG direction
-  float qx[VPT], qy[VPT], qz[VPT];      // A p, then the preconditioned residual z
+  // g: distance gradient; r: PCG residual; p: direction; q = A p;
+  // d = lambda^2 deg + LM diagonal / radius (the diagonal of A without
+  // g g^T, also the preconditioner's diagonal); e: the step delta.
+  float gx[VPT], gy[VPT], gz[VPT];
+  float rx[VPT], ry[VPT], rz_[VPT];
+  float px[VPT], py[VPT], pz[VPT];
+  float qx[VPT], qy[VPT], qz[VPT];
+  float dx[VPT], dy[VPT], dz[VPT];
+  float ex[VPT], ey[VPT], ez[VPT];
 
-  // z = M^-1 r with M = g g^T + diag(c), c = lambda^2 deg + D^2 (Sherman-Morrison)
-  auto precond = [&](int k, const float4 t4, float r0, float r1, float r2, float& z0, float& z1, float& z2) {
-    const float i0 = __builtin_amdgcn_rcpf(t4.w + t4.x), i1 = __builtin_amdgcn_rcpf(t4.w + t4.y),
-                i2 = __builtin_amdgcn_rcpf(t4.w + t4.z);
+  // z = M^-1 r with M = g g^T + diag(d) (Sherman-Morrison)
+  auto precond = [&](int k, float r0, float r1, float r2, float& z0, float& z1, float& z2) {
+    const float i0 = __builtin_amdgcn_rcpf(dx[k]), i1 = __builtin_amdgcn_rcpf(dy[k]),
+                i2 = __builtin_amdgcn_rcpf(dz[k]);
     const float h0 = gx[k] * i0, h1 = gy[k] * i1, h2 = gz[k] * i2;
     const float sig = 1.0f + gx[k] * h0 + gy[k] * h1 + gz[k] * h2;
     const float c = (h0 * r0 + h1 * r1 + h2 * r2) * __builtin_amdgcn_rcpf(sig);
     z0 = r0 * i0 - h0 * c;
     z1 = r1 * i1 - h1 * c;
     z2 = r2 * i2 - h2 * c;
+  };
+
+  // sum over the vertex's neighbours of multiplicity * VEC[neighbour].xyz
+  auto gather = [&](int k, float& sx, float& sy, float& sz) {
+    sx = 0.f; sy = 0.f; sz = 0.f;
+    const int cs = opaque((int)csr[k]);
+    const int o = (int)((unsigned)cs >> 16), c = cs & 0xFFFF;
+    for (int q = 0; q < c; ++q) {
+      const unsigned w = LCSR[o + q];
+      const float m = (float)((w & 1u) + 1u);
+      const float4 vj = VEC[w >> 1];
+      sx += m * vj.x; sy += m * vj.y; sz += m * vj.z;
+    }
   };
 
   auto eval = [&](bool first, double& cv, double& ce, double& gmax) {
@@ -188,16 +206,25 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
         S[i] = make_float4(1.0f / (1.0f + sqrtf(c0)), 1.0f / (1.0f + sqrtf(c1)), 1.0f / (1.0f + sqrtf(c2)), 0.f);
       }
     }
-    block_reduce<2, 1>(v, red, sel);
+    block_reduce_dpp<2, 1>(v, red, sel);
     cv = v[0]; ce = v[1]; gmax = v[2];
   };
 
+  // PCG on (J^T J + D^2 / radius) delta = -J^T f.  Each iteration publishes
+  // z = M^-1 r once and forms A p as A z + beta A p_prev (p = z + beta
+  // p_prev), so an iteration has two barriers: the reduction of p.Ap, and
+  // the reduction of r.z and r.r, which also publishes the new z.
   auto solve = [&](bool reuse, double radius) -> int {
     const double inv_radius = 1.0 / radius;
     double v[2] = {0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
       const int i = opaque(tid + k * T);
+      px[k] = py[k] = pz[k] = 0.f;
+      qx[k] = qy[k] = qz[k] = 0.f;
+      ex[k] = ey[k] = ez[k] = 0.f;
+      rx[k] = ry[k] = rz_[k] = 0.f;
+      dx[k] = dy[k] = dz[k] = 1.f;
       if (i >= nv) continue;
       const float4 s = S[i];
       const float ldk = l2 * X0[i].w;
@@ -221,48 +248,42 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       const double t0 = (double)dg.x / ((double)s.x * s.x) * inv_radius;
       const double t1 = (double)dg.y / ((double)s.y * s.y) * inv_radius;
       const double t2 = (double)dg.z / ((double)s.z * s.z) * inv_radius;
-      const float4 t4 = make_float4((float)(t0 < cap ? t0 : cap), (float)(t1 < cap ? t1 : cap),
-                                    (float)(t2 < cap ? t2 : cap), ldk);
-      TV[i] = t4;
+      dx[k] = ldk + (float)(t0 < cap ? t0 : cap);
+      dy[k] = ldk + (float)(t1 < cap ? t1 : cap);
+      dz[k] = ldk + (float)(t2 < cap ? t2 : cap);
       const float4 gr = GR[i];
       rx[k] = -gr.x; ry[k] = -gr.y; rz_[k] = -gr.z;
       float z0, z1, z2;
-      precond(k, t4, rx[k], ry[k], rz_[k], z0, z1, z2);
-      px[k] = z0; py[k] = z1; pz[k] = z2;
-      EV[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      precond(k, rx[k], ry[k], rz_[k], z0, z1, z2);
       VEC[i] = make_float4(z0, z1, z2, 0.f);
-      v[0] += (double)rx[k] * z0 + (double)ry[k] * z1 + (double)rz_[k] * z2;
-      v[1] += (double)rx[k] * rx[k] + (double)ry[k] * ry[k] + (double)rz_[k] * rz_[k];
+      v[0] += (double)(rx[k] * z0 + ry[k] * z1 + rz_[k] * z2);
+      v[1] += (double)(rx[k] * rx[k] + ry[k] * ry[k] + rz_[k] * rz_[k]);
     }
-    block_reduce<2, 0>(v, red, sel);   // also publishes p to LDS
+    block_reduce_dpp<2, 0>(v, red, sel);   // also publishes z
     double rz = v[0];
     const double rr0 = v[1];
     if (!(rr0 > 0.0)) return 0;
     const double tol2 = (double)o_pcg_tol * o_pcg_tol * rr0;
+    float beta = 0.f;
     int it = 0;
-    for (; it < o_pcg_max; ++it) {
+    while (it < o_pcg_max) {
       double w1[1] = {0.0};
 #pragma unroll
       for (int k = 0; k < VPT; ++k) {
         const int i = opaque(tid + k * T);
         if (i >= nv) continue;
-        const float gp = gx[k] * px[k] + gy[k] * py[k] + gz[k] * pz[k];
-        float lx = 0.f, ly = 0.f, lz = 0.f;
-        const int cs = opaque((int)csr[k]);
-      const int o = (int)((unsigned)cs >> 16), c = cs & 0xFFFF;
-        for (int q = 0; q < c; ++q) {
-          const unsigned w = LCSR[o + q];
-          const float m = (float)((w & 1u) + 1u);
-          const float4 pj = VEC[w >> 1];
-          lx += m * (px[k] - pj.x); ly += m * (py[k] - pj.y); lz += m * (pz[k] - pj.z);
-        }
-        const float4 t4 = TV[i];
-        qx[k] = gx[k] * gp + l2 * lx + t4.x * px[k];
-        qy[k] = gy[k] * gp + l2 * ly + t4.y * py[k];
-        qz[k] = gz[k] * gp + l2 * lz + t4.z * pz[k];
-        w1[0] += (double)px[k] * qx[k] + (double)py[k] * qy[k] + (double)pz[k] * qz[k];
+        const float4 zi = VEC[i];
+        float sx, sy, sz;
+        gather(k, sx, sy, sz);
+        const float gzz = gx[k] * zi.x + gy[k] * zi.y + gz[k] * zi.z;
+        const float wx = gx[k] * gzz + dx[k] * zi.x - l2 * sx;
+        const float wy = gy[k] * gzz + dy[k] * zi.y - l2 * sy;
+        const float wz = gz[k] * gzz + dz[k] * zi.z - l2 * sz;
+        px[k] = zi.x + beta * px[k]; py[k] = zi.y + beta * py[k]; pz[k] = zi.z + beta * pz[k];
+        qx[k] = wx + beta * qx[k]; qy[k] = wy + beta * qy[k]; qz[k] = wz + beta * qz[k];
+        w1[0] += (double)(px[k] * qx[k] + py[k] * qy[k] + pz[k] * qz[k]);
       }
-      block_reduce<1, 0>(w1, red, sel);
+      block_reduce_dpp<1, 0>(w1, red, sel);
       const double pq = w1[0];
       if (!(pq > 0.0)) break;
       const float fa = (float)(rz / pq);
@@ -271,28 +292,19 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       for (int k = 0; k < VPT; ++k) {
         const int i = opaque(tid + k * T);
         if (i >= nv) continue;
-        float4 e4 = EV[i];
-        e4.x += fa * px[k]; e4.y += fa * py[k]; e4.z += fa * pz[k];
-        EV[i] = e4;
+        ex[k] += fa * px[k]; ey[k] += fa * py[k]; ez[k] += fa * pz[k];
         rx[k] -= fa * qx[k]; ry[k] -= fa * qy[k]; rz_[k] -= fa * qz[k];
         float z0, z1, z2;
-        precond(k, TV[i], rx[k], ry[k], rz_[k], z0, z1, z2);
-        qx[k] = z0; qy[k] = z1; qz[k] = z2;
-        w2[0] += (double)rx[k] * z0 + (double)ry[k] * z1 + (double)rz_[k] * z2;
-        w2[1] += (double)rx[k] * rx[k] + (double)ry[k] * ry[k] + (double)rz_[k] * rz_[k];
+        precond(k, rx[k], ry[k], rz_[k], z0, z1, z2);
+        VEC[i] = make_float4(z0, z1, z2, 0.f);
+        w2[0] += (double)(rx[k] * z0 + ry[k] * z1 + rz_[k] * z2);
+        w2[1] += (double)(rx[k] * rx[k] + ry[k] * ry[k] + rz_[k] * rz_[k]);
       }
-      block_reduce<2, 0>(w2, red, sel);
-      const float fb = (float)(w2[0] / rz);
+      block_reduce_dpp<2, 0>(w2, red, sel);   // also publishes z
+      ++it;
+      if (w2[1] <= tol2) break;
+      beta = (float)(w2[0] / rz);
       rz = w2[0];
-      if (w2[1] <= tol2) { ++it; break; }
-#pragma unroll
-      for (int k = 0; k < VPT; ++k) {
-        const int i = opaque(tid + k * T);
-        if (i >= nv) continue;
-        px[k] = qx[k] + fb * px[k]; py[k] = qy[k] + fb * py[k]; pz[k] = qz[k] + fb * pz[k];
-        VEC[i] = make_float4(px[k], py[k], pz[k], 0.f);
-      }
-      __syncthreads();
     }
     return it;
   };
@@ -304,8 +316,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       const int i = opaque(tid + k * T);
       if (i < nv) {
         const float4 ui = U[i];
-        const float4 e4 = EV[i];
-        VEC[i] = make_float4(ui.x + e4.x, ui.y + e4.y, ui.z + e4.z, 0.f);
+        VEC[i] = make_float4(ui.x + ex[k], ui.y + ey[k], ui.z + ez[k], 0.f);
       }
     }
     __syncthreads();
@@ -316,8 +327,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       if (i >= nv) continue;
       const float4 x0 = X0[i];
       const float dres = GR[i].w;
-      const float4 e4 = EV[i];
-      const float gd = gx[k] * e4.x + gy[k] * e4.y + gz[k] * e4.z;
+      const float gd = gx[k] * ex[k] + gy[k] * ey[k] + gz[k] * ez[k];
       v[0] += (double)gd * ((double)dres + 0.5 * (double)gd);
       const float4 wv = VEC[i];
       const float wx = wv.x, wy = wv.y, wz = wv.z;
@@ -334,12 +344,12 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
         e += (double)m * ((double)ax * ax + (double)ay * ay + (double)az * az);
       }
       v[2] += 0.25 * (double)l2 * e;
-      v[3] += (double)e4.x * e4.x + (double)e4.y * e4.y + (double)e4.z * e4.z;
+      v[3] += (double)ex[k] * ex[k] + (double)ey[k] * ey[k] + (double)ez[k] * ez[k];
       const float4 ui = U[i];
       const float qx0 = x0.x + ui.x, qy0 = x0.y + ui.y, qz0 = x0.z + ui.z;
       v[4] += (double)qx0 * qx0 + (double)qy0 * qy0 + (double)qz0 * qz0;
     }
-    block_reduce<5, 0>(v, red, sel);
+    block_reduce_dpp<5, 0>(v, red, sel);
     // rigidity residuals are linear: their model change equals the actual one
     return LMCandidate{-v[0] - (v[2] - ce_now), v[1] + v[2], sqrt(v[3]), sqrt(v[4])};
   };

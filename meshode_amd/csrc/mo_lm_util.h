@@ -41,6 +41,80 @@ __device__ inline void block_reduce(double (&v)[K + M], double* red, int& sel) {
   sel ^= 1;
 }
 
+// Wavefront reductions of FP64 values through DPP lane moves (no LDS).  A
+// DPP control moves each 32-bit half: quad_perm [1,0,3,2] (0xB1) and
+// [2,3,0,1] (0x4E), row_half_mirror (0x141), row_mirror (0x140).  After the
+// four butterfly steps every lane of a 16-lane row holds the row's total,
+// formed by the same additions in every lane (IEEE addition commutes), and
+// the four row totals are combined in a fixed order: every lane of the wave
+// gets the same bits.  All 64 lanes must be active.
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, (unsigned long long)(unsigned)lo | ((unsigned long long)(unsigned)hi << 32));
+}
+
+__device__ inline double readlane_f64(double v, int lane) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+  return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+
+__device__ inline double row_sum_f64(double v) {
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  v += dpp_f64<0x140>(v);
+  return v;
+}
+
+__device__ inline double row_max_f64(double v) {
+  v = fmax(v, dpp_f64<0xB1>(v));
+  v = fmax(v, dpp_f64<0x4E>(v));
+  v = fmax(v, dpp_f64<0x141>(v));
+  v = fmax(v, dpp_f64<0x140>(v));
+  return v;
+}
+
+__device__ inline double wave_sum_f64(double v) {
+  v = row_sum_f64(v);
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+__device__ inline double wave_max_f64(double v) {
+  v = row_max_f64(v);
+  return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)), fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+
+// Block-wide reduction of K sums then M maxima with DPP: each wave reduces
+// its own lanes, lane 0 writes the wave's totals to LDS, and after one
+// barrier every wave combines the (at most 16) wave totals in lanes 0..15
+// with the same row butterfly, so every thread gets the same bits.  Two LDS
+// buffers alternate (sel), so consecutive reductions need only the barrier
+// inside each.
+template <int K, int M>
+__device__ inline void block_reduce_dpp(double (&v)[K + M], double* red, int& sel) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int k = 0; k < K; ++k) v[k] = wave_sum_f64(v[k]);
+  for (int k = K; k < K + M; ++k) v[k] = wave_max_f64(v[k]);
+  double* buf = red + sel * (8 * kMaxWaves);
+  if (lane == 0)
+    for (int k = 0; k < K + M; ++k) buf[k * kMaxWaves + w] = v[k];
+  __syncthreads();
+  for (int k = 0; k < K; ++k) {
+    const double x = lane < nw ? buf[k * kMaxWaves + lane] : 0.0;
+    v[k] = readlane_f64(row_sum_f64(x), 0);
+  }
+  for (int k = K; k < K + M; ++k) {
+    const double x = lane < nw ? buf[k * kMaxWaves + lane] : -INFINITY;
+    v[k] = readlane_f64(row_max_f64(x), 0);
+  }
+  sel ^= 1;
+}
+
 __device__ inline unsigned nb_index(unsigned e) { return e & 0xFFFFFFu; }
 __device__ inline float nb_mult(unsigned e) { return (float)(e >> 24); }
 
