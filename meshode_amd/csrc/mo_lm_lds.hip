@@ -156,6 +156,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
     sx = 0.f; sy = 0.f; sz = 0.f;
     const int cs = opaque((int)csr[k]);
     const int o = (int)((unsigned)cs >> 16), c = cs & 0xFFFF;
+    #pragma unroll 1
     for (int q = 0; q < c; ++q) {
       const unsigned w = LCSR[o + q];
       const float m = (float)((w & 1u) + 1u);
@@ -178,13 +179,14 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       if (i >= nv) continue;
       const float4 x0 = X0[i];
       const float4 ui = VEC[i];
-      const JetF J = lookup_jet(grid, N, x0.x + ui.x, x0.y + ui.y, x0.z + ui.z);
+      const JetF J = lookup_value_grad(grid, N, x0.x + ui.x, x0.y + ui.y, x0.z + ui.z);
       gx[k] = J.v0; gy[k] = J.v1; gz[k] = J.v2;
       const float ldk = l2 * x0.w;
       float sx = 0.f, sy = 0.f, sz = 0.f;
       double e = 0.0;
       const int cs = opaque((int)csr[k]);
       const int o = (int)((unsigned)cs >> 16), c = cs & 0xFFFF;
+      #pragma unroll 1
       for (int q = 0; q < c; ++q) {
         const unsigned w = LCSR[o + q];
         const float m = (float)((w & 1u) + 1u);
@@ -215,7 +217,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
   // p_prev), so an iteration has two barriers: the reduction of p.Ap, and
   // the reduction of r.z and r.r, which also publishes the new z.
   auto solve = [&](bool reuse, double radius) -> int {
-    const double inv_radius = 1.0 / radius;
+    const float finv_radius = (float)(1.0 / radius);
     double v[2] = {0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
@@ -244,13 +246,12 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       } else {
         dg = DG[i];
       }
-      const double cap = 1e30;
-      const double t0 = (double)dg.x / ((double)s.x * s.x) * inv_radius;
-      const double t1 = (double)dg.y / ((double)s.y * s.y) * inv_radius;
-      const double t2 = (double)dg.z / ((double)s.z * s.z) * inv_radius;
-      dx[k] = ldk + (float)(t0 < cap ? t0 : cap);
-      dy[k] = ldk + (float)(t1 < cap ? t1 : cap);
-      dz[k] = ldk + (float)(t2 < cap ? t2 : cap);
+      // LM regularization D^2 / radius in the unscaled variables (the
+      // diagonal over s^2), capped at 1e30; FP32 with a reciprocal (the
+      // FP64 divisions' temporaries would set the kernel's register peak)
+      dx[k] = ldk + fminf(dg.x * __builtin_amdgcn_rcpf(s.x * s.x) * finv_radius, 1e30f);
+      dy[k] = ldk + fminf(dg.y * __builtin_amdgcn_rcpf(s.y * s.y) * finv_radius, 1e30f);
+      dz[k] = ldk + fminf(dg.z * __builtin_amdgcn_rcpf(s.z * s.z) * finv_radius, 1e30f);
       const float4 gr = GR[i];
       rx[k] = -gr.x; ry[k] = -gr.y; rz_[k] = -gr.z;
       float z0, z1, z2;
@@ -336,6 +337,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       double e = 0.0;
       const int cs = opaque((int)csr[k]);
       const int o = (int)((unsigned)cs >> 16), c = cs & 0xFFFF;
+      #pragma unroll 1
       for (int q = 0; q < c; ++q) {
         const unsigned w = LCSR[o + q];
         const float m = (float)((w & 1u) + 1u);

@@ -67,6 +67,55 @@ __device__ inline float lookup_f32(const float* __restrict__ grid, int N, float 
   return res;
 }
 
+// Value and gradient for the LM solver: the value is formed exactly as in
+// lookup_f32 (the solver's candidate costs use lookup_f32, so the cost at a
+// point does not depend on which of the two evaluated it); the gradient is
+// the trilinear interpolant's analytic derivative written compactly, e.g.
+// d/dx = N [(c001 - c000) oy oz + (c011 - c010) wy oz + (c101 - c100) oy wz
+// + (c111 - c110) wy wz], which is DistanceFloat<Jet<float,3>>'s derivative
+// up to FP32 rounding order; +-N per axis outside the box and 0 when the
+// value is truncated, as the Jet gives.  About half the registers of
+// lookup_jet, whose Jet products keep every zero derivative term.
+__device__ inline JetF lookup_value_grad(const float* __restrict__ grid, int N, float p0, float p1, float p2) {
+  const float fN = (float)N;
+  const int px = trunc_i32f(p0 * fN), py = trunc_i32f(p1 * fN), pz = trunc_i32f(p2 * fN);
+  if (px < 0 || py < 0 || pz < 0 || px >= N - 1 || py >= N - 1 || pz >= N - 1) {
+    const float edge = (float)(N - 1 - 1e-3);
+    float l = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+    if (px < 0) { l = l + (-p0) * fN; g0 = -fN; }
+    else if (px >= N) { l = l + (p0 * fN - edge); g0 = fN; }
+    if (py < 0) { l = l + (-p1) * fN; g1 = -fN; }
+    else if (py >= N) { l = l + (p1 * fN - edge); g1 = fN; }
+    if (pz < 0) { l = l + (-p2) * fN; g2 = -fN; }
+    else if (pz >= N) { l = l + (p2 * fN - edge); g2 = fN; }
+    return JetF{l, g0, g1, g2};
+  }
+  const float wx = p0 * fN - (float)px, wy = p1 * fN - (float)py, wz = p2 * fN - (float)pz;
+  const long long base = ((long long)pz * N + py) * N + px;
+  const long long sy = N, sz = (long long)N * N;
+  const float c000 = grid[base], c001 = grid[base + 1], c010 = grid[base + sy], c011 = grid[base + sy + 1];
+  const float c100 = grid[base + sz], c101 = grid[base + sz + 1], c110 = grid[base + sz + sy],
+              c111 = grid[base + sz + sy + 1];
+  const float ox = 1.0f - wx, oy = 1.0f - wy, oz = 1.0f - wz;
+  float w0 = ox * oy * oz * c000;
+  float w1 = wx * oy * oz * c001;
+  float w2 = ox * wy * oz * c010;
+  float w3 = wx * wy * oz * c011;
+  float w4 = ox * oy * wz * c100;
+  float w5 = wx * oy * wz * c101;
+  float w6 = ox * wy * wz * c110;
+  float w7 = wx * wy * wz * c111;
+  float res = w0 + w1 + w2 + w3 + w4 + w5 + w6 + w7;
+  if (res > 0.2f) return JetF{0.0f, 0.0f, 0.0f, 0.0f};
+  const float yz00 = oy * oz, yz10 = wy * oz, yz01 = oy * wz, yz11 = wy * wz;
+  const float g0 = fN * ((c001 - c000) * yz00 + (c011 - c010) * yz10 + (c101 - c100) * yz01 + (c111 - c110) * yz11);
+  const float xz00 = ox * oz, xz10 = wx * oz, xz01 = ox * wz, xz11 = wx * wz;
+  const float g1 = fN * ((c010 - c000) * xz00 + (c011 - c001) * xz10 + (c110 - c100) * xz01 + (c111 - c101) * xz11);
+  const float xy00 = ox * oy, xy10 = wx * oy, xy01 = ox * wy, xy11 = wx * wy;
+  const float g2 = fN * ((c100 - c000) * xy00 + (c101 - c001) * xy10 + (c110 - c010) * xy01 + (c111 - c011) * xy11);
+  return JetF{res, g0, g1, g2};
+}
+
 // DistanceFloat<Jet<float,3>> seeded with the identity: value and gradient.
 __device__ inline JetF lookup_jet(const float* __restrict__ grid, int N, float p0, float p1, float p2) {
   const float fNs = (float)N;
