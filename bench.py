@@ -24,26 +24,54 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFLOPS = 78.6    # MI355X FP64 vector (FMA = 2 flops)
+FP32_VALU_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (published; packed FP32)
+
+# Flop model of the register/LDS LM kernel (FP32 vector work per vertex):
+# per PCG iteration 84 + 6 E_v (matrix-free A z with E_v adjacency entries
+# per vertex at 3 FMAs each, recurrences, Sherman-Morrison preconditioner,
+# three dot products); per LM iteration 150 + 20 E_v (distance lookup and
+# gradient, edge energy in the Jacobian and candidate evaluations).
+# rocprofv3's SQ_INSTS_VALU_FLOPS_FP32/FP64 on the headline launch read
+# 6.6e11 flops against this model's 6.1e11.
+LM_PCG_FLOPS_BASE, LM_PCG_FLOPS_PER_ENTRY = 84, 6
+LM_ITER_FLOPS_BASE, LM_ITER_FLOPS_PER_ENTRY = 150, 20
 
 
-def lm_algorithmic_bytes(nv, nh, lm_iters, pcg_iters):
-    """Compulsory traffic of the LM solve (FP32 vectors, int32 indices):
-    per LM iteration 96 V + 44 H bytes (lookup + gradient, edge residuals and
-    J^T r, update), per PCG iteration 184 V + 4 H bytes (matrix-free product,
-    preconditioner apply, fused axpy/dot passes)."""
-    return lm_iters * (96 * nv + 44 * nh) + pcg_iters * (184 * nv + 4 * nh)
+def lm_algorithmic_flops(nv, entries, lm_iters, pcg_iters):
+    """Flops of one pair's LM solve: nv vertices, `entries` adjacency
+    entries (distinct neighbours summed over vertices)."""
+    import numpy as np
+    ev = np.asarray(entries, dtype=np.float64) / np.maximum(nv, 1)
+    return nv * (pcg_iters * (LM_PCG_FLOPS_BASE + LM_PCG_FLOPS_PER_ENTRY * ev) +
+                 lm_iters * (LM_ITER_FLOPS_BASE + LM_ITER_FLOPS_PER_ENTRY * ev))
 
 
-# HBM bytes per LM-solver launch measured with rocprofv3 counter passes
-# (FETCH_SIZE x 1.99975 + WRITE_SIZE x 1.0, factors from a 256 MiB copy) on
-# the default workload; profiles/k_lm_solve_lds_hbm_traffic_batch512.txt.
-# Keyed by (pairs, distinct pairs, source verts, grid); other workloads
-# report null.
-MEASURED_LM_TRAFFIC = {(512, 128, 5000, 64): 270.58e9}
+# Flop model of the grid build (FP64, no FMA contraction): 60 flops per
+# point-triangle distance evaluation (face-region test, or the three
+# canonical edge distances) and 25 per face bound check (point-to-AABB and
+# plane bounds); both counts come live from the kernel (mo_last_grid_stats).
+# rocprofv3's SQ_INSTS_VALU_FLOPS_FP64 on the headline launch reads 9.3e11
+# flops against this model's ~7e11 (the rest: tile scans and seeding).
+GRID_FLOPS_PER_EXACT, GRID_FLOPS_PER_BOUND = 60, 25
 
 
-def grid_algorithmic_bytes(N, nf_t):
-    return 4 * N ** 3 + 36 * nf_t
+def adjacency_entries(F):
+    """Distinct undirected neighbours summed over vertices (the LM kernel's
+    adjacency entries) of a triangle mesh."""
+    import numpy as np
+    a = np.concatenate([F[:, 0], F[:, 1], F[:, 2], F[:, 1], F[:, 2], F[:, 0]]).astype(np.int64)
+    b = np.concatenate([F[:, 1], F[:, 2], F[:, 0], F[:, 0], F[:, 1], F[:, 2]]).astype(np.int64)
+    keep = a != b
+    return int(len(np.unique(a[keep] * (int(F.max()) + 1) + b[keep])))
+
+
+# HBM bytes per launch measured with rocprofv3 counter passes (FETCH_SIZE x
+# 1.99975 and WRITE_SIZE x 1.0, the factors a plain 256 MiB copy reads) on
+# the default workload; keyed by (kernel, pairs, distinct pairs, source
+# verts, grid); other workloads report null.
+MEASURED_TRAFFIC = {("k_lm_solve_lds", 512, 128, 5000, 64): 37.48e9,
+                    ("k_grid_build", 512, 128, 5000, 64): 12.85e9}
 
 
 def cpu_baseline(pair, N, lam):
@@ -97,7 +125,9 @@ def main():
     ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "flow256", "neural2", "rot", "cfg1", "cfg4"],
                     help="batch: the headline line; serve / adam / ode: the serving, pyDeform-Adam and "
                          "NeuralODE workloads (their own JSON line)")
-    ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of Poisson arrivals")
+    ap.add_argument("--duration", type=float, default=None, help="serve: seconds of Poisson arrivals (default 60)")
+    ap.add_argument("--serve-pairs", type=int, default=96, help="serve: distinct (source, target) pairs in the pool")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: template cache / admission budget")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: requests per second")
     ap.add_argument("--max-verts", type=int, default=100000, help="serve: largest source size")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -162,6 +192,11 @@ def main():
 
     value = args.pairs * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
+    # one more step, after the timed region, with the grid build's work
+    # counters on (timing 2: their atomics cost a few percent of the phase)
+    opt.timing = 2
+    batch.run(opt)
+    opt.timing = 1
 
     # per-pair latency inside the batch: completion time since the call began
     lat = []
@@ -170,16 +205,38 @@ def main():
             lat.append((s["clock_end"] - tm["clock_start"]) / tm["clock_hz"] * 1e3)
     lat = np.array(lat)
 
-    # roofline of the dominant kernel (LM solver), algorithmic bytes per launch
+    # rooflines: the grid build (FP64 VALU, work counted live by the
+    # kernel) and the LM solver (FP32 VALU, algorithmic flop model); the one
+    # with the longer live time is the dominant kernel
     nv = np.array([len(p[0]) for p in pairs])
-    nh = np.array([3 * len(p[1]) for p in pairs])
-    lm_bytes = []
+    ent_d = [adjacency_entries(p[1]) for p in distinct]
+    ent = np.array([ent_d[i % len(distinct)] for i in range(len(pairs))])
+    lm_flops = []
     for st in stats_all:
         it = np.array([s["iterations"] for s in st])
         pc = np.array([s["pcg_iterations"] for s in st])
-        lm_bytes.append(float(np.sum(lm_algorithmic_bytes(nv, nh, it, pc))))
-    lm_ms = np.array([t["lm_ms"] for t in timings])
-    achieved = float(np.mean(lm_bytes) / (lm_ms.mean() / 1e3) / 1e9)
+        lm_flops.append(float(np.sum(lm_algorithmic_flops(nv, ent, it, pc))))
+    lm_ms = float(np.mean([t["lm_ms"] for t in timings]))
+    grid_ms = float(np.mean([t["grid_ms"] for t in timings]))
+    gs = _lib.last_grid_stats()
+    grid_flops = GRID_FLOPS_PER_EXACT * gs["exact_tests"] + GRID_FLOPS_PER_BOUND * gs["bound_checks"]
+    lm_tf = float(np.mean(lm_flops)) / (lm_ms / 1e3) / 1e12
+    grid_tf = grid_flops / (grid_ms / 1e3) / 1e12
+    cells = int(sum(args.grid ** 3 for _ in pairs))
+    wkey = (args.pairs, len(distinct), args.verts, args.grid)
+    roof_grid = {"kernel": "k_grid_build", "bound": "fp64-valu", "achieved": grid_tf, "peak": FP64_VALU_PEAK_TFLOPS,
+                 "unit": "TFLOP/s", "frac": grid_tf / FP64_VALU_PEAK_TFLOPS,
+                 "traffic": MEASURED_TRAFFIC.get(("k_grid_build",) + wkey),
+                 "flops_per_launch": grid_flops, "launch_ms": grid_ms,
+                 "exact_tests_per_cell": gs["exact_tests"] / cells,
+                 "exact_tests_needed_per_cell": gs["exact_tests_needed"] / cells,
+                 "bound_checks_per_cell": gs["bound_checks"] / cells,
+                 "bound_check_flop_share": GRID_FLOPS_PER_BOUND * gs["bound_checks"] / max(grid_flops, 1)}
+    roof_lm = {"kernel": "k_lm_solve_lds", "bound": "fp32-valu", "achieved": lm_tf, "peak": FP32_VALU_PEAK_TFLOPS,
+               "unit": "TFLOP/s", "frac": lm_tf / FP32_VALU_PEAK_TFLOPS,
+               "traffic": MEASURED_TRAFFIC.get(("k_lm_solve_lds",) + wkey),
+               "flops_per_launch": float(np.mean(lm_flops)), "launch_ms": lm_ms}
+    assert roof_grid["frac"] <= 1.0 and roof_lm["frac"] <= 1.0, (roof_grid, roof_lm)
     last = stats_all[-1]
 
     # plain path: one pair per call, sequentially
@@ -228,16 +285,9 @@ def main():
         "lm_iterations_mean": float(np.mean([s["iterations"] for s in last])),
         "pcg_iterations_per_lm_iteration": float(np.sum([s["pcg_iterations"] for s in last]) /
                                                  max(1, np.sum([s["iterations"] for s in last]))),
-        "roofline": {
-            "kernel": "k_lm_solve_lds",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": MEASURED_LM_TRAFFIC.get((args.pairs, len(distinct), args.verts, args.grid)),
-            "algorithmic_bytes_per_launch": float(np.mean(lm_bytes)),
-        },
+        "roofline": roof_grid if grid_ms >= lm_ms else roof_lm,
+        "roofline_grid": roof_grid,
+        "roofline_lm": roof_lm,
     }
 
     if not args.no_cpu_baseline:
@@ -383,51 +433,106 @@ def bench_cfg4(args):
                        "block_threads": opt.block_threads, "team_vertices": opt.team_vertices}}
 
 
+def _serve_pair(job):
+    """One (source, target) pair of the serving pool: config-1-like
+    (icosphere source, displaced icosphere target one level finer, capped at
+    6) or config-2-like (box union of exactly nv vertices vs its jittered
+    boxes); run in a worker process."""
+    kind, seed, nv = job
+    from meshode_amd import synth
+    if kind == 0:
+        lvl = 4 if nv < 6000 else 5 if nv < 25000 else 6
+        return synth.icosphere_pair(seed, lvl)
+    return synth.box_union_pair(seed, n_verts=max(1000, nv - nv % 2))
+
+
 def bench_serve(args):
-    """Poisson arrivals of config-1/config-2-like pairs of uneven sizes and
-    grids through the continuous-batching server; p50/p99 latency and pairs/s."""
+    """Config 5: Poisson arrivals at args.rate requests/s for args.duration
+    seconds through the continuous-batching server, cache starting cold.
+    A pool of args.serve_pairs distinct pairs (half built like config 1, half
+    like config 2; source sizes log-uniform 2k-args.max_verts vertices; each
+    pair's grid drawn from {64, 128, 256}^3) is generated on the host first;
+    every request takes a pool pair uniformly at random and moves its source
+    by a fresh random rotation (up to 3 degrees), scale (+-2 %) and shift
+    (+-1 % of the extent), so no two requests are the same problem.  The
+    first request for a target builds its template (normalization + grid);
+    the LRU cache works under args.budget_gb.  Reports pairs/s, p50/p99
+    latency (all requests and cache misses), builds and evictions."""
     import numpy as np
+    from concurrent.futures import ProcessPoolExecutor
     from meshode_amd import serve, synth
     rng = np.random.default_rng(7)
-    pool = []
-    for i in range(24):
-        nv = int(np.exp(rng.uniform(np.log(2000), np.log(args.max_verts))))
-        if i % 2 == 0:
-            p = synth.box_union_pair(1000 + i, n_verts=max(nv - nv % 2, 1000))   # box unions have even counts
-        else:
-            lvl = 3 if nv < 4000 else 4 if nv < 15000 else 5 if nv < 60000 else 6
-            p = synth.displaced_sphere_pair(1000 + i, src_level=lvl, tgt_level=min(lvl + 1, 6))
-        grid = int(rng.choice([64, 128, 256]))
-        pool.append((p, grid))
-    n_req = int(args.rate * args.duration)
-    gaps = rng.exponential(1.0 / args.rate, n_req)
-    picks = rng.integers(0, len(pool), n_req)
-    srv = serve.DeformServer(mem_budget_bytes=200e9, max_batch=512, workers=2)
-    # warm the template cache and the kernels on every pool entry
-    warm = [srv.submit(serve.DeformRequest(p[0], p[1], i, p[2], p[3], grid=g)) for i, (p, g) in enumerate(pool)]
-    for f in warm:
-        f.result()
-    futs = []
+    n_pool = args.serve_pairs
+    sizes = np.exp(rng.uniform(np.log(2000), np.log(args.max_verts), n_pool)).astype(int)
+    kinds = np.arange(n_pool) % 2
+    grids = [int(g) for g in rng.choice([64, 128, 256], n_pool)]
     t0 = time.perf_counter()
-    t_next = t0
-    for gap, k in zip(gaps, picks):
+    workers = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        pool = list(ex.map(_serve_pair, [(int(k), 5000 + i, int(n)) for i, (k, n) in enumerate(zip(kinds, sizes))]))
+    print(f"[bench] serve: {n_pool} pool pairs generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    duration = args.duration if args.duration is not None else 60.0
+    n_req = int(args.rate * duration)
+    gaps = rng.exponential(1.0 / args.rate, n_req)
+    picks = rng.integers(0, n_pool, n_req)
+    ang = np.radians(rng.uniform(-3, 3, (n_req, 3)))
+    scl = rng.uniform(0.98, 1.02, n_req)
+    shf = rng.uniform(-0.01, 0.01, (n_req, 3))
+
+    def moved(V, k):
+        a, b, c = ang[k]
+        Rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+        Ry = np.array([[np.cos(b), 0, np.sin(b)], [0, 1, 0], [-np.sin(b), 0, np.cos(b)]])
+        Rz = np.array([[np.cos(c), -np.sin(c), 0], [np.sin(c), np.cos(c), 0], [0, 0, 1]])
+        ctr = V.mean(0)
+        ext = V.max(0) - V.min(0)
+        return (V - ctr) @ (Rz @ Ry @ Rx).T * scl[k] + ctr + shf[k] * ext
+
+    srv = serve.DeformServer(mem_budget_bytes=args.budget_gb * 1e9)
+    # load the kernels (untimed): one tiny request on a target outside the pool
+    w = synth.box_union_pair(4, n_boxes=4, n_verts=1000)
+    srv.submit(serve.DeformRequest(w[0], w[1], "warm", w[2], w[3], grid=32)).result()
+    futs, first = [], set()
+    miss = np.zeros(n_req, bool)
+    t_start = time.perf_counter()
+    t_next = t_start
+    for k, (gap, i) in enumerate(zip(gaps, picks)):
         t_next += gap
         now = time.perf_counter()
         if t_next > now:
             time.sleep(t_next - now)
-        (p, g) = pool[k]
-        futs.append(srv.submit(serve.DeformRequest(p[0], p[1], int(k), p[2], p[3], grid=g)))
+        p = pool[i]
+        miss[k] = i not in first
+        first.add(i)
+        futs.append(srv.submit(serve.DeformRequest(moved(p[0], k), p[1], int(i), p[2], p[3], grid=grids[i])))
+        if (k + 1) % int(max(args.rate * 10, 1)) == 0:
+            print(f"[bench] serve: {k + 1}/{n_req} submitted, {srv.completed} done", file=sys.stderr, flush=True)
     res = [f.result() for f in futs]
-    t1 = time.perf_counter()
+    t_end = time.perf_counter()
     srv.close()
     lat = np.array([r.latency_s for r in res]) * 1e3
-    return {"metric": "serving: deformation pairs/sec with p50/p99 latency (Poisson arrivals)",
-            "value": len(res) / (t1 - t0), "unit": "pairs/s", "requests": len(res),
-            "offered_rate": args.rate, "duration_s": t1 - t0,
-            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-            "mean_batch": float(np.mean(srv.batches)) if srv.batches else 0.0,
-            "config": {"workload": "serve", "sources": "2k-%d verts log-uniform, box/sphere 50/50" % args.max_verts,
-                       "grids": [64, 128, 256], "distinct_targets": len(pool), "template_cache": "LRU, 200 GB"}}
+    nv = np.array([len(pool[i][0]) for i in picks])
+    classes = np.array([c for c, _ in srv.batches])
+    sizes_b = np.array([n for _, n in srv.batches])
+    pct = lambda x, q: float(np.percentile(x, q)) if len(x) else None   # noqa: E731
+    return {"metric": "serving: deformation pairs/sec with p50/p99 latency (config 5, Poisson arrivals, cold cache)",
+            "value": len(res) / (t_end - t_start), "unit": "pairs/s", "higher_is_better": True,
+            "requests": len(res), "offered_rate": args.rate, "arrival_s": duration,
+            "wall_s": t_end - t_start, "dtype": "f32 (LM), f64 (grid)", "data": "synthetic",
+            "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": float(lat.max()),
+                           "p50_small_pairs": pct(lat[nv <= 8192], 50), "p99_small_pairs": pct(lat[nv <= 8192], 99),
+                           "p50_large_pairs": pct(lat[nv > 8192], 50), "p99_large_pairs": pct(lat[nv > 8192], 99),
+                           "p50_cache_miss": pct(lat[miss], 50), "p99_cache_miss": pct(lat[miss], 99)},
+            "template_builds": srv.builds, "template_rebuilds_in_place": srv.rebuilds, "evictions": srv.evictions,
+            "cache_misses": int(miss.sum()),
+            "batches": {"small": int((classes == 0).sum()), "large": int((classes == 1).sum()),
+                        "mean_small": float(sizes_b[classes == 0].mean()) if (classes == 0).any() else 0.0,
+                        "mean_large": float(sizes_b[classes == 1].mean()) if (classes == 1).any() else 0.0},
+            "config": {"workload": "config 5 serving: config-1/config-2-like pairs 50/50, sources log-uniform "
+                                   "2k-%d verts, grids 64/128/256^3, fresh rigid motion per request" % args.max_verts,
+                       "pool_pairs": n_pool, "grids": {str(g): grids.count(g) for g in (64, 128, 256)},
+                       "budget_gb": args.budget_gb, "workers": "3 small + 1 large", "seed": 7}}
 
 
 def bench_adam(args):

@@ -164,7 +164,9 @@ typedef struct mo_lm_options {
   int32_t pcg_max_iterations;            /* inner PCG cap per LM step (1000) */
   float pcg_relative_tolerance;          /* ||r|| <= tol ||r0|| ends PCG (1e-5) */
   int32_t block_threads;                 /* threads per pair of the global-memory kernel (1024) */
-  int32_t timing;                        /* nonzero: record phase times (mo_last_timings) */
+  int32_t timing;                        /* nonzero: record phase times (mo_last_timings); 2: also
+                                            count the grid build's work (mo_last_grid_stats; costs a
+                                            few percent of the grid phase) */
   int32_t kernel_variant;                /* 0: register/LDS kernel when a pair fits, else the
                                             global-memory kernels (a team of workgroups per pair
                                             above team_vertices); 1: one-workgroup global-memory
@@ -297,6 +299,13 @@ int mo_deform_subdivision(int32_t param_id, const double* V_rest, double* V, int
  * wall clock rate (Hz).  No counterpart in the reference (its only timing
  * is Ceres' progress output, deformer.cc:58). */
 int mo_last_timings(double* out6);
+
+/* Work counters of the calling thread's last mo_rigid_deform_batch call made
+ * with opt->timing = 2 (no counterpart in the reference): out4 = point-
+ * triangle distance evaluations executed by the grid build (64 lanes per
+ * wavefront evaluation), the evaluations some lane needed, face bound checks
+ * executed (64 lanes each) and face tiles staged, summed over the batch. */
+int mo_last_grid_stats(int64_t* out4);
 
 /* ---------------------------------------------------------------------------
  * CAD chain (cad_deform.cc:21-120, LoadCadMesh mesh_tensor.cc:102-178).

@@ -32,6 +32,7 @@ SYMBOLS = [
     "mo_flow256_step_backward", "mo_flow256_small_grad_floats", "mo_flow256_small_grads",
     "mo_flow256_dw_work_floats", "mo_flow256_dw_grads",
     "mo_deform_batch_noalloc_max_bytes", "mo_deform_batch_noalloc", "mo_template_rebuild_f64",
+    "mo_last_grid_stats",
 ]
 
 
@@ -125,6 +126,7 @@ _SIGS = {
     "mo_rigid_deform_batch": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                         C.POINTER(LMOptions), _vp, _vp, _vp]),
     "mo_last_timings": (C.c_int, [_vp]),
+    "mo_last_grid_stats": (C.c_int, [_vp]),
     "mo_lm_shape_count": (_i32, []),
     "mo_lm_shape_info": (C.c_int, [_i32, C.POINTER(_i32), C.POINTER(_i32)]),
     "mo_lm_set_shape_table": (C.c_int, [_vp, _vp, _i32]),
@@ -198,6 +200,17 @@ def last_timings():
     call("mo_last_timings", C.c_void_p(out.ctypes.data))
     return dict(grid_ms=out[0], setup_ms=out[1], lm_ms=out[2], total_ms=out[3], clock_start=out[4],
                 clock_hz=out[5])
+
+
+def last_grid_stats():
+    """Grid-build work counters of this thread's last rigid_deform batch call
+    made with options.timing=2: dict(exact_tests, exact_tests_needed,
+    bound_checks, tiles) (lane-level counts, summed over the batch)."""
+    import numpy as np
+    out = np.zeros(4, np.int64)
+    call("mo_last_grid_stats", C.c_void_p(out.ctypes.data))
+    return dict(exact_tests=int(out[0]), exact_tests_needed=int(out[1]), bound_checks=int(out[2]),
+                tiles=int(out[3]))
 
 
 def default_lm_options() -> LMOptions:

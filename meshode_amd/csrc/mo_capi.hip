@@ -26,6 +26,7 @@ namespace mo {
 
 static thread_local std::string g_last_error;
 static thread_local double g_timings[6] = {0, 0, 0, 0, 0, 0};
+static thread_local long long g_grid_stats[kGridStatCount] = {0, 0, 0, 0};
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
@@ -878,6 +879,12 @@ MO_EXPORT int mo_lm_set_shape_table(const int32_t* max_vertices, const int32_t* 
   return lm_lds_set_table(max_vertices, shape, n);
 }
 
+MO_EXPORT int mo_last_grid_stats(int64_t* out4) {
+  MO_CHECK(out4, MO_ERR_ARG, "null output");
+  for (int k = 0; k < kGridStatCount; ++k) out4[k] = g_grid_stats[k];
+  return MO_OK;
+}
+
 MO_EXPORT int mo_last_timings(double* out6) {
   MO_CHECK(out6, MO_ERR_ARG, "null output");
   for (int k = 0; k < 6; ++k) out6[k] = g_timings[k];
@@ -982,13 +989,23 @@ MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, co
                          lookup_band2((int)N)};
   }
   const size_t gws = grid_batch_workspace_bytes(gspecs.data(), npairs);
-  bytes += al(gws) + 2 * al(xform_workspace_bytes(npairs));
+  // grid work counters: one 64-byte line per pair (atomics from that
+  // pair's wavefronts only), summed on the host
+  constexpr int kStatStride = 8;
+  bytes += al(gws) + 2 * al(xform_workspace_bytes(npairs)) +
+           al((size_t)npairs * kStatStride * sizeof(unsigned long long));
   Arena ar;
   int rc;
   PhaseTimer timer;
   if ((rc = timer.start(o.timing != 0, s))) return rc;
   if ((rc = ar.reserve(bytes, s))) return rc;
   int* err = ar.take<int>(1);
+  // with timing: the grid build's work counters (summed over the batch)
+  unsigned long long* gstats = ar.take<unsigned long long>((size_t)npairs * kStatStride);
+  if (o.timing == 2) {
+    if ((rc = zero_fill(gstats, (size_t)npairs * kStatStride * sizeof(unsigned long long), s))) return rc;
+    for (int p = 0; p < npairs; ++p) gspecs[p].stats = gstats + (size_t)p * kStatStride;
+  }
   void* gws_ptr = ar.take<char>(gws);
   void* xws_t = ar.take<char>(xform_workspace_bytes(npairs));
   void* xws_s = ar.take<char>(xform_workspace_bytes(npairs));
@@ -1013,6 +1030,15 @@ MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, co
   if ((rc = timer.mark(1))) return rc;
   if ((rc = lm_core(pairs, o, stats, s, timer))) return rc;
   if ((rc = timer.finish())) return rc;
+  if (o.timing == 2) {
+    std::vector<unsigned long long> h((size_t)npairs * kStatStride);
+    MO_HIP(hipMemcpyAsync(h.data(), gstats, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    MO_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < kGridStatCount; ++k) {
+      g_grid_stats[k] = 0;
+      for (int p = 0; p < npairs; ++p) g_grid_stats[k] += (long long)h[(size_t)p * kStatStride + k];
+    }
+  }
   return check_err_flag(err, s, "target faces");
 }
 

@@ -37,7 +37,15 @@ struct GridSpec {
   float* dist32;
   int* nearest;
   double band2 = INFINITY;
+  unsigned long long* stats = nullptr;   // optional work counters (kGridStat*), added to
 };
+
+// Work counters of a grid build (per launch, summed over its grids), added
+// with one atomic per wavefront when GridSpec::stats is set: point-triangle
+// distance evaluations executed (64 lanes per wavefront evaluation) and the
+// ones some lane needed, face bound checks executed (64 lanes each), and
+// tiles of faces staged into LDS.
+enum GridStat { kGridStatExact = 0, kGridStatExactNeeded, kGridStatBound, kGridStatTiles, kGridStatCount };
 
 // The band a template grid needs: the lookups (uniformgrid.cc:18-150)
 // interpolate the 8 corners of a cell and return 0 when the result exceeds
@@ -66,6 +74,7 @@ struct GridJob {
   float* dist32;
   int* nearest;
   double band2;
+  unsigned long long* stats;
 };
 
 size_t grid_batch_workspace_bytes(const GridSpec* specs, int n);

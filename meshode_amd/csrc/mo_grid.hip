@@ -291,6 +291,9 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
   // points farther than the band only need to be shown farther (GridSpec)
   double best = JJ.band2;
   int bidx = INT_MAX;
+  // work counters (wave-uniform): exact evaluations, lanes that needed
+  // one, face bound checks, staged tiles
+  unsigned long long n_exact = 0, n_needed = 0, n_bound = 0, n_tiles = 0;
 
   auto process_tile = [&](int t) {
     const long long f0 = (long long)t * kTile;
@@ -309,6 +312,8 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
     const unsigned long long m = __ballot(cand);
     const int ns = __popcll(m);
     if (ns == 0) return;
+    ++n_tiles;
+    n_bound += (unsigned long long)ns;
     // compact the survivors (in face order) and stage their records in LDS
     const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
     if (cand) s_list[rank] = lane;
@@ -333,7 +338,10 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
       // the face plane is a second lower bound (0 for degenerate faces)
       const double dp = g.nx * (qx - g.ax) + g.ny * (qy - g.ay) + g.nz * (qz - g.az);
       lb = dmax(lb, (dp * dp) * g.inv_nn);
-      if (__ballot(valid && !(lb > best * (1.0 + 1e-9)))) {
+      const unsigned long long need = __ballot(valid && !(lb > best * (1.0 + 1e-9)));
+      if (need) {
+        ++n_exact;
+        n_needed += (unsigned long long)__popcll(need);
         double d = face_d2(g, qx, qy, qz);
         if (d < best || (d == best && g.face < bidx)) { best = d; bidx = g.face; }
       }
@@ -390,6 +398,12 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
       if (!__ballot(valid && !(pl > best * (1.0 + 1e-9)))) continue;
       process_tile(tt);
     }
+  }
+  if (JJ.stats && lane == 0) {
+    atomicAdd(JJ.stats + kGridStatExact, n_exact * 64ull);
+    atomicAdd(JJ.stats + kGridStatExactNeeded, n_needed);
+    atomicAdd(JJ.stats + kGridStatBound, n_bound * 64ull);
+    atomicAdd(JJ.stats + kGridStatTiles, n_tiles);
   }
   if (valid) {
     long long cell = ((long long)i * N + j) * N + k;
@@ -485,6 +499,7 @@ int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipSt
     J.dist64 = sp.dist64; J.dist32 = sp.dist32; J.nearest = sp.nearest;
     MO_CHECK(sp.band2 > 0.0, MO_ERR_ARG, "grid band must be positive");
     J.band2 = sp.band2;
+    J.stats = sp.stats;
     const long long bpa = (sp.N + 3) / 4;
     max_nf = std::max(max_nf, sp.nf);
     max_tiles = std::max(max_tiles, J.ntiles);
