@@ -204,6 +204,7 @@ typedef struct mo_lm_stats {
 #define MO_PATH_ONE_WORKGROUP 1  /* global-memory kernel, one workgroup */
 #define MO_PATH_TEAM 2           /* multi-workgroup (cooperative) kernel */
 #define MO_PATH_COOP_REFUSED 3   /* one workgroup after a refused cooperative launch */
+#define MO_PATH_LDS_TEAM 4       /* register/LDS kernel on a team of workgroups (cooperative) */
 
 /* Fills the defaults listed above. */
 void mo_lm_default_options(mo_lm_options* opt);

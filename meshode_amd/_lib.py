@@ -91,7 +91,7 @@ class LMStats(C.Structure):
 
 
 # mo_lm_stats.solver_path (MO_PATH_* in include/meshode.h)
-SOLVER_PATHS = {0: "lds", 1: "one_workgroup", 2: "team", 3: "one_workgroup_coop_refused"}
+SOLVER_PATHS = {0: "lds", 1: "one_workgroup", 2: "team", 3: "one_workgroup_coop_refused", 4: "lds_team"}
 
 
 _lib = None

@@ -18,6 +18,7 @@ inline const char* solver_path_name(int p) {
     case 1: return "one-workgroup";
     case 2: return "team";
     case 3: return "one-workgroup (cooperative launch refused)";
+    case 4: return "lds-team";
     default: return "unknown";
   }
 }

@@ -471,7 +471,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
                  al(std::max<long long>(2 * nedges_tot, 1) * sizeof(unsigned)) +
                  al(std::max<long long>(2 * nedges_tot, 1) * sizeof(float)) + al(topo_workspace_bytes(nvalloc)) +
                  al(np * sizeof(LMPairDesc)) + al(np * sizeof(TopoPairDesc)) + al(np * sizeof(LMResultDev)) +
-                 al(2 * np * sizeof(int)) + al(np * sizeof(int)) + 512;
+                 al(2 * np * sizeof(int)) + al(np * sizeof(int)) + al(3 * np * sizeof(int)) + 512;
   Arena ar;
   int rc;
   if ((rc = ext ? ar.attach(ext, ext_cap, bytes, s) : ar.reserve(bytes, s))) return rc;
@@ -488,6 +488,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   LMResultDev* d_res = ar.take<LMResultDev>(np);
   int* d_pstat = ar.take<int>(2 * np);
   int* d_plist = ar.take<int>(np);
+  int* d_chunk = ar.take<int>(3 * np);
   int* err = ar.take<int>(1);
 
   std::vector<LMPairDesc> h_pairs(np);
@@ -513,8 +514,11 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
                        u + vbase[p], c.xstart);
   }
   MO_HIP(hipGetLastError());
-  std::vector<int> pstat(2 * np);
+  // per pair and register/LDS team shape: the largest chunk's adjacency entries
+  if ((rc = lm_lds_chunk_entries(d_pairs, adj, np, d_chunk, s))) return rc;
+  std::vector<int> pstat(2 * np), hchunk(3 * np);
   int herr = 0;
+  MO_HIP(hipMemcpyAsync(hchunk.data(), d_chunk, 3 * np * sizeof(int), hipMemcpyDeviceToHost, s));
   MO_HIP(hipMemcpyAsync(pstat.data(), d_pstat, 2 * np * sizeof(int), hipMemcpyDeviceToHost, s));
   MO_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
   MO_HIP(hipStreamSynchronize(s));
@@ -592,9 +596,79 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   {
     const int bt = opt.block_threads;
     MO_CHECK(opt.team_vertices >= 0, MO_ERR_ARG, "team_vertices must be >= 0");
+    // Pairs above one register/LDS workgroup go to the register/LDS team
+    // kernel when their multiplicities allow it (kind 0/1, <= 2), with the
+    // first team shape (largest chunk) whose chunk fits the LDS, a team of
+    // at most kMaxTeam workgroups, and the whole team co-resident.  The
+    // rest take the global-memory kernels below.
+    std::vector<int> lds_cand;
+    std::vector<int> rest;
+    // a pair that no team shape can hold alone (its team would exceed
+    // kMaxTeam workgroups, a chunk's entries the 16-bit LDS offsets, or its
+    // LDS the budget) takes the global-memory kernels
+    auto lds_team_fits = [&](int p, int b) {
+      int T = 0, V = 0;
+      if (lm_lds_team_shape(b, &T, &V)) return false;
+      const long long chcap = (long long)T * V, nvp = pairs[p].nv;
+      const long long c = (nvp + chcap - 1) / chcap, ch = (nvp + c - 1) / c;
+      const long long ent = hchunk[3 * p + b];
+      return c <= lm_lds_team_max() && ent < 65536 && lm_lds_team_bytes((int)ch, ent) <= lds_cap;
+    };
+    for (int p : bucket[nb]) {
+      const CorePair& c = pairs[p];
+      bool ok = opt.kernel_variant == 0 && c.kind != 2 && c.nv > 0 && pstat[2 * p + 1] <= 2;
+      if (ok) {
+        ok = false;
+        for (int b = 0; b < lm_lds_team_shapes() && !ok; ++b) ok = lds_team_fits(p, b);
+      }
+      (ok ? lds_cand : rest).push_back(p);
+    }
+    // each pair takes the first shape (largest chunk) that holds it, so its
+    // chunking -- and its result -- does not depend on the rest of the
+    // batch; the teams are grouped by shape, one cooperative grid per
+    // group, its LDS size the group's largest
+    const int nls = lm_lds_team_shapes();
+    std::vector<int> pshape(lds_cand.size(), -1);
+    std::vector<size_t> lds_of(nls, 0);
+    for (size_t k = 0; k < lds_cand.size(); ++k) {
+      const int p = lds_cand[k];
+      for (int b = 0; b < nls && pshape[k] < 0; ++b)
+        if (lds_team_fits(p, b)) {
+          int T = 0, V = 0;
+          if ((rc = lm_lds_team_shape(b, &T, &V))) return rc;
+          const long long chcap = (long long)T * V, nvp = pairs[p].nv;
+          const long long c = (nvp + chcap - 1) / chcap, ch = (nvp + c - 1) / c;
+          pshape[k] = b;
+          lds_of[b] = std::max(lds_of[b], lm_lds_team_bytes((int)ch, hchunk[3 * p + b]));
+        }
+    }
+    std::vector<int> cap_of(nls, 0);
+    for (int b = 0; b < nls; ++b)
+      if (lds_of[b] > 0) cap_of[b] = lm_lds_team_capacity(b, lds_of[b]);
+    std::vector<int> lteams, lteam_c, lteam_ch, lteam_b;
+    for (int b = 0; b < nls; ++b) {
+      int T = 0, V = 0;
+      if ((rc = lm_lds_team_shape(b, &T, &V))) return rc;
+      const long long chcap = (long long)T * V;
+      for (size_t k = 0; k < lds_cand.size(); ++k) {
+        if (pshape[k] != b) continue;
+        const int p = lds_cand[k];
+        const long long nvp = pairs[p].nv;
+        const long long c = (nvp + chcap - 1) / chcap;
+        if (c > cap_of[b]) {   // the team cannot be co-resident
+          rest.push_back(p);
+          continue;
+        }
+        lteams.push_back(p);
+        lteam_c.push_back((int)c);
+        lteam_ch.push_back((int)((nvp + c - 1) / c));
+        lteam_b.push_back(b);
+        path[p] = MO_PATH_LDS_TEAM;
+      }
+    }
     const long long vpw = opt.team_vertices > 0 ? opt.team_vertices : (opt.kernel_variant == 2 ? bt : 4LL * bt);
     const int cap = opt.kernel_variant == 1 ? 0 : lm_team_capacity(bt);
-    for (int p : bucket[nb]) {
+    for (int p : rest) {
       const long long c = std::min<long long>(std::max<long long>((pairs[p].nv + vpw - 1) / vpw, 1), lm_team_max());
       if (cap >= lm_team_max() && (c > 1 || opt.kernel_variant == 2)) {
         teams.push_back(p);
@@ -609,10 +683,14 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
       if ((rc = lm_solve_launch(d_pairs, d_plist + start[nb], (int)solo.size(), A, dopt, d_res, nvalloc, bt, s)))
         return rc;
     }
-    if (!teams.empty()) {
-      const int nt = (int)teams.size();
-      const int per = 2 * lm_team_max() * 8;
+    // team arrays: the register/LDS teams first, then the global-memory teams
+    const int nlt = (int)lteams.size(), nt = nlt + (int)teams.size();
+    if (nt > 0) {
+      std::vector<int> all_t(lteams);
+      all_t.insert(all_t.end(), teams.begin(), teams.end());
+      const int per = lm_team_part_doubles();
       long long nwg_all = 0;
+      for (int c : lteam_c) nwg_all += c;
       for (int c : team_c) nwg_all += c;
       const size_t team_bytes = al(nt * sizeof(int)) + al(nwg_all * sizeof(int4)) +
                                 al((size_t)nt * 32 * sizeof(unsigned) + sizeof(int)) +
@@ -626,36 +704,46 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
       double* d_part = tar.take<double>((size_t)nt * per);
       d_abort = reinterpret_cast<int*>(d_cnt + (size_t)nt * 32);
       if ((rc = zero_fill(d_cnt, ((size_t)nt * 32 + 1) * sizeof(unsigned), s))) return rc;
-      // chunks: consecutive teams (largest first) whose workgroups fit one grid
+      // chunks: consecutive teams (largest first) whose workgroups fit one
+      // grid; a chunk never mixes the two kernels
       std::vector<int> chunk_team{0}, chunk_wg{0};
       int used = 0;
       for (int t = 0; t < nt; ++t) {
-        if (used + team_c[t] > cap) {
+        const bool lt = t < nlt;
+        const int c = lt ? lteam_c[t] : team_c[t - nlt];
+        if (t > 0 && (used + c > (lt ? cap_of[lteam_b[t]] : cap) || t == nlt || (lt && lteam_b[t] != lteam_b[t - 1]))) {
           chunk_team.push_back(t);
           chunk_wg.push_back((int)wg.size());
           used = 0;
         }
         const int rel = t - chunk_team.back();
-        for (int r = 0; r < team_c[t]; ++r) wg.push_back(make_int4(rel, r, team_c[t], 0));
-        used += team_c[t];
+        for (int r = 0; r < c; ++r) wg.push_back(make_int4(rel, r, c, lt ? lteam_ch[t] : 0));
+        used += c;
       }
       chunk_team.push_back(nt);
       chunk_wg.push_back((int)wg.size());
-      MO_HIP(hipMemcpyAsync(d_tp, teams.data(), nt * sizeof(int), hipMemcpyHostToDevice, s));
+      MO_HIP(hipMemcpyAsync(d_tp, all_t.data(), nt * sizeof(int), hipMemcpyHostToDevice, s));
       MO_HIP(hipMemcpyAsync(d_wg, wg.data(), wg.size() * sizeof(int4), hipMemcpyHostToDevice, s));
       team_lock.lock();
       for (size_t k = 0; k + 1 < chunk_team.size(); ++k) {
-        const int t0 = chunk_team[k];
-        rc = lm_team_launch(d_pairs, d_tp + t0, d_wg + chunk_wg[k], chunk_wg[k + 1] - chunk_wg[k], A, dopt, d_res,
-                            nvalloc, bt, d_cnt + (size_t)t0 * 32, d_part + (size_t)t0 * per, d_abort, s);
+        const int t0 = chunk_team[k], t1 = chunk_team[k + 1];
+        const bool lt = t0 < nlt;
+        if (lt)
+          rc = lm_lds_team_launch(lteam_b[t0], d_pairs, d_tp + t0, d_wg + chunk_wg[k], chunk_wg[k + 1] - chunk_wg[k],
+                                  A, dopt, d_res, nvalloc, lds_of[lteam_b[t0]], d_cnt + (size_t)t0 * 32,
+                                  d_part + (size_t)t0 * per, d_abort, s);
+        else
+          rc = lm_team_launch(d_pairs, d_tp + t0, d_wg + chunk_wg[k], chunk_wg[k + 1] - chunk_wg[k], A, dopt, d_res,
+                              nvalloc, bt, d_cnt + (size_t)t0 * 32, d_part + (size_t)t0 * per, d_abort, s);
         if (rc == kCoopRefused) {
           // the runtime refused the cooperative grid: this chunk's and the
           // remaining teams' pairs run one workgroup each (d_tp lists them)
           if ((rc = lm_solve_launch(d_pairs, d_tp + t0, nt - t0, A, dopt, d_res, nvalloc, bt, s))) return rc;
-          for (int t = t0; t < nt; ++t) path[teams[t]] = MO_PATH_COOP_REFUSED;
+          for (int t = t0; t < nt; ++t) path[all_t[t]] = MO_PATH_COOP_REFUSED;
           break;
         }
         if (rc) return rc;
+        (void)t1;
       }
     }
   }
@@ -1295,7 +1383,7 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
     }
     chunk_team.push_back(npairs);
     chunk_wg.push_back((int)wg.size());
-    const int per = 2 * lm_team_max() * 8;
+    const int per = lm_team_part_doubles();
     const size_t b_tp = al(npairs * sizeof(int)), b_wg = al(nwg_all * sizeof(int4)),
                  b_cnt = al(((size_t)npairs * 32 + 1) * sizeof(unsigned)), b_part = al((size_t)npairs * per * 8);
     if (int e_ = d_team.alloc(b_tp + b_wg + b_cnt + b_part)) return e_;

@@ -81,6 +81,7 @@ int lm_solve_launch(const LMPairDesc* pairs_dev, const int* plist, int npairs, c
 // caller runs the one-workgroup kernel instead.
 constexpr int kCoopRefused = 1;
 int lm_team_max();
+int lm_team_part_doubles();   // per-team FP64 partial storage (all team kernels)
 int lm_team_capacity(int block_threads);
 // Held from a multi-workgroup (cooperative) launch until its stream is
 // synchronized: two cooperative grids on different streams must not wait on
@@ -101,5 +102,25 @@ size_t lm_lds_bytes(int nv, long long entries);      // dynamic LDS for one pair
 int lm_lds_launch(int bucket, const LMPairDesc* pairs_dev, const int* plist, int count, const LMArrays& arrays,
                   const LMOptionsDev& opt, LMResultDev* results_dev, long long total_vertices, size_t lds_bytes,
                   hipStream_t s);
+
+// Multi-workgroup register/LDS kernel (k_lm_solve_lds_team) for pairs
+// above one workgroup's capacity with multiplicities <= 2: workgroup r of a
+// team owns a contiguous chunk of the pair's vertices.  Team shape b (0..2)
+// holds 4096, 3072 or 2048 vertices per workgroup; lm_lds_chunk_entries
+// writes, per pair, the largest chunk's adjacency entry count for each
+// shape (3 ints per pair); lm_lds_team_bytes gives the dynamic LDS of a
+// chunk; wg_dev entries are (team, rank, team size, chunk); cnt / part /
+// abort_flag as for lm_team_launch.  Returns kCoopRefused like
+// lm_team_launch.
+int lm_lds_team_shapes();
+int lm_lds_team_max();   // largest team of the register/LDS team kernel
+int lm_lds_team_shape(int b, int* threads, int* vertices_per_thread);
+size_t lm_lds_team_bytes(int chunk, long long entries);
+int lm_lds_team_capacity(int b, size_t lds_bytes);
+int lm_lds_chunk_entries(const LMPairDesc* pairs_dev, const int2* adj, int np, int* out_dev, hipStream_t s);
+int lm_lds_team_launch(int b, const LMPairDesc* pairs_dev, const int* plist, const int4* wg_dev, int nwg,
+                       const LMArrays& arrays, const LMOptionsDev& opt, LMResultDev* results_dev,
+                       long long total_vertices, size_t lds_bytes, unsigned* cnt, double* part, int* abort_flag,
+                       hipStream_t s);
 
 }  // namespace mo

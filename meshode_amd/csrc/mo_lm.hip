@@ -340,7 +340,10 @@ int lm_solve_launch(const LMPairDesc* pairs_dev, const int* plist, int npairs, c
   return MO_OK;
 }
 
-int lm_team_max() { return kMaxTeam; }
+// Team size cap of the global-memory team kernels (the register/LDS team
+// kernel's teams may reach kMaxTeam, the partials' stride).
+int lm_team_max() { return 32; }
+int lm_team_part_doubles() { return 2 * kMaxTeam * 8; }
 
 std::mutex& team_launch_mutex() {
   static std::mutex m;

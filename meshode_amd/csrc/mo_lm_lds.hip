@@ -38,7 +38,7 @@ namespace mo {
 
 namespace {
 
-enum GSlot { gGR = 1, gS = 2, gDG = 3 };   // global slots of LMArrays::work used here
+enum GSlot { gGR = 1, gS = 2, gDG = 3, gZ = 4 };   // global slots of LMArrays::work used here
 
 // Block-wide exclusive scan of one int per thread (T <= 1024).
 __device__ inline int block_exclusive_scan(int x, int* buf, int& total) {
@@ -69,20 +69,24 @@ __device__ inline int opaque(int x) {
   return x;
 }
 
-template <int T, int VPT>
-__global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict__ pairs,
-                                                    const int* __restrict__ plist, LMArrays A, LMOptionsDev opt,
-                                                    LMResultDev* __restrict__ res, long long totalV) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[2 * 8 * kMaxWaves];
-  __shared__ int scan_buf[kMaxWaves];
+// The solver body, shared by the one-workgroup kernel (Team = LdsSolo: the
+// workgroup owns all nv vertices, neighbours come from its LDS array) and
+// the multi-workgroup kernel (Team = LdsTeam: workgroup r owns the
+// contiguous chunk [lo, hi) of the pair's vertices; a neighbour in another
+// chunk is read from the pair's global publication array, which every
+// owner writes next to its LDS copy before the team barrier).  Entry: an
+// adjacency entry type (uint16 local indices for the solo kernel, uint32
+// pair indices for the team kernel).
+template <int T, int VPT, class Team, class Entry>
+__device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const LMArrays& A,
+                                 const LMOptionsDev& opt, LMResultDev* __restrict__ res, long long totalV,
+                                 unsigned char* smem, double* red, int* scan_buf) {
   const double o_min_diag = opt.min_lm_diagonal, o_max_diag = opt.max_lm_diagonal;
   const float o_pcg_tol = opt.pcg_relative_tolerance;
   const int o_pcg_max = opt.pcg_max_iterations;
-  const int pid = plist[blockIdx.x];
-  const LMPairDesc P = pairs[pid];
   const int tid = threadIdx.x;
-  const int nv = P.nv;
+  const int lo = team.lo;                  // owned vertices: lo + t + k T < hi
+  const int own = team.hi - team.lo;
   const long long vb = P.vbase;
   const float l2 = P.lambda * P.lambda;
   const float* __restrict__ grid = P.grid;
@@ -94,8 +98,9 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
   float4* __restrict__ GR = A.work + (size_t)gGR * totalV + vb;
   float4* __restrict__ S = A.work + (size_t)gS * totalV + vb;
   float4* __restrict__ DG = A.work + (size_t)gDG * totalV + vb;
-  float4* VEC = (float4*)smem;
-  unsigned short* LCSR = (unsigned short*)(smem + (size_t)nv * sizeof(float4));
+  float4* __restrict__ ZG = A.work + (size_t)gZ * totalV + vb;   // team: published vectors
+  float4* VEC = (float4*)smem;                                      // own vertices, local index
+  Entry* LCSR = (Entry*)(smem + (size_t)own * sizeof(float4));
   int sel = 0;
   const long long clock_start = wall_clock64();
 
@@ -105,10 +110,10 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
     int cnt[VPT], gst[VPT], mine = 0;
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = opaque(tid + k * T);
+      const int il = opaque(tid + k * T), i = lo + il;
       cnt[k] = 0;
       gst[k] = 0;
-      if (i < nv) {
+      if (il < own) {
         const int2 a = ADJ[i];
         gst[k] = a.x;
         cnt[k] = a.y;
@@ -122,7 +127,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       csr[k] = ((unsigned)off << 16) | (unsigned)cnt[k];
       for (int e = 0; e < cnt[k]; ++e) {
         const unsigned en = NB[gst[k] + e];
-        LCSR[off + e] = (unsigned short)((nb_index(en) << 1) | ((en >> 24) - 1u));
+        LCSR[off + e] = (Entry)((nb_index(en) << 1) | ((en >> 24) - 1u));
       }
       off += cnt[k];
     }
@@ -160,7 +165,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
     for (int q = 0; q < c; ++q) {
       const unsigned w = LCSR[o + q];
       const float m = (float)((w & 1u) + 1u);
-      const float4 vj = VEC[w >> 1];
+      const float4 vj = team.neighbour(VEC, ZG, w >> 1);
       sx += m * vj.x; sy += m * vj.y; sz += m * vj.z;
     }
   };
@@ -168,17 +173,17 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
   auto eval = [&](bool first, double& cv, double& ce, double& gmax) {
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = opaque(tid + k * T);
-      if (i < nv) VEC[i] = U[i];
+      const int il = opaque(tid + k * T), i = lo + il;
+      if (il < own) team.publish(VEC, ZG, il, i, U[i]);
     }
-    __syncthreads();
+    team.sync();
     double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = opaque(tid + k * T);
-      if (i >= nv) continue;
+      const int il = opaque(tid + k * T), i = lo + il;
+      if (il >= own) continue;
       const float4 x0 = X0[i];
-      const float4 ui = VEC[i];
+      const float4 ui = VEC[il];
       const JetF J = lookup_value_grad(grid, N, x0.x + ui.x, x0.y + ui.y, x0.z + ui.z);
       gx[k] = J.v0; gy[k] = J.v1; gz[k] = J.v2;
       const float ldk = l2 * x0.w;
@@ -190,7 +195,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       for (int q = 0; q < c; ++q) {
         const unsigned w = LCSR[o + q];
         const float m = (float)((w & 1u) + 1u);
-        const float4 uj = VEC[w >> 1];
+        const float4 uj = team.neighbour(VEC, ZG, w >> 1);
         const float ax = ui.x - uj.x, ay = ui.y - uj.y, az = ui.z - uj.z;
         sx += m * ax; sy += m * ay; sz += m * az;
         e += (double)m * ((double)ax * ax + (double)ay * ay + (double)az * az);
@@ -208,7 +213,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
         S[i] = make_float4(1.0f / (1.0f + sqrtf(c0)), 1.0f / (1.0f + sqrtf(c1)), 1.0f / (1.0f + sqrtf(c2)), 0.f);
       }
     }
-    block_reduce_dpp<2, 1>(v, red, sel);
+    team.template reduce<2, 1>(v, red, sel);
     cv = v[0]; ce = v[1]; gmax = v[2];
   };
 
@@ -221,13 +226,13 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
     double v[2] = {0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = opaque(tid + k * T);
+      const int il = opaque(tid + k * T), i = lo + il;
       px[k] = py[k] = pz[k] = 0.f;
       qx[k] = qy[k] = qz[k] = 0.f;
       ex[k] = ey[k] = ez[k] = 0.f;
       rx[k] = ry[k] = rz_[k] = 0.f;
       dx[k] = dy[k] = dz[k] = 1.f;
-      if (i >= nv) continue;
+      if (il >= own) continue;
       const float4 s = S[i];
       const float ldk = l2 * X0[i].w;
       float4 dg;
@@ -256,11 +261,11 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       rx[k] = -gr.x; ry[k] = -gr.y; rz_[k] = -gr.z;
       float z0, z1, z2;
       precond(k, rx[k], ry[k], rz_[k], z0, z1, z2);
-      VEC[i] = make_float4(z0, z1, z2, 0.f);
+      team.publish(VEC, ZG, il, i, make_float4(z0, z1, z2, 0.f));
       v[0] += (double)(rx[k] * z0 + ry[k] * z1 + rz_[k] * z2);
       v[1] += (double)(rx[k] * rx[k] + ry[k] * ry[k] + rz_[k] * rz_[k]);
     }
-    block_reduce_dpp<2, 0>(v, red, sel);   // also publishes z
+    team.template reduce<2, 0>(v, red, sel);   // also publishes z
     double rz = v[0];
     const double rr0 = v[1];
     if (!(rr0 > 0.0)) return 0;
@@ -271,9 +276,9 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       double w1[1] = {0.0};
 #pragma unroll
       for (int k = 0; k < VPT; ++k) {
-        const int i = opaque(tid + k * T);
-        if (i >= nv) continue;
-        const float4 zi = VEC[i];
+        const int il = opaque(tid + k * T), i = lo + il;
+        if (il >= own) continue;
+        const float4 zi = VEC[il];
         float sx, sy, sz;
         gather(k, sx, sy, sz);
         const float gzz = gx[k] * zi.x + gy[k] * zi.y + gz[k] * zi.z;
@@ -284,24 +289,24 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
         qx[k] = wx + beta * qx[k]; qy[k] = wy + beta * qy[k]; qz[k] = wz + beta * qz[k];
         w1[0] += (double)(px[k] * qx[k] + py[k] * qy[k] + pz[k] * qz[k]);
       }
-      block_reduce_dpp<1, 0>(w1, red, sel);
+      team.template reduce<1, 0>(w1, red, sel);
       const double pq = w1[0];
       if (!(pq > 0.0)) break;
       const float fa = (float)(rz / pq);
       double w2[2] = {0.0, 0.0};
 #pragma unroll
       for (int k = 0; k < VPT; ++k) {
-        const int i = opaque(tid + k * T);
-        if (i >= nv) continue;
+        const int il = opaque(tid + k * T), i = lo + il;
+        if (il >= own) continue;
         ex[k] += fa * px[k]; ey[k] += fa * py[k]; ez[k] += fa * pz[k];
         rx[k] -= fa * qx[k]; ry[k] -= fa * qy[k]; rz_[k] -= fa * qz[k];
         float z0, z1, z2;
         precond(k, rx[k], ry[k], rz_[k], z0, z1, z2);
-        VEC[i] = make_float4(z0, z1, z2, 0.f);
+        team.publish(VEC, ZG, il, i, make_float4(z0, z1, z2, 0.f));
         w2[0] += (double)(rx[k] * z0 + ry[k] * z1 + rz_[k] * z2);
         w2[1] += (double)(rx[k] * rx[k] + ry[k] * ry[k] + rz_[k] * rz_[k]);
       }
-      block_reduce_dpp<2, 0>(w2, red, sel);   // also publishes z
+      team.template reduce<2, 0>(w2, red, sel);   // also publishes z
       ++it;
       if (w2[1] <= tol2) break;
       beta = (float)(w2[0] / rz);
@@ -314,23 +319,23 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
   auto candidate = [&]() -> LMCandidate {
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = opaque(tid + k * T);
-      if (i < nv) {
+      const int il = opaque(tid + k * T), i = lo + il;
+      if (il < own) {
         const float4 ui = U[i];
-        VEC[i] = make_float4(ui.x + ex[k], ui.y + ey[k], ui.z + ez[k], 0.f);
+        team.publish(VEC, ZG, il, i, make_float4(ui.x + ex[k], ui.y + ey[k], ui.z + ez[k], 0.f));
       }
     }
-    __syncthreads();
+    team.sync();
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = opaque(tid + k * T);
-      if (i >= nv) continue;
+      const int il = opaque(tid + k * T), i = lo + il;
+      if (il >= own) continue;
       const float4 x0 = X0[i];
       const float dres = GR[i].w;
       const float gd = gx[k] * ex[k] + gy[k] * ey[k] + gz[k] * ez[k];
       v[0] += (double)gd * ((double)dres + 0.5 * (double)gd);
-      const float4 wv = VEC[i];
+      const float4 wv = VEC[il];
       const float wx = wv.x, wy = wv.y, wz = wv.z;
       const float dc = lookup_f32(grid, N, x0.x + wx, x0.y + wy, x0.z + wz);
       v[1] += 0.5 * (double)dc * (double)dc;
@@ -341,7 +346,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       for (int q = 0; q < c; ++q) {
         const unsigned w = LCSR[o + q];
         const float m = (float)((w & 1u) + 1u);
-        const float4 wj = VEC[w >> 1];
+        const float4 wj = team.neighbour(VEC, ZG, w >> 1);
         const float ax = wx - wj.x, ay = wy - wj.y, az = wz - wj.z;
         e += (double)m * ((double)ax * ax + (double)ay * ay + (double)az * az);
       }
@@ -351,7 +356,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
       const float qx0 = x0.x + ui.x, qy0 = x0.y + ui.y, qz0 = x0.z + ui.z;
       v[4] += (double)qx0 * qx0 + (double)qy0 * qy0 + (double)qz0 * qz0;
     }
-    block_reduce_dpp<5, 0>(v, red, sel);
+    team.template reduce<5, 0>(v, red, sel);
     // rigidity residuals are linear: their model change equals the actual one
     return LMCandidate{-v[0] - (v[2] - ce_now), v[1] + v[2], sqrt(v[3]), sqrt(v[4])};
   };
@@ -364,11 +369,139 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict
     // u <- u + delta: exactly the candidate published to LDS
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      const int i = opaque(tid + k * T);
-      if (i < nv) U[i] = VEC[i];
+      const int il = opaque(tid + k * T), i = lo + il;
+      if (il < own) U[i] = VEC[il];
     }
   };
-  lm_trust_region_loop(opt, eval_tracked, solve, candidate, accept, res + pid, clock_start);
+  lm_trust_region_loop(opt, eval_tracked, solve, candidate, accept, team.leader() ? res + pid : nullptr,
+                       clock_start);
+}
+
+// One workgroup per pair.
+struct LdsSolo {
+  int lo, hi;
+  __device__ bool leader() const { return true; }
+  __device__ void sync() { __syncthreads(); }
+  template <int K, int M>
+  __device__ void reduce(double (&v)[K + M], double* red, int& sel) { block_reduce_dpp<K, M>(v, red, sel); }
+  __device__ float4 neighbour(const float4* VEC, const float4* ZG, unsigned j) const { return VEC[j]; }
+  __device__ void publish(float4* VEC, float4* ZG, int il, int i, float4 v) const { VEC[il] = v; }
+};
+
+// A team of workgroups per pair (co-resident: cooperative launch).
+// Workgroup r owns [lo, hi).  The publication array ZG is written and read
+// only through device-coherent stores and loads, and the team barrier and
+// reductions are CoherentTeam's, so no barrier writes back or invalidates
+// the L2.
+struct LdsTeam {
+  CoherentTeam g;
+  int lo, hi;
+  __device__ bool leader() const { return g.leader(); }
+  __device__ void sync() { g.barrier(); }
+  template <int K, int M>
+  __device__ void reduce(double (&v)[K + M], double* red, int& sel) { g.template reduce_dpp<K, M>(v, red, sel); }
+  __device__ float4 neighbour(const float4* VEC, const float4* ZG, unsigned j) const {
+    const unsigned jl = j - (unsigned)lo;
+    return jl < (unsigned)(hi - lo) ? VEC[jl] : load_coherent(ZG + j);
+  }
+  __device__ void publish(float4* VEC, float4* ZG, int il, int i, float4 v) const {
+    VEC[il] = v;
+    store_coherent(ZG + i, v);
+  }
+};
+
+template <int T, int VPT>
+__global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict__ pairs,
+                                                    const int* __restrict__ plist, LMArrays A, LMOptionsDev opt,
+                                                    LMResultDev* __restrict__ res, long long totalV) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[2 * 8 * kMaxWaves];
+  __shared__ int scan_buf[kMaxWaves];
+  const int pid = plist[blockIdx.x];
+  const LMPairDesc P = pairs[pid];
+  LdsSolo team{0, P.nv};
+  lds_solve<T, VPT, LdsSolo, unsigned short>(P, pid, team, A, opt, res, totalV, smem, red, scan_buf);
+}
+
+// wg[b] = (team, rank, team size, chunk): team t solves pair plist[t];
+// workgroup `rank` owns vertices [rank chunk, min(nv, (rank + 1) chunk)).
+template <int T, int VPT>
+__global__ void __launch_bounds__(T) k_lm_solve_lds_team(const LMPairDesc* __restrict__ pairs,
+                                                         const int* __restrict__ plist, const int4* __restrict__ wg,
+                                                         LMArrays A, LMOptionsDev opt, LMResultDev* __restrict__ res,
+                                                         long long totalV, unsigned* cnt, double* part,
+                                                         int* abort_flag, long long limit) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[2 * 8 * kMaxWaves];
+  __shared__ int scan_buf[kMaxWaves];
+  const int4 m = wg[blockIdx.x];
+  const int pid = plist[m.x];
+  const LMPairDesc P = pairs[pid];
+  const int lo = min(P.nv, m.y * m.w), hi = min(P.nv, lo + m.w);
+  LdsTeam team{CoherentTeam{m.y, m.z, cnt + (size_t)m.x * 32, part + (size_t)m.x * 2 * kMaxTeam * 8, abort_flag, limit, 0u, 0},
+               lo, hi};
+  lds_solve<T, VPT, LdsTeam, unsigned>(P, pid, team, A, opt, res, totalV, smem, red, scan_buf);
+}
+
+// Team launch of the register/LDS kernel: nwg workgroups (wg_dev entries)
+// in one cooperative grid.
+template <int T, int VPT>
+int launch_team_bucket(const LMPairDesc* pairs_dev, const int* plist, const int4* wg_dev, int nwg, const LMArrays& A,
+                       const LMOptionsDev& opt, LMResultDev* res, long long totalV, size_t lds_bytes, unsigned* cnt,
+                       double* part, int* abort_flag, hipStream_t s) {
+  auto kern = k_lm_solve_lds_team<T, VPT>;
+  MO_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+  int dev = 0, khz = 0;
+  MO_HIP(hipGetDevice(&dev));
+  MO_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  long long limit = (long long)std::max(khz, 1) * 1000LL * 20;   // a 20 s bound on any one wait
+  if (const char* w = getenv("MO_LM_TEAM_WAIT_US")) limit = atoll(w) * (long long)std::max(khz, 1) / 1000LL;
+  if (const char* f = getenv("MO_LM_COOP_REFUSE"))
+    if (f[0] == '1') return kCoopRefused;
+  LMArrays a = A;
+  LMOptionsDev o = opt;
+  long long tv = totalV;
+  void* args[] = {(void*)&pairs_dev, (void*)&plist, (void*)&wg_dev, (void*)&a, (void*)&o, (void*)&res, (void*)&tv,
+                  (void*)&cnt, (void*)&part, (void*)&abort_flag, (void*)&limit};
+  if (hipLaunchCooperativeKernel((const void*)kern, dim3(nwg), dim3(T), args, (unsigned)lds_bytes, s) != hipSuccess) {
+    (void)hipGetLastError();   // the runtime refused the cooperative grid: nothing was launched
+    return kCoopRefused;
+  }
+  return MO_OK;
+}
+
+template <int T, int VPT>
+int team_bucket_capacity(size_t lds_bytes) {
+  auto kern = k_lm_solve_lds_team<T, VPT>;
+  int dev = 0, cus = 0, per = 0;
+  if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)kern, T, lds_bytes) != hipSuccess)
+    return 0;
+  return per * cus;
+}
+
+__global__ void k_chunk_entries(const LMPairDesc* __restrict__ pairs, const int2* __restrict__ adj, int np,
+                                int c0, int c1, int c2, int* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int nv = pairs[p].nv;
+  const long long vb = pairs[p].vbase;
+  const int caps[3] = {c0, c1, c2};
+  for (int b = 0; b < 3; ++b) {
+    int mx = 0;
+    if (nv > 0) {
+      const int c = (nv + caps[b] - 1) / caps[b], ch = (nv + c - 1) / c;
+      for (int r = 0; r < c; ++r) {
+        const int lo = r * ch, hi = min(nv, lo + ch);
+        if (lo >= hi) continue;
+        const int2 a = adj[vb + hi - 1], z = adj[vb + lo];
+        mx = max(mx, a.x + a.y - z.x);
+      }
+    }
+    out[3 * p + b] = mx;
+  }
 }
 
 template <int T, int VPT>
@@ -457,6 +590,58 @@ int lm_lds_bucket_for(int nv) {
   for (int b : shape_preference())
     if (nv <= kShapeT[b] * kShapeVPT[b]) return b;
   return -1;
+}
+
+// Team shapes of the register/LDS kernel (threads x vertices per thread per
+// workgroup), in preference order: the largest chunk whose LDS fits.
+static const int kTeamT[] = {1024, 1024, 1024};
+static const int kTeamVPT[] = {4, 3, 2};
+
+int lm_lds_team_shapes() { return 3; }
+
+int lm_lds_team_max() { return kMaxTeam; }
+
+int lm_lds_team_shape(int b, int* threads, int* vpt) {
+  MO_CHECK(b >= 0 && b < 3, MO_ERR_ARG, "team shape %d out of range", b);
+  *threads = kTeamT[b];
+  *vpt = kTeamVPT[b];
+  return MO_OK;
+}
+
+size_t lm_lds_team_bytes(int chunk, long long entries) {
+  return (((size_t)chunk * 16 + (size_t)entries * 4) + 15) & ~(size_t)15;
+}
+
+int lm_lds_team_capacity(int b, size_t lds_bytes) {
+  switch (b) {
+    case 0: return team_bucket_capacity<1024, 4>(lds_bytes);
+    case 1: return team_bucket_capacity<1024, 3>(lds_bytes);
+    case 2: return team_bucket_capacity<1024, 2>(lds_bytes);
+    default: return 0;
+  }
+}
+
+int lm_lds_chunk_entries(const LMPairDesc* pairs_dev, const int2* adj, int np, int* out_dev, hipStream_t s) {
+  if (np <= 0) return MO_OK;
+  hipLaunchKernelGGL(k_chunk_entries, dim3((np + 255) / 256), dim3(256), 0, s, pairs_dev, adj, np,
+                     kTeamT[0] * kTeamVPT[0], kTeamT[1] * kTeamVPT[1], kTeamT[2] * kTeamVPT[2], out_dev);
+  MO_HIP(hipGetLastError());
+  return MO_OK;
+}
+
+int lm_lds_team_launch(int b, const LMPairDesc* pairs, const int* plist, const int4* wg_dev, int nwg,
+                       const LMArrays& A, const LMOptionsDev& opt, LMResultDev* res, long long totalV,
+                       size_t lds_bytes, unsigned* cnt, double* part, int* abort_flag, hipStream_t s) {
+  if (nwg <= 0) return MO_OK;
+  switch (b) {
+    case 0: return launch_team_bucket<1024, 4>(pairs, plist, wg_dev, nwg, A, opt, res, totalV, lds_bytes, cnt, part,
+                                               abort_flag, s);
+    case 1: return launch_team_bucket<1024, 3>(pairs, plist, wg_dev, nwg, A, opt, res, totalV, lds_bytes, cnt, part,
+                                               abort_flag, s);
+    case 2: return launch_team_bucket<1024, 2>(pairs, plist, wg_dev, nwg, A, opt, res, totalV, lds_bytes, cnt, part,
+                                               abort_flag, s);
+    default: return fail(MO_ERR_ARG, "bad register/LDS team shape %d", b);
+  }
 }
 
 size_t lm_lds_bytes(int nv, long long entries) {

@@ -140,7 +140,7 @@ struct SoloTeam {
 // `limit` device clock ticks; a wait that runs out sets the shared abort
 // flag, after which no barrier waits (the solve finishes in bounded time
 // and the host reports the error).
-constexpr int kMaxTeam = 32;
+constexpr int kMaxTeam = 64;
 
 struct GridTeam {
   int r, c;
@@ -176,6 +176,19 @@ struct GridTeam {
   template <int K, int M>
   __device__ void reduce(double (&v)[K + M], double* red, int& sel) {
     block_reduce<K, M>(v, red, sel);
+    exchange<K, M>(v);
+  }
+  // The same with the workgroup's DPP reduction.
+  template <int K, int M>
+  __device__ void reduce_dpp(double (&v)[K + M], double* red, int& sel) {
+    block_reduce_dpp<K, M>(v, red, sel);
+    exchange<K, M>(v);
+  }
+  // Team part of a reduction: every workgroup's total (identical in all its
+  // threads) goes to its rank's slot; after the barrier each workgroup sums
+  // the c slots in rank order.
+  template <int K, int M>
+  __device__ void exchange(double (&v)[K + M]) {
     double* slot = part + gsel * (kMaxTeam * 8);
     gsel ^= 1;
     if (threadIdx.x == 0)
@@ -191,6 +204,90 @@ struct GridTeam {
       for (int q = 1; q < c; ++q) t = slot[q * 8 + k] > t ? slot[q * 8 + k] : t;
       v[k] = t;
     }
+  }
+};
+
+// Device-coherent accesses.  Relaxed agent-scope atomics compile to loads
+// and stores with the device-scope cache policy (sc1 on gfx950): they skip
+// the CU's L1 and are coherent across the XCDs' L2 caches, so data moved
+// through them needs no L2 write-back (buffer_wbl2) or invalidation
+// (buffer_inv) -- the cache maintenance an agent-scope fence performs on the
+// whole L2 of its XCD, which dominates a fenced team barrier.
+__device__ inline void store_coherent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double load_coherent(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_coherent(float4* p, float4 v) {
+  unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+  const unsigned long long a = (unsigned long long)__float_as_uint(v.x) | ((unsigned long long)__float_as_uint(v.y) << 32);
+  const unsigned long long b = (unsigned long long)__float_as_uint(v.z) | ((unsigned long long)__float_as_uint(v.w) << 32);
+  __hip_atomic_store(q, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline float4 load_coherent(const float4* p) {
+  unsigned long long* q = reinterpret_cast<unsigned long long*>(const_cast<float4*>(p));
+  const unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return make_float4(__uint_as_float((unsigned)a), __uint_as_float((unsigned)(a >> 32)), __uint_as_float((unsigned)b),
+                     __uint_as_float((unsigned)(b >> 32)));
+}
+
+// CoherentTeam: GridTeam's protocol (per-problem arrival counter, FP64
+// partials in rank slots, bounded waits, shared abort flag; the slots are
+// combined by one wave-wide DPP reduction, the same in every member) for
+// teams whose exchanged data -- the partials and whatever the
+// caller publishes -- goes only through store_coherent / load_coherent.
+// Its barrier needs no fence: each wave waits for its own stores to
+// complete (s_waitcnt vmcnt(0)), the workgroup barrier then orders them
+// before thread 0's arrival, and a coherent load issued after the barrier
+// that ends the wait reads the stored value.  Plain global data a member
+// reads is only data it wrote itself.
+struct CoherentTeam {
+  int r, c;
+  unsigned* cnt;        // this problem's arrival counter
+  double* part;         // this problem's partials [2][kMaxTeam][8]
+  int* abort_flag;
+  long long limit;
+  unsigned arrivals;
+  int gsel;
+  __device__ bool leader() const { return r == 0; }
+  __device__ void barrier() {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned target = (arrivals + 1u) * (unsigned)c;
+      const long long t0 = wall_clock64();
+      while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (wall_clock64() - t0 > limit) {
+          __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    ++arrivals;
+    __syncthreads();
+  }
+  template <int K, int M>
+  __device__ void reduce_dpp(double (&v)[K + M], double* red, int& sel) {
+    block_reduce_dpp<K, M>(v, red, sel);
+    double* slot = part + gsel * (kMaxTeam * 8);
+    gsel ^= 1;
+    if (threadIdx.x == 0)
+      for (int k = 0; k < K + M; ++k) store_coherent(slot + r * 8 + k, v[k]);
+    barrier();
+    // lane q of every wave loads rank q's partials (c <= kMaxTeam = 64), all
+    // loads in flight at once; the wave's DPP reduction then gives every
+    // lane of every member the same bits
+    const int lane = threadIdx.x & 63;
+    double x[K + M];
+    for (int k = 0; k < K + M; ++k) x[k] = lane < c ? load_coherent(slot + lane * 8 + k) : (k < K ? 0.0 : -INFINITY);
+    for (int k = 0; k < K; ++k) v[k] = wave_sum_f64(x[k]);
+    for (int k = K; k < K + M; ++k) v[k] = wave_max_f64(x[k]);
   }
 };
 

@@ -14,7 +14,7 @@
   bit-exact at every iterate, and the same loop on the CPU driven by the
   oracle's layers;
 * config 4 (icosphere levels 3-6, 64^3 / 128^3, lambda in [0.5, 2]): the
-  level-6 pair (40962 vertices, 128^3) on the multi-workgroup kernel against
+  level-6 pair (40962 vertices, 128^3) on the register/LDS team kernel against
   the oracle's converged energy (golden fixture), and a mixed batch whose
   every pair is bit-identical alone and in the mix."""
 import os
@@ -209,7 +209,7 @@ def test_config2_adam_steps(config2, oracle, gpu):
 def test_config4_level6_pair_golden(gpu, oracle):
     """The largest config-4 pair: a 40962-vertex icosphere source against a
     displaced level-6 target at 128^3 (lambda 1.3), on the multi-workgroup
-    kernel, within 1e-3 of the oracle's converged energy
+    register/LDS team kernel, within 1e-3 of the oracle's converged energy
     (tests/golden/lm_cfg4_l6_n128.npz, written by make_golden.py cfg4_l6)."""
     z = np.load(os.path.join(GOLDEN, "lm_cfg4_l6_n128.npz"))
     N, lam = int(z["N"]), float(z["lam"])
@@ -222,7 +222,7 @@ def test_config4_level6_pair_golden(gpu, oracle):
     sn = oracle.apply_transform(pair[0], scale, pos)
     V = torch.from_numpy(sn.copy()).cuda()
     st = deform.deform_batch([pid], [V], [pair[1]], [0], [lam])[0]
-    assert st["solver_path_name"] == "team"
+    assert st["solver_path_name"] == "lds_team"
     eg, _, _ = oracle.lm_cost(grid, N, V.cpu().numpy(), sn, oracle.face_edges(pair[1]), lam)
     ref = float(z["final_cost"])
     assert float(z["initial_cost"]) == pytest.approx(st["initial_cost"], rel=1e-4)
@@ -232,8 +232,9 @@ def test_config4_level6_pair_golden(gpu, oracle):
 
 def test_config4_mixed_batch_alone_equals_mix(gpu):
     """Eight config-4 pairs (levels 3-6, 64^3 / 128^3, lambda in [0.5, 2],
-    50 LM x 30 PCG iterations): the register/LDS kernel and the team kernel
-    in one call; each pair's result is bit-identical when solved alone."""
+    50 LM x 30 PCG iterations): the one-workgroup and the team register/LDS
+    kernels in one call; each pair's result is bit-identical when solved
+    alone."""
     _say("[cfg] config 4: mixed L3-L6 batch, alone vs in the mix")
     rng = np.random.Generator(np.random.PCG64(4))
     levels = [3, 4, 5, 6, 3, 4, 5, 6]
@@ -244,7 +245,7 @@ def test_config4_mixed_batch_alone_equals_mix(gpu):
                                  gradient_tolerance=0.0, parameter_tolerance=0.0, pcg_relative_tolerance=0.0)
     mix, st_mix = deform.rigid_deform_batch(pairs, grid_resolution=grids, lambdas=lams, options=opt)
     paths = {s["solver_path_name"] for s in st_mix}
-    assert {"lds", "team"} <= paths, paths
+    assert {"lds", "lds_team"} <= paths, paths
     for i, p in enumerate(pairs):
         alone, st = deform.rigid_deform_batch([p], grid_resolution=grids[i], lambdas=lams[i], options=opt)
         assert torch.equal(alone[0], mix[i]), i

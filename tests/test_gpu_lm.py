@@ -304,3 +304,61 @@ def test_dense_graph_beyond_lds_offsets(gpu, oracle):
          f"path={st['solver_path_name']}")
     assert st["solver_path_name"] == "one_workgroup"
     assert abs(eg - so["final_cost"]) <= ENERGY_RTOL * so["final_cost"]
+
+
+def test_lds_team_barrier_timeout_is_reported(gpu, monkeypatch):
+    """The register/LDS team kernel (a 12000-vertex pair: a team of 3
+    workgroups) with every barrier wait forced to give up at once
+    (MO_LM_TEAM_WAIT_US=-1, from the host): the kernel finishes, the call
+    raises and leaves the caller's vertices untouched; the next call solves
+    bit-identically to one made before."""
+    from meshode_amd import _lib
+    _say("[lm] case=register/LDS team barrier bound forced")
+    pair = synth.box_union_pair(5, n_verts=12000)
+    pid = deform.template_f64(pair[2], pair[3], 32)
+    _, scale, pos = pyDeform.TemplateInfo(pid)
+    x0 = (pair[0] - pos) / scale
+    opt = deform.default_options(max_num_iterations=5)
+    V = torch.from_numpy(x0.copy()).cuda()
+    st = deform.deform_batch([pid], [V], [pair[1]], [0], [1.0], options=opt)[0]
+    assert st["solver_path_name"] == "lds_team"
+    monkeypatch.setenv("MO_LM_TEAM_WAIT_US", "-1")
+    W = torch.from_numpy(x0.copy()).cuda()
+    with pytest.raises(_lib.MoError, match="outputs not written"):
+        deform.deform_batch([pid], [W], [pair[1]], [0], [1.0], options=opt)
+    assert torch.equal(W, torch.from_numpy(x0).cuda())
+    monkeypatch.delenv("MO_LM_TEAM_WAIT_US")
+    deform.deform_batch([pid], [W], [pair[1]], [0], [1.0], options=opt)
+    assert torch.equal(W, V)
+
+
+def test_lds_team_kernel_energy_parity(gpu, oracle):
+    """A 12000-vertex box union at 48^3 on the register/LDS team kernel
+    (3 workgroups of 4096 vertices, neighbours across chunks read from the
+    published array) against the oracle's exact-step LM."""
+    _say("[lm] case=box12000_n48 register/LDS team")
+    pair = synth.box_union_pair(6, n_verts=12000)
+    so, eg, st = _parity(oracle, pair, 48)
+    _say(f"[lm] lds team: oracle E={so['final_cost']:.8g}  gpu E={eg:.8g} it={st['iterations']} "
+         f"path={st['solver_path_name']}")
+    assert st["solver_path_name"] == "lds_team"
+    assert abs(eg - so["final_cost"]) <= ENERGY_RTOL * so["final_cost"]
+
+
+def test_lds_team_many_teams_deterministic(gpu):
+    """Eight pairs of 9000-30000 vertices (teams of 3 to 8 workgroups in one
+    cooperative grid) solved twice: every pair's vertices are bit-identical
+    between the calls, and each pair matches its own solve alone (the
+    teams' exchanges through the publication array and the partial slots
+    carry no stale value)."""
+    _say("[lm] case=8 register/LDS teams, repeat and alone")
+    pairs = [synth.box_union_pair(40 + k, n_verts=9000 + 3000 * k) for k in range(8)]
+    opt = deform.default_options(max_num_iterations=8)
+    a, sa = deform.rigid_deform_batch(pairs, grid_resolution=32, options=opt)
+    b, _ = deform.rigid_deform_batch(pairs, grid_resolution=32, options=opt)
+    assert all(s["solver_path_name"] == "lds_team" for s in sa), [s["solver_path_name"] for s in sa]
+    for k in range(8):
+        assert torch.equal(a[k], b[k]), k
+    for k in (0, 7):
+        alone, _ = deform.rigid_deform_batch([pairs[k]], grid_resolution=32, options=opt)
+        assert torch.equal(alone[0], a[k]), k

@@ -100,7 +100,7 @@ def test_served_100k_request_at_256_equals_alone(gpu):
         res = [f.result(timeout=180) for f in futs]
     finally:
         srv.close()
-    assert res[0].stats["solver_path_name"] == "team"
+    assert res[0].stats["solver_path_name"] == "lds_team"
     for r, out in zip(reqs, res):
         x, st = _alone(r)
         assert np.array_equal(out.V, x)

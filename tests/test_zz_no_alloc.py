@@ -1,7 +1,7 @@
 """mo_deform_batch_noalloc (the batched LM with its scratch in the
 library's fixed per-GPU array instead of device allocations) against
 mo_deform_batch, bit for bit: two calls in a row, the second smaller, on a
-batch that takes the register/LDS kernel, the multi-workgroup team kernel
+batch that takes the register/LDS kernel, its multi-workgroup team form
 (a 10242-vertex source) and an explicit edge list; a batch over the array's
 size is refused before anything runs."""
 import numpy as np
@@ -62,7 +62,7 @@ def test_no_alloc_matches_allocating_entry(gpu):
     _say("[noalloc] call 1: 5 pairs (LDS kernel, team kernel, edge list)")
     first = _solve(cases, True)
     paths = {s["solver_path_name"] for s in first[1]}
-    assert {"lds", "team"} <= paths, paths
+    assert {"lds", "lds_team"} <= paths, paths
     _same(cases, first, _solve(cases, False))
     _say("[noalloc] call 2: 2 smaller pairs")
     small = [cases[2], cases[4]]
