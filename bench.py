@@ -120,6 +120,7 @@ def main():
                     help="0: register/LDS kernel when a pair fits, else teams; 1: one-workgroup global; 2: teams")
     ap.add_argument("--pcg-tol", type=float, default=None, help="inner PCG relative tolerance (library default if unset)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the captured-graph replay measurement")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the launch-shape autotuner (untimed, before warm-up) and use the built-in order")
     ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "flow256", "neural2", "rot", "cfg1", "cfg4"],
@@ -192,6 +193,32 @@ def main():
 
     value = args.pairs * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
+
+    # the same step as a captured HIP graph (mo_rigid_deform_batch_graph):
+    # every kernel of the chain replays on the same inputs; timed like the
+    # eager steps (timing events are not captured, so opt.timing is 0)
+    graph_line = None
+    if not args.no_graph:
+        gopt = deform.default_options(block_threads=args.block_threads, timing=0,
+                                      kernel_variant=args.kernel_variant)
+        if args.pcg_tol is not None:
+            gopt.pcg_relative_tolerance = args.pcg_tol
+        gbatch = deform.PairBatch(pairs, grid_resolution=args.grid, lambdas=args.lam)
+        gst = gbatch.capture(gopt)
+        gbatch.replay()
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        for k in range(args.steps):
+            gst = gbatch.replay()
+        torch.cuda.synchronize()
+        gel = time.perf_counter() - tg
+        same = all(torch.equal(a, b) for a, b in zip(gbatch.out, batch.out))
+        graph_line = {"pairs_per_s": args.pairs * args.steps / gel, "ms_per_step": 1e3 * gel / args.steps,
+                      "outputs_equal_eager": bool(same),
+                      "lm_iterations_mean": float(np.mean([s["iterations"] for s in gst]))}
+        gbatch.release()
+        print(f"[bench] graph replay: {graph_line['ms_per_step']:.1f} ms/step, outputs equal eager: {same}",
+              file=sys.stderr, flush=True)
     # one more step, after the timed region, with the grid build's work
     # counters on (timing 2: their atomics cost a few percent of the phase)
     opt.timing = 2
@@ -288,6 +315,7 @@ def main():
         "roofline": roof_grid if grid_ms >= lm_ms else roof_lm,
         "roofline_grid": roof_grid,
         "roofline_lm": roof_lm,
+        "graph_replay": graph_line,
     }
 
     if not args.no_cpu_baseline:

@@ -192,7 +192,7 @@ typedef struct mo_lm_stats {
   int64_t clock_start;   /* device wall clock (ticks) when the pair's solve began */
   int64_t clock_end;     /* ... and ended; see mo_last_timings for the rate */
   int32_t solver_path;   /* MO_PATH_*: which kernel solved the pair */
-  int32_t reserved;
+  int32_t workgroups;    /* workgroups that solved it (1, or the team's size) */
 } mo_lm_stats;
 
 /* mo_lm_stats.solver_path.  MO_PATH_COOP_REFUSED marks a pair meant for a
@@ -271,6 +271,35 @@ int mo_rigid_deform_batch(int32_t npairs, const double* const* src, const int64_
                           const int64_t* tgt_nv, const int32_t* const* tgt_faces, const int64_t* tgt_nf,
                           const int32_t* grid_resolution, const double* lambda, const mo_lm_options* opt,
                           double* const* out, mo_lm_stats* stats, void* stream);
+
+/* The same pipeline as a captured HIP graph (no counterpart in the
+ * reference, which runs one ceres::Solve per process, rigid_deform.cc:12-63:
+ * the launch-chain replay for callers that solve the same batch shape again
+ * and again).  mo_rigid_deform_batch_graph runs the batch once, exactly as
+ * mo_rigid_deform_batch does (out and stats are that run's), then captures
+ * the whole chain -- target normalization, grid builds, source transform,
+ * topology, LM, write-back -- into a graph with its own device scratch and
+ * pinned staging (allocated here, once), and returns it in *graph.
+ * mo_batch_graph_launch replays it on `stream` and waits for it: every
+ * kernel runs again on the current contents of the same src / tgt buffers
+ * (positions may change between replays; faces, sizes, resolutions and
+ * lambdas are the captured ones) and writes the same out buffers, and stats
+ * (optional) receive that replay's solver statistics.  A replay checks the
+ * values that steered the captured chain (adjacency entry counts, error
+ * flags) and returns MO_ERR_STATE when they differ.  out[p] must not alias
+ * src[p]; options.timing must be 0; a batch with a pair that needs a
+ * cooperative multi-workgroup launch (MO_PATH_TEAM / MO_PATH_LDS_TEAM) is
+ * not captured (MO_ERR_ARG after the eager run, whose results are valid).
+ * A graph belongs to the device current at capture; replays of one graph
+ * must not overlap.  mo_batch_graph_destroy frees it. */
+typedef struct mo_batch_graph mo_batch_graph;
+int mo_rigid_deform_batch_graph(int32_t npairs, const double* const* src, const int64_t* src_nv,
+                                const int32_t* const* src_faces, const int64_t* src_nf, const double* const* tgt,
+                                const int64_t* tgt_nv, const int32_t* const* tgt_faces, const int64_t* tgt_nf,
+                                const int32_t* grid_resolution, const double* lambda, const mo_lm_options* opt,
+                                double* const* out, mo_lm_stats* stats, void* stream, mo_batch_graph** graph);
+int mo_batch_graph_launch(mo_batch_graph* graph, mo_lm_stats* stats, void* stream);
+int mo_batch_graph_destroy(mo_batch_graph* graph);
 
 /* Deformer::DeformSubdivision with a TerminateWhenSuccessCallback
  * (deformer.cc:173-263, callback.h:9-39, as cad_deform.cc:87-114 runs it):

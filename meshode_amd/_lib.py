@@ -32,7 +32,7 @@ SYMBOLS = [
     "mo_flow256_step_backward", "mo_flow256_small_grad_floats", "mo_flow256_small_grads",
     "mo_flow256_dw_work_floats", "mo_flow256_dw_grads",
     "mo_deform_batch_noalloc_max_bytes", "mo_deform_batch_noalloc", "mo_template_rebuild_f64",
-    "mo_last_grid_stats",
+    "mo_last_grid_stats", "mo_rigid_deform_batch_graph", "mo_batch_graph_launch", "mo_batch_graph_destroy",
 ]
 
 
@@ -81,11 +81,11 @@ class LMStats(C.Structure):
         ("clock_start", C.c_int64),
         ("clock_end", C.c_int64),
         ("solver_path", C.c_int32),
-        ("reserved", C.c_int32),
+        ("workgroups", C.c_int32),
     ]
 
     def as_dict(self):
-        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        d = {f: getattr(self, f) for f, _ in self._fields_}
         d["solver_path_name"] = SOLVER_PATHS.get(self.solver_path, "unknown")
         return d
 
@@ -125,6 +125,10 @@ _SIGS = {
                                           _vp]),
     "mo_rigid_deform_batch": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                         C.POINTER(LMOptions), _vp, _vp, _vp]),
+    "mo_rigid_deform_batch_graph": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                              C.POINTER(LMOptions), _vp, _vp, _vp, C.POINTER(_vp)]),
+    "mo_batch_graph_launch": (C.c_int, [_vp, _vp, _vp]),
+    "mo_batch_graph_destroy": (C.c_int, [_vp]),
     "mo_last_timings": (C.c_int, [_vp]),
     "mo_last_grid_stats": (C.c_int, [_vp]),
     "mo_lm_shape_count": (_i32, []),

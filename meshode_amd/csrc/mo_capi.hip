@@ -40,6 +40,101 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+// ---------------------------------------------------------------------------
+// Batch graphs: a batch's launch chain (normalization, grid build, topology,
+// LM, write-back) is run twice by mo_rigid_deform_batch_graph -- once
+// eagerly in record mode, then under hipStreamBeginCapture in capture mode --
+// and the captured graph is replayed by mo_batch_graph_launch.  The chain's
+// host code reaches the stream through these helpers where the modes differ:
+//   Arena::reserve  record: sizes summed; capture: carved from the graph's
+//                   device pool (no stream allocation inside the graph)
+//   chain_h2d       capture: the bytes are staged in the graph's pinned
+//                   buffer first (a replay copies the same bytes)
+//   chain_read      device values the host reads to steer the chain (entry
+//                   counts, error flags).  eager: copy + synchronize; record:
+//                   the same, values kept in order; capture: the recorded
+//                   values are returned and the device copy goes to the
+//                   pinned buffer, compared with the record after each replay
+// The LM results are copied to the pinned buffer at the end of the graph.
+struct ChainCtx {
+  enum Mode { kRecord, kCapture };
+  Mode mode = kRecord;
+  std::vector<std::vector<char>> reads;   // record: steering values, in order
+  size_t pool_bytes = 0;                  // record: device scratch of the chain
+  size_t pinned_bytes = 0;                // record: staged copies + reads + results
+  bool team_launch = false;               // record: a cooperative team launch was made
+  size_t next_read = 0;                   // capture
+  char* pool = nullptr;
+  size_t pool_cap = 0, pool_used = 0;
+  char* pinned = nullptr;
+  size_t pinned_cap = 0, pinned_used = 0;
+  std::vector<std::pair<size_t, size_t>> checks;   // (pinned offset, index into reads)
+  size_t res_off = 0;
+  int res_np = 0;
+  std::vector<int> res_path, res_wgs;
+};
+
+thread_local ChainCtx* g_chain = nullptr;
+
+inline size_t al256c(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static int pinned_take(ChainCtx* c, size_t bytes, size_t* off) {
+  MO_CHECK(c->pinned_used + al256c(bytes) <= c->pinned_cap, MO_ERR_STATE,
+           "batch graph capture: pinned staging exceeds the recorded size");
+  *off = c->pinned_used;
+  c->pinned_used += al256c(bytes);
+  return MO_OK;
+}
+
+int chain_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return MO_OK;
+  ChainCtx* c = g_chain;
+  if (c && c->mode == ChainCtx::kCapture) {
+    size_t off = 0;
+    int rc = pinned_take(c, bytes, &off);
+    if (rc) return rc;
+    memcpy(c->pinned + off, src, bytes);
+    src = c->pinned + off;
+  } else if (c) {
+    c->pinned_bytes += al256c(bytes);
+  }
+  MO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  return MO_OK;
+}
+
+struct ChainRead {
+  void* host;
+  const void* dev;
+  size_t bytes;
+};
+
+static int chain_read(const ChainRead* r, int n, hipStream_t s) {
+  ChainCtx* c = g_chain;
+  if (c && c->mode == ChainCtx::kCapture) {
+    for (int k = 0; k < n; ++k) {
+      MO_CHECK(c->next_read < c->reads.size() && c->reads[c->next_read].size() == r[k].bytes, MO_ERR_STATE,
+               "batch graph capture diverged from the recorded run");
+      memcpy(r[k].host, c->reads[c->next_read].data(), r[k].bytes);
+      size_t off = 0;
+      int rc = pinned_take(c, r[k].bytes, &off);
+      if (rc) return rc;
+      MO_HIP(hipMemcpyAsync(c->pinned + off, r[k].dev, r[k].bytes, hipMemcpyDeviceToHost, s));
+      c->checks.emplace_back(off, c->next_read);
+      ++c->next_read;
+    }
+    return MO_OK;
+  }
+  for (int k = 0; k < n; ++k) MO_HIP(hipMemcpyAsync(r[k].host, r[k].dev, r[k].bytes, hipMemcpyDeviceToHost, s));
+  MO_HIP(hipStreamSynchronize(s));
+  if (c)
+    for (int k = 0; k < n; ++k) {
+      const char* h = (const char*)r[k].host;
+      c->reads.emplace_back(h, h + r[k].bytes);
+      c->pinned_bytes += al256c(r[k].bytes);
+    }
+  return MO_OK;
+}
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -82,6 +177,17 @@ struct Arena {
   int reserve(size_t bytes, hipStream_t s) {
     stream = s;
     size = bytes + 256;
+    if (ChainCtx* c = g_chain) {
+      if (c->mode == ChainCtx::kCapture) {
+        MO_CHECK(c->pool_used + al256c(size) <= c->pool_cap, MO_ERR_STATE,
+                 "batch graph capture: scratch exceeds the recorded size");
+        base = c->pool + c->pool_used;
+        c->pool_used += al256c(size);
+        owned = false;
+        return MO_OK;
+      }
+      c->pool_bytes += al256c(size);
+    }
     hipError_t e = hipMallocAsync((void**)&base, size, s);
     if (e != hipSuccess) {
       base = nullptr;
@@ -226,8 +332,9 @@ int template_create(const T* V, int64_t nv, const int32_t* F, int64_t nf, int32_
 
 int check_err_flag(int* err_dev, hipStream_t s, const char* what) {
   int h = 0;
-  MO_HIP(hipMemcpyAsync(&h, err_dev, sizeof(int), hipMemcpyDeviceToHost, s));
-  MO_HIP(hipStreamSynchronize(s));
+  const ChainRead rd{&h, err_dev, sizeof(int)};
+  int rc = chain_read(&rd, 1, s);
+  if (rc) return rc;
   MO_CHECK(h == 0, MO_ERR_RANGE, "%s: index out of range (flags 0x%x)", what, h);
   return MO_OK;
 }
@@ -304,16 +411,32 @@ __global__ void k_stamp(long long* out) { *out = wall_clock64(); }
 
 // x0 = rest position (w: sum of edge weights), u = start - rest (0 when the
 // solve starts at the rest shape).
-__global__ void k_make_x0(const double* __restrict__ V, int nv, const float* __restrict__ deg, float4* __restrict__ x0,
-                          float4* __restrict__ u, const double* __restrict__ xs) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  x0[i] = make_float4((float)V[i * 3], (float)V[i * 3 + 1], (float)V[i * 3 + 2], deg[i]);
-  if (xs)
-    u[i] = make_float4((float)(xs[i * 3] - V[i * 3]), (float)(xs[i * 3 + 1] - V[i * 3 + 1]),
-                       (float)(xs[i * 3 + 2] - V[i * 3 + 2]), 0.f);
-  else
-    u[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+// Per-pair inputs and outputs of the batched set-up / write-back kernels.
+struct PairIO {
+  const double* x0;       // normalized FP64 source [nv,3]
+  const double* xstart;   // start positions, or null (the rest shape)
+  const double* sp;       // (scale, pos) for denormalized output, or null
+  double* out;
+  long long vbase;
+  long long nv;
+};
+
+// blockIdx.y = pair; x0 = (V, degree), u = xstart - V (or 0).
+__global__ void k_make_x0(const PairIO* __restrict__ io, const float* __restrict__ deg, float4* __restrict__ x0,
+                          float4* __restrict__ u) {
+  const PairIO P = io[blockIdx.y];
+  const double* __restrict__ V = P.x0;
+  const double* __restrict__ xs = P.xstart;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < P.nv;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long g = P.vbase + i;
+    x0[g] = make_float4((float)V[i * 3], (float)V[i * 3 + 1], (float)V[i * 3 + 2], deg[g]);
+    if (xs)
+      u[g] = make_float4((float)(xs[i * 3] - V[i * 3]), (float)(xs[i * 3 + 1] - V[i * 3 + 1]),
+                         (float)(xs[i * 3 + 2] - V[i * 3 + 2]), 0.f);
+    else
+      u[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 }
 
 // AdaptiveEdgeLoss weights (edgeloss.h:35-61): every adjacency entry of a
@@ -343,13 +466,18 @@ __global__ void k_adaptive_weights(const double* __restrict__ V, int nv, const i
 
 // out = x0_64 + u (normalized frame), or (x0_64 + u) * scale + pos when sp
 // is given (Mesh::WriteOBJ, mesh.cc:46-64).
-__global__ void k_finish(const double* __restrict__ X, int nv, const float4* __restrict__ u,
-                         const double* __restrict__ sp, double* __restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  const float4 d = u[i];
-  double p[3] = {X[i * 3] + (double)d.x, X[i * 3 + 1] + (double)d.y, X[i * 3 + 2] + (double)d.z};
-  for (int k = 0; k < 3; ++k) out[i * 3 + k] = sp ? p[k] * sp[0] + sp[1 + k] : p[k];
+// blockIdx.y = pair: out = denormalized (x0 + u).
+__global__ void k_finish(const PairIO* __restrict__ io, const float4* __restrict__ u) {
+  const PairIO P = io[blockIdx.y];
+  const double* __restrict__ X = P.x0;
+  const double* __restrict__ sp = P.sp;
+  double* __restrict__ out = P.out;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < P.nv;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float4 d = u[P.vbase + i];
+    double p[3] = {X[i * 3] + (double)d.x, X[i * 3 + 1] + (double)d.y, X[i * 3 + 2] + (double)d.z};
+    for (int k = 0; k < 3; ++k) out[i * 3 + k] = sp ? p[k] * sp[0] + sp[1 + k] : p[k];
+  }
 }
 
 struct CorePair {
@@ -443,6 +571,26 @@ int device_lds_limit() {
   return cached;
 }
 
+void fill_stats(const LMResultDev* res, const int* path, const int* wgs, int np, mo_lm_stats* stats) {
+  for (int p = 0; p < np; ++p) {
+    const LMResultDev& r = res[p];
+    mo_lm_stats& o = stats[p];
+    o.initial_cost = r.initial_cost;
+    o.final_cost = r.final_cost;
+    o.vertex_cost = r.vertex_cost;
+    o.rigidity_cost = r.rigidity_cost;
+    o.iterations = r.iterations;
+    o.successful_steps = r.successful_steps;
+    o.unsuccessful_steps = r.unsuccessful_steps;
+    o.termination = r.termination;
+    o.pcg_iterations = r.pcg_iterations;
+    o.clock_start = r.clock_start;
+    o.clock_end = r.clock_end;
+    o.solver_path = path[p];
+    o.workgroups = wgs[p];
+  }
+}
+
 // ext / ext_cap: a caller-provided device region for all of the solve's
 // scratch (the no-allocation entry point); null: stream-ordered allocations.
 int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_stats* stats, hipStream_t s,
@@ -471,7 +619,9 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
                  al(std::max<long long>(2 * nedges_tot, 1) * sizeof(unsigned)) +
                  al(std::max<long long>(2 * nedges_tot, 1) * sizeof(float)) + al(topo_workspace_bytes(nvalloc)) +
                  al(np * sizeof(LMPairDesc)) + al(np * sizeof(TopoPairDesc)) + al(np * sizeof(LMResultDev)) +
-                 al(2 * np * sizeof(int)) + al(np * sizeof(int)) + al(3 * np * sizeof(int)) + 512;
+                 al(2 * np * sizeof(int)) + al(np * sizeof(int)) + al(3 * np * sizeof(int)) +
+                 al(np * sizeof(PairIO)) + 512;
+  MO_CHECK(np <= 65535, MO_ERR_ARG, "at most 65535 pairs per batch (got %d)", np);
   Arena ar;
   int rc;
   if ((rc = ext ? ar.attach(ext, ext_cap, bytes, s) : ar.reserve(bytes, s))) return rc;
@@ -489,39 +639,45 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   int* d_pstat = ar.take<int>(2 * np);
   int* d_plist = ar.take<int>(np);
   int* d_chunk = ar.take<int>(3 * np);
+  PairIO* d_io = ar.take<PairIO>(np);
   int* err = ar.take<int>(1);
 
   std::vector<LMPairDesc> h_pairs(np);
   std::vector<TopoPairDesc> h_topo(np);
+  std::vector<PairIO> h_io(np);
+  long long max_nv_io = 1;
   for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
+    h_io[p] = PairIO{c.x0, c.xstart, c.sp, c.out, vbase[p], c.nv};
+    max_nv_io = std::max(max_nv_io, c.nv);
     const bool weighted = c.kind == 2;
     h_pairs[p] = LMPairDesc{c.grid, c.N, (int)c.nv, vbase[p], 0, weighted ? 1.0f : (float)c.lambda, weighted ? 1 : 0};
     h_topo[p] = TopoPairDesc{c.idx, c.kind == 0 ? 3 * c.nidx : c.nidx, vbase[p], (int)c.nv, c.kind == 0 ? 0 : 1};
   }
-  MO_HIP(hipMemcpyAsync(d_pairs, h_pairs.data(), np * sizeof(LMPairDesc), hipMemcpyHostToDevice, s));
-  MO_HIP(hipMemcpyAsync(d_topo, h_topo.data(), np * sizeof(TopoPairDesc), hipMemcpyHostToDevice, s));
+  if ((rc = chain_h2d(d_pairs, h_pairs.data(), np * sizeof(LMPairDesc), s))) return rc;
+  if ((rc = chain_h2d(d_topo, h_topo.data(), np * sizeof(TopoPairDesc), s))) return rc;
+  if ((rc = chain_h2d(d_io, h_io.data(), np * sizeof(PairIO), s))) return rc;
   if ((rc = zero_fill(err, sizeof(int), s))) return rc;
   if ((rc = topo_build_launch(d_topo, np, max_edges, max_nv, nvtot, raw, adj, deg, d_pstat, tws, err, s))) return rc;
   for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
-    if (c.nv == 0) continue;
-    const dim3 grid((unsigned)((c.nv + 255) / 256));
-    if (c.kind == 2)
-      hipLaunchKernelGGL(k_adaptive_weights, grid, dim3(256), 0, s, c.x0, (int)c.nv, adj + vbase[p], raw, c.lambda,
-                         wt, deg + vbase[p]);
-    hipLaunchKernelGGL(k_make_x0, grid, dim3(256), 0, s, c.x0, (int)c.nv, deg + vbase[p], x0 + vbase[p],
-                       u + vbase[p], c.xstart);
+    if (c.nv == 0 || c.kind != 2) continue;
+    hipLaunchKernelGGL(k_adaptive_weights, dim3((unsigned)((c.nv + 255) / 256)), dim3(256), 0, s, c.x0, (int)c.nv,
+                       adj + vbase[p], raw, c.lambda, wt, deg + vbase[p]);
   }
+  const dim3 io_grid((unsigned)std::min<long long>((max_nv_io + 255) / 256, 64), (unsigned)np);
+  hipLaunchKernelGGL(k_make_x0, io_grid, dim3(256), 0, s, d_io, deg, x0, u);
   MO_HIP(hipGetLastError());
   // per pair and register/LDS team shape: the largest chunk's adjacency entries
   if ((rc = lm_lds_chunk_entries(d_pairs, adj, np, d_chunk, s))) return rc;
   std::vector<int> pstat(2 * np), hchunk(3 * np);
   int herr = 0;
-  MO_HIP(hipMemcpyAsync(hchunk.data(), d_chunk, 3 * np * sizeof(int), hipMemcpyDeviceToHost, s));
-  MO_HIP(hipMemcpyAsync(pstat.data(), d_pstat, 2 * np * sizeof(int), hipMemcpyDeviceToHost, s));
-  MO_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
-  MO_HIP(hipStreamSynchronize(s));
+  {
+    const ChainRead rd[3] = {{hchunk.data(), d_chunk, 3 * np * sizeof(int)},
+                             {pstat.data(), d_pstat, 2 * np * sizeof(int)},
+                             {&herr, err, sizeof(int)}};
+    if ((rc = chain_read(rd, 3, s))) return rc;
+  }
   MO_CHECK(herr == 0, MO_ERR_RANGE, "source topology: index out of range or too many vertices (flags 0x%x)", herr);
 
   // Schedule: largest pairs first within every launch (workgroups are
@@ -557,7 +713,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
     plist.insert(plist.end(), bucket[b].begin(), bucket[b].end());
   }
   start[nb + 1] = (int)plist.size();
-  MO_HIP(hipMemcpyAsync(d_plist, plist.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+  if ((rc = chain_h2d(d_plist, plist.data(), np * sizeof(int), s))) return rc;
   LMArrays A{x0, u, adj, raw, work, wt};
   const LMOptionsDev dopt = to_dev_options(opt);
   if ((rc = timer.mark(2))) return rc;
@@ -580,6 +736,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   std::vector<int> solo, teams, team_c;
   std::vector<int4> wg;
   std::vector<int> path(np, MO_PATH_LDS);
+  std::vector<int> nwgs(np, 1);   // workgroups that solve each pair
   for (int p : bucket[nb]) path[p] = MO_PATH_ONE_WORKGROUP;
   std::unique_lock<std::mutex> team_lock(team_launch_mutex(), std::defer_lock);
   // leaving with the lock held (an error after a cooperative launch): wait
@@ -598,9 +755,9 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
     MO_CHECK(opt.team_vertices >= 0, MO_ERR_ARG, "team_vertices must be >= 0");
     // Pairs above one register/LDS workgroup go to the register/LDS team
     // kernel when their multiplicities allow it (kind 0/1, <= 2), with the
-    // first team shape (largest chunk) whose chunk fits the LDS, a team of
-    // at most kMaxTeam workgroups, and the whole team co-resident.  The
-    // rest take the global-memory kernels below.
+    // first team shape in preference order (lm_lds_team_order) whose chunk
+    // fits the LDS, a team of at most kMaxTeam workgroups, and the whole
+    // team co-resident.  The rest take the global-memory kernels below.
     std::vector<int> lds_cand;
     std::vector<int> rest;
     // a pair that no team shape can hold alone (its team would exceed
@@ -619,11 +776,11 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
       bool ok = opt.kernel_variant == 0 && c.kind != 2 && c.nv > 0 && pstat[2 * p + 1] <= 2;
       if (ok) {
         ok = false;
-        for (int b = 0; b < lm_lds_team_shapes() && !ok; ++b) ok = lds_team_fits(p, b);
+        for (int b : lm_lds_team_order()) ok = ok || lds_team_fits(p, b);
       }
       (ok ? lds_cand : rest).push_back(p);
     }
-    // each pair takes the first shape (largest chunk) that holds it, so its
+    // each pair takes the first shape in preference order that holds it, so its
     // chunking -- and its result -- does not depend on the rest of the
     // batch; the teams are grouped by shape, one cooperative grid per
     // group, its LDS size the group's largest
@@ -632,8 +789,8 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
     std::vector<size_t> lds_of(nls, 0);
     for (size_t k = 0; k < lds_cand.size(); ++k) {
       const int p = lds_cand[k];
-      for (int b = 0; b < nls && pshape[k] < 0; ++b)
-        if (lds_team_fits(p, b)) {
+      for (int b : lm_lds_team_order())
+        if (pshape[k] < 0 && lds_team_fits(p, b)) {
           int T = 0, V = 0;
           if ((rc = lm_lds_team_shape(b, &T, &V))) return rc;
           const long long chcap = (long long)T * V, nvp = pairs[p].nv;
@@ -664,6 +821,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
         lteam_ch.push_back((int)((nvp + c - 1) / c));
         lteam_b.push_back(b);
         path[p] = MO_PATH_LDS_TEAM;
+        nwgs[p] = (int)c;
       }
     }
     const long long vpw = opt.team_vertices > 0 ? opt.team_vertices : (opt.kernel_variant == 2 ? bt : 4LL * bt);
@@ -674,17 +832,23 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
         teams.push_back(p);
         team_c.push_back((int)c);
         path[p] = MO_PATH_TEAM;
+        nwgs[p] = (int)c;
       } else {
         solo.push_back(p);
       }
     }
     if (!solo.empty()) {
-      MO_HIP(hipMemcpyAsync(d_plist + start[nb], solo.data(), solo.size() * sizeof(int), hipMemcpyHostToDevice, s));
+      if ((rc = chain_h2d(d_plist + start[nb], solo.data(), solo.size() * sizeof(int), s))) return rc;
       if ((rc = lm_solve_launch(d_pairs, d_plist + start[nb], (int)solo.size(), A, dopt, d_res, nvalloc, bt, s)))
         return rc;
     }
     // team arrays: the register/LDS teams first, then the global-memory teams
     const int nlt = (int)lteams.size(), nt = nlt + (int)teams.size();
+    if (nt > 0 && g_chain) {
+      // cooperative grids are not captured into batch graphs
+      MO_CHECK(g_chain->mode == ChainCtx::kRecord, MO_ERR_STATE, "batch graph capture: unexpected team launch");
+      g_chain->team_launch = true;
+    }
     if (nt > 0) {
       std::vector<int> all_t(lteams);
       all_t.insert(all_t.end(), teams.begin(), teams.end());
@@ -739,7 +903,10 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
           // the runtime refused the cooperative grid: this chunk's and the
           // remaining teams' pairs run one workgroup each (d_tp lists them)
           if ((rc = lm_solve_launch(d_pairs, d_tp + t0, nt - t0, A, dopt, d_res, nvalloc, bt, s))) return rc;
-          for (int t = t0; t < nt; ++t) path[all_t[t]] = MO_PATH_COOP_REFUSED;
+          for (int t = t0; t < nt; ++t) {
+            path[all_t[t]] = MO_PATH_COOP_REFUSED;
+            nwgs[all_t[t]] = 1;
+          }
           break;
         }
         if (rc) return rc;
@@ -757,35 +924,27 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
     MO_CHECK(h_abort == 0, MO_ERR_HIP,
              "LM team solve: a barrier wait exceeded its bound (results invalid; outputs not written)");
   }
-  for (int p = 0; p < np; ++p) {
-    const CorePair& c = pairs[p];
-    if (c.nv == 0) continue;
-    hipLaunchKernelGGL(k_finish, dim3((unsigned)((c.nv + 255) / 256)), dim3(256), 0, s, c.x0, (int)c.nv,
-                       u + vbase[p], c.sp, c.out);
-  }
+  hipLaunchKernelGGL(k_finish, io_grid, dim3(256), 0, s, d_io, u);
   MO_HIP(hipGetLastError());
+  if (ChainCtx* c = g_chain) {
+    if (c->mode == ChainCtx::kCapture) {
+      // the graph ends with the results in its pinned buffer; stats are
+      // filled after each replay
+      size_t off = 0;
+      if ((rc = pinned_take(c, np * sizeof(LMResultDev), &off))) return rc;
+      MO_HIP(hipMemcpyAsync(c->pinned + off, d_res, np * sizeof(LMResultDev), hipMemcpyDeviceToHost, s));
+      c->res_off = off;
+      c->res_np = np;
+      c->res_path = path;
+      c->res_wgs = nwgs;
+      return MO_OK;
+    }
+    c->pinned_bytes += al256c(np * sizeof(LMResultDev));
+  }
   std::vector<LMResultDev> h_res(np);
   MO_HIP(hipMemcpyAsync(h_res.data(), d_res, np * sizeof(LMResultDev), hipMemcpyDeviceToHost, s));
   MO_HIP(hipStreamSynchronize(s));
-  if (stats) {
-    for (int p = 0; p < np; ++p) {
-      const LMResultDev& r = h_res[p];
-      mo_lm_stats& o = stats[p];
-      o.initial_cost = r.initial_cost;
-      o.final_cost = r.final_cost;
-      o.vertex_cost = r.vertex_cost;
-      o.rigidity_cost = r.rigidity_cost;
-      o.iterations = r.iterations;
-      o.successful_steps = r.successful_steps;
-      o.unsuccessful_steps = r.unsuccessful_steps;
-      o.termination = r.termination;
-      o.pcg_iterations = r.pcg_iterations;
-      o.clock_start = r.clock_start;
-      o.clock_end = r.clock_end;
-      o.solver_path = path[p];
-      o.reserved = 0;
-    }
-  }
+  if (stats) fill_stats(h_res.data(), path.data(), nwgs.data(), np, stats);
   return MO_OK;
 }
 
@@ -1050,19 +1209,13 @@ MO_EXPORT int mo_deform_batch_noalloc(int32_t npairs, const int32_t* param_ids, 
   return rc;
 }
 
-MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, const int64_t* src_nv,
-                                    const int32_t* const* src_faces, const int64_t* src_nf, const double* const* tgt,
-                                    const int64_t* tgt_nv, const int32_t* const* tgt_faces, const int64_t* tgt_nf,
-                                    const int32_t* grid_resolution, const double* lambda, const mo_lm_options* opt,
-                                    double* const* out, mo_lm_stats* stats, void* stream) {
-  MO_CHECK(npairs >= 0, MO_ERR_ARG, "npairs < 0");
-  if (npairs == 0) return MO_OK;
-  MO_CHECK(src && src_nv && src_faces && src_nf && tgt && tgt_nv && tgt_faces && tgt_nf && grid_resolution &&
-               lambda && out,
-           MO_ERR_ARG, "null per-pair array");
-  mo_lm_options o;
-  if (opt) o = *opt; else mo_lm_default_options(&o);
-  hipStream_t s = as_stream(stream);
+namespace {
+
+int rigid_deform_batch_impl(int32_t npairs, const double* const* src, const int64_t* src_nv,
+                            const int32_t* const* src_faces, const int64_t* src_nf, const double* const* tgt,
+                            const int64_t* tgt_nv, const int32_t* const* tgt_faces, const int64_t* tgt_nf,
+                            const int32_t* grid_resolution, const double* lambda, const mo_lm_options& o,
+                            double* const* out, mo_lm_stats* stats, hipStream_t s) {
   // per-pair scratch: normalized target, (scale, pos), normalized source, FP32 grid
   std::vector<GridSpec> gspecs(npairs);
   size_t bytes = 512;
@@ -1128,6 +1281,166 @@ MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, co
     }
   }
   return check_err_flag(err, s, "target faces");
+}
+
+int check_rigid_args(int32_t npairs, const double* const* src, const int64_t* src_nv, const int32_t* const* src_faces,
+                     const int64_t* src_nf, const double* const* tgt, const int64_t* tgt_nv,
+                     const int32_t* const* tgt_faces, const int64_t* tgt_nf, const int32_t* grid_resolution,
+                     const double* lambda, double* const* out) {
+  MO_CHECK(npairs >= 0, MO_ERR_ARG, "npairs < 0");
+  if (npairs == 0) return MO_OK;
+  MO_CHECK(src && src_nv && src_faces && src_nf && tgt && tgt_nv && tgt_faces && tgt_nf && grid_resolution &&
+               lambda && out,
+           MO_ERR_ARG, "null per-pair array");
+  return MO_OK;
+}
+
+}  // namespace
+
+MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, const int64_t* src_nv,
+                                    const int32_t* const* src_faces, const int64_t* src_nf, const double* const* tgt,
+                                    const int64_t* tgt_nv, const int32_t* const* tgt_faces, const int64_t* tgt_nf,
+                                    const int32_t* grid_resolution, const double* lambda, const mo_lm_options* opt,
+                                    double* const* out, mo_lm_stats* stats, void* stream) {
+  int rc = check_rigid_args(npairs, src, src_nv, src_faces, src_nf, tgt, tgt_nv, tgt_faces, tgt_nf, grid_resolution,
+                            lambda, out);
+  if (rc || npairs == 0) return rc;
+  mo_lm_options o;
+  if (opt) o = *opt; else mo_lm_default_options(&o);
+  return rigid_deform_batch_impl(npairs, src, src_nv, src_faces, src_nf, tgt, tgt_nv, tgt_faces, tgt_nf,
+                                 grid_resolution, lambda, o, out, stats, as_stream(stream));
+}
+
+// ---------------------------------------------------------------------------
+// Batch graphs (see ChainCtx).
+
+struct mo_batch_graph {
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  char* pool = nullptr;
+  char* pinned = nullptr;
+  mo::ChainCtx ctx;
+  ~mo_batch_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (pool) (void)hipFree(pool);
+    if (pinned) (void)hipHostFree(pinned);
+  }
+};
+
+namespace {
+
+struct ChainScope {
+  explicit ChainScope(ChainCtx* c) { g_chain = c; }
+  ~ChainScope() { g_chain = nullptr; }
+};
+
+int batch_graph_finish(mo_batch_graph* g, mo_lm_stats* stats) {
+  const ChainCtx& c = g->ctx;
+  for (const auto& ck : c.checks) {
+    const std::vector<char>& want = c.reads[ck.second];
+    MO_CHECK(memcmp(g->pinned + ck.first, want.data(), want.size()) == 0, MO_ERR_STATE,
+             "batch graph: the batch's topology, faces or an error flag differ from the captured run "
+             "(results invalid; the inputs' faces must not change between replays)");
+  }
+  if (stats)
+    fill_stats(reinterpret_cast<const LMResultDev*>(g->pinned + c.res_off), c.res_path.data(), c.res_wgs.data(),
+               c.res_np, stats);
+  return MO_OK;
+}
+
+}  // namespace
+
+MO_EXPORT int mo_rigid_deform_batch_graph(int32_t npairs, const double* const* src, const int64_t* src_nv,
+                                          const int32_t* const* src_faces, const int64_t* src_nf,
+                                          const double* const* tgt, const int64_t* tgt_nv,
+                                          const int32_t* const* tgt_faces, const int64_t* tgt_nf,
+                                          const int32_t* grid_resolution, const double* lambda,
+                                          const mo_lm_options* opt, double* const* out, mo_lm_stats* stats,
+                                          void* stream, mo_batch_graph** graph) {
+  MO_CHECK(graph, MO_ERR_ARG, "graph must not be null");
+  *graph = nullptr;
+  int rc = check_rigid_args(npairs, src, src_nv, src_faces, src_nf, tgt, tgt_nv, tgt_faces, tgt_nf, grid_resolution,
+                            lambda, out);
+  if (rc) return rc;
+  MO_CHECK(npairs > 0, MO_ERR_ARG, "a batch graph needs at least one pair");
+  for (int p = 0; p < npairs; ++p)
+    MO_CHECK((const void*)out[p] != (const void*)src[p], MO_ERR_ARG,
+             "pair %d: out aliases src (a replay reads the sources again)", p);
+  mo_lm_options o;
+  if (opt) o = *opt; else mo_lm_default_options(&o);
+  MO_CHECK(o.timing == 0, MO_ERR_ARG, "options.timing must be 0 for a batch graph");
+  std::unique_ptr<mo_batch_graph> g(new mo_batch_graph);
+  MO_HIP(hipGetDevice(&g->device));
+  hipStream_t s = as_stream(stream);
+  // 1. the eager run (its results and stats are the call's), recording the
+  //    steering values and sizes
+  g->ctx.mode = ChainCtx::kRecord;
+  {
+    ChainScope scope(&g->ctx);
+    if ((rc = rigid_deform_batch_impl(npairs, src, src_nv, src_faces, src_nf, tgt, tgt_nv, tgt_faces, tgt_nf,
+                                      grid_resolution, lambda, o, out, stats, s)))
+      return rc;
+  }
+  MO_CHECK(!g->ctx.team_launch, MO_ERR_ARG,
+           "batch graph: a pair of this batch needs a cooperative multi-workgroup launch, which is not captured "
+           "(the eager run's results and stats are valid; use mo_rigid_deform_batch for such batches)");
+  // 2. the graph's own memory, then the capture on a private stream
+  ChainCtx& c = g->ctx;
+  c.pool_cap = c.pool_bytes;
+  c.pinned_cap = c.pinned_bytes;
+  MO_HIP(hipMalloc((void**)&g->pool, std::max<size_t>(c.pool_cap, 256)));
+  MO_HIP(hipHostMalloc((void**)&g->pinned, std::max<size_t>(c.pinned_cap, 256), hipHostMallocDefault));
+  c.pool = g->pool;
+  c.pinned = g->pinned;
+  c.mode = ChainCtx::kCapture;
+  hipStream_t cs = nullptr;
+  MO_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } sg{cs};
+  MO_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  {
+    ChainScope scope(&c);
+    rc = rigid_deform_batch_impl(npairs, src, src_nv, src_faces, src_nf, tgt, tgt_nv, tgt_faces, tgt_nf,
+                                 grid_resolution, lambda, o, out, nullptr, cs);
+  }
+  hipGraph_t captured = nullptr;
+  const hipError_t ec = hipStreamEndCapture(cs, &captured);
+  if (rc) {
+    if (captured) (void)hipGraphDestroy(captured);
+    return rc;
+  }
+  MO_CHECK(ec == hipSuccess && captured, MO_ERR_HIP, "batch graph: stream capture failed: %s", hipGetErrorString(ec));
+  g->graph = captured;
+  MO_CHECK(c.next_read == c.reads.size(), MO_ERR_STATE, "batch graph capture diverged from the recorded run");
+  MO_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  *graph = g.release();
+  return MO_OK;
+}
+
+MO_EXPORT int mo_batch_graph_launch(mo_batch_graph* graph, mo_lm_stats* stats, void* stream) {
+  MO_CHECK(graph && graph->exec, MO_ERR_ARG, "null batch graph");
+  int dev = 0;
+  MO_HIP(hipGetDevice(&dev));
+  MO_CHECK(dev == graph->device, MO_ERR_STATE, "batch graph was captured on device %d, current device is %d",
+           graph->device, dev);
+  hipStream_t s = as_stream(stream);
+  MO_HIP(hipGraphLaunch(graph->exec, s));
+  MO_HIP(hipStreamSynchronize(s));
+  return batch_graph_finish(graph, stats);
+}
+
+MO_EXPORT int mo_batch_graph_destroy(mo_batch_graph* graph) {
+  if (!graph) return MO_OK;
+  int dev = 0;
+  MO_HIP(hipGetDevice(&dev));
+  MO_CHECK(dev == graph->device, MO_ERR_STATE, "batch graph belongs to device %d, current device is %d",
+           graph->device, dev);
+  delete graph;
+  return MO_OK;
 }
 
 namespace {
@@ -1353,6 +1666,7 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
     }
   } team_drain{team_lock, s};
   int path = MO_PATH_ONE_WORKGROUP;
+  std::vector<int> rot_wgs(npairs, 1);   // workgroups that solve each problem
   DevPtr d_team;
   int* d_abort = nullptr;
   int rc;
@@ -1368,6 +1682,7 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
       tc[t] = (int)std::min<long long>(std::max<long long>((nv[order[t]] + o.team_vertices - 1) / o.team_vertices, 1),
                                        lm_team_max());
       nwg_all += tc[t];
+      rot_wgs[order[t]] = tc[t];
     }
     std::vector<int4> wg;
     std::vector<int> chunk_team{0}, chunk_wg{0};
@@ -1411,6 +1726,7 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
                                 hslots, rbt, s)))
           return rc;
         path = MO_PATH_COOP_REFUSED;
+        std::fill(rot_wgs.begin(), rot_wgs.end(), 1);
         break;
       }
       if (rc == kCoopRefused) return fail(MO_ERR_HIP, "DeformWithRot: cooperative launch refused after %zu chunks", k);
@@ -1454,7 +1770,7 @@ MO_EXPORT int mo_deform_rot_batch(int32_t npairs, const int32_t* param_ids, doub
       o2.clock_start = r[p].clock_start;
       o2.clock_end = r[p].clock_end;
       o2.solver_path = path;
-      o2.reserved = 0;
+      o2.workgroups = rot_wgs[p];
     }
   return MO_OK;
 }

@@ -33,6 +33,13 @@ int fail(int code, const char* fmt, ...);
   } while (0)
 
 // ---------------------------------------------------------------------------
+// Host->device copies of a batch's launch chain.  Outside graph capture this
+// is hipMemcpyAsync; while a batch graph is being captured (mo_capi.hip) the
+// bytes are first copied into pinned memory the graph owns, so that every
+// replay copies the same bytes.
+int chain_h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// ---------------------------------------------------------------------------
 // Float -> int conversion with x86 cvtt semantics (out of int32 range or NaN
 // -> INT_MIN).  The reference computes `int px = p * N` (uniformgrid.cc:20-22,
 // 87-89); on its x86 build an out-of-range value yields INT_MIN.

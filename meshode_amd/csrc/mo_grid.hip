@@ -505,7 +505,8 @@ int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipSt
     max_tiles = std::max(max_tiles, J.ntiles);
     max_bricks = std::max(max_bricks, bpa * bpa * bpa);
   }
-  MO_HIP(hipMemcpyAsync(d_jobs, jobs.data(), (size_t)n * sizeof(GridJob), hipMemcpyHostToDevice, stream));
+  int rc = chain_h2d(d_jobs, jobs.data(), (size_t)n * sizeof(GridJob), stream);
+  if (rc) return rc;
   const int T = 256;
   const unsigned fb = (unsigned)std::min<long long>((max_nf + T - 1) / T, 1024);
   hipLaunchKernelGGL(k_face_bins, dim3(fb, n), dim3(T), 0, stream, d_jobs, err_dev);
@@ -530,7 +531,8 @@ int xform_batch_launch(const XformJob* jobs, int n, bool compute_stats, void* wo
   XformJob* d_jobs = (XformJob*)workspace;
   long long max_n = 0;
   for (int q = 0; q < n; ++q) max_n = std::max(max_n, jobs[q].n);
-  MO_HIP(hipMemcpyAsync(d_jobs, jobs, (size_t)n * sizeof(XformJob), hipMemcpyHostToDevice, stream));
+  int rc = chain_h2d(d_jobs, jobs, (size_t)n * sizeof(XformJob), stream);
+  if (rc) return rc;
   if (compute_stats) hipLaunchKernelGGL(k_normalize_stats, dim3(n), dim3(1024), 0, stream, d_jobs);
   if (max_n > 0) {
     const unsigned bx = (unsigned)std::min<long long>((max_n * 3 + 255) / 256, 1024);

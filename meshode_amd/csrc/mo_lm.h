@@ -2,6 +2,8 @@
 // Deformer::DeformGraph, deformer.cc:18-92 and 370-442) on gfx950.
 #pragma once
 
+#include <vector>
+
 #include <mutex>
 
 #include "mo_common.h"
@@ -113,6 +115,9 @@ int lm_lds_launch(int bucket, const LMPairDesc* pairs_dev, const int* plist, int
 // abort_flag as for lm_team_launch.  Returns kCoopRefused like
 // lm_team_launch.
 int lm_lds_team_shapes();
+// Team shapes in preference order (MO_LM_TEAM_VPT overrides the default
+// largest-chunk-first order).
+const std::vector<int>& lm_lds_team_order();
 int lm_lds_team_max();   // largest team of the register/LDS team kernel
 int lm_lds_team_shape(int b, int* threads, int* vertices_per_thread);
 size_t lm_lds_team_bytes(int chunk, long long entries);

@@ -80,7 +80,7 @@ __device__ inline int opaque(int x) {
 template <int T, int VPT, class Team, class Entry>
 __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const LMArrays& A,
                                  const LMOptionsDev& opt, LMResultDev* __restrict__ res, long long totalV,
-                                 unsigned char* smem, double* red, int* scan_buf) {
+                                 unsigned char* smem, double* red, int* scan_buf, unsigned lds_bytes) {
   const double o_min_diag = opt.min_lm_diagonal, o_max_diag = opt.max_lm_diagonal;
   const float o_pcg_tol = opt.pcg_relative_tolerance;
   const int o_pcg_max = opt.pcg_max_iterations;
@@ -122,6 +122,10 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
     }
     int total;
     int off = block_exclusive_scan(mine, scan_buf, total);
+    // the launch's LDS was sized from the same adjacency; a mismatch (a
+    // replayed graph whose inputs' faces changed) leaves the pair unsolved
+    // and is reported by the host, never written past the allocation
+    if ((size_t)own * sizeof(float4) + (size_t)total * sizeof(Entry) > lds_bytes) return;
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
       csr[k] = ((unsigned)off << 16) | (unsigned)cnt[k];
@@ -413,14 +417,15 @@ struct LdsTeam {
 template <int T, int VPT>
 __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict__ pairs,
                                                     const int* __restrict__ plist, LMArrays A, LMOptionsDev opt,
-                                                    LMResultDev* __restrict__ res, long long totalV) {
+                                                    LMResultDev* __restrict__ res, long long totalV,
+                                                    unsigned lds_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[2 * 8 * kMaxWaves];
   __shared__ int scan_buf[kMaxWaves];
   const int pid = plist[blockIdx.x];
   const LMPairDesc P = pairs[pid];
   LdsSolo team{0, P.nv};
-  lds_solve<T, VPT, LdsSolo, unsigned short>(P, pid, team, A, opt, res, totalV, smem, red, scan_buf);
+  lds_solve<T, VPT, LdsSolo, unsigned short>(P, pid, team, A, opt, res, totalV, smem, red, scan_buf, lds_bytes);
 }
 
 // wg[b] = (team, rank, team size, chunk): team t solves pair plist[t];
@@ -430,7 +435,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds_team(const LMPairDesc* __res
                                                          const int* __restrict__ plist, const int4* __restrict__ wg,
                                                          LMArrays A, LMOptionsDev opt, LMResultDev* __restrict__ res,
                                                          long long totalV, unsigned* cnt, double* part,
-                                                         int* abort_flag, long long limit) {
+                                                         int* abort_flag, long long limit, unsigned lds_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[2 * 8 * kMaxWaves];
   __shared__ int scan_buf[kMaxWaves];
@@ -440,7 +445,7 @@ __global__ void __launch_bounds__(T) k_lm_solve_lds_team(const LMPairDesc* __res
   const int lo = min(P.nv, m.y * m.w), hi = min(P.nv, lo + m.w);
   LdsTeam team{CoherentTeam{m.y, m.z, cnt + (size_t)m.x * 32, part + (size_t)m.x * 2 * kMaxTeam * 8, abort_flag, limit, 0u, 0},
                lo, hi};
-  lds_solve<T, VPT, LdsTeam, unsigned>(P, pid, team, A, opt, res, totalV, smem, red, scan_buf);
+  lds_solve<T, VPT, LdsTeam, unsigned>(P, pid, team, A, opt, res, totalV, smem, red, scan_buf, lds_bytes);
 }
 
 // Team launch of the register/LDS kernel: nwg workgroups (wg_dev entries)
@@ -461,8 +466,10 @@ int launch_team_bucket(const LMPairDesc* pairs_dev, const int* plist, const int4
   LMArrays a = A;
   LMOptionsDev o = opt;
   long long tv = totalV;
-  void* args[] = {(void*)&pairs_dev, (void*)&plist, (void*)&wg_dev, (void*)&a, (void*)&o, (void*)&res, (void*)&tv,
-                  (void*)&cnt, (void*)&part, (void*)&abort_flag, (void*)&limit};
+  unsigned lb = (unsigned)lds_bytes;
+  void* args[] = {(void*)&pairs_dev, (void*)&plist, (void*)&wg_dev, (void*)&a,           (void*)&o,
+                  (void*)&res,       (void*)&tv,    (void*)&cnt,    (void*)&part,        (void*)&abort_flag,
+                  (void*)&limit,     (void*)&lb};
   if (hipLaunchCooperativeKernel((const void*)kern, dim3(nwg), dim3(T), args, (unsigned)lds_bytes, s) != hipSuccess) {
     (void)hipGetLastError();   // the runtime refused the cooperative grid: nothing was launched
     return kCoopRefused;
@@ -482,24 +489,47 @@ int team_bucket_capacity(size_t lds_bytes) {
   return per * cus;
 }
 
-__global__ void k_chunk_entries(const LMPairDesc* __restrict__ pairs, const int2* __restrict__ adj, int np,
-                                int c0, int c1, int c2, int* __restrict__ out) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
+// Per pair and team shape b: the largest chunk's adjacency entries (the sum
+// of its vertices' neighbour counts; adjacency rows may have unused slots
+// after merging, so row offsets do not give it).  One workgroup per pair.
+__global__ void __launch_bounds__(256) k_chunk_entries(const LMPairDesc* __restrict__ pairs,
+                                                       const int2* __restrict__ adj, int np, int c0, int c1, int c2,
+                                                       int* __restrict__ out) {
+  __shared__ int acc[3][kMaxTeam];
+  const int p = blockIdx.x;
   const int nv = pairs[p].nv;
   const long long vb = pairs[p].vbase;
   const int caps[3] = {c0, c1, c2};
+  int c[3], ch[3];
   for (int b = 0; b < 3; ++b) {
-    int mx = 0;
-    if (nv > 0) {
-      const int c = (nv + caps[b] - 1) / caps[b], ch = (nv + c - 1) / c;
-      for (int r = 0; r < c; ++r) {
-        const int lo = r * ch, hi = min(nv, lo + ch);
-        if (lo >= hi) continue;
-        const int2 a = adj[vb + hi - 1], z = adj[vb + lo];
-        mx = max(mx, a.x + a.y - z.x);
+    c[b] = nv > 0 ? (nv + caps[b] - 1) / caps[b] : 0;
+    ch[b] = c[b] > 0 ? (nv + c[b] - 1) / c[b] : 1;
+  }
+  for (int k = threadIdx.x; k < 3 * kMaxTeam; k += blockDim.x) acc[k / kMaxTeam][k % kMaxTeam] = 0;
+  __syncthreads();
+  int cur[3] = {-1, -1, -1}, sum[3] = {0, 0, 0};
+  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+    const int n = adj[vb + i].y;
+    for (int b = 0; b < 3; ++b) {
+      if (c[b] > kMaxTeam) continue;
+      const int r = i / ch[b];
+      if (r != cur[b]) {
+        if (cur[b] >= 0) atomicAdd(&acc[b][cur[b]], sum[b]);
+        cur[b] = r;
+        sum[b] = 0;
       }
+      sum[b] += n;
     }
+  }
+  for (int b = 0; b < 3; ++b)
+    if (cur[b] >= 0) atomicAdd(&acc[b][cur[b]], sum[b]);
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int b = threadIdx.x;
+    int mx = 0;
+    if (c[b] > kMaxTeam) mx = INT_MAX;   // no team of this shape holds the pair
+    else
+      for (int r = 0; r < c[b]; ++r) mx = max(mx, acc[b][r]);
     out[3 * p + b] = mx;
   }
 }
@@ -509,7 +539,7 @@ int launch_bucket(const LMPairDesc* pairs, const int* plist, int count, const LM
                   LMResultDev* res, long long totalV, size_t lds_bytes, hipStream_t s) {
   auto kern = k_lm_solve_lds<T, VPT>;
   MO_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
-  hipLaunchKernelGGL(kern, dim3(count), dim3(T), lds_bytes, s, pairs, plist, A, opt, res, totalV);
+  hipLaunchKernelGGL(kern, dim3(count), dim3(T), lds_bytes, s, pairs, plist, A, opt, res, totalV, (unsigned)lds_bytes);
   MO_HIP(hipGetLastError());
   return MO_OK;
 }
@@ -599,6 +629,33 @@ static const int kTeamVPT[] = {4, 3, 2};
 
 int lm_lds_team_shapes() { return 3; }
 
+// Preference order of the team shapes: 1024x3, then 1024x4 (only for pairs
+// whose 3072-vertex team would exceed kMaxTeam), then 1024x2 (chunks whose
+// adjacency does not fit the LDS otherwise) -- 1024x3 solved 12k-100k vertex
+// pairs as fast as 1024x4 and 4% faster on 41k-vertex icospheres (its
+// register allocation spills 6 values against 33) -- or
+// MO_LM_TEAM_VPT="v,v,..." (vertices per thread, e.g. "2,4") read once.
+const std::vector<int>& lm_lds_team_order() {
+  static const std::vector<int> order = [] {
+    std::vector<int> o;
+    if (const char* env = getenv("MO_LM_TEAM_VPT")) {
+      std::string e(env);
+      size_t pos = 0;
+      while (pos <= e.size()) {
+        const size_t comma = e.find(',', pos);
+        const int v = atoi(e.substr(pos, comma == std::string::npos ? std::string::npos : comma - pos).c_str());
+        for (int b = 0; b < 3; ++b)
+          if (kTeamVPT[b] == v) o.push_back(b);
+        if (comma == std::string::npos) break;
+        pos = comma + 1;
+      }
+    }
+    if (o.empty()) o = {1, 0, 2};
+    return o;
+  }();
+  return order;
+}
+
 int lm_lds_team_max() { return kMaxTeam; }
 
 int lm_lds_team_shape(int b, int* threads, int* vpt) {
@@ -623,7 +680,7 @@ int lm_lds_team_capacity(int b, size_t lds_bytes) {
 
 int lm_lds_chunk_entries(const LMPairDesc* pairs_dev, const int2* adj, int np, int* out_dev, hipStream_t s) {
   if (np <= 0) return MO_OK;
-  hipLaunchKernelGGL(k_chunk_entries, dim3((np + 255) / 256), dim3(256), 0, s, pairs_dev, adj, np,
+  hipLaunchKernelGGL(k_chunk_entries, dim3(np), dim3(256), 0, s, pairs_dev, adj, np,
                      kTeamT[0] * kTeamVPT[0], kTeamT[1] * kTeamVPT[1], kTeamT[2] * kTeamVPT[2], out_dev);
   MO_HIP(hipGetLastError());
   return MO_OK;

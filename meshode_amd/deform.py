@@ -154,6 +154,35 @@ class PairBatch:
         _lib.call("mo_rigid_deform_batch", *self._args, C.byref(opt), self._out_ptrs, stats, _stream())
         return [s.as_dict() for s in stats]
 
+    def capture(self, options=None):
+        """mo_rigid_deform_batch_graph: solve the batch once (returns its
+        stats, like run) and capture the whole launch chain into a HIP graph
+        that replay() re-runs on the current contents of the same inputs."""
+        self.release()
+        opt = options or default_options()
+        stats = (_lib.LMStats * self.n)()
+        g = C.c_void_p()
+        _lib.call("mo_rigid_deform_batch_graph", *self._args, C.byref(opt), self._out_ptrs, stats, _stream(),
+                  C.byref(g))
+        self._graph = g
+        return [s.as_dict() for s in stats]
+
+    def replay(self):
+        """mo_batch_graph_launch: one replay of the captured chain (waits for
+        it); returns the replay's stats."""
+        if getattr(self, "_graph", None) is None:
+            raise RuntimeError("no captured graph: call capture() first")
+        stats = (_lib.LMStats * self.n)()
+        _lib.call("mo_batch_graph_launch", self._graph, stats, _stream())
+        return [s.as_dict() for s in stats]
+
+    def release(self):
+        """mo_batch_graph_destroy of the captured graph, if any."""
+        g = getattr(self, "_graph", None)
+        if g is not None:
+            self._graph = None
+            _lib.call("mo_batch_graph_destroy", g)
+
 
 def rigid_deform_batch(pairs, grid_resolution=64, lambdas=1.0, options=None):
     """Run the rigid_deform pipeline on [(src_V, src_F, tgt_V, tgt_F), ...].

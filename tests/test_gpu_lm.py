@@ -307,8 +307,8 @@ def test_dense_graph_beyond_lds_offsets(gpu, oracle):
 
 
 def test_lds_team_barrier_timeout_is_reported(gpu, monkeypatch):
-    """The register/LDS team kernel (a 12000-vertex pair: a team of 3
-    workgroups) with every barrier wait forced to give up at once
+    """The register/LDS team kernel (a 12000-vertex pair: a team of 4
+    workgroups of 3000 vertices) with every barrier wait forced to give up at once
     (MO_LM_TEAM_WAIT_US=-1, from the host): the kernel finishes, the call
     raises and leaves the caller's vertices untouched; the next call solves
     bit-identically to one made before."""
@@ -334,14 +334,14 @@ def test_lds_team_barrier_timeout_is_reported(gpu, monkeypatch):
 
 def test_lds_team_kernel_energy_parity(gpu, oracle):
     """A 12000-vertex box union at 48^3 on the register/LDS team kernel
-    (3 workgroups of 4096 vertices, neighbours across chunks read from the
+    (4 workgroups of 3000 vertices, neighbours across chunks read from the
     published array) against the oracle's exact-step LM."""
     _say("[lm] case=box12000_n48 register/LDS team")
     pair = synth.box_union_pair(6, n_verts=12000)
     so, eg, st = _parity(oracle, pair, 48)
     _say(f"[lm] lds team: oracle E={so['final_cost']:.8g}  gpu E={eg:.8g} it={st['iterations']} "
          f"path={st['solver_path_name']}")
-    assert st["solver_path_name"] == "lds_team"
+    assert st["solver_path_name"] == "lds_team" and st["workgroups"] == 4
     assert abs(eg - so["final_cost"]) <= ENERGY_RTOL * so["final_cost"]
 
 
