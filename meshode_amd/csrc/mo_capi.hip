@@ -619,7 +619,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
                  al(std::max<long long>(2 * nedges_tot, 1) * sizeof(unsigned)) +
                  al(std::max<long long>(2 * nedges_tot, 1) * sizeof(float)) + al(topo_workspace_bytes(nvalloc)) +
                  al(np * sizeof(LMPairDesc)) + al(np * sizeof(TopoPairDesc)) + al(np * sizeof(LMResultDev)) +
-                 al(2 * np * sizeof(int)) + al(np * sizeof(int)) + al(3 * np * sizeof(int)) +
+                 al(2 * np * sizeof(int)) + al(np * sizeof(int)) + al(4 * np * sizeof(int)) +
                  al(np * sizeof(PairIO)) + 512;
   MO_CHECK(np <= 65535, MO_ERR_ARG, "at most 65535 pairs per batch (got %d)", np);
   Arena ar;
@@ -638,7 +638,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   LMResultDev* d_res = ar.take<LMResultDev>(np);
   int* d_pstat = ar.take<int>(2 * np);
   int* d_plist = ar.take<int>(np);
-  int* d_chunk = ar.take<int>(3 * np);
+  int* d_chunk = ar.take<int>(4 * np);
   PairIO* d_io = ar.take<PairIO>(np);
   int* err = ar.take<int>(1);
 
@@ -670,10 +670,10 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   MO_HIP(hipGetLastError());
   // per pair and register/LDS team shape: the largest chunk's adjacency entries
   if ((rc = lm_lds_chunk_entries(d_pairs, adj, np, d_chunk, s))) return rc;
-  std::vector<int> pstat(2 * np), hchunk(3 * np);
+  std::vector<int> pstat(2 * np), hchunk(4 * np);
   int herr = 0;
   {
-    const ChainRead rd[3] = {{hchunk.data(), d_chunk, 3 * np * sizeof(int)},
+    const ChainRead rd[3] = {{hchunk.data(), d_chunk, 4 * np * sizeof(int)},
                              {pstat.data(), d_pstat, 2 * np * sizeof(int)},
                              {&herr, err, sizeof(int)}};
     if ((rc = chain_read(rd, 3, s))) return rc;
@@ -695,16 +695,17 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
     const CorePair& c = pairs[p];
     int b = -1;
     // the register/LDS kernel packs a vertex's LDS adjacency offset and count
-    // into 16 bits each and a neighbour index into 15: a pair qualifies only
-    // when all of them fit
+    // into 16 bits each and a neighbour index (or the zero slot, index nv)
+    // into 15: a pair qualifies only when all of them fit
+    const int padded = hchunk[4 * p + 3];   // entries with every count rounded up to even
     if (opt.kernel_variant == 0 && c.kind != 2 && c.nv > 0 && c.nv <= 32767 && pstat[2 * p + 1] <= 2 &&
-        pstat[2 * p] < 65536) {
+        padded < 65536) {
       b = lm_lds_bucket_for((int)c.nv);
-      if (b >= 0 && lm_lds_bytes((int)c.nv, pstat[2 * p]) > lds_cap) b = -1;
+      if (b >= 0 && lm_lds_bytes((int)c.nv, padded) > lds_cap) b = -1;
     }
     if (b < 0) b = nb;
     bucket[b].push_back(p);
-    if (b < nb) bucket_lds[b] = std::max(bucket_lds[b], lm_lds_bytes((int)c.nv, pstat[2 * p]));
+    if (b < nb) bucket_lds[b] = std::max(bucket_lds[b], lm_lds_bytes((int)c.nv, padded));
   }
   std::vector<int> plist;
   std::vector<int> start(nb + 2, 0);
@@ -768,7 +769,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
       if (lm_lds_team_shape(b, &T, &V)) return false;
       const long long chcap = (long long)T * V, nvp = pairs[p].nv;
       const long long c = (nvp + chcap - 1) / chcap, ch = (nvp + c - 1) / c;
-      const long long ent = hchunk[3 * p + b];
+      const long long ent = hchunk[4 * p + b];
       return c <= lm_lds_team_max() && ent < 65536 && lm_lds_team_bytes((int)ch, ent) <= lds_cap;
     };
     for (int p : bucket[nb]) {
@@ -796,7 +797,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
           const long long chcap = (long long)T * V, nvp = pairs[p].nv;
           const long long c = (nvp + chcap - 1) / chcap, ch = (nvp + c - 1) / c;
           pshape[k] = b;
-          lds_of[b] = std::max(lds_of[b], lm_lds_team_bytes((int)ch, hchunk[3 * p + b]));
+          lds_of[b] = std::max(lds_of[b], lm_lds_team_bytes((int)ch, hchunk[4 * p + b]));
         }
     }
     std::vector<int> cap_of(nls, 0);

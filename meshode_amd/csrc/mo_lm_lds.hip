@@ -60,6 +60,14 @@ __device__ inline int block_exclusive_scan(int x, int* buf, int& total) {
   return base + v - x;
 }
 
+// x, y, z of an LDS vertex record (w, the vertex's adjacency word, stays).
+typedef float mo_v3f __attribute__((ext_vector_type(3)));
+__device__ inline void store_xyz(float4* p, float4 v) {
+  mo_v3f t;
+  t.x = v.x; t.y = v.y; t.z = v.z;
+  *reinterpret_cast<mo_v3f*>(p) = t;
+}
+
 // An index the optimizer cannot see through: per-vertex addresses are then
 // formed at each use (base + index) instead of being hoisted out of the
 // solver loops, where they would hold dozens of registers for the whole
@@ -99,13 +107,20 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
   float4* __restrict__ S = A.work + (size_t)gS * totalV + vb;
   float4* __restrict__ DG = A.work + (size_t)gDG * totalV + vb;
   float4* __restrict__ ZG = A.work + (size_t)gZ * totalV + vb;   // team: published vectors
+  // paired layout (one workgroup per pair): a zero vector at VEC[own], and
+  // every vertex's entry list padded to even length with it, so the PCG
+  // product reads two entries per 32-bit LDS word
+  constexpr bool kPaired = Team::kPaired;
+  constexpr int kZeroSlots = kPaired ? 1 : 0;
   float4* VEC = (float4*)smem;                                      // own vertices, local index
-  Entry* LCSR = (Entry*)(smem + (size_t)own * sizeof(float4));
+  Entry* LCSR = (Entry*)(smem + (size_t)(own + kZeroSlots) * sizeof(float4));
   int sel = 0;
   const long long clock_start = wall_clock64();
 
   // ---- adjacency -> LDS (packed, compacted by a block scan) -------------
-  unsigned csr[VPT];   // (LDS offset << 16) | count
+  // each own vertex's (LDS offset << 16) | count lives in VEC[il].w for the
+  // whole solve (publications write x, y, z only), so it costs no register
+  float uniform_m = 0.f;   // paired layout: the pair's multiplicity when all entries share it, else 0
   {
     int cnt[VPT], gst[VPT], mine = 0;
 #pragma unroll
@@ -117,7 +132,7 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
         const int2 a = ADJ[i];
         gst[k] = a.x;
         cnt[k] = a.y;
-        mine += a.y;
+        mine += kPaired ? a.y + (a.y & 1) : a.y;
       }
     }
     int total;
@@ -125,24 +140,36 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
     // the launch's LDS was sized from the same adjacency; a mismatch (a
     // replayed graph whose inputs' faces changed) leaves the pair unsolved
     // and is reported by the host, never written past the allocation
-    if ((size_t)own * sizeof(float4) + (size_t)total * sizeof(Entry) > lds_bytes) return;
+    if ((size_t)(own + kZeroSlots) * sizeof(float4) + (size_t)total * sizeof(Entry) > lds_bytes) return;
+    int has1 = 0, has2 = 0;
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
-      csr[k] = ((unsigned)off << 16) | (unsigned)cnt[k];
+      {
+        const int il = opaque(tid + k * T);
+        if (il < own) reinterpret_cast<float*>(VEC + il)[3] = __uint_as_float(((unsigned)off << 16) | (unsigned)cnt[k]);
+      }
       for (int e = 0; e < cnt[k]; ++e) {
         const unsigned en = NB[gst[k] + e];
+        has1 |= (en >> 24) == 1u;
+        has2 |= (en >> 24) == 2u;
         LCSR[off + e] = (Entry)((nb_index(en) << 1) | ((en >> 24) - 1u));
       }
-      off += cnt[k];
+      if (kPaired && (cnt[k] & 1)) LCSR[off + cnt[k]] = (Entry)((unsigned)own << 1);
+      off += kPaired ? cnt[k] + (cnt[k] & 1) : cnt[k];
+    }
+    if constexpr (kPaired) {
+      const int any1 = __syncthreads_or(has1), any2 = __syncthreads_or(has2);
+      uniform_m = any1 && !any2 ? 1.f : (any2 && !any1 ? 2.f : 0.f);
+      if (tid == 0) VEC[own] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
 
   // ---- per-vertex register state ----------------------------------------
-  // g: distance gradient; r: PCG residual; p: direction; q = A p;
-  // d = lambda^2 deg + LM diagonal / radius (the diagonal of A without
-  // g g^T, also the preconditioner's diagonal); e: the step delta.
+  // g: distance gradient; p: direction; q = A p; d = lambda^2 deg + LM
+  // diagonal / radius (the diagonal of A without g g^T, also the
+  // preconditioner's diagonal); e: the step delta.  The PCG residual r is
+  // not stored: r = M z for the z = M^-1 r held in LDS.
   float gx[VPT], gy[VPT], gz[VPT];
-  float rx[VPT], ry[VPT], rz_[VPT];
   float px[VPT], py[VPT], pz[VPT];
   float qx[VPT], qy[VPT], qz[VPT];
   float dx[VPT], dy[VPT], dz[VPT];
@@ -161,16 +188,39 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
   };
 
   // sum over the vertex's neighbours of multiplicity * VEC[neighbour].xyz
-  auto gather = [&](int k, float& sx, float& sy, float& sz) {
+  auto gather = [&](int cs, float& sx, float& sy, float& sz) {
     sx = 0.f; sy = 0.f; sz = 0.f;
-    const int cs = opaque((int)csr[k]);
     const int o = (int)((unsigned)cs >> 16), c = cs & 0xFFFF;
-    #pragma unroll 1
-    for (int q = 0; q < c; ++q) {
-      const unsigned w = LCSR[o + q];
-      const float m = (float)((w & 1u) + 1u);
-      const float4 vj = team.neighbour(VEC, ZG, w >> 1);
-      sx += m * vj.x; sy += m * vj.y; sz += m * vj.z;
+    if constexpr (kPaired) {
+      // two entries per LDS word (offsets are even; an odd list ends with
+      // the zero slot), two independent neighbour reads in flight
+      const unsigned* __restrict__ L2 = reinterpret_cast<const unsigned*>(LCSR) + (o >> 1);
+      const int n2 = (c + 1) >> 1;
+      if (uniform_m != 0.f) {
+        #pragma unroll 1
+        for (int q = 0; q < n2; ++q) {
+          const unsigned w = L2[q];
+          const float4 a = VEC[__builtin_amdgcn_ubfe(w, 1, 15)], b = VEC[__builtin_amdgcn_ubfe(w, 17, 15)];
+          sx += a.x + b.x; sy += a.y + b.y; sz += a.z + b.z;
+        }
+        sx *= uniform_m; sy *= uniform_m; sz *= uniform_m;
+      } else {
+        #pragma unroll 1
+        for (int q = 0; q < n2; ++q) {
+          const unsigned w = L2[q];
+          const float4 a = VEC[__builtin_amdgcn_ubfe(w, 1, 15)], b = VEC[__builtin_amdgcn_ubfe(w, 17, 15)];
+          const float ma = (float)((w & 1u) + 1u), mb = (float)(((w >> 16) & 1u) + 1u);
+          sx += ma * a.x + mb * b.x; sy += ma * a.y + mb * b.y; sz += ma * a.z + mb * b.z;
+        }
+      }
+    } else {
+      #pragma unroll 1
+      for (int q = 0; q < c; ++q) {
+        const unsigned w = LCSR[o + q];
+        const float m = (float)((w & 1u) + 1u);
+        const float4 vj = team.neighbour(VEC, ZG, w >> 1);
+        sx += m * vj.x; sy += m * vj.y; sz += m * vj.z;
+      }
     }
   };
 
@@ -188,12 +238,12 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
       if (il >= own) continue;
       const float4 x0 = X0[i];
       const float4 ui = VEC[il];
+      const int cs = __float_as_int(ui.w);
       const JetF J = lookup_value_grad(grid, N, x0.x + ui.x, x0.y + ui.y, x0.z + ui.z);
       gx[k] = J.v0; gy[k] = J.v1; gz[k] = J.v2;
       const float ldk = l2 * x0.w;
       float sx = 0.f, sy = 0.f, sz = 0.f;
       double e = 0.0;
-      const int cs = opaque((int)csr[k]);
       const int o = (int)((unsigned)cs >> 16), c = cs & 0xFFFF;
       #pragma unroll 1
       for (int q = 0; q < c; ++q) {
@@ -234,7 +284,6 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
       px[k] = py[k] = pz[k] = 0.f;
       qx[k] = qy[k] = qz[k] = 0.f;
       ex[k] = ey[k] = ez[k] = 0.f;
-      rx[k] = ry[k] = rz_[k] = 0.f;
       dx[k] = dy[k] = dz[k] = 1.f;
       if (il >= own) continue;
       const float4 s = S[i];
@@ -262,12 +311,12 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
       dy[k] = ldk + fminf(dg.y * __builtin_amdgcn_rcpf(s.y * s.y) * finv_radius, 1e30f);
       dz[k] = ldk + fminf(dg.z * __builtin_amdgcn_rcpf(s.z * s.z) * finv_radius, 1e30f);
       const float4 gr = GR[i];
-      rx[k] = -gr.x; ry[k] = -gr.y; rz_[k] = -gr.z;
+      const float r0 = -gr.x, r1 = -gr.y, r2 = -gr.z;
       float z0, z1, z2;
-      precond(k, rx[k], ry[k], rz_[k], z0, z1, z2);
+      precond(k, r0, r1, r2, z0, z1, z2);
       team.publish(VEC, ZG, il, i, make_float4(z0, z1, z2, 0.f));
-      v[0] += (double)(rx[k] * z0 + ry[k] * z1 + rz_[k] * z2);
-      v[1] += (double)(rx[k] * rx[k] + ry[k] * ry[k] + rz_[k] * rz_[k]);
+      v[0] += (double)(r0 * z0 + r1 * z1 + r2 * z2);
+      v[1] += (double)(r0 * r0 + r1 * r1 + r2 * r2);
     }
     team.template reduce<2, 0>(v, red, sel);   // also publishes z
     double rz = v[0];
@@ -284,7 +333,7 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
         if (il >= own) continue;
         const float4 zi = VEC[il];
         float sx, sy, sz;
-        gather(k, sx, sy, sz);
+        gather(__float_as_int(zi.w), sx, sy, sz);
         const float gzz = gx[k] * zi.x + gy[k] * zi.y + gz[k] * zi.z;
         const float wx = gx[k] * gzz + dx[k] * zi.x - l2 * sx;
         const float wy = gy[k] * gzz + dy[k] * zi.y - l2 * sy;
@@ -303,12 +352,18 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
         const int il = opaque(tid + k * T), i = lo + il;
         if (il >= own) continue;
         ex[k] += fa * px[k]; ey[k] += fa * py[k]; ez[k] += fa * pz[k];
-        rx[k] -= fa * qx[k]; ry[k] -= fa * qy[k]; rz_[k] -= fa * qz[k];
+        // the residual is not kept: r = M z for the z still published
+        // (M = g g^T + diag(d), the preconditioner), then r -= alpha q
+        const float4 zo = VEC[il];
+        const float gzo = gx[k] * zo.x + gy[k] * zo.y + gz[k] * zo.z;
+        const float r0 = gx[k] * gzo + dx[k] * zo.x - fa * qx[k];
+        const float r1 = gy[k] * gzo + dy[k] * zo.y - fa * qy[k];
+        const float r2 = gz[k] * gzo + dz[k] * zo.z - fa * qz[k];
         float z0, z1, z2;
-        precond(k, rx[k], ry[k], rz_[k], z0, z1, z2);
+        precond(k, r0, r1, r2, z0, z1, z2);
         team.publish(VEC, ZG, il, i, make_float4(z0, z1, z2, 0.f));
-        w2[0] += (double)(rx[k] * z0 + ry[k] * z1 + rz_[k] * z2);
-        w2[1] += (double)(rx[k] * rx[k] + ry[k] * ry[k] + rz_[k] * rz_[k]);
+        w2[0] += (double)(r0 * z0 + r1 * z1 + r2 * z2);
+        w2[1] += (double)(r0 * r0 + r1 * r1 + r2 * r2);
       }
       team.template reduce<2, 0>(w2, red, sel);   // also publishes z
       ++it;
@@ -340,11 +395,11 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
       const float gd = gx[k] * ex[k] + gy[k] * ey[k] + gz[k] * ez[k];
       v[0] += (double)gd * ((double)dres + 0.5 * (double)gd);
       const float4 wv = VEC[il];
+      const int cs = __float_as_int(wv.w);
       const float wx = wv.x, wy = wv.y, wz = wv.z;
       const float dc = lookup_f32(grid, N, x0.x + wx, x0.y + wy, x0.z + wz);
       v[1] += 0.5 * (double)dc * (double)dc;
       double e = 0.0;
-      const int cs = opaque((int)csr[k]);
       const int o = (int)((unsigned)cs >> 16), c = cs & 0xFFFF;
       #pragma unroll 1
       for (int q = 0; q < c; ++q) {
@@ -374,7 +429,10 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
       const int il = opaque(tid + k * T), i = lo + il;
-      if (il < own) U[i] = VEC[il];
+      if (il < own) {
+        const float4 v = VEC[il];
+        U[i] = make_float4(v.x, v.y, v.z, 0.f);
+      }
     }
   };
   lm_trust_region_loop(opt, eval_tracked, solve, candidate, accept, team.leader() ? res + pid : nullptr,
@@ -383,13 +441,14 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
 
 // One workgroup per pair.
 struct LdsSolo {
+  static constexpr bool kPaired = true;
   int lo, hi;
   __device__ bool leader() const { return true; }
   __device__ void sync() { __syncthreads(); }
   template <int K, int M>
   __device__ void reduce(double (&v)[K + M], double* red, int& sel) { block_reduce_dpp<K, M>(v, red, sel); }
   __device__ float4 neighbour(const float4* VEC, const float4* ZG, unsigned j) const { return VEC[j]; }
-  __device__ void publish(float4* VEC, float4* ZG, int il, int i, float4 v) const { VEC[il] = v; }
+  __device__ void publish(float4* VEC, float4* ZG, int il, int i, float4 v) const { store_xyz(VEC + il, v); }
 };
 
 // A team of workgroups per pair (co-resident: cooperative launch).
@@ -398,6 +457,7 @@ struct LdsSolo {
 // reductions are CoherentTeam's, so no barrier writes back or invalidates
 // the L2.
 struct LdsTeam {
+  static constexpr bool kPaired = false;
   CoherentTeam g;
   int lo, hi;
   __device__ bool leader() const { return g.leader(); }
@@ -409,7 +469,7 @@ struct LdsTeam {
     return jl < (unsigned)(hi - lo) ? VEC[jl] : load_coherent(ZG + j);
   }
   __device__ void publish(float4* VEC, float4* ZG, int il, int i, float4 v) const {
-    VEC[il] = v;
+    store_xyz(VEC + il, v);
     store_coherent(ZG + i, v);
   }
 };
@@ -489,9 +549,11 @@ int team_bucket_capacity(size_t lds_bytes) {
   return per * cus;
 }
 
-// Per pair and team shape b: the largest chunk's adjacency entries (the sum
-// of its vertices' neighbour counts; adjacency rows may have unused slots
-// after merging, so row offsets do not give it).  One workgroup per pair.
+// Per pair: for each team shape b, the largest chunk's adjacency entries
+// (the sum of its vertices' neighbour counts; adjacency rows may have unused
+// slots after merging, so row offsets do not give it), then the one-
+// workgroup kernel's LDS entries (every vertex's count rounded up to even).
+// out[4 p + 0..3].  One workgroup per pair.
 __global__ void __launch_bounds__(256) k_chunk_entries(const LMPairDesc* __restrict__ pairs,
                                                        const int2* __restrict__ adj, int np, int c0, int c1, int c2,
                                                        int* __restrict__ out) {
@@ -507,9 +569,12 @@ __global__ void __launch_bounds__(256) k_chunk_entries(const LMPairDesc* __restr
   }
   for (int k = threadIdx.x; k < 3 * kMaxTeam; k += blockDim.x) acc[k / kMaxTeam][k % kMaxTeam] = 0;
   __syncthreads();
-  int cur[3] = {-1, -1, -1}, sum[3] = {0, 0, 0};
+  __shared__ int padded;
+  if (threadIdx.x == 0) padded = 0;
+  int cur[3] = {-1, -1, -1}, sum[3] = {0, 0, 0}, mine = 0;
   for (int i = threadIdx.x; i < nv; i += blockDim.x) {
     const int n = adj[vb + i].y;
+    mine += n + (n & 1);
     for (int b = 0; b < 3; ++b) {
       if (c[b] > kMaxTeam) continue;
       const int r = i / ch[b];
@@ -523,6 +588,7 @@ __global__ void __launch_bounds__(256) k_chunk_entries(const LMPairDesc* __restr
   }
   for (int b = 0; b < 3; ++b)
     if (cur[b] >= 0) atomicAdd(&acc[b][cur[b]], sum[b]);
+  atomicAdd(&padded, mine);
   __syncthreads();
   if (threadIdx.x < 3) {
     const int b = threadIdx.x;
@@ -530,8 +596,9 @@ __global__ void __launch_bounds__(256) k_chunk_entries(const LMPairDesc* __restr
     if (c[b] > kMaxTeam) mx = INT_MAX;   // no team of this shape holds the pair
     else
       for (int r = 0; r < c[b]; ++r) mx = max(mx, acc[b][r]);
-    out[3 * p + b] = mx;
+    out[4 * p + b] = mx;
   }
+  if (threadIdx.x == 3) out[4 * p + 3] = padded;
 }
 
 template <int T, int VPT>
@@ -553,9 +620,9 @@ int launch_bucket(const LMPairDesc* pairs, const int* plist, int count, const LM
 //      what the autotuner installs after timing the candidates);
 //   2. else the first fitting shape in the preference order: the built-in
 //      order below, or MO_LM_SHAPES="TxVPT,TxVPT,..." read once.
-static const int kShapeT[] = {1024, 1024, 1024, 1024, 512, 512, 512, 512, 256, 256, 256};
-static const int kShapeVPT[] = {1, 2, 4, 5, 8, 10, 12, 16, 20, 24, 32};
-constexpr int kNumShapes = 11;
+static const int kShapeT[] = {1024, 1024, 1024, 1024, 512, 512, 512, 512, 256, 256, 256, 768, 768};
+static const int kShapeVPT[] = {1, 2, 4, 5, 8, 10, 12, 16, 20, 24, 32, 6, 7};
+constexpr int kNumShapes = 13;
 
 static std::mutex g_shape_mu;
 static std::vector<int> g_table_nv, g_table_shape;   // guarded by g_shape_mu
@@ -701,8 +768,8 @@ int lm_lds_team_launch(int b, const LMPairDesc* pairs, const int* plist, const i
   }
 }
 
-size_t lm_lds_bytes(int nv, long long entries) {
-  return (((size_t)nv * 16 + (size_t)entries * 2) + 15) & ~(size_t)15;
+size_t lm_lds_bytes(int nv, long long padded_entries) {
+  return (((size_t)nv * 16 + 16 + (size_t)padded_entries * 2) + 15) & ~(size_t)15;
 }
 
 int lm_lds_launch(int bucket, const LMPairDesc* pairs, const int* plist, int count, const LMArrays& A,
@@ -713,7 +780,7 @@ int lm_lds_launch(int bucket, const LMPairDesc* pairs, const int* plist, int cou
     case ID: return launch_bucket<T, V>(pairs, plist, count, A, opt, res, totalV, lds_bytes, s);
     MO_SHAPE(0, 1024, 1) MO_SHAPE(1, 1024, 2) MO_SHAPE(2, 1024, 4) MO_SHAPE(3, 1024, 5)
     MO_SHAPE(4, 512, 8) MO_SHAPE(5, 512, 10) MO_SHAPE(6, 512, 12) MO_SHAPE(7, 512, 16)
-    MO_SHAPE(8, 256, 20) MO_SHAPE(9, 256, 24) MO_SHAPE(10, 256, 32)
+    MO_SHAPE(8, 256, 20) MO_SHAPE(9, 256, 24) MO_SHAPE(10, 256, 32) MO_SHAPE(11, 768, 6) MO_SHAPE(12, 768, 7)
 #undef MO_SHAPE
     default: return fail(MO_ERR_ARG, "bad LDS solver shape %d", bucket);
   }

@@ -49,8 +49,8 @@ def test_lm_shape_table_host_calls():
     import pytest
     from meshode_amd import autotune
     shp = autotune.shapes()
-    assert len(shp) == 11 and shp[0] == (1024, 1) and max(t * v for t, v in shp) == autotune.LDS_MAX_VERTICES
-    assert autotune.size_classes() == [1024, 2048, 4096, 5120, 6144, 8192]
+    assert len(shp) == 13 and shp[0] == (1024, 1) and max(t * v for t, v in shp) == autotune.LDS_MAX_VERTICES
+    assert autotune.size_classes() == [1024, 2048, 4096, 4608, 5120, 5376, 6144, 8192]
     try:
         autotune.set_table([(1024, 4), (5120, 3)])          # 512x8 for <= 1024, 1024x5 for <= 5120
         with pytest.raises(_lib.MoError):
@@ -58,8 +58,8 @@ def test_lm_shape_table_host_calls():
         with pytest.raises(_lib.MoError):
             autotune.set_table([(4096, 2), (2048, 1)])      # not ascending
         with pytest.raises(_lib.MoError):
-            autotune.set_table([(100, 11)])                 # no such shape
+            autotune.set_table([(100, 13)])                 # no such shape
         with pytest.raises(_lib.MoError):
-            _lib.call("mo_lm_shape_info", 11, None, None)
+            _lib.call("mo_lm_shape_info", 13, None, None)
     finally:
         autotune.set_table([])

@@ -18,7 +18,7 @@ def _say(msg):
 
 
 def test_every_lds_shape_meets_the_energy_bar(gpu, oracle):
-    """A 1000-vertex box union fits all 11 shapes (T * VPT >= 1024)."""
+    """A 1000-vertex box union fits all 13 shapes (T * VPT >= 1024)."""
     _say("[autotune] case=box1000_n32 every shape")
     sV, sF, tV, tF = synth.box_union_pair(5, n_boxes=4, n_verts=1000)
     N = 32
