@@ -5,15 +5,20 @@
 // One workgroup of T threads solves one pair whose nv <= T * VPT vertices
 // are statically owned (vertex i by thread i % T, slot i / T).  Everything
 // the PCG iterates on lives in VGPRs for the whole solve: the distance
-// gradient g, the residual r, the direction p, q = A p, the step delta and
+// gradient g, the direction p, q = A p, the step delta and
 // d = lambda^2 deg + LM diagonal / radius (the diagonal of A apart from
 // g g^T, which is also the block-Jacobi preconditioner's diagonal).  The
 // one vector neighbours read (u while evaluating the Jacobian, z = M^-1 r
 // inside PCG, u + delta for the candidate) is published to a
-// float4-per-vertex LDS array, and the pair's adjacency is copied once into
-// LDS as uint16 entries (neighbour << 1 | multiplicity - 1), so the PCG loop
-// touches no global memory.  Reductions are FP64, through DPP lane moves
-// and one LDS exchange between waves.
+// float4-per-vertex LDS array whose w holds the vertex's adjacency word
+// (LDS offset << 16 | count) for the whole solve; the PCG residual is not
+// stored but recovered as r = M z from the published z.  The pair's
+// adjacency is copied once into LDS as uint16 entries (neighbour << 1 |
+// multiplicity - 1), each vertex's list padded to even length with a zero
+// vector slot so the PCG product reads two entries per 32-bit word; pairs
+// whose entries all share one multiplicity skip the per-entry weight.  The
+// PCG loop touches no global memory.  Reductions are FP64, through DPP lane
+// moves and one LDS exchange between waves.
 //
 // Differences from the global kernel that do not change the algorithm:
 //   * the preconditioner M = g g^T + diag(d) is applied with the
