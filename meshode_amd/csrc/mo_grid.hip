@@ -239,12 +239,71 @@ __device__ inline double face_d2(const GFace& g, double qx, double qy, double qz
   return m < d0 ? m : d0;
 }
 
-__device__ inline double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = dmax(v, __shfl_xor(v, o, 64));
-  return v;
+// Device memory reached through the job table's pointers: the explicit
+// global address space makes the accesses global_* instead of flat_*
+// instructions (a flat load also counts against the LDS wait counter, so
+// every LDS wait would wait for it).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MO_GLOBAL __attribute__((address_space(1)))
+#else
+#define MO_GLOBAL
+#endif
+template <class T>
+__device__ inline MO_GLOBAL T* gptr(T* p) {
+  return (MO_GLOBAL T*)p;
 }
 
-__device__ inline double box_gap2(const double* __restrict__ tb, long long t, double lx, double ly, double lz,
+// Wave-wide reductions through DPP lane moves (quad_perm [1,0,3,2] and
+// [2,3,0,1], row_half_mirror, row_mirror), then the four row results by
+// readlane: every lane gets the same value, with no LDS permutes.
+template <int CTRL>
+__device__ inline unsigned dpp_u32(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = dpp_u32<CTRL>((unsigned)b), hi = dpp_u32<CTRL>((unsigned)(b >> 32));
+  return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+__device__ inline double readlane_f64(double v, int l) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+
+__device__ inline double wave_max(double v) {
+  v = dmax(v, dpp_f64<0xB1>(v));
+  v = dmax(v, dpp_f64<0x4E>(v));
+  v = dmax(v, dpp_f64<0x141>(v));
+  v = dmax(v, dpp_f64<0x140>(v));
+  return dmax(dmax(readlane_f64(v, 0), readlane_f64(v, 16)), dmax(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+
+// (value, index) with the smaller value, ties to the smaller non-negative
+// index (-1: none)
+__device__ inline void argmin_step(double& ol, int& ot, double xl, int xt) {
+  if (xl < ol || (xl == ol && xt >= 0 && (ot < 0 || xt < ot))) { ol = xl; ot = xt; }
+}
+template <int CTRL>
+__device__ inline void argmin_dpp(double& ol, int& ot) {
+  argmin_step(ol, ot, dpp_f64<CTRL>(ol), (int)dpp_u32<CTRL>((unsigned)ot));
+}
+// the wave's argmin (the same in every lane)
+__device__ inline void wave_argmin(double& ol, int& ot) {
+  argmin_dpp<0xB1>(ol, ot);
+  argmin_dpp<0x4E>(ol, ot);
+  argmin_dpp<0x141>(ol, ot);
+  argmin_dpp<0x140>(ol, ot);
+  double rl = readlane_f64(ol, 0);
+  int rt = __builtin_amdgcn_readlane(ot, 0);
+  for (int r = 16; r < 64; r += 16) argmin_step(rl, rt, readlane_f64(ol, r), __builtin_amdgcn_readlane(ot, r));
+  ol = rl;
+  ot = rt;
+}
+
+__device__ inline double box_gap2(const MO_GLOBAL double* __restrict__ tb, long long t, double lx, double ly, double lz,
                                   double hx, double hy, double hz) {
   double tl[3] = {tb[t * 6 + 0], tb[t * 6 + 1], tb[t * 6 + 2]};
   double th[3] = {tb[t * 6 + 3], tb[t * 6 + 4], tb[t * 6 + 5]};
@@ -264,16 +323,16 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
   __shared__ GFace s_face[kTile];
   __shared__ int s_list[kTile];
   const GridJob& JJ = jobs[blockIdx.y];
-  const GFace* __restrict__ faces = JJ.faces;
-  const GCull* __restrict__ cull = JJ.cull;
+  const MO_GLOBAL GFace* __restrict__ faces = gptr(JJ.faces);
+  const MO_GLOBAL GCull* __restrict__ cull = gptr(JJ.cull);
   const long long nf = JJ.nf;
-  const double* __restrict__ tb = JJ.tb;
+  const MO_GLOBAL double* __restrict__ tb = gptr(JJ.tb);
   const long long ntiles = JJ.ntiles;
   const int N = JJ.N;
   const int bricks_per_axis = (N + 3) / 4;
-  double* __restrict__ dist64 = JJ.dist64;
-  float* __restrict__ dist32 = JJ.dist32;
-  int* __restrict__ nearest = JJ.nearest;
+  MO_GLOBAL double* __restrict__ dist64 = gptr(JJ.dist64);
+  MO_GLOBAL float* __restrict__ dist32 = gptr(JJ.dist32);
+  MO_GLOBAL int* __restrict__ nearest = gptr(JJ.nearest);
   const int lane = threadIdx.x;
   long long b = blockIdx.x;
   if (b >= (long long)bricks_per_axis * bricks_per_axis * bricks_per_axis) return;
@@ -303,7 +362,7 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
     const double bound = wave_max(valid ? best : -INFINITY);
     bool cand = false;
     if (lane < cnt) {
-      const GCull* gp = cull + f0 + lane;
+      const MO_GLOBAL GCull* gp = cull + f0 + lane;
       const double ax = dmax(0.0, dmax(gp->lox - hx, lx - gp->hix));
       const double ay = dmax(0.0, dmax(gp->loy - hy, ly - gp->hiy));
       const double az = dmax(0.0, dmax(gp->loz - hz, lz - gp->hiz));
@@ -318,10 +377,11 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
     const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
     if (cand) s_list[rank] = lane;
     __syncthreads();
-    constexpr int kChunks = (int)(sizeof(GFace) / sizeof(uint4));
-    static_assert(sizeof(GFace) % sizeof(uint4) == 0, "face records are copied in 16-byte chunks");
-    const uint4* __restrict__ src = reinterpret_cast<const uint4*>(faces + f0);
-    uint4* dst = reinterpret_cast<uint4*>(s_face);
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    constexpr int kChunks = (int)(sizeof(GFace) / sizeof(u32x4));
+    static_assert(sizeof(GFace) % sizeof(u32x4) == 0, "face records are copied in 16-byte chunks");
+    const MO_GLOBAL u32x4* __restrict__ src = reinterpret_cast<const MO_GLOBAL u32x4*>(faces + f0);
+    u32x4* dst = reinterpret_cast<u32x4*>(s_face);
     for (int c = lane; c < ns * kChunks; c += 64) {
       const int r = c / kChunks, w = c - r * kChunks;
       dst[c] = src[s_list[r] * kChunks + w];
@@ -365,14 +425,10 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
   int nseed = 0;
   for (int r = 0; r < kSeedTiles; ++r) {
     double ol = mlb;
-    long long ot = mt;
-    for (int o = 32; o > 0; o >>= 1) {
-      const double xl = __shfl_xor(ol, o, 64);
-      const long long xt = __shfl_xor(ot, o, 64);
-      if (xl < ol || (xl == ol && xt >= 0 && (ot < 0 || xt < ot))) { ol = xl; ot = xt; }
-    }
+    int ot = (int)mt;
+    wave_argmin(ol, ot);
     if (ot < 0) break;
-    const int st = __builtin_amdgcn_readfirstlane((int)ot);
+    const int st = ot;
     if (mt == st) { mlb = INFINITY; mt = -1; }
     seeds[nseed++] = st;
     process_tile(st);
