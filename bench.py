@@ -28,13 +28,13 @@ FP64_VALU_PEAK_TFLOPS = 78.6    # MI355X FP64 vector (FMA = 2 flops)
 FP32_VALU_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (published; packed FP32)
 
 # Flop model of the register/LDS LM kernel (FP32 vector work per vertex):
-# per PCG iteration 84 + 6 E_v (matrix-free A z with E_v adjacency entries
-# per vertex at 3 FMAs each, recurrences, Sherman-Morrison preconditioner,
-# three dot products); per LM iteration 150 + 20 E_v (distance lookup and
-# gradient, edge energy in the Jacobian and candidate evaluations).
-# rocprofv3's SQ_INSTS_VALU_FLOPS_FP32/FP64 on the headline launch read
-# 6.6e11 flops against this model's 6.1e11.
-LM_PCG_FLOPS_BASE, LM_PCG_FLOPS_PER_ENTRY = 84, 6
+# per PCG iteration 93 + 6 E_v (matrix-free A z with E_v adjacency entries
+# per vertex at 3 FMAs each, recurrences, the residual recovered as M z,
+# Sherman-Morrison preconditioner, three dot products); per LM iteration
+# 150 + 20 E_v (distance lookup and gradient, edge energy in the Jacobian
+# and candidate evaluations).  rocprofv3's SQ_INSTS_VALU_FLOPS_FP32/FP64 on
+# the headline launch read 6.4e11 flops against this model's 6.4e11.
+LM_PCG_FLOPS_BASE, LM_PCG_FLOPS_PER_ENTRY = 93, 6
 LM_ITER_FLOPS_BASE, LM_ITER_FLOPS_PER_ENTRY = 150, 20
 
 
