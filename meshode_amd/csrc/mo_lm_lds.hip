@@ -201,19 +201,24 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
       // the zero slot), two independent neighbour reads in flight
       const unsigned* __restrict__ L2 = reinterpret_cast<const unsigned*>(LCSR) + (o >> 1);
       const int n2 = (c + 1) >> 1;
+      // (the entry indices pass through opaque() so each address is one
+      // bit-field extract and one shift-add)
+      const unsigned* const L2e = L2 + n2;
       if (uniform_m != 0.f) {
         #pragma unroll 1
-        for (int q = 0; q < n2; ++q) {
-          const unsigned w = L2[q];
-          const float4 a = VEC[__builtin_amdgcn_ubfe(w, 1, 15)], b = VEC[__builtin_amdgcn_ubfe(w, 17, 15)];
+        for (const unsigned* pw = L2; pw < L2e; ++pw) {
+          const unsigned w = *pw;
+          const float4 a = VEC[opaque((int)__builtin_amdgcn_ubfe(w, 1, 15))];
+          const float4 b = VEC[opaque((int)__builtin_amdgcn_ubfe(w, 17, 15))];
           sx += a.x + b.x; sy += a.y + b.y; sz += a.z + b.z;
         }
         sx *= uniform_m; sy *= uniform_m; sz *= uniform_m;
       } else {
         #pragma unroll 1
-        for (int q = 0; q < n2; ++q) {
-          const unsigned w = L2[q];
-          const float4 a = VEC[__builtin_amdgcn_ubfe(w, 1, 15)], b = VEC[__builtin_amdgcn_ubfe(w, 17, 15)];
+        for (const unsigned* pw = L2; pw < L2e; ++pw) {
+          const unsigned w = *pw;
+          const float4 a = VEC[opaque((int)__builtin_amdgcn_ubfe(w, 1, 15))];
+          const float4 b = VEC[opaque((int)__builtin_amdgcn_ubfe(w, 17, 15))];
           const float ma = (float)((w & 1u) + 1u), mb = (float)(((w >> 16) & 1u) + 1u);
           sx += ma * a.x + mb * b.x; sy += ma * a.y + mb * b.y; sz += ma * a.z + mb * b.z;
         }
