@@ -398,12 +398,18 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
       // the face plane is a second lower bound (0 for degenerate faces)
       const double dp = g.nx * (qx - g.ax) + g.ny * (qy - g.ay) + g.nz * (qz - g.az);
       lb = dmax(lb, (dp * dp) * g.inv_nn);
-      const unsigned long long need = __ballot(valid && !(lb > best * (1.0 + 1e-9)));
+      const bool mine = valid && !(lb > best * (1.0 + 1e-9));
+      const unsigned long long need = __ballot(mine);
       if (need) {
         ++n_exact;
         n_needed += (unsigned long long)__popcll(need);
-        double d = face_d2(g, qx, qy, qz);
-        if (d < best || (d == best && g.face < bidx)) { best = d; bidx = g.face; }
+        // only the lanes that need it evaluate (a lane whose bound exceeds
+        // its best cannot improve or tie it), so the face-region / edge
+        // branches diverge over those lanes alone
+        if (mine) {
+          const double d = face_d2(g, qx, qy, qz);
+          if (d < best || (d == best && g.face < bidx)) { best = d; bidx = g.face; }
+        }
       }
     }
     __syncthreads();  // the next tile overwrites s_face / s_list
