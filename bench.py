@@ -15,6 +15,7 @@ oracle (FP64 restatement of the reference pipeline, 1 core) on the same
 pair, and the energy agreement of that pair with the oracle.
 """
 import argparse
+import collections
 import json
 import os
 import sys
@@ -123,7 +124,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the captured-graph replay measurement")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the launch-shape autotuner (untimed, before warm-up) and use the built-in order")
-    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "adam", "ode", "flow256", "neural2", "rot", "cfg1", "cfg4"],
+    ap.add_argument("--mode", default="batch", choices=["batch", "serve", "serve5k", "adam", "ode", "flow256", "neural2", "rot", "cfg1", "cfg4"],
                     help="batch: the headline line; serve / adam / ode: the serving, pyDeform-Adam and "
                          "NeuralODE workloads (their own JSON line)")
     ap.add_argument("--duration", type=float, default=None, help="serve: seconds of Poisson arrivals (default 60)")
@@ -156,7 +157,7 @@ def main():
         torch.cuda.synchronize()
         del a, b
     if args.mode != "batch":
-        print(json.dumps({"serve": bench_serve, "adam": bench_adam, "ode": bench_ode, "flow256": bench_flow256,
+        print(json.dumps({"serve": bench_serve, "serve5k": bench_serve5k, "adam": bench_adam, "ode": bench_ode, "flow256": bench_flow256,
                           "neural2": bench_neural2, "rot": bench_rot, "cfg1": bench_cfg1,
                           "cfg4": bench_cfg4}[args.mode](args)),
               flush=True)
@@ -561,6 +562,112 @@ def bench_serve(args):
                                    "2k-%d verts, grids 64/128/256^3, fresh rigid motion per request" % args.max_verts,
                        "pool_pairs": n_pool, "grids": {str(g): grids.count(g) for g in (64, 128, 256)},
                        "budget_gb": args.budget_gb, "workers": "3 small + 1 large", "seed": 7}}
+
+
+def bench_serve5k(args):
+    """The headline workload in serving form: Poisson arrivals at args.rate
+    requests/s (default 4000) for args.duration seconds (default 10) of
+    5000-vertex box-union pairs at 64^3 (128 distinct targets, the template
+    cache starting cold; every request moves its pool source by a fresh
+    rigid motion) through the continuous-batching server.  Beside it, in the
+    same run, the plain one-pair-at-a-time GPU path on the same kind of
+    requests: one pair per mo_deform_batch call on the cached template, and
+    one pair per rigid_deform pipeline call (normalization and grid per
+    call).  Energies are checked against the oracle by the serving and LM
+    parity tests; here every result must report convergence or the
+    iteration cap."""
+    import numpy as np
+    import torch
+    from meshode_amd import deform, pyDeform, serve, synth
+    rate = args.rate if args.rate != 200.0 else 4000.0
+    duration = args.duration if args.duration is not None else 10.0
+    rng = np.random.default_rng(11)
+    pool = [synth.box_union_pair(s) for s in range(128)]
+    n_req = int(rate * duration)
+    gaps = rng.exponential(1.0 / rate, n_req)
+    picks = rng.integers(0, len(pool), n_req)
+    ang = np.radians(rng.uniform(-3, 3, (n_req, 3)))
+    scl = rng.uniform(0.98, 1.02, n_req)
+    shf = rng.uniform(-0.01, 0.01, (n_req, 3))
+
+    def moved(V, k):
+        a, b, c = ang[k]
+        Rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+        Ry = np.array([[np.cos(b), 0, np.sin(b)], [0, 1, 0], [-np.sin(b), 0, np.cos(b)]])
+        Rz = np.array([[np.cos(c), -np.sin(c), 0], [np.sin(c), np.cos(c), 0], [0, 0, 1]])
+        ctr = V.mean(0)
+        ext = V.max(0) - V.min(0)
+        return (V - ctr) @ (Rz @ Ry @ Rx).T * scl[k] + ctr + shf[k] * ext
+
+    # the requests' sources are generated before the clock starts (the
+    # server's own work per request -- admission, upload, solve, download --
+    # is what is timed)
+    srcs = [moved(pool[i][0], k) for k, i in enumerate(picks)]
+    srv = serve.DeformServer(mem_budget_bytes=args.budget_gb * 1e9)
+    w = synth.box_union_pair(4, n_boxes=4, n_verts=1000)
+    srv.submit(serve.DeformRequest(w[0], w[1], "warm", w[2], w[3], grid=32)).result()
+    futs = []
+    t_start = time.perf_counter()
+    t_next = t_start
+    for k, (gap, i) in enumerate(zip(gaps, picks)):
+        t_next += gap
+        now = time.perf_counter()
+        if t_next > now:
+            time.sleep(t_next - now)
+        p = pool[i]
+        futs.append(srv.submit(serve.DeformRequest(srcs[k], p[1], int(i), p[2], p[3], grid=64)))
+        if (k + 1) % int(max(rate * 2, 1)) == 0:
+            print(f"[bench] serve5k: {k + 1}/{n_req} submitted, {srv.completed} done", file=sys.stderr, flush=True)
+    res = [f.result() for f in futs]
+    t_end = time.perf_counter()
+    srv.close()
+    lat = np.array([r.latency_s for r in res]) * 1e3
+    terms = collections.Counter(int(r.stats["termination"]) for r in res)
+    served = len(res) / (t_end - t_start)
+    sizes_b = np.array([n for _, n in srv.batches])
+    # plain path 1: one pair per mo_deform_batch call, template cached
+    pids = {}
+    n_plain = 48
+    torch.cuda.synchronize()
+    for k in range(4):   # templates and kernels (untimed)
+        p = pool[k]
+        pids[k] = deform.template_f64(p[2], p[3], 64)
+    V0 = []
+    for k in range(n_plain):
+        p = pool[k % 4]
+        _, scale, pos = pyDeform.TemplateInfo(pids[k % 4])
+        V0.append(torch.from_numpy((moved(p[0], k) - pos) / scale).cuda())
+    deform.deform_batch([pids[0]], [V0[0].clone()], [pool[0][1]], [0], [1.0])
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for k in range(n_plain):
+        deform.deform_batch([pids[k % 4]], [V0[k]], [pool[k % 4][1]], [0], [1.0])
+    torch.cuda.synchronize()
+    plain_cached = n_plain / (time.perf_counter() - tp)
+    # plain path 2: one pair per rigid_deform pipeline call (grid built per call)
+    n_pipe = 16
+    deform.rigid_deform_batch([pool[0]], grid_resolution=64)
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for k in range(n_pipe):
+        deform.rigid_deform_batch([pool[k]], grid_resolution=64)
+    torch.cuda.synchronize()
+    plain_pipe = n_pipe / (time.perf_counter() - tp)
+    pct = lambda x, q: float(np.percentile(x, q))   # noqa: E731
+    return {"metric": "serving: deformation pairs/sec with p50/p99 latency (5k verts, 64^3, Poisson arrivals)",
+            "value": served, "unit": "pairs/s", "higher_is_better": True,
+            "requests": len(res), "offered_rate": rate, "arrival_s": duration, "wall_s": t_end - t_start,
+            "dtype": "f32 (LM), f64 (grid)", "data": "synthetic",
+            "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": float(lat.max())},
+            "terminations": {str(k): v for k, v in sorted(terms.items())},
+            "plain_one_pair_per_call_cached_template_pairs_per_s": plain_cached,
+            "plain_one_pair_rigid_deform_pipeline_pairs_per_s": plain_pipe,
+            "served_over_plain_cached": served / plain_cached, "served_over_plain_pipeline": served / plain_pipe,
+            "template_builds": srv.builds, "batches": int(len(sizes_b)),
+            "mean_batch": float(sizes_b.mean()) if len(sizes_b) else 0.0,
+            "config": {"workload": "headline pairs served: 12-box unions of 5000 verts vs jittered targets, 64^3, "
+                                   "128 distinct targets (cold cache), fresh rigid motion per request",
+                       "budget_gb": args.budget_gb, "workers": "3 small + 1 large", "seed": 11}}
 
 
 def bench_adam(args):
