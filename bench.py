@@ -72,7 +72,7 @@ def adjacency_entries(F):
 # the default workload; keyed by (kernel, pairs, distinct pairs, source
 # verts, grid); other workloads report null.
 MEASURED_TRAFFIC = {("k_lm_solve_lds", 512, 128, 5000, 64): 26.08e9,
-                    ("k_grid_build", 512, 128, 5000, 64): 13.90e9}
+                    ("k_grid_build", 512, 128, 5000, 64): 11.66e9}
 
 
 def cpu_baseline(pair, N, lam):
