@@ -64,6 +64,12 @@ __device__ inline unsigned spread3(unsigned v) {
 
 __device__ inline double dmin(double a, double b) { return b < a ? b : a; }
 __device__ inline double dmax(double a, double b) { return b > a ? b : a; }
+// One-instruction v_max_f64 / v_min_f64 for the culling bounds and the
+// minimum of the three edge distances: the operands are never NaN there, and
+// where two operands are equal (including +0 / -0) either result gives the
+// same squared distance and the same comparisons.
+__device__ inline double vmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ inline double vmin(double a, double b) { return __builtin_fmin(a, b); }
 
 // Pass 1: validate indices and compute the centroid bin (Morton order of a
 // 16^3 lattice over [0,1]^3, clamped).  blockIdx.y = job.
@@ -239,8 +245,7 @@ __device__ inline double face_d2(const GFace& g, double qx, double qy, double qz
                            : edge_d2(qx, qy, qz, g.bx, g.by, g.bz, g.cx, g.cy, g.cz, g.ll1, g.il1);
   double d2 = (g.flip & 4) ? edge_d2(qx, qy, qz, g.ax, g.ay, g.az, g.cx, g.cy, g.cz, g.ll2, g.il2)
                            : edge_d2(qx, qy, qz, g.cx, g.cy, g.cz, g.ax, g.ay, g.az, g.ll2, g.il2);
-  double m = d2 < d1 ? d2 : d1;
-  return m < d0 ? m : d0;
+  return vmin(vmin(d2, d1), d0);
 }
 
 // Device memory reached through the job table's pointers: the explicit
@@ -278,11 +283,11 @@ __device__ inline double readlane_f64(double v, int l) {
 }
 
 __device__ inline double wave_max(double v) {
-  v = dmax(v, dpp_f64<0xB1>(v));
-  v = dmax(v, dpp_f64<0x4E>(v));
-  v = dmax(v, dpp_f64<0x141>(v));
-  v = dmax(v, dpp_f64<0x140>(v));
-  return dmax(dmax(readlane_f64(v, 0), readlane_f64(v, 16)), dmax(readlane_f64(v, 32), readlane_f64(v, 48)));
+  v = vmax(v, dpp_f64<0xB1>(v));
+  v = vmax(v, dpp_f64<0x4E>(v));
+  v = vmax(v, dpp_f64<0x141>(v));
+  v = vmax(v, dpp_f64<0x140>(v));
+  return vmax(vmax(readlane_f64(v, 0), readlane_f64(v, 16)), vmax(readlane_f64(v, 32), readlane_f64(v, 48)));
 }
 
 // (value, index) with the smaller value, ties to the smaller non-negative
@@ -314,7 +319,7 @@ __device__ inline double box_gap2(const MO_GLOBAL double* __restrict__ tb, long 
   double ql[3] = {lx, ly, lz}, qh[3] = {hx, hy, hz};
   double s = 0.0;
   for (int a = 0; a < 3; ++a) {
-    double g = dmax(0.0, dmax(tl[a] - qh[a], ql[a] - th[a]));
+    double g = vmax(0.0, vmax(tl[a] - qh[a], ql[a] - th[a]));
     s += g * g;
   }
   return s;
@@ -367,9 +372,9 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
     bool cand = false;
     if (lane < cnt) {
       const MO_GLOBAL GCull* gp = cull + f0 + lane;
-      const double ax = dmax(0.0, dmax(gp->lox - hx, lx - gp->hix));
-      const double ay = dmax(0.0, dmax(gp->loy - hy, ly - gp->hiy));
-      const double az = dmax(0.0, dmax(gp->loz - hz, lz - gp->hiz));
+      const double ax = vmax(0.0, vmax(gp->lox - hx, lx - gp->hix));
+      const double ay = vmax(0.0, vmax(gp->loy - hy, ly - gp->hiy));
+      const double az = vmax(0.0, vmax(gp->loz - hz, lz - gp->hiz));
       cand = !(ax * ax + ay * ay + az * az > bound * (1.0 + 1e-9));
     }
     const unsigned long long m = __ballot(cand);
@@ -395,13 +400,13 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
     // then the exact test when some lane can still improve or tie its best
     for (int r = 0; r < ns; ++r) {
       const GFace& g = s_face[r];
-      double gx = dmax(0.0, dmax(g.lox - qx, qx - g.hix));
-      double gy = dmax(0.0, dmax(g.loy - qy, qy - g.hiy));
-      double gz = dmax(0.0, dmax(g.loz - qz, qz - g.hiz));
+      double gx = vmax(0.0, vmax(g.lox - qx, qx - g.hix));
+      double gy = vmax(0.0, vmax(g.loy - qy, qy - g.hiy));
+      double gz = vmax(0.0, vmax(g.loz - qz, qz - g.hiz));
       double lb = gx * gx + gy * gy + gz * gz;
       // the face plane is a second lower bound (0 for degenerate faces)
       const double dp = g.nx * (qx - g.ax) + g.ny * (qy - g.ay) + g.nz * (qz - g.az);
-      lb = dmax(lb, (dp * dp) * g.inv_nn);
+      lb = vmax(lb, (dp * dp) * g.inv_nn);
       const bool mine = valid && !(lb > best * (1.0 + 1e-9));
       const unsigned long long need = __ballot(mine);
       if (need) {
