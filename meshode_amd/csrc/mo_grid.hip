@@ -238,13 +238,17 @@ __device__ inline double face_d2(const GFace& g, double qx, double qy, double qz
       return (dp * dp) * g.inv_nn;
     }
   }
-  // edge j joins p_j and p_{j+1}; flip bit j set when p_{j+1} has the lower index
-  double d0 = (g.flip & 1) ? edge_d2(qx, qy, qz, g.bx, g.by, g.bz, g.ax, g.ay, g.az, g.ll0, g.il0)
-                           : edge_d2(qx, qy, qz, g.ax, g.ay, g.az, g.bx, g.by, g.bz, g.ll0, g.il0);
-  double d1 = (g.flip & 2) ? edge_d2(qx, qy, qz, g.cx, g.cy, g.cz, g.bx, g.by, g.bz, g.ll1, g.il1)
-                           : edge_d2(qx, qy, qz, g.bx, g.by, g.bz, g.cx, g.cy, g.cz, g.ll1, g.il1);
-  double d2 = (g.flip & 4) ? edge_d2(qx, qy, qz, g.ax, g.ay, g.az, g.cx, g.cy, g.cz, g.ll2, g.il2)
-                           : edge_d2(qx, qy, qz, g.cx, g.cy, g.cz, g.ax, g.ay, g.az, g.ll2, g.il2);
+  // edge j joins p_j and p_{j+1}; flip bit j set when p_{j+1} has the lower
+  // index.  The face is the same in every lane, so the flip bits are read
+  // into a scalar register and each edge takes one uniform branch.
+  const int flip = __builtin_amdgcn_readfirstlane(g.flip);
+  double d0, d1, d2;
+  if (flip & 1) d0 = edge_d2(qx, qy, qz, g.bx, g.by, g.bz, g.ax, g.ay, g.az, g.ll0, g.il0);
+  else d0 = edge_d2(qx, qy, qz, g.ax, g.ay, g.az, g.bx, g.by, g.bz, g.ll0, g.il0);
+  if (flip & 2) d1 = edge_d2(qx, qy, qz, g.cx, g.cy, g.cz, g.bx, g.by, g.bz, g.ll1, g.il1);
+  else d1 = edge_d2(qx, qy, qz, g.bx, g.by, g.bz, g.cx, g.cy, g.cz, g.ll1, g.il1);
+  if (flip & 4) d2 = edge_d2(qx, qy, qz, g.ax, g.ay, g.az, g.cx, g.cy, g.cz, g.ll2, g.il2);
+  else d2 = edge_d2(qx, qy, qz, g.cx, g.cy, g.cz, g.ax, g.ay, g.az, g.ll2, g.il2);
   return vmin(vmin(d2, d1), d0);
 }
 
@@ -358,6 +362,7 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
 
   // points farther than the band only need to be shown farther (GridSpec)
   double best = JJ.band2;
+  double bcut = best * (1.0 + 1e-9);   // the culling threshold, kept with best
   int bidx = INT_MAX;
   // work counters (wave-uniform): exact evaluations, lanes that needed
   // one, face bound checks, staged tiles
@@ -407,7 +412,7 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
       // the face plane is a second lower bound (0 for degenerate faces)
       const double dp = g.nx * (qx - g.ax) + g.ny * (qy - g.ay) + g.nz * (qz - g.az);
       lb = vmax(lb, (dp * dp) * g.inv_nn);
-      const bool mine = valid && !(lb > best * (1.0 + 1e-9));
+      const bool mine = valid && !(lb > bcut);
       const unsigned long long need = __ballot(mine);
       if (need) {
         ++n_exact;
@@ -417,7 +422,11 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
         // branches diverge over those lanes alone
         if (mine) {
           const double d = face_d2(g, qx, qy, qz);
-          if (d < best || (d == best && g.face < bidx)) { best = d; bidx = g.face; }
+          if (d < best || (d == best && g.face < bidx)) {
+            best = d;
+            bcut = d * (1.0 + 1e-9);
+            bidx = g.face;
+          }
         }
       }
     }
@@ -466,7 +475,7 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
       mask &= mask - 1;
       const int tt = __builtin_amdgcn_readfirstlane((int)(c + l));
       const double pl = box_gap2(tb, tt, qx, qy, qz, qx, qy, qz);
-      if (!__ballot(valid && !(pl > best * (1.0 + 1e-9)))) continue;
+      if (!__ballot(valid && !(pl > bcut))) continue;
       process_tile(tt);
     }
   }
