@@ -71,8 +71,8 @@ def adjacency_entries(F):
 # 1.99975 and WRITE_SIZE x 1.0, the factors a plain 256 MiB copy reads) on
 # the default workload; keyed by (kernel, pairs, distinct pairs, source
 # verts, grid); other workloads report null.
-MEASURED_TRAFFIC = {("k_lm_solve_lds", 512, 128, 5000, 64): 26.08e9,
-                    ("k_grid_build", 512, 128, 5000, 64): 11.66e9}
+MEASURED_TRAFFIC = {("k_lm_solve_lds", 512, 128, 5000, 64): 26.23e9,
+                    ("k_grid_build", 512, 128, 5000, 64): 11.62e9}
 
 
 def cpu_baseline(pair, N, lam):
