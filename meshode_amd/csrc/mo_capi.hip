@@ -593,6 +593,8 @@ void fill_stats(const LMResultDev* res, const int* path, const int* wgs, int np,
 
 // ext / ext_cap: a caller-provided device region for all of the solve's
 // scratch (the no-allocation entry point); null: stream-ordered allocations.
+constexpr int kMaxLdsLaunches = 32;   // register/LDS launches (shapes) per batch
+
 int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_stats* stats, hipStream_t s,
             PhaseTimer& timer, char* ext = nullptr, size_t ext_cap = 0) {
   const int np = (int)pairs.size();
@@ -620,7 +622,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
                  al(std::max<long long>(2 * nedges_tot, 1) * sizeof(float)) + al(topo_workspace_bytes(nvalloc)) +
                  al(np * sizeof(LMPairDesc)) + al(np * sizeof(TopoPairDesc)) + al(np * sizeof(LMResultDev)) +
                  al(2 * np * sizeof(int)) + al(np * sizeof(int)) + al(4 * np * sizeof(int)) +
-                 al(np * sizeof(PairIO)) + 512;
+                 al(np * sizeof(PairIO)) + al(kMaxLdsLaunches * sizeof(int)) + 512;
   MO_CHECK(np <= 65535, MO_ERR_ARG, "at most 65535 pairs per batch (got %d)", np);
   Arena ar;
   int rc;
@@ -640,6 +642,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   int* d_plist = ar.take<int>(np);
   int* d_chunk = ar.take<int>(4 * np);
   PairIO* d_io = ar.take<PairIO>(np);
+  int* d_next = ar.take<int>(kMaxLdsLaunches);   // work counters of the register/LDS launches
   int* err = ar.take<int>(1);
 
   std::vector<LMPairDesc> h_pairs(np);
@@ -717,11 +720,14 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   if ((rc = chain_h2d(d_plist, plist.data(), np * sizeof(int), s))) return rc;
   LMArrays A{x0, u, adj, raw, work, wt};
   const LMOptionsDev dopt = to_dev_options(opt);
+  MO_CHECK(nb <= kMaxLdsLaunches, MO_ERR_STATE, "%d register/LDS shapes exceed the %d work counters", nb,
+           kMaxLdsLaunches);
+  if ((rc = zero_fill(d_next, nb * sizeof(int), s))) return rc;
   if ((rc = timer.mark(2))) return rc;
   for (int b = 0; b < nb; ++b)
     if (!bucket[b].empty())
-      if ((rc = lm_lds_launch(b, d_pairs, d_plist + start[b], (int)bucket[b].size(), A, dopt, d_res, nvalloc,
-                              bucket_lds[b], s)))
+      if ((rc = lm_lds_launch(b, d_pairs, d_plist + start[b], (int)bucket[b].size(), d_next + b, A, dopt, d_res,
+                              nvalloc, bucket_lds[b], s)))
         return rc;
   // Pairs the register/LDS kernel does not take.  A pair of nv vertices
   // gets a team of ceil(nv / team_vertices) co-resident workgroups (at most

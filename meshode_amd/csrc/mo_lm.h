@@ -101,9 +101,10 @@ int lm_lds_bucket_for(int nv);                       // -1: too large
 int lm_lds_shape(int bucket, int* threads, int* vertices_per_thread);
 int lm_lds_set_table(const int* max_nv, const int* shape, int n);   // n = 0: built-in order
 size_t lm_lds_bytes(int nv, long long entries);      // dynamic LDS for one pair
-int lm_lds_launch(int bucket, const LMPairDesc* pairs_dev, const int* plist, int count, const LMArrays& arrays,
-                  const LMOptionsDev& opt, LMResultDev* results_dev, long long total_vertices, size_t lds_bytes,
-                  hipStream_t s);
+// next: one device int per launch, zero before it (the launch's work counter)
+int lm_lds_launch(int bucket, const LMPairDesc* pairs_dev, const int* plist, int count, int* next,
+                  const LMArrays& arrays, const LMOptionsDev& opt, LMResultDev* results_dev,
+                  long long total_vertices, size_t lds_bytes, hipStream_t s);
 
 // Multi-workgroup register/LDS kernel (k_lm_solve_lds_team) for pairs
 // above one workgroup's capacity with multiplicities <= 2: workgroup r of a

@@ -484,18 +484,33 @@ struct LdsTeam {
   }
 };
 
+// A resident grid of workgroups (as many as the chip holds at once) works
+// through plist[0, count): workgroup b solves item b, then takes the next
+// unsolved item from the launch's counter `next` (zero before the launch)
+// until none is left.  Pairs are solved in list order (largest first), a
+// new pair starts on a workgroup as soon as its previous one ends, and
+// every XCD draws from the one queue; each pair's result is independent of
+// which workgroup solves it.
 template <int T, int VPT>
 __global__ void __launch_bounds__(T) k_lm_solve_lds(const LMPairDesc* __restrict__ pairs,
-                                                    const int* __restrict__ plist, LMArrays A, LMOptionsDev opt,
-                                                    LMResultDev* __restrict__ res, long long totalV,
-                                                    unsigned lds_bytes) {
+                                                    const int* __restrict__ plist, int count, int* __restrict__ next,
+                                                    LMArrays A, LMOptionsDev opt, LMResultDev* __restrict__ res,
+                                                    long long totalV, unsigned lds_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[2 * 8 * kMaxWaves];
   __shared__ int scan_buf[kMaxWaves];
-  const int pid = plist[blockIdx.x];
-  const LMPairDesc P = pairs[pid];
-  LdsSolo team{0, P.nv};
-  lds_solve<T, VPT, LdsSolo, unsigned short>(P, pid, team, A, opt, res, totalV, smem, red, scan_buf, lds_bytes);
+  __shared__ int s_item;
+  int w = blockIdx.x;
+  while (w < count) {
+    const int pid = plist[w];
+    const LMPairDesc P = pairs[pid];
+    LdsSolo team{0, P.nv};
+    lds_solve<T, VPT, LdsSolo, unsigned short>(P, pid, team, A, opt, res, totalV, smem, red, scan_buf, lds_bytes);
+    __syncthreads();   // every thread is done with this pair's LDS
+    if (threadIdx.x == 0) s_item = (int)gridDim.x + atomicAdd(next, 1);
+    __syncthreads();
+    w = s_item;
+  }
 }
 
 // wg[b] = (team, rank, team size, chunk): team t solves pair plist[t];
@@ -612,11 +627,18 @@ __global__ void __launch_bounds__(256) k_chunk_entries(const LMPairDesc* __restr
 }
 
 template <int T, int VPT>
-int launch_bucket(const LMPairDesc* pairs, const int* plist, int count, const LMArrays& A, const LMOptionsDev& opt,
-                  LMResultDev* res, long long totalV, size_t lds_bytes, hipStream_t s) {
+int launch_bucket(const LMPairDesc* pairs, const int* plist, int count, int* next, const LMArrays& A,
+                  const LMOptionsDev& opt, LMResultDev* res, long long totalV, size_t lds_bytes, hipStream_t s) {
   auto kern = k_lm_solve_lds<T, VPT>;
   MO_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
-  hipLaunchKernelGGL(kern, dim3(count), dim3(T), lds_bytes, s, pairs, plist, A, opt, res, totalV, (unsigned)lds_bytes);
+  // one wave of workgroups: the chip's capacity for this shape and LDS size
+  int dev = 0, cus = 0, per = 0;
+  MO_HIP(hipGetDevice(&dev));
+  MO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  MO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)kern, T, lds_bytes));
+  const int nwg = std::max(1, std::min(count, std::max(per, 1) * std::max(cus, 1)));
+  hipLaunchKernelGGL(kern, dim3(nwg), dim3(T), lds_bytes, s, pairs, plist, count, next, A, opt, res, totalV,
+                     (unsigned)lds_bytes);
   MO_HIP(hipGetLastError());
   return MO_OK;
 }
@@ -784,12 +806,12 @@ size_t lm_lds_bytes(int nv, long long padded_entries) {
   return (((size_t)nv * 16 + 16 + (size_t)padded_entries * 2) + 15) & ~(size_t)15;
 }
 
-int lm_lds_launch(int bucket, const LMPairDesc* pairs, const int* plist, int count, const LMArrays& A,
+int lm_lds_launch(int bucket, const LMPairDesc* pairs, const int* plist, int count, int* next, const LMArrays& A,
                   const LMOptionsDev& opt, LMResultDev* res, long long totalV, size_t lds_bytes, hipStream_t s) {
   if (count <= 0) return MO_OK;
   switch (bucket) {
 #define MO_SHAPE(ID, T, V) \
-    case ID: return launch_bucket<T, V>(pairs, plist, count, A, opt, res, totalV, lds_bytes, s);
+    case ID: return launch_bucket<T, V>(pairs, plist, count, next, A, opt, res, totalV, lds_bytes, s);
     MO_SHAPE(0, 1024, 1) MO_SHAPE(1, 1024, 2) MO_SHAPE(2, 1024, 4) MO_SHAPE(3, 1024, 5)
     MO_SHAPE(4, 512, 8) MO_SHAPE(5, 512, 10) MO_SHAPE(6, 512, 12) MO_SHAPE(7, 512, 16)
     MO_SHAPE(8, 256, 20) MO_SHAPE(9, 256, 24) MO_SHAPE(10, 256, 32) MO_SHAPE(11, 768, 6) MO_SHAPE(12, 768, 7)
