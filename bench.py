@@ -9,7 +9,10 @@ Deformer::Deform (Levenberg-Marquardt, <= 100 iterations, lambda = 1) and write
 the result in the target frame.  Pairs are the seeded 12-box unions of
 5000 vertices against their jittered targets (meshode_amd/synth.py).
 
-Reports pairs/s (whole batch), p50/p99 per-pair latency inside the batch, the
+Reports pairs/s over K steps with two batches in flight on two HIP streams
+(mo_rigid_deform_batch_async: one batch's grid builds fill the CUs that the
+previous batch's last solves leave idle), the same steps one at a time on one
+stream (with per-phase times), p50/p99 per-pair latency inside the batch, the
 one-pair-at-a-time GPU rate, the LM kernel's roofline position, the CPU
 oracle (FP64 restatement of the reference pipeline, 1 core) on the same
 pair, and the energy agreement of that pair with the oracle.
@@ -108,7 +111,7 @@ def cpu_baseline_all_cores(pairs, N, lam):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=512, help="pairs per step (batch)")
     ap.add_argument("--distinct", type=int, default=128, help="distinct seeded pairs (repeated to fill the batch)")
@@ -181,19 +184,51 @@ def main():
         batch.run(opt)
         print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
 
+    # the headline: K steps, each one batch of args.pairs pairs, with two
+    # batches in flight on two streams (mo_rigid_deform_batch_async); step
+    # k+1's call returns once its solver launches are queued, and step k is
+    # waited for after that, so its last solves overlap the next grid builds
+    aopt = deform.default_options(block_threads=args.block_threads, timing=0, kernel_variant=args.kernel_variant)
+    if args.pcg_tol is not None:
+        aopt.pcg_relative_tolerance = args.pcg_tol
+    pipe = [deform.PairBatch(pairs, grid_resolution=args.grid, lambdas=args.lam) for _ in range(2)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for j in range(2):
+        with torch.cuda.stream(streams[j]):
+            pipe[j].run_async(aopt)
+        pipe[j].wait()
+    torch.cuda.synchronize()
+    pipe_stats = []
+    t_start = time.perf_counter()
+    prev = None
+    for k in range(args.steps):
+        with torch.cuda.stream(streams[k % 2]):
+            pipe[k % 2].run_async(aopt, after=pipe[prev] if prev is not None else None)
+        if prev is not None:
+            pipe_stats.append(pipe[prev].wait())
+        prev = k % 2
+    pipe_stats.append(pipe[prev].wait())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    value = args.pairs * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    print(f"[bench] pipelined: {ms_per_step:.1f} ms/step, {value:.0f} pairs/s", file=sys.stderr, flush=True)
+
+    # the same steps one at a time on one stream, with HIP events around the
+    # phases (grid, set-up, LM)
     torch.cuda.synchronize()
     stats_all, timings = [], []
-    t_start = time.perf_counter()
+    t1 = time.perf_counter()
     for k in range(args.steps):
         st = batch.run(opt)
         stats_all.append(st)
         timings.append(_lib.last_timings())
         print(f"[bench] step {k + 1}/{args.steps}: {timings[-1]['total_ms']:.1f} ms", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-
-    value = args.pairs * args.steps / elapsed
-    ms_per_step = 1e3 * elapsed / args.steps
+    el1 = time.perf_counter() - t1
+    pipe_equal = all(torch.equal(a, b) for pb in pipe for a, b in zip(pb.out, batch.out))
+    one_stream = {"pairs_per_s": args.pairs * args.steps / el1, "ms_per_step": 1e3 * el1 / args.steps,
+                  "outputs_equal_pipelined": bool(pipe_equal)}
 
     # the same step as a captured HIP graph (mo_rigid_deform_batch_graph):
     # every kernel of the chain replays on the same inputs; timed like the
@@ -294,6 +329,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "rigid_deform pipeline, batch of seeded 12-box unions (5000 verts) vs jittered targets",
+            "batches_in_flight": 2,
             "pairs_per_step": args.pairs,
             "distinct_pairs": len(distinct),
             "source_verts": args.verts,
@@ -308,6 +344,7 @@ def main():
                        "max": float(lat.max())},
         "plain_one_pair_per_call_pairs_per_s": plain_rate,
         "batched_over_plain": value / plain_rate,
+        "one_stream": one_stream,
         "phase_ms_mean": {k: float(np.mean([t[k] for t in timings])) for k in ("grid_ms", "setup_ms", "lm_ms",
                                                                                  "total_ms")},
         "lm_iterations_mean": float(np.mean([s["iterations"] for s in last])),

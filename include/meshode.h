@@ -301,6 +301,33 @@ int mo_rigid_deform_batch_graph(int32_t npairs, const double* const* src, const 
 int mo_batch_graph_launch(mo_batch_graph* graph, mo_lm_stats* stats, void* stream);
 int mo_batch_graph_destroy(mo_batch_graph* graph);
 
+/* mo_rigid_deform_batch without the final wait (no counterpart in the
+ * reference, which runs one ceres::Solve per process, rigid_deform.cc:12-63:
+ * for callers that keep several batches in flight on several streams, so
+ * that one batch's grid builds fill the CUs that the previous batch's last
+ * solves leave idle).  The call enqueues the whole pipeline on `stream`
+ * and returns once the solver launches are queued (it waits only for the
+ * sources' topology, whose sizes steer those launches); *handle then owns
+ * the batch.  after (optional): a handle not yet waited for, on another
+ * stream; this batch's grid builds start only once that batch's solver
+ * launches are about to start, so the solvers are placed on the CUs first
+ * and the grid builds take the CUs they leave.  mo_batch_wait(handle,
+ * stats) waits for the batch, fills stats (optional; host, npairs entries)
+ * exactly as mo_rigid_deform_batch would, reports a bad target face index
+ * as that call would, and frees the handle (call it once per handle, from
+ * any thread).  out[p] is written when the batch completes and the
+ * caller's buffers must stay alive until then.  options.timing must be 0.
+ * A batch with a pair that needs a cooperative multi-workgroup launch
+ * (MO_PATH_TEAM / MO_PATH_LDS_TEAM) completes inside the call. */
+typedef struct mo_batch_handle mo_batch_handle;
+int mo_rigid_deform_batch_async(int32_t npairs, const double* const* src, const int64_t* src_nv,
+                                const int32_t* const* src_faces, const int64_t* src_nf, const double* const* tgt,
+                                const int64_t* tgt_nv, const int32_t* const* tgt_faces, const int64_t* tgt_nf,
+                                const int32_t* grid_resolution, const double* lambda, const mo_lm_options* opt,
+                                double* const* out, void* stream, mo_batch_handle* after,
+                                mo_batch_handle** handle);
+int mo_batch_wait(mo_batch_handle* handle, mo_lm_stats* stats);
+
 /* Deformer::DeformSubdivision with a TerminateWhenSuccessCallback
  * (deformer.cc:173-263, callback.h:9-39, as cad_deform.cc:87-114 runs it):
  * distance residuals on every vertex of the subdivided mesh plus

@@ -33,6 +33,7 @@ SYMBOLS = [
     "mo_flow256_dw_work_floats", "mo_flow256_dw_grads",
     "mo_deform_batch_noalloc_max_bytes", "mo_deform_batch_noalloc", "mo_template_rebuild_f64",
     "mo_last_grid_stats", "mo_rigid_deform_batch_graph", "mo_batch_graph_launch", "mo_batch_graph_destroy",
+    "mo_rigid_deform_batch_async", "mo_batch_wait",
 ]
 
 
@@ -129,6 +130,9 @@ _SIGS = {
                                               C.POINTER(LMOptions), _vp, _vp, _vp, C.POINTER(_vp)]),
     "mo_batch_graph_launch": (C.c_int, [_vp, _vp, _vp]),
     "mo_batch_graph_destroy": (C.c_int, [_vp]),
+    "mo_rigid_deform_batch_async": (C.c_int, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                              C.POINTER(LMOptions), _vp, _vp, _vp, C.POINTER(_vp)]),
+    "mo_batch_wait": (C.c_int, [_vp, _vp]),
     "mo_last_timings": (C.c_int, [_vp]),
     "mo_last_grid_stats": (C.c_int, [_vp]),
     "mo_lm_shape_count": (_i32, []),

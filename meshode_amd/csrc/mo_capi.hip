@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -76,6 +77,124 @@ struct ChainCtx {
 
 thread_local ChainCtx* g_chain = nullptr;
 
+// Set while mo_rigid_deform_batch_async enqueues a batch: the LM results
+// and the error flag are copied into `pinned` (page-locked, so the copies
+// stay asynchronous) instead of being read back, and the call returns with
+// the work in flight.  deferred stays false when the batch needed a
+// cooperative team launch: that path completes inside the call.
+struct AsyncCtx {
+  char* pinned = nullptr;
+  size_t err_off = 0;
+  bool deferred = false;
+  int np = 0;
+  std::vector<int> path, wgs;
+  hipEvent_t lm_ready = nullptr;    // recorded just before the batch's solver launches
+  hipEvent_t after = nullptr;       // the grid builds wait for this event (another batch's lm_ready)
+  ~AsyncCtx() {
+    if (lm_ready) (void)hipEventDestroy(lm_ready);
+  }
+};
+
+thread_local AsyncCtx* g_async = nullptr;
+
+// page-locked blocks, reused across calls (freeing page-locked memory
+// synchronizes the device, which would stall the batches in flight)
+std::mutex g_pinned_mu;
+std::vector<std::pair<size_t, char*>> g_pinned_free;
+
+int pinned_get(size_t bytes, char** p, size_t* cap) {
+  {
+    std::lock_guard<std::mutex> lk(g_pinned_mu);
+    for (size_t k = 0; k < g_pinned_free.size(); ++k)
+      if (g_pinned_free[k].first >= bytes) {
+        *cap = g_pinned_free[k].first;
+        *p = g_pinned_free[k].second;
+        g_pinned_free.erase(g_pinned_free.begin() + (long)k);
+        return MO_OK;
+      }
+  }
+  const size_t c = std::max<size_t>((bytes + 65535) & ~(size_t)65535, 65536);
+  MO_HIP(hipHostMalloc((void**)p, c, hipHostMallocDefault));
+  *cap = c;
+  return MO_OK;
+}
+
+void pinned_put(char* p, size_t cap) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(g_pinned_mu);
+  g_pinned_free.emplace_back(cap, p);
+}
+
+// Page-locked staging of one batch call's host <-> device copies, so that
+// they stay asynchronous (a copy from pageable memory returns only once it
+// has run, i.e. once the stream has drained up to it).  The blocks go back
+// to the pool when the call ends -- for an asynchronous batch, when it is
+// waited for -- after the stream has passed every copy that reads them.
+struct HostStage {
+  hipStream_t stream = nullptr;
+  std::vector<std::pair<char*, size_t>> blocks;
+  size_t used = 0;                  // bytes taken from the last block
+  hipEvent_t ev = nullptr;          // read-back fence
+  bool drained = false;             // set when the stream is known to be past every copy
+  explicit HostStage(hipStream_t s) : stream(s) {}
+  ~HostStage() {
+    if (!drained && !blocks.empty()) (void)hipStreamSynchronize(stream);
+    if (ev) (void)hipEventDestroy(ev);
+    for (auto& b : blocks) pinned_put(b.first, b.second);
+  }
+  int take(size_t bytes, char** p) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (blocks.empty() || used + bytes > blocks.back().second) {
+      char* b = nullptr;
+      size_t cap = 0;
+      int rc = pinned_get(std::max<size_t>(bytes, 1 << 20), &b, &cap);
+      if (rc) return rc;
+      blocks.emplace_back(b, cap);
+      used = 0;
+    }
+    *p = blocks.back().first + used;
+    used += bytes;
+    return MO_OK;
+  }
+};
+
+thread_local HostStage* g_stage = nullptr;
+
+struct StageScope {
+  explicit StageScope(HostStage* st) { g_stage = st; }
+  ~StageScope() { g_stage = nullptr; }
+};
+
+// Second streams for work of one call that may overlap its main stream's
+// (a batch's source topology beside its grid builds): taken from a pool per
+// device, returned when the call ends, kept for the process's lifetime.
+std::mutex g_aux_mu;
+std::vector<std::pair<int, hipStream_t>> g_aux_free;
+
+struct AuxStream {
+  int device = -1;
+  hipStream_t s = nullptr;
+  int acquire() {
+    MO_HIP(hipGetDevice(&device));
+    {
+      std::lock_guard<std::mutex> lk(g_aux_mu);
+      for (size_t k = 0; k < g_aux_free.size(); ++k)
+        if (g_aux_free[k].first == device) {
+          s = g_aux_free[k].second;
+          g_aux_free.erase(g_aux_free.begin() + (long)k);
+          return MO_OK;
+        }
+    }
+    MO_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return MO_OK;
+  }
+  ~AuxStream() {
+    if (!s) return;
+    std::lock_guard<std::mutex> lk(g_aux_mu);
+    g_aux_free.emplace_back(device, s);
+  }
+};
+
 inline size_t al256c(size_t b) { return (b + 255) & ~(size_t)255; }
 
 static int pinned_take(ChainCtx* c, size_t bytes, size_t* off) {
@@ -95,8 +214,15 @@ int chain_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
     if (rc) return rc;
     memcpy(c->pinned + off, src, bytes);
     src = c->pinned + off;
-  } else if (c) {
-    c->pinned_bytes += al256c(bytes);
+  } else {
+    if (c) c->pinned_bytes += al256c(bytes);
+    if (HostStage* st = g_stage) {
+      char* p = nullptr;
+      int rc = st->take(bytes, &p);
+      if (rc) return rc;
+      memcpy(p, src, bytes);
+      src = p;
+    }
   }
   MO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
   return MO_OK;
@@ -108,7 +234,11 @@ struct ChainRead {
   size_t bytes;
 };
 
-static int chain_read(const ChainRead* r, int n, hipStream_t s) {
+// Reads n device values into host memory.  between(), when given, is
+// called after the reads are enqueued and before the host waits for them:
+// it enqueues work that the reads do not depend on (so that it runs while
+// the host waits and decides).
+static int chain_read(const ChainRead* r, int n, hipStream_t s, const std::function<int()>& between = {}) {
   ChainCtx* c = g_chain;
   if (c && c->mode == ChainCtx::kCapture) {
     for (int k = 0; k < n; ++k) {
@@ -122,10 +252,25 @@ static int chain_read(const ChainRead* r, int n, hipStream_t s) {
       c->checks.emplace_back(off, c->next_read);
       ++c->next_read;
     }
-    return MO_OK;
+    return between ? between() : MO_OK;
   }
-  for (int k = 0; k < n; ++k) MO_HIP(hipMemcpyAsync(r[k].host, r[k].dev, r[k].bytes, hipMemcpyDeviceToHost, s));
-  MO_HIP(hipStreamSynchronize(s));
+  int rc = MO_OK;
+  if (HostStage* st = g_stage) {
+    std::vector<char*> tmp(n);
+    for (int k = 0; k < n; ++k) {
+      if ((rc = st->take(r[k].bytes, &tmp[k]))) return rc;
+      MO_HIP(hipMemcpyAsync(tmp[k], r[k].dev, r[k].bytes, hipMemcpyDeviceToHost, s));
+    }
+    if (!st->ev) MO_HIP(hipEventCreateWithFlags(&st->ev, hipEventDisableTiming));
+    MO_HIP(hipEventRecord(st->ev, s));
+    if (between && (rc = between())) return rc;
+    MO_HIP(hipEventSynchronize(st->ev));
+    for (int k = 0; k < n; ++k) memcpy(r[k].host, tmp[k], r[k].bytes);
+  } else {
+    for (int k = 0; k < n; ++k) MO_HIP(hipMemcpyAsync(r[k].host, r[k].dev, r[k].bytes, hipMemcpyDeviceToHost, s));
+    if (between && (rc = between())) return rc;
+    MO_HIP(hipStreamSynchronize(s));
+  }
   if (c)
     for (int k = 0; k < n; ++k) {
       const char* h = (const char*)r[k].host;
@@ -516,6 +661,7 @@ LMOptionsDev to_dev_options(const mo_lm_options& o) {
 // HIP events around the phases of a batch call (only when opt.timing is set).
 struct PhaseTimer {
   bool on = false;
+  bool setup_first = false;   // marks 1 and 2 close the set-up and then the grid span
   hipStream_t s = nullptr;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   long long* stamp = nullptr;   // device: wall clock at the start of the call
@@ -542,6 +688,7 @@ struct PhaseTimer {
     MO_HIP(hipEventElapsedTime(&ms[1], ev[1], ev[2]));
     MO_HIP(hipEventElapsedTime(&ms[2], ev[2], ev[3]));
     MO_HIP(hipEventElapsedTime(&ms[3], ev[0], ev[4]));
+    if (setup_first) std::swap(ms[0], ms[1]);
     long long st = 0;
     MO_HIP(hipMemcpy(&st, stamp, sizeof(long long), hipMemcpyDeviceToHost));
     int dev = 0, rate_khz = 0;
@@ -596,7 +743,8 @@ void fill_stats(const LMResultDev* res, const int* path, const int* wgs, int np,
 constexpr int kMaxLdsLaunches = 32;   // register/LDS launches (shapes) per batch
 
 int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_stats* stats, hipStream_t s,
-            PhaseTimer& timer, char* ext = nullptr, size_t ext_cap = 0) {
+            PhaseTimer& timer, char* ext = nullptr, size_t ext_cap = 0,
+            const std::function<int()>& between = {}) {
   const int np = (int)pairs.size();
   if (np == 0) return MO_OK;
   MO_CHECK(opt.block_threads >= 64 && opt.block_threads <= 1024 && opt.block_threads % 64 == 0, MO_ERR_ARG,
@@ -657,29 +805,53 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
     h_pairs[p] = LMPairDesc{c.grid, c.N, (int)c.nv, vbase[p], 0, weighted ? 1.0f : (float)c.lambda, weighted ? 1 : 0};
     h_topo[p] = TopoPairDesc{c.idx, c.kind == 0 ? 3 * c.nidx : c.nidx, vbase[p], (int)c.nv, c.kind == 0 ? 0 : 1};
   }
-  if ((rc = chain_h2d(d_pairs, h_pairs.data(), np * sizeof(LMPairDesc), s))) return rc;
-  if ((rc = chain_h2d(d_topo, h_topo.data(), np * sizeof(TopoPairDesc), s))) return rc;
-  if ((rc = chain_h2d(d_io, h_io.data(), np * sizeof(PairIO), s))) return rc;
-  if ((rc = zero_fill(err, sizeof(int), s))) return rc;
-  if ((rc = topo_build_launch(d_topo, np, max_edges, max_nv, nvtot, raw, adj, deg, d_pstat, tws, err, s))) return rc;
+  // With work to enqueue in between (a batch's grid builds), the topology
+  // runs on a second stream, forked from s here and joined back into s
+  // after its sizes are read, so that the two overlap (not inside a batch
+  // graph's record / capture: those keep one stream).
+  AuxStream aux;
+  struct Ev {
+    hipEvent_t e = nullptr;
+    ~Ev() {
+      if (e) (void)hipEventDestroy(e);
+    }
+  } fork_ev, join_ev;
+  hipStream_t ts = s;
+  if (between && !g_chain) {
+    if ((rc = aux.acquire())) return rc;
+    MO_HIP(hipEventCreateWithFlags(&fork_ev.e, hipEventDisableTiming));
+    MO_HIP(hipEventCreateWithFlags(&join_ev.e, hipEventDisableTiming));
+    MO_HIP(hipEventRecord(fork_ev.e, s));
+    MO_HIP(hipStreamWaitEvent(aux.s, fork_ev.e, 0));
+    ts = aux.s;
+  }
+  if ((rc = chain_h2d(d_pairs, h_pairs.data(), np * sizeof(LMPairDesc), ts))) return rc;
+  if ((rc = chain_h2d(d_topo, h_topo.data(), np * sizeof(TopoPairDesc), ts))) return rc;
+  if ((rc = chain_h2d(d_io, h_io.data(), np * sizeof(PairIO), ts))) return rc;
+  if ((rc = zero_fill(err, sizeof(int), ts))) return rc;
+  if ((rc = topo_build_launch(d_topo, np, max_edges, max_nv, nvtot, raw, adj, deg, d_pstat, tws, err, ts))) return rc;
   for (int p = 0; p < np; ++p) {
     const CorePair& c = pairs[p];
     if (c.nv == 0 || c.kind != 2) continue;
-    hipLaunchKernelGGL(k_adaptive_weights, dim3((unsigned)((c.nv + 255) / 256)), dim3(256), 0, s, c.x0, (int)c.nv,
+    hipLaunchKernelGGL(k_adaptive_weights, dim3((unsigned)((c.nv + 255) / 256)), dim3(256), 0, ts, c.x0, (int)c.nv,
                        adj + vbase[p], raw, c.lambda, wt, deg + vbase[p]);
   }
   const dim3 io_grid((unsigned)std::min<long long>((max_nv_io + 255) / 256, 64), (unsigned)np);
-  hipLaunchKernelGGL(k_make_x0, io_grid, dim3(256), 0, s, d_io, deg, x0, u);
+  hipLaunchKernelGGL(k_make_x0, io_grid, dim3(256), 0, ts, d_io, deg, x0, u);
   MO_HIP(hipGetLastError());
   // per pair and register/LDS team shape: the largest chunk's adjacency entries
-  if ((rc = lm_lds_chunk_entries(d_pairs, adj, np, d_chunk, s))) return rc;
+  if ((rc = lm_lds_chunk_entries(d_pairs, adj, np, d_chunk, ts))) return rc;
   std::vector<int> pstat(2 * np), hchunk(4 * np);
   int herr = 0;
   {
     const ChainRead rd[3] = {{hchunk.data(), d_chunk, 4 * np * sizeof(int)},
                              {pstat.data(), d_pstat, 2 * np * sizeof(int)},
                              {&herr, err, sizeof(int)}};
-    if ((rc = chain_read(rd, 3, s))) return rc;
+    if ((rc = chain_read(rd, 3, ts, between))) return rc;
+  }
+  if (ts != s) {
+    MO_HIP(hipEventRecord(join_ev.e, ts));
+    MO_HIP(hipStreamWaitEvent(s, join_ev.e, 0));
   }
   MO_CHECK(herr == 0, MO_ERR_RANGE, "source topology: index out of range or too many vertices (flags 0x%x)", herr);
 
@@ -723,6 +895,10 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   MO_CHECK(nb <= kMaxLdsLaunches, MO_ERR_STATE, "%d register/LDS shapes exceed the %d work counters", nb,
            kMaxLdsLaunches);
   if ((rc = zero_fill(d_next, nb * sizeof(int), s))) return rc;
+  if (AsyncCtx* a = g_async) {
+    if (!a->lm_ready) MO_HIP(hipEventCreateWithFlags(&a->lm_ready, hipEventDisableTiming));
+    MO_HIP(hipEventRecord(a->lm_ready, s));
+  }
   if ((rc = timer.mark(2))) return rc;
   for (int b = 0; b < nb; ++b)
     if (!bucket[b].empty())
@@ -947,6 +1123,16 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
       return MO_OK;
     }
     c->pinned_bytes += al256c(np * sizeof(LMResultDev));
+  }
+  if (AsyncCtx* a = g_async) {
+    if (!d_abort) {
+      MO_HIP(hipMemcpyAsync(a->pinned, d_res, np * sizeof(LMResultDev), hipMemcpyDeviceToHost, s));
+      a->deferred = true;
+      a->np = np;
+      a->path = path;
+      a->wgs = nwgs;
+      return MO_OK;
+    }
   }
   std::vector<LMResultDev> h_res(np);
   MO_HIP(hipMemcpyAsync(h_res.data(), d_res, np * sizeof(LMResultDev), hipMemcpyDeviceToHost, s));
@@ -1272,11 +1458,20 @@ int rigid_deform_batch_impl(int32_t npairs, const double* const* src, const int6
     gspecs[p].dist32 = g32;
     pairs[p] = CorePair{g32, (int)N, sn, src_nv[p], src_faces[p], src_nf[p], 0, lambda[p], sp, out[p]};
   }
+  // normalization and the sources' topology first; the grid builds are
+  // enqueued while the host waits for the topology's sizes (they steer the
+  // solver launches, which then queue behind the grids)
   if ((rc = xform_batch_launch(tj.data(), npairs, true, xws_t, s))) return rc;
-  if ((rc = grid_build_batch_launch(gspecs.data(), npairs, gws_ptr, s, err))) return rc;
   if ((rc = xform_batch_launch(sj.data(), npairs, false, xws_s, s))) return rc;
-  if ((rc = timer.mark(1))) return rc;
-  if ((rc = lm_core(pairs, o, stats, s, timer))) return rc;
+  timer.setup_first = true;
+  auto grids = [&]() -> int {
+    int r = timer.mark(1);
+    if (r) return r;
+    // after another batch's solver launches (asynchronous batches): the
+    // face preparation runs at once, the grid build proper waits
+    return grid_build_batch_launch(gspecs.data(), npairs, gws_ptr, s, err, g_async ? g_async->after : nullptr);
+  };
+  if ((rc = lm_core(pairs, o, stats, s, timer, nullptr, 0, grids))) return rc;
   if ((rc = timer.finish())) return rc;
   if (o.timing == 2) {
     std::vector<unsigned long long> h((size_t)npairs * kStatStride);
@@ -1285,6 +1480,12 @@ int rigid_deform_batch_impl(int32_t npairs, const double* const* src, const int6
     for (int k = 0; k < kGridStatCount; ++k) {
       g_grid_stats[k] = 0;
       for (int p = 0; p < npairs; ++p) g_grid_stats[k] += (long long)h[(size_t)p * kStatStride + k];
+    }
+  }
+  if (AsyncCtx* a = g_async) {
+    if (a->deferred) {
+      MO_HIP(hipMemcpyAsync(a->pinned + a->err_off, err, sizeof(int), hipMemcpyDeviceToHost, s));
+      return MO_OK;
     }
   }
   return check_err_flag(err, s, "target faces");
@@ -1314,8 +1515,12 @@ MO_EXPORT int mo_rigid_deform_batch(int32_t npairs, const double* const* src, co
   if (rc || npairs == 0) return rc;
   mo_lm_options o;
   if (opt) o = *opt; else mo_lm_default_options(&o);
-  return rigid_deform_batch_impl(npairs, src, src_nv, src_faces, src_nf, tgt, tgt_nv, tgt_faces, tgt_nf,
-                                 grid_resolution, lambda, o, out, stats, as_stream(stream));
+  HostStage stage(as_stream(stream));
+  StageScope scope(&stage);
+  rc = rigid_deform_batch_impl(npairs, src, src_nv, src_faces, src_nf, tgt, tgt_nv, tgt_faces, tgt_nf,
+                               grid_resolution, lambda, o, out, stats, as_stream(stream));
+  stage.drained = rc == MO_OK;   // a successful call ends with a synchronization
+  return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -1447,6 +1652,93 @@ MO_EXPORT int mo_batch_graph_destroy(mo_batch_graph* graph) {
   MO_CHECK(dev == graph->device, MO_ERR_STATE, "batch graph belongs to device %d, current device is %d",
            graph->device, dev);
   delete graph;
+  return MO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Asynchronous batches (see AsyncCtx).
+
+namespace {
+
+struct AsyncScope {
+  explicit AsyncScope(AsyncCtx* a) { g_async = a; }
+  ~AsyncScope() { g_async = nullptr; }
+};
+
+}  // namespace
+
+struct mo_batch_handle {
+  hipEvent_t done = nullptr;
+  char* pinned = nullptr;
+  size_t pinned_cap = 0;
+  mo::AsyncCtx ctx;
+  std::vector<mo_lm_stats> stats;   // the batch completed inside the call (team launches)
+  int npairs = 0;
+  std::unique_ptr<mo::HostStage> stage;
+  ~mo_batch_handle() {
+    stage.reset();   // waits for the stream unless the batch is known to be complete
+    if (done) (void)hipEventDestroy(done);
+    pinned_put(pinned, pinned_cap);
+  }
+};
+
+MO_EXPORT int mo_rigid_deform_batch_async(int32_t npairs, const double* const* src, const int64_t* src_nv,
+                                          const int32_t* const* src_faces, const int64_t* src_nf,
+                                          const double* const* tgt, const int64_t* tgt_nv,
+                                          const int32_t* const* tgt_faces, const int64_t* tgt_nf,
+                                          const int32_t* grid_resolution, const double* lambda,
+                                          const mo_lm_options* opt, double* const* out, void* stream,
+                                          mo_batch_handle* after, mo_batch_handle** handle) {
+  MO_CHECK(handle, MO_ERR_ARG, "handle must not be null");
+  *handle = nullptr;
+  int rc = check_rigid_args(npairs, src, src_nv, src_faces, src_nf, tgt, tgt_nv, tgt_faces, tgt_nf, grid_resolution,
+                            lambda, out);
+  if (rc) return rc;
+  mo_lm_options o;
+  if (opt) o = *opt; else mo_lm_default_options(&o);
+  MO_CHECK(o.timing == 0, MO_ERR_ARG, "options.timing must be 0 for an asynchronous batch");
+  std::unique_ptr<mo_batch_handle> h(new mo_batch_handle);
+  const size_t res_bytes = al256c((size_t)std::max(npairs, 1) * sizeof(LMResultDev));
+  if ((rc = pinned_get(res_bytes + 256, &h->pinned, &h->pinned_cap))) return rc;
+  h->ctx.pinned = h->pinned;
+  h->ctx.err_off = res_bytes;
+  h->stats.resize((size_t)std::max(npairs, 1));
+  h->npairs = npairs;
+  hipStream_t s = as_stream(stream);
+  h->stage.reset(new HostStage(s));
+  if (after) h->ctx.after = after->ctx.lm_ready;   // null: that batch launched no solver
+  {
+    AsyncScope scope(&h->ctx);
+    StageScope stage_scope(h->stage.get());
+    if ((rc = rigid_deform_batch_impl(npairs, src, src_nv, src_faces, src_nf, tgt, tgt_nv, tgt_faces, tgt_nf,
+                                      grid_resolution, lambda, o, out, h->stats.data(), s)))
+      return rc;
+  }
+  if (!h->ctx.deferred) {
+    h->stage->drained = true;   // completed inside the call
+  } else {
+    MO_HIP(hipEventCreateWithFlags(&h->done, hipEventDisableTiming));
+    MO_HIP(hipEventRecord(h->done, s));
+  }
+  h->ctx.after = nullptr;
+  *handle = h.release();
+  return MO_OK;
+}
+
+MO_EXPORT int mo_batch_wait(mo_batch_handle* handle, mo_lm_stats* stats) {
+  MO_CHECK(handle, MO_ERR_ARG, "null batch handle");
+  std::unique_ptr<mo_batch_handle> h(handle);
+  const AsyncCtx& a = h->ctx;
+  if (!a.deferred) {
+    if (stats) std::copy(h->stats.begin(), h->stats.begin() + h->npairs, stats);
+    return MO_OK;
+  }
+  MO_HIP(hipEventSynchronize(h->done));
+  h->stage->drained = true;   // every staged copy precedes the event
+  int herr = 0;
+  memcpy(&herr, h->pinned + a.err_off, sizeof(int));
+  MO_CHECK(herr == 0, MO_ERR_RANGE, "target faces: index out of range (flags 0x%x)", herr);
+  if (stats) fill_stats(reinterpret_cast<const LMResultDev*>(h->pinned), a.path.data(), a.wgs.data(), a.np, stats);
   return MO_OK;
 }
 

@@ -81,7 +81,10 @@ size_t grid_batch_workspace_bytes(const GridSpec* specs, int n);
 
 // Builds every grid of the batch with one launch per stage.  err_dev: device
 // int, bit 0 set when a face index is out of range.
-int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipStream_t stream, int* err_dev);
+// build_after (optional): the grid build proper (not the face preparation
+// before it) waits for this event on `stream`.
+int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipStream_t stream, int* err_dev,
+                            hipEvent_t build_after = nullptr);
 
 // Mesh::Normalize / Mesh::ApplyTransform jobs: V [n,3] (FP32 or FP64),
 // sp = (scale, pos[3]) in device memory, out = (V - pos) / scale (FP64).

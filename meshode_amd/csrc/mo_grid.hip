@@ -94,19 +94,30 @@ __global__ void k_face_bins(const GridJob* __restrict__ jobs, int* __restrict__ 
 }
 
 // Pass 2: one workgroup per job counting-sorts face indices by bin (LDS
-// histogram, block-wide scan of the 4096 bins, LDS-atomic scatter).
-__global__ void __launch_bounds__(1024) k_bin_sort(const GridJob* __restrict__ jobs) {
+// histogram, block-wide scan of the 4096 bins, LDS-atomic scatter).  The
+// set-up kernels of a batch use workgroups of at most 256 threads (one
+// wavefront per SIMD), which fit beside a running grid build's wavefronts,
+// so a second batch's set-up is not held back behind them.
+constexpr int kSortT = 256;
+constexpr int kBinsPerThread = kBins / kSortT;   // 16 consecutive bins each
+__global__ void __launch_bounds__(kSortT) k_bin_sort(const GridJob* __restrict__ jobs) {
   const GridJob J = jobs[blockIdx.x];
   __shared__ unsigned hist[kBins];
-  __shared__ unsigned wsum[16];
+  __shared__ unsigned wsum[kSortT / 64];
   for (int b = threadIdx.x; b < kBins; b += blockDim.x) hist[b] = 0;
   __syncthreads();
   for (long long f = threadIdx.x; f < J.nf; f += blockDim.x) atomicAdd(&hist[J.bin[f]], 1u);
   __syncthreads();
-  // exclusive scan: each of the 1024 threads owns 4 consecutive bins
+  // exclusive scan: each thread owns kBinsPerThread consecutive bins
   const int t = threadIdx.x;
-  unsigned v0 = hist[4 * t], v1 = hist[4 * t + 1], v2 = hist[4 * t + 2], v3 = hist[4 * t + 3];
-  unsigned mine = v0 + v1 + v2 + v3, incl = mine;
+  unsigned v[kBinsPerThread];
+  unsigned mine = 0;
+#pragma unroll
+  for (int q = 0; q < kBinsPerThread; ++q) {
+    v[q] = hist[kBinsPerThread * t + q];
+    mine += v[q];
+  }
+  unsigned incl = mine;
   const int lane = t & 63, w = t >> 6;
   for (int o = 1; o < 64; o <<= 1) {
     unsigned y = __shfl_up(incl, o, 64);
@@ -117,10 +128,11 @@ __global__ void __launch_bounds__(1024) k_bin_sort(const GridJob* __restrict__ j
   unsigned base = 0;
   for (int q = 0; q < w; ++q) base += wsum[q];
   unsigned ex = base + incl - mine;
-  hist[4 * t] = ex;
-  hist[4 * t + 1] = ex + v0;
-  hist[4 * t + 2] = ex + v0 + v1;
-  hist[4 * t + 3] = ex + v0 + v1 + v2;
+#pragma unroll
+  for (int q = 0; q < kBinsPerThread; ++q) {
+    hist[kBinsPerThread * t + q] = ex;
+    ex += v[q];
+  }
   __syncthreads();
   for (long long f = threadIdx.x; f < J.nf; f += blockDim.x) {
     unsigned pos = atomicAdd(&hist[J.bin[f]], 1u);
@@ -129,7 +141,10 @@ __global__ void __launch_bounds__(1024) k_bin_sort(const GridJob* __restrict__ j
 }
 
 // Pass 3: FP64 face record of face perm[pos], written at its sorted position.
-__global__ void k_face_prep(const GridJob* __restrict__ jobs) {
+// At most 64 VGPRs (8 wavefronts per SIMD): the kernel then fits beside a
+// running grid build's wavefronts (85 VGPRs, 5 per SIMD), so a second
+// batch's face preparation is not held back until that build ends.
+__global__ void __launch_bounds__(256, 8) k_face_prep(const GridJob* __restrict__ jobs) {
   const GridJob J = jobs[blockIdx.y];
   const double* __restrict__ V = J.V;
   const int* __restrict__ F = J.F;
@@ -501,10 +516,12 @@ __device__ inline double xf_read(const XformJob& J, long long i) {
   return J.f64 ? ((const double*)J.V)[i] : (double)((const float*)J.V)[i];
 }
 
-// One workgroup per job: min / max per axis, then scale and pos.
-__global__ void __launch_bounds__(1024) k_normalize_stats(const XformJob* __restrict__ jobs) {
+// One workgroup per job: min / max per axis, then scale and pos (exact
+// whatever the workgroup size: minima and maxima do not round).
+constexpr int kStatT = 256;
+__global__ void __launch_bounds__(kStatT) k_normalize_stats(const XformJob* __restrict__ jobs) {
   const XformJob J = jobs[blockIdx.x];
-  __shared__ double smn[3][1024], smx[3][1024];
+  __shared__ double smn[3][kStatT], smx[3][kStatT];
   double mn[3] = {1e30, 1e30, 1e30}, mx[3] = {-1e30, -1e30, -1e30};
   for (long long i = threadIdx.x; i < J.n; i += blockDim.x)
     for (int a = 0; a < 3; ++a) {
@@ -557,7 +574,8 @@ size_t grid_job_workspace_bytes(long long nf) {
          al256((size_t)nf * sizeof(int)) + al256((size_t)ntiles * 6 * sizeof(double));
 }
 
-int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipStream_t stream, int* err_dev) {
+int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipStream_t stream, int* err_dev,
+                            hipEvent_t build_after) {
   if (n <= 0) return MO_OK;
   std::vector<GridJob> jobs(n);
   char* w = (char*)workspace;
@@ -590,9 +608,10 @@ int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipSt
   const int T = 256;
   const unsigned fb = (unsigned)std::min<long long>((max_nf + T - 1) / T, 1024);
   hipLaunchKernelGGL(k_face_bins, dim3(fb, n), dim3(T), 0, stream, d_jobs, err_dev);
-  hipLaunchKernelGGL(k_bin_sort, dim3(n), dim3(1024), 0, stream, d_jobs);
+  hipLaunchKernelGGL(k_bin_sort, dim3(n), dim3(kSortT), 0, stream, d_jobs);
   hipLaunchKernelGGL(k_face_prep, dim3(fb, n), dim3(T), 0, stream, d_jobs);
   hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((max_tiles + 3) / 4), n), dim3(256), 0, stream, d_jobs);
+  if (build_after) MO_HIP(hipStreamWaitEvent(stream, build_after, 0));
   hipLaunchKernelGGL(k_grid_build, dim3((unsigned)max_bricks, n), dim3(64), 0, stream, d_jobs);
   MO_HIP(hipGetLastError());
   return MO_OK;
@@ -613,7 +632,7 @@ int xform_batch_launch(const XformJob* jobs, int n, bool compute_stats, void* wo
   for (int q = 0; q < n; ++q) max_n = std::max(max_n, jobs[q].n);
   int rc = chain_h2d(d_jobs, jobs, (size_t)n * sizeof(XformJob), stream);
   if (rc) return rc;
-  if (compute_stats) hipLaunchKernelGGL(k_normalize_stats, dim3(n), dim3(1024), 0, stream, d_jobs);
+  if (compute_stats) hipLaunchKernelGGL(k_normalize_stats, dim3(n), dim3(kStatT), 0, stream, d_jobs);
   if (max_n > 0) {
     const unsigned bx = (unsigned)std::min<long long>((max_n * 3 + 255) / 256, 1024);
     hipLaunchKernelGGL(k_apply_transform, dim3(bx, n), dim3(256), 0, stream, d_jobs);

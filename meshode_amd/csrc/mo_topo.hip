@@ -102,41 +102,44 @@ __global__ void k_topo_finish(const TopoPairDesc* __restrict__ pairs, const long
 }
 
 // Exclusive scan of unsigned counts into 64-bit offsets: per-block scan,
-// block totals scanned by one block, then added back.
-__global__ void __launch_bounds__(1024) k_scan_blocks(const unsigned* __restrict__ in, long long n,
-                                                      long long* __restrict__ out, long long* __restrict__ bsum) {
-  __shared__ long long s[1024];
-  long long i = (long long)blockIdx.x * 1024 + threadIdx.x;
+// block totals scanned by one block, then added back.  Workgroups of 256
+// threads (one wavefront per SIMD) fit beside a running grid build.
+constexpr int kScanT = 256;
+
+__global__ void __launch_bounds__(kScanT) k_scan_blocks(const unsigned* __restrict__ in, long long n,
+                                                        long long* __restrict__ out, long long* __restrict__ bsum) {
+  __shared__ long long s[kScanT];
+  long long i = (long long)blockIdx.x * kScanT + threadIdx.x;
   long long x = i < n ? (long long)in[i] : 0;
   s[threadIdx.x] = x;
   __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
+  for (int d = 1; d < kScanT; d <<= 1) {
     long long t = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0;
     __syncthreads();
     s[threadIdx.x] += t;
     __syncthreads();
   }
   if (i < n) out[i] = s[threadIdx.x] - x;
-  if (threadIdx.x == 1023) bsum[blockIdx.x] = s[1023];
+  if (threadIdx.x == kScanT - 1) bsum[blockIdx.x] = s[kScanT - 1];
 }
 
-__global__ void __launch_bounds__(1024) k_scan_sums(long long* __restrict__ bsum, long long nb,
-                                                    long long* __restrict__ total) {
-  __shared__ long long s[1024];
+__global__ void __launch_bounds__(kScanT) k_scan_sums(long long* __restrict__ bsum, long long nb,
+                                                      long long* __restrict__ total) {
+  __shared__ long long s[kScanT];
   long long carry = 0;
-  for (long long base = 0; base < nb; base += 1024) {
+  for (long long base = 0; base < nb; base += kScanT) {
     long long i = base + threadIdx.x;
     long long x = i < nb ? bsum[i] : 0;
     s[threadIdx.x] = x;
     __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
+    for (int d = 1; d < kScanT; d <<= 1) {
       long long t = threadIdx.x >= (unsigned)d ? s[threadIdx.x - d] : 0;
       __syncthreads();
       s[threadIdx.x] += t;
       __syncthreads();
     }
     if (i < nb) bsum[i] = carry + s[threadIdx.x] - x;
-    long long tot = s[1023];
+    long long tot = s[kScanT - 1];
     __syncthreads();
     carry += tot;
   }
@@ -144,22 +147,22 @@ __global__ void __launch_bounds__(1024) k_scan_sums(long long* __restrict__ bsum
 }
 
 __global__ void k_scan_add(long long* __restrict__ out, long long n, const long long* __restrict__ bsum) {
-  long long i = (long long)blockIdx.x * 1024 + threadIdx.x;
+  long long i = (long long)blockIdx.x * kScanT + threadIdx.x;
   if (i < n) out[i] += bsum[blockIdx.x];
 }
 
 }  // namespace
 
-size_t scan_workspace_bytes(long long n) { return (size_t)(((n + 1023) / 1024) + 1) * sizeof(long long); }
+size_t scan_workspace_bytes(long long n) { return (size_t)(((n + kScanT - 1) / kScanT) + 1) * sizeof(long long); }
 
 int exclusive_scan_launch(const unsigned* in, long long n, long long* out, long long* total_dev, void* ws,
                           hipStream_t s) {
   if (n <= 0) return MO_OK;
-  long long nb = (n + 1023) / 1024;
+  long long nb = (n + kScanT - 1) / kScanT;
   long long* bsum = (long long*)ws;
-  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(1024), 0, s, in, n, out, bsum);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, bsum, nb, total_dev);
-  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(1024), 0, s, out, n, bsum);
+  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(kScanT), 0, s, in, n, out, bsum);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanT), 0, s, bsum, nb, total_dev);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kScanT), 0, s, out, n, bsum);
   MO_HIP(hipGetLastError());
   return MO_OK;
 }

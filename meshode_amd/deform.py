@@ -154,6 +154,37 @@ class PairBatch:
         _lib.call("mo_rigid_deform_batch", *self._args, C.byref(opt), self._out_ptrs, stats, _stream())
         return [s.as_dict() for s in stats]
 
+    def run_async(self, options=None, after=None):
+        """mo_rigid_deform_batch_async: enqueue the batch on the current
+        stream and return once its solver launches are queued; wait()
+        returns its stats.  after: another PairBatch with a batch in flight
+        (on another stream) whose solver launches this batch's grid builds
+        follow.  One batch of this object in flight at a time (its outputs
+        are self.out)."""
+        if getattr(self, "_pending", None) is not None:
+            raise RuntimeError("a batch of this PairBatch is still in flight: call wait() first")
+        prev = None
+        if after is not None:
+            prev = getattr(after, "_pending", None)
+            if prev is None:
+                raise RuntimeError("`after` has no batch in flight")
+        opt = options or default_options()
+        h = C.c_void_p()
+        _lib.call("mo_rigid_deform_batch_async", *self._args, C.byref(opt), self._out_ptrs, _stream(), prev,
+                  C.byref(h))
+        self._pending = h
+
+    def wait(self):
+        """mo_batch_wait on the batch in flight: waits for it and returns its
+        stats (as run does)."""
+        h = getattr(self, "_pending", None)
+        if h is None:
+            raise RuntimeError("no batch in flight: call run_async() first")
+        self._pending = None
+        stats = (_lib.LMStats * self.n)()
+        _lib.call("mo_batch_wait", h, stats)
+        return [s.as_dict() for s in stats]
+
     def capture(self, options=None):
         """mo_rigid_deform_batch_graph: solve the batch once (returns its
         stats, like run) and capture the whole launch chain into a HIP graph
