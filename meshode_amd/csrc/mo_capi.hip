@@ -808,7 +808,8 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
   // With work to enqueue in between (a batch's grid builds), the topology
   // runs on a second stream, forked from s here and joined back into s
   // after its sizes are read, so that the two overlap (not inside a batch
-  // graph's record / capture: those keep one stream).
+  // graph's record / capture, which keep one stream, and not with phase
+  // timing on, whose events time the phases one after the other).
   AuxStream aux;
   struct Ev {
     hipEvent_t e = nullptr;
@@ -817,7 +818,7 @@ int lm_core(const std::vector<CorePair>& pairs, const mo_lm_options& opt, mo_lm_
     }
   } fork_ev, join_ev;
   hipStream_t ts = s;
-  if (between && !g_chain) {
+  if (between && !g_chain && opt.timing == 0) {
     if ((rc = aux.acquire())) return rc;
     MO_HIP(hipEventCreateWithFlags(&fork_ev.e, hipEventDisableTiming));
     MO_HIP(hipEventCreateWithFlags(&join_ev.e, hipEventDisableTiming));
