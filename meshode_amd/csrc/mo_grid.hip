@@ -416,18 +416,19 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
       dst[c] = src[s_list[r] * kChunks + w];
     }
     __syncthreads();
-    // stage 2, per surviving face: every lane's own AABB and plane bounds,
-    // then the exact test when some lane can still improve or tie its best
+    // stage 2, per surviving face: every lane's own AABB bound, then (when
+    // it leaves some lane) the plane bound, then the exact test when some
+    // lane can still improve or tie its best
     for (int r = 0; r < ns; ++r) {
       const GFace& g = s_face[r];
       double gx = vmax(0.0, vmax(g.lox - qx, qx - g.hix));
       double gy = vmax(0.0, vmax(g.loy - qy, qy - g.hiy));
       double gz = vmax(0.0, vmax(g.loz - qz, qz - g.hiz));
-      double lb = gx * gx + gy * gy + gz * gz;
+      bool mine = valid && !(gx * gx + gy * gy + gz * gz > bcut);
+      if (!__ballot(mine)) continue;
       // the face plane is a second lower bound (0 for degenerate faces)
       const double dp = g.nx * (qx - g.ax) + g.ny * (qy - g.ay) + g.nz * (qz - g.az);
-      lb = vmax(lb, (dp * dp) * g.inv_nn);
-      const bool mine = valid && !(lb > bcut);
+      mine = mine && !((dp * dp) * g.inv_nn > bcut);
       const unsigned long long need = __ballot(mine);
       if (need) {
         ++n_exact;
