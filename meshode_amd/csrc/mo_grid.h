@@ -70,6 +70,8 @@ struct GridJob {
   unsigned* bin;
   int* perm;
   double* tb;
+  long long nsuper;   // super-tiles: kSuperTiles consecutive tiles each
+  double* sb;         // their AABBs [nsuper][6]
   double* dist64;
   float* dist32;
   int* nearest;

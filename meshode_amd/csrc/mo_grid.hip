@@ -52,6 +52,11 @@ namespace {
 // smaller first-stage cull loses); 8 was slower again
 constexpr int kTile = 16;
 constexpr int kSeedTiles = 4;  // nearest tiles evaluated before the culled walk
+// tiles per super-tile: the walk culls 256-face super-tiles against the
+// brick first, then the tiles of the kept ones (four super-tiles, 64 tiles,
+// per wavefront pass), so the walk's cost grows with the super-tiles near
+// the brick rather than with the whole target
+constexpr int kSuperTiles = 16;
 constexpr int kBinBits = 4;                  // 16 bins per axis
 constexpr int kBins = 1 << (3 * kBinBits);   // 4096
 
@@ -222,6 +227,21 @@ __global__ void __launch_bounds__(256) k_tile_bounds(const GridJob* __restrict__
   }
 }
 
+// One thread per super-tile: the union of its tiles' AABBs.
+__global__ void k_super_bounds(const GridJob* __restrict__ jobs) {
+  const GridJob J = jobs[blockIdx.y];
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= J.nsuper) return;
+  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  const long long t1 = (s + 1) * kSuperTiles < J.ntiles ? (s + 1) * kSuperTiles : J.ntiles;
+  for (long long t = s * kSuperTiles; t < t1; ++t)
+    for (int a = 0; a < 3; ++a) {
+      v[a] = dmin(v[a], J.tb[t * 6 + a]);
+      v[3 + a] = dmax(v[3 + a], J.tb[t * 6 + 3 + a]);
+    }
+  for (int a = 0; a < 6; ++a) J.sb[s * 6 + a] = v[a];
+}
+
 // Canonical edge squared distance (endpoints ordered by vertex index).
 __device__ inline double edge_d2(double qx, double qy, double qz, double ux, double uy, double uz,
                                  double wx, double wy, double wz, double ll, double il) {
@@ -356,6 +376,8 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
   const long long nf = JJ.nf;
   const MO_GLOBAL double* __restrict__ tb = gptr(JJ.tb);
   const long long ntiles = JJ.ntiles;
+  const MO_GLOBAL double* __restrict__ sb = gptr(JJ.sb);
+  const long long nsuper = JJ.nsuper;
   const int N = JJ.N;
   const int bricks_per_axis = (N + 3) / 4;
   MO_GLOBAL double* __restrict__ dist64 = gptr(JJ.dist64);
@@ -449,50 +471,92 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
     __syncthreads();  // the next tile overwrites s_face / s_list
   };
 
-  // 1. seeds: one scan of the tile AABBs; each lane keeps its nearest tile,
-  //    and the kSeedTiles nearest of those are evaluated first.  They give
-  //    every lane a near-exact distance, which makes the culling below tight
-  //    (the choice of seeds affects speed only).
-  double mlb = INFINITY;
-  long long mt = -1;
-  for (long long c = 0; c < ntiles; c += 64) {
-    const long long t = c + lane;
-    if (t >= ntiles) continue;
-    const double lb = box_gap2(tb, t, lx, ly, lz, hx, hy, hz);
-    if (lb < mlb) { mlb = lb; mt = t; }
-  }
+  // 1. seeds: the four super-tiles nearest to the brick's centre (one scan of the
+  //    super-tile AABBs; each lane keeps its nearest), then the kSeedTiles
+  //    nearest of their 64 tiles, are evaluated first.  They give every
+  //    lane a near-exact distance, which makes the culling below tight (the
+  //    choice of seeds affects speed only).
   int seeds[kSeedTiles];
   int nseed = 0;
-  for (int r = 0; r < kSeedTiles; ++r) {
-    double ol = mlb;
-    int ot = (int)mt;
-    wave_argmin(ol, ot);
-    if (ot < 0) break;
-    const int st = ot;
-    if (mt == st) { mlb = INFINITY; mt = -1; }
-    seeds[nseed++] = st;
-    process_tile(st);
+  {
+    // ranked by the AABB gap from the brick's centre (a brick inside
+    // several AABBs has gap 0 to all of them)
+    const double cx = 0.5 * (lx + hx), cy = 0.5 * (ly + hy), cz = 0.5 * (lz + hz);
+    double slb = INFINITY;
+    int ss = -1;
+    for (long long c = 0; c < nsuper; c += 64) {
+      const long long sidx = c + lane;
+      if (sidx >= nsuper) continue;
+      const double lb = box_gap2(sb, sidx, cx, cy, cz, cx, cy, cz);
+      if (lb < slb) { slb = lb; ss = (int)sidx; }
+    }
+    int s4[4] = {-1, -1, -1, -1};
+    for (int q = 0; q < 4; ++q) {
+      double ol = slb;
+      int os = ss;
+      wave_argmin(ol, os);
+      if (os < 0) break;
+      if (ss == os) { slb = INFINITY; ss = -1; }
+      s4[q] = os;
+    }
+    const int q = lane >> 4;
+    const int sq = q == 0 ? s4[0] : q == 1 ? s4[1] : q == 2 ? s4[2] : s4[3];
+    double mlb = INFINITY;
+    int mt = -1;
+    if (sq >= 0) {
+      const long long t = (long long)sq * kSuperTiles + (lane & (kSuperTiles - 1));
+      if (t < ntiles) {
+        mlb = box_gap2(tb, t, cx, cy, cz, cx, cy, cz);
+        mt = (int)t;
+      }
+    }
+    for (int r = 0; r < kSeedTiles; ++r) {
+      double ol = mlb;
+      int ot = mt;
+      wave_argmin(ol, ot);
+      if (ot < 0) break;
+      if (mt == ot) { mlb = INFINITY; mt = -1; }
+      seeds[nseed++] = ot;
+    }
   }
+  for (int r = 0; r < nseed; ++r) process_tile(seeds[r]);
 
   // 2. every other tile that can still improve (or tie) some lane's best:
-  //    a brick-level prefilter (brick AABB to tile AABB against the wave's
-  //    worst best) selects candidates 64 at a time, then each candidate is
-  //    kept only if some lane's own point-to-AABB bound reaches its own best.
-  for (long long c = 0; c < ntiles; c += 64) {
-    long long t = c + lane;
-    bool taken = false;
-    for (int q = 0; q < nseed; ++q) taken |= (seeds[q] == (int)t);
-    double lb = t < ntiles ? box_gap2(tb, t, lx, ly, lz, hx, hy, hz) : INFINITY;
-    double bound = wave_max(valid ? best : -INFINITY);
-    bool need = (t < ntiles) && !taken && !(lb > bound * (1.0 + 1e-9));
-    unsigned long long mask = __ballot(need);
-    while (mask) {
-      int l = __ffsll((long long)mask) - 1;
-      mask &= mask - 1;
-      const int tt = __builtin_amdgcn_readfirstlane((int)(c + l));
-      const double pl = box_gap2(tb, tt, qx, qy, qz, qx, qy, qz);
-      if (!__ballot(valid && !(pl > bcut))) continue;
-      process_tile(tt);
+  //    super-tiles whose AABB is within the wave's worst best of the brick
+  //    AABB, then (four kept super-tiles per pass, lane 16 q + j on tile j
+  //    of the q-th) their tiles by the same test, and each such tile is
+  //    kept only if some lane's own point-to-AABB bound reaches its own
+  //    best.
+  for (long long c = 0; c < nsuper; c += 64) {
+    const long long sidx = c + lane;
+    const double slb = sidx < nsuper ? box_gap2(sb, sidx, lx, ly, lz, hx, hy, hz) : INFINITY;
+    const double sbound = wave_max(valid ? best : -INFINITY);
+    unsigned long long smask = __ballot(sidx < nsuper && !(slb > sbound * (1.0 + 1e-9)));
+    while (smask) {
+      int s4[4] = {-1, -1, -1, -1};
+      for (int q = 0; q < 4 && smask; ++q) {
+        s4[q] = (int)c + __ffsll((long long)smask) - 1;
+        smask &= smask - 1;
+      }
+      const int q = lane >> 4;
+      const int sq = q == 0 ? s4[0] : q == 1 ? s4[1] : q == 2 ? s4[2] : s4[3];
+      const long long t = sq < 0 ? -1 : (long long)sq * kSuperTiles + (lane & (kSuperTiles - 1));
+      const double bound = wave_max(valid ? best : -INFINITY);
+      bool need = false;
+      if (t >= 0 && t < ntiles) {
+        bool taken = false;
+        for (int r = 0; r < nseed; ++r) taken |= (seeds[r] == (int)t);
+        need = !taken && !(box_gap2(tb, t, lx, ly, lz, hx, hy, hz) > bound * (1.0 + 1e-9));
+      }
+      unsigned long long tmask = __ballot(need);
+      while (tmask) {
+        const int l = __ffsll((long long)tmask) - 1;
+        tmask &= tmask - 1;
+        const int tt = __builtin_amdgcn_readlane((int)t, l);
+        const double pl = box_gap2(tb, tt, qx, qy, qz, qx, qy, qz);
+        if (!__ballot(valid && !(pl > bcut))) continue;
+        process_tile(tt);
+      }
     }
   }
   if (JJ.stats && lane == 0) {
@@ -571,8 +635,10 @@ static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 size_t grid_job_workspace_bytes(long long nf) {
   const long long ntiles = (nf + kTile - 1) / kTile;
+  const long long nsuper = (ntiles + kSuperTiles - 1) / kSuperTiles;
   return al256((size_t)nf * sizeof(GFace)) + al256((size_t)nf * sizeof(GCull)) + al256((size_t)nf * sizeof(unsigned)) +
-         al256((size_t)nf * sizeof(int)) + al256((size_t)ntiles * 6 * sizeof(double));
+         al256((size_t)nf * sizeof(int)) + al256((size_t)ntiles * 6 * sizeof(double)) +
+         al256((size_t)nsuper * 6 * sizeof(double));
 }
 
 int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipStream_t stream, int* err_dev,
@@ -582,7 +648,7 @@ int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipSt
   char* w = (char*)workspace;
   GridJob* d_jobs = (GridJob*)w;
   w += al256((size_t)n * sizeof(GridJob));
-  long long max_nf = 0, max_tiles = 0, max_bricks = 0;
+  long long max_nf = 0, max_tiles = 0, max_bricks = 0, max_super = 0;
   for (int q = 0; q < n; ++q) {
     const GridSpec& sp = specs[q];
     MO_CHECK(sp.N >= 2 && sp.N <= 2048, MO_ERR_ARG, "grid resolution %d out of range [2, 2048]", sp.N);
@@ -595,6 +661,8 @@ int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipSt
     J.bin = (unsigned*)w; w += al256((size_t)sp.nf * sizeof(unsigned));
     J.perm = (int*)w; w += al256((size_t)sp.nf * sizeof(int));
     J.tb = (double*)w; w += al256((size_t)J.ntiles * 6 * sizeof(double));
+    J.nsuper = (J.ntiles + kSuperTiles - 1) / kSuperTiles;
+    J.sb = (double*)w; w += al256((size_t)J.nsuper * 6 * sizeof(double));
     J.dist64 = sp.dist64; J.dist32 = sp.dist32; J.nearest = sp.nearest;
     MO_CHECK(sp.band2 > 0.0, MO_ERR_ARG, "grid band must be positive");
     J.band2 = sp.band2;
@@ -602,6 +670,7 @@ int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipSt
     const long long bpa = (sp.N + 3) / 4;
     max_nf = std::max(max_nf, sp.nf);
     max_tiles = std::max(max_tiles, J.ntiles);
+    max_super = std::max(max_super, J.nsuper);
     max_bricks = std::max(max_bricks, bpa * bpa * bpa);
   }
   int rc = chain_h2d(d_jobs, jobs.data(), (size_t)n * sizeof(GridJob), stream);
@@ -612,6 +681,7 @@ int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipSt
   hipLaunchKernelGGL(k_bin_sort, dim3(n), dim3(kSortT), 0, stream, d_jobs);
   hipLaunchKernelGGL(k_face_prep, dim3(fb, n), dim3(T), 0, stream, d_jobs);
   hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)((max_tiles + 3) / 4), n), dim3(256), 0, stream, d_jobs);
+  hipLaunchKernelGGL(k_super_bounds, dim3((unsigned)((max_super + 63) / 64), n), dim3(64), 0, stream, d_jobs);
   if (build_after) MO_HIP(hipStreamWaitEvent(stream, build_after, 0));
   hipLaunchKernelGGL(k_grid_build, dim3((unsigned)max_bricks, n), dim3(64), 0, stream, d_jobs);
   MO_HIP(hipGetLastError());
