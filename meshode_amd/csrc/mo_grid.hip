@@ -16,25 +16,30 @@
 //   1. k_face_bins   : per face, index check + Morton bin of the centroid
 //                      (16^3 bins).
 //   2. k_bin_sort    : one workgroup counting-sorts face indices by bin (LDS
-//                      histogram + scan) so 64-face tiles are spatially tight.
+//                      histogram + scan) so tiles of consecutive faces are
+//                      spatially tight.
 //   3. k_face_prep   : FP64 face record per face, written in sorted order.
-//   4. k_tile_bounds : AABB per tile of 16 sorted faces.
+//   4. k_tile_bounds : AABB per tile of kTile (8) sorted faces;
+//      k_super_bounds: AABB per super-tile of kSuperTiles (16) tiles.
 //   5. k_grid_build  : one wavefront per 4x4x4 brick of grid points.  The
-//                      wave first evaluates the 4 tiles closest to its brick
-//                      (which seed every lane with a near-exact distance),
-//                      then walks all tiles 64 at a time: a brick-level
-//                      ballot drops tiles whose AABB is farther from the
-//                      brick than the wave's worst best distance, and each
-//                      remaining tile is kept only if some lane's own
-//                      point-to-AABB bound still reaches that lane's best.
-//                      For a kept tile, one lane per face first drops the
-//                      faces whose AABB is out of reach of the whole brick;
-//                      the survivors' records are copied into LDS with
-//                      coalesced loads, and the wave walks them (one LDS
-//                      address for all lanes), testing a face exactly only
-//                      when its AABB and plane bounds can still improve
-//                      some lane.  Ties keep the lowest face index, so the
-//                      visiting order never changes the result.
+//                      wave first evaluates the 4 tiles nearest its brick's
+//                      centre among the tiles of the 4 nearest super-tiles
+//                      (they seed every lane with a near-exact distance),
+//                      then walks the super-tiles 64 at a time: a
+//                      brick-level ballot drops super-tiles whose AABB is
+//                      farther from the brick than the wave's worst best
+//                      distance, the tiles of the kept ones (64 per pass)
+//                      go through the same test, and each remaining tile is
+//                      kept only if some lane's own point-to-AABB bound
+//                      still reaches that lane's best.  For a kept tile,
+//                      one lane per face first drops the faces whose AABB
+//                      is out of reach of the whole brick; the survivors'
+//                      records are copied into LDS with coalesced loads,
+//                      and the wave walks them (one LDS address for all
+//                      lanes), testing a face exactly only when its AABB
+//                      and plane bounds can still improve some lane.  Ties
+//                      keep the lowest face index, so the visiting order
+//                      never changes the result.
 #include "mo_common.h"
 #include "mo_grid.h"
 
@@ -46,17 +51,22 @@ namespace mo {
 
 namespace {
 
-// faces per culling tile: 16 against 32 gave 51.9 against 59.4 ms for 512
-// box-union grids at 64^3 and 265 against 278 ms for 64 icosphere targets
-// of 81,920 faces at 128^3 (tighter tile AABBs cull more faces than the
-// smaller first-stage cull loses); 8 was slower again
-constexpr int kTile = 16;
+// faces per culling tile: with the super-tile level below, 8 against 16
+// gave 41.4 against 42.7 ms for 512 box-union grids at 64^3 (65 against
+// 90 bound checks per cell: tighter tile AABBs) and the same time for the
+// mixed 64^3 / 128^3 icosphere targets of up to 81,920 faces; 4 was
+// slower (43.8 ms).  Without super-tiles 16 beat 32 and 8 (the scan over
+// all tiles grew with their number).
+constexpr int kTile = 8;
 constexpr int kSeedTiles = 4;  // nearest tiles evaluated before the culled walk
-// tiles per super-tile: the walk culls 256-face super-tiles against the
+// tiles per super-tile: the walk culls 128-face super-tiles against the
 // brick first, then the tiles of the kept ones (four super-tiles, 64 tiles,
 // per wavefront pass), so the walk's cost grows with the super-tiles near
-// the brick rather than with the whole target
+// the brick rather than with the whole target (32 tiles per super-tile was
+// slower: 47.7 ms, the seeds then come from two super-tiles)
 constexpr int kSuperTiles = 16;
+constexpr int kSupPass = 64 / kSuperTiles;   // super-tiles whose tiles one wavefront pass covers
+static_assert(kSuperTiles * kSupPass == 64 && kSuperTiles <= 64, "a pass covers 64 tiles");
 constexpr int kBinBits = 4;                  // 16 bins per axis
 constexpr int kBins = 1 << (3 * kBinBits);   // 4096
 
@@ -471,9 +481,9 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
     __syncthreads();  // the next tile overwrites s_face / s_list
   };
 
-  // 1. seeds: the four super-tiles nearest to the brick's centre (one scan of the
-  //    super-tile AABBs; each lane keeps its nearest), then the kSeedTiles
-  //    nearest of their 64 tiles, are evaluated first.  They give every
+  // 1. seeds: the kSupPass super-tiles nearest to the brick's centre (one
+  //    scan of the super-tile AABBs; each lane keeps its nearest), then the
+  //    kSeedTiles nearest of their 64 tiles, are evaluated first.  They give every
   //    lane a near-exact distance, which makes the culling below tight (the
   //    choice of seeds affects speed only).
   int seeds[kSeedTiles];
@@ -490,17 +500,20 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
       const double lb = box_gap2(sb, sidx, cx, cy, cz, cx, cy, cz);
       if (lb < slb) { slb = lb; ss = (int)sidx; }
     }
-    int s4[4] = {-1, -1, -1, -1};
-    for (int q = 0; q < 4; ++q) {
+    int sp[kSupPass];
+    for (int u = 0; u < kSupPass; ++u) sp[u] = -1;
+    for (int u = 0; u < kSupPass; ++u) {
       double ol = slb;
       int os = ss;
       wave_argmin(ol, os);
       if (os < 0) break;
       if (ss == os) { slb = INFINITY; ss = -1; }
-      s4[q] = os;
+      sp[u] = os;
     }
-    const int q = lane >> 4;
-    const int sq = q == 0 ? s4[0] : q == 1 ? s4[1] : q == 2 ? s4[2] : s4[3];
+    const int q = lane / kSuperTiles;
+    int sq = -1;
+#pragma unroll
+    for (int u = 0; u < kSupPass; ++u) sq = q == u ? sp[u] : sq;
     double mlb = INFINITY;
     int mt = -1;
     if (sq >= 0) {
@@ -523,8 +536,8 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
 
   // 2. every other tile that can still improve (or tie) some lane's best:
   //    super-tiles whose AABB is within the wave's worst best of the brick
-  //    AABB, then (four kept super-tiles per pass, lane 16 q + j on tile j
-  //    of the q-th) their tiles by the same test, and each such tile is
+  //    AABB, then (kSupPass kept super-tiles per pass, lane kSuperTiles q + j
+  //    on tile j of the q-th) their tiles by the same test, and each such tile is
   //    kept only if some lane's own point-to-AABB bound reaches its own
   //    best.
   for (long long c = 0; c < nsuper; c += 64) {
@@ -533,13 +546,15 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
     const double sbound = wave_max(valid ? best : -INFINITY);
     unsigned long long smask = __ballot(sidx < nsuper && !(slb > sbound * (1.0 + 1e-9)));
     while (smask) {
-      int s4[4] = {-1, -1, -1, -1};
-      for (int q = 0; q < 4 && smask; ++q) {
-        s4[q] = (int)c + __ffsll((long long)smask) - 1;
+      int sp[kSupPass];
+      for (int u = 0; u < kSupPass; ++u) {
+        sp[u] = smask ? (int)c + __ffsll((long long)smask) - 1 : -1;
         smask &= smask - 1;
       }
-      const int q = lane >> 4;
-      const int sq = q == 0 ? s4[0] : q == 1 ? s4[1] : q == 2 ? s4[2] : s4[3];
+      const int q = lane / kSuperTiles;
+      int sq = -1;
+#pragma unroll
+      for (int u = 0; u < kSupPass; ++u) sq = q == u ? sp[u] : sq;
       const long long t = sq < 0 ? -1 : (long long)sq * kSuperTiles + (lane & (kSuperTiles - 1));
       const double bound = wave_max(valid ? best : -INFINITY);
       bool need = false;
