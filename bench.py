@@ -74,8 +74,8 @@ def adjacency_entries(F):
 # 1.99975 and WRITE_SIZE x 1.0, the factors a plain 256 MiB copy reads) on
 # the default workload; keyed by (kernel, pairs, distinct pairs, source
 # verts, grid); other workloads report null.
-MEASURED_TRAFFIC = {("k_lm_solve_lds", 512, 128, 5000, 64): 26.23e9,
-                    ("k_grid_build", 512, 128, 5000, 64): 11.62e9}
+MEASURED_TRAFFIC = {("k_lm_solve_lds", 512, 128, 5000, 64): 26.69e9,
+                    ("k_grid_build", 512, 128, 5000, 64): 9.79e9}
 
 
 def cpu_baseline(pair, N, lam):
