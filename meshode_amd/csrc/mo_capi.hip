@@ -190,6 +190,9 @@ struct AuxStream {
   }
   ~AuxStream() {
     if (!s) return;
+    // nothing of this call may still run on it when the call's scratch and
+    // staging are released (a no-op once its sizes have been read)
+    (void)hipStreamSynchronize(s);
     std::lock_guard<std::mutex> lk(g_aux_mu);
     g_aux_free.emplace_back(device, s);
   }
