@@ -76,6 +76,19 @@ def adjacency_entries(F):
 # verts, grid); other workloads report null.
 MEASURED_TRAFFIC = {("k_lm_solve_lds", 512, 128, 5000, 64): 26.69e9,
                     ("k_grid_build", 512, 128, 5000, 64): 9.79e9}
+# Wavefront VALU instructions per launch (rocprofv3 SQ_INSTS_VALU, same
+# workload and key): with the live launch time they give the fraction of the
+# SIMDs' issue cycles the kernel's VALU work fills (a wave64 VALU
+# instruction occupies a SIMD 4 cycles; 1024 SIMDs at 2.4 GHz), which is
+# what bounds both kernels.
+MEASURED_VALU_INSTS = {("k_lm_solve_lds", 512, 128, 5000, 64): 1.305e10,
+                       ("k_grid_build", 512, 128, 5000, 64): 1.772e10}
+SIMDS, ENGINE_HZ = 1024, 2.4e9
+
+
+def valu_issue_frac(kernel, wkey, launch_ms):
+    n = MEASURED_VALU_INSTS.get((kernel,) + wkey)
+    return None if n is None else n * 4 / (SIMDS * launch_ms * 1e-3 * ENGINE_HZ)
 
 
 def cpu_baseline(pair, N, lam):
@@ -290,6 +303,7 @@ def main():
     roof_grid = {"kernel": "k_grid_build", "bound": "fp64-valu", "achieved": grid_tf, "peak": FP64_VALU_PEAK_TFLOPS,
                  "unit": "TFLOP/s", "frac": grid_tf / FP64_VALU_PEAK_TFLOPS,
                  "traffic": MEASURED_TRAFFIC.get(("k_grid_build",) + wkey),
+                 "valu_issue_frac": valu_issue_frac("k_grid_build", wkey, grid_ms),
                  "flops_per_launch": grid_flops, "launch_ms": grid_ms,
                  "exact_tests_per_cell": gs["exact_tests"] / cells,
                  "exact_tests_needed_per_cell": gs["exact_tests_needed"] / cells,
@@ -298,6 +312,7 @@ def main():
     roof_lm = {"kernel": "k_lm_solve_lds", "bound": "fp32-valu", "achieved": lm_tf, "peak": FP32_VALU_PEAK_TFLOPS,
                "unit": "TFLOP/s", "frac": lm_tf / FP32_VALU_PEAK_TFLOPS,
                "traffic": MEASURED_TRAFFIC.get(("k_lm_solve_lds",) + wkey),
+               "valu_issue_frac": valu_issue_frac("k_lm_solve_lds", wkey, lm_ms),
                "flops_per_launch": float(np.mean(lm_flops)), "launch_ms": lm_ms}
     assert roof_grid["frac"] <= 1.0 and roof_lm["frac"] <= 1.0, (roof_grid, roof_lm)
     last = stats_all[-1]
