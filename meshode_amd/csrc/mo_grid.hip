@@ -415,12 +415,13 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
   // one, face bound checks, staged tiles
   unsigned long long n_exact = 0, n_needed = 0, n_bound = 0, n_tiles = 0;
 
-  auto process_tile = [&](int t) {
+  // bound: an upper bound of every valid lane's best (the wave's worst best
+  // when it was taken; bests only shrink, so an older value stays valid)
+  auto process_tile = [&](int t, double bound) {
     const long long f0 = (long long)t * kTile;
     const int cnt = (int)((nf - f0) < kTile ? (nf - f0) : kTile);
     // stage 1, one face per lane: drop faces whose AABB is farther from the
     // brick AABB than the wave's worst best distance
-    const double bound = wave_max(valid ? best : -INFINITY);
     bool cand = false;
     if (lane < cnt) {
       const MO_GLOBAL GCull* gp = cull + f0 + lane;
@@ -532,7 +533,7 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
       seeds[nseed++] = ot;
     }
   }
-  for (int r = 0; r < nseed; ++r) process_tile(seeds[r]);
+  for (int r = 0; r < nseed; ++r) process_tile(seeds[r], wave_max(valid ? best : -INFINITY));
 
   // 2. every other tile that can still improve (or tie) some lane's best:
   //    super-tiles whose AABB is within the wave's worst best of the brick
@@ -570,7 +571,7 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
         const int tt = __builtin_amdgcn_readlane((int)t, l);
         const double pl = box_gap2(tb, tt, qx, qy, qz, qx, qy, qz);
         if (!__ballot(valid && !(pl > bcut))) continue;
-        process_tile(tt);
+        process_tile(tt, bound);
       }
     }
   }
