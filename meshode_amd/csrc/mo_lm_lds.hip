@@ -73,6 +73,10 @@ __device__ inline void store_xyz(float4* p, float4 v) {
   *reinterpret_cast<mo_v3f*>(p) = t;
 }
 
+// A solve after a rejected step starts from that step (scaled) instead of
+// from zero (see solve below).
+constexpr bool kPcgWarmStart = true;
+
 // An index the optimizer cannot see through: per-vertex addresses are then
 // formed at each use (base + index) instead of being hoisted out of the
 // solver loops, where they would hold dozens of registers for the whole
@@ -96,6 +100,7 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
                                  unsigned char* smem, double* red, int* scan_buf, unsigned lds_bytes) {
   const double o_min_diag = opt.min_lm_diagonal, o_max_diag = opt.max_lm_diagonal;
   const float o_pcg_tol = opt.pcg_relative_tolerance;
+  const bool o_pcg_warm = kPcgWarmStart;
   const int o_pcg_max = opt.pcg_max_iterations;
   const int tid = threadIdx.x;
   const int lo = team.lo;                  // owned vertices: lo + t + k T < hi
@@ -285,15 +290,22 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
   // z = M^-1 r once and forms A p as A z + beta A p_prev (p = z + beta
   // p_prev), so an iteration has two barriers: the reduction of p.Ap, and
   // the reduction of r.z and r.r, which also publishes the new z.
+  //
+  // After a rejected step (reuse: the same J^T J and J^T f, only the LM
+  // diagonal grew) the solve starts from the rejected step e scaled by
+  // alpha = (b . A e) / (e . A e), the best multiple of it in the A-norm,
+  // instead of from zero; the stopping test stays |r| <= tol |b|, the same
+  // absolute accuracy as a solve started from zero.
   auto solve = [&](bool reuse, double radius) -> int {
     const float finv_radius = (float)(1.0 / radius);
-    double v[2] = {0.0, 0.0};
+    const bool warm = reuse && o_pcg_warm;
+    double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < VPT; ++k) {
       const int il = opaque(tid + k * T), i = lo + il;
       px[k] = py[k] = pz[k] = 0.f;
       qx[k] = qy[k] = qz[k] = 0.f;
-      ex[k] = ey[k] = ez[k] = 0.f;
+      if (!warm) ex[k] = ey[k] = ez[k] = 0.f;
       dx[k] = dy[k] = dz[k] = 1.f;
       if (il >= own) continue;
       const float4 s = S[i];
@@ -320,6 +332,10 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
       dx[k] = ldk + fminf(dg.x * __builtin_amdgcn_rcpf(s.x * s.x) * finv_radius, 1e30f);
       dy[k] = ldk + fminf(dg.y * __builtin_amdgcn_rcpf(s.y * s.y) * finv_radius, 1e30f);
       dz[k] = ldk + fminf(dg.z * __builtin_amdgcn_rcpf(s.z * s.z) * finv_radius, 1e30f);
+      if (warm) {
+        team.publish(VEC, ZG, il, i, make_float4(ex[k], ey[k], ez[k], 0.f));
+        continue;
+      }
       const float4 gr = GR[i];
       const float r0 = -gr.x, r1 = -gr.y, r2 = -gr.z;
       float z0, z1, z2;
@@ -328,11 +344,49 @@ __device__ inline void lds_solve(const LMPairDesc& P, int pid, Team& team, const
       v[0] += (double)(r0 * z0 + r1 * z1 + r2 * z2);
       v[1] += (double)(r0 * r0 + r1 * r1 + r2 * r2);
     }
-    team.template reduce<2, 0>(v, red, sel);   // also publishes z
+    if (warm) {
+      // A e (into q) and the scaling alpha
+      team.sync();
+      double a2[2] = {0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < VPT; ++k) {
+        const int il = opaque(tid + k * T), i = lo + il;
+        if (il >= own) continue;
+        const float4 ei = VEC[il];
+        float sx, sy, sz;
+        gather(__float_as_int(ei.w), sx, sy, sz);
+        const float gee = gx[k] * ei.x + gy[k] * ei.y + gz[k] * ei.z;
+        qx[k] = gx[k] * gee + dx[k] * ei.x - l2 * sx;
+        qy[k] = gy[k] * gee + dy[k] * ei.y - l2 * sy;
+        qz[k] = gz[k] * gee + dz[k] * ei.z - l2 * sz;
+        const float4 gr = GR[i];
+        a2[0] -= (double)(gr.x * qx[k] + gr.y * qy[k] + gr.z * qz[k]);
+        a2[1] += (double)(ex[k] * qx[k] + ey[k] * qy[k] + ez[k] * qz[k]);
+      }
+      team.template reduce<2, 0>(a2, red, sel);
+      const float al = a2[1] > 0.0 ? (float)(a2[0] / a2[1]) : 0.f;
+#pragma unroll
+      for (int k = 0; k < VPT; ++k) {
+        const int il = opaque(tid + k * T), i = lo + il;
+        if (il >= own) continue;
+        ex[k] *= al; ey[k] *= al; ez[k] *= al;
+        const float4 gr = GR[i];
+        const float r0 = -gr.x - al * qx[k], r1 = -gr.y - al * qy[k], r2 = -gr.z - al * qz[k];
+        qx[k] = qy[k] = qz[k] = 0.f;
+        float z0, z1, z2;
+        precond(k, r0, r1, r2, z0, z1, z2);
+        team.publish(VEC, ZG, il, i, make_float4(z0, z1, z2, 0.f));
+        v[0] += (double)(r0 * z0 + r1 * z1 + r2 * z2);
+        v[1] += (double)(r0 * r0 + r1 * r1 + r2 * r2);
+        v[2] += (double)(gr.x * gr.x + gr.y * gr.y + gr.z * gr.z);
+      }
+    }
+    team.template reduce<3, 0>(v, red, sel);   // also publishes z
     double rz = v[0];
-    const double rr0 = v[1];
+    const double rr0 = warm ? v[2] : v[1];     // |b|^2
     if (!(rr0 > 0.0)) return 0;
     const double tol2 = (double)o_pcg_tol * o_pcg_tol * rr0;
+    if (warm && v[1] <= tol2) return 0;
     float beta = 0.f;
     int it = 0;
     while (it < o_pcg_max) {
