@@ -308,7 +308,7 @@ def main():
                  "exact_tests_per_cell": gs["exact_tests"] / cells,
                  "exact_tests_needed_per_cell": gs["exact_tests_needed"] / cells,
                  "bound_checks_per_cell": gs["bound_checks"] / cells,
-                 "tiles_per_brick": gs["tiles"] / max(1, sum(((args.grid + 3) // 4) ** 3 for _ in pairs)),
+                 "tiles_per_64_cells": 64 * gs["tiles"] / max(1, cells),
                  "bound_check_flop_share": GRID_FLOPS_PER_BOUND * gs["bound_checks"] / max(grid_flops, 1)}
     roof_lm = {"kernel": "k_lm_solve_lds", "bound": "fp32-valu", "achieved": lm_tf, "peak": FP32_VALU_PEAK_TFLOPS,
                "unit": "TFLOP/s", "frac": lm_tf / FP32_VALU_PEAK_TFLOPS,

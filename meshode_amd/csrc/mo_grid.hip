@@ -21,7 +21,8 @@
 //   3. k_face_prep   : FP64 face record per face, written in sorted order.
 //   4. k_tile_bounds : AABB per tile of kTile (8) sorted faces;
 //      k_super_bounds: AABB per super-tile of kSuperTiles (16) tiles.
-//   5. k_grid_build  : one wavefront per 4x4x4 brick of grid points.  The
+//   5. k_grid_build  : one wavefront per 4x4x8 brick of grid points (two
+//                      points per lane, 4 apart in z).  The
 //                      wave first evaluates the 4 tiles nearest its brick's
 //                      centre among the tiles of the 4 nearest super-tiles
 //                      (they seed every lane with a near-exact distance),
@@ -374,7 +375,11 @@ __device__ inline double box_gap2(const MO_GLOBAL double* __restrict__ tb, long 
   return s;
 }
 
-// One wavefront (64 lanes) per 4x4x4 brick.
+// One wavefront (64 lanes) per 4x4x8 brick: lane l holds the two grid
+// points (x, y, z) and (x, y, z + 4) of it.  The two points share the
+// brick's culling, the staged face records, every LDS read of a record and
+// the x / y parts of their bounds.
+constexpr int kZPts = 2;   // points per lane, 4 apart in z
 __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ jobs) {
   // survivors of a tile's brick-level cull: their full records, copied with
   // coalesced vector loads, then read by the whole wave at one LDS address
@@ -389,34 +394,54 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
   const MO_GLOBAL double* __restrict__ sb = gptr(JJ.sb);
   const long long nsuper = JJ.nsuper;
   const int N = JJ.N;
-  const int bricks_per_axis = (N + 3) / 4;
+  const int bpa = (N + 3) / 4, bpz = (N + 4 * kZPts - 1) / (4 * kZPts);
   MO_GLOBAL double* __restrict__ dist64 = gptr(JJ.dist64);
   MO_GLOBAL float* __restrict__ dist32 = gptr(JJ.dist32);
   MO_GLOBAL int* __restrict__ nearest = gptr(JJ.nearest);
   const int lane = threadIdx.x;
   long long b = blockIdx.x;
-  if (b >= (long long)bricks_per_axis * bricks_per_axis * bricks_per_axis) return;
-  const int bx = (int)(b % bricks_per_axis);
-  const int by = (int)((b / bricks_per_axis) % bricks_per_axis);
-  const int bz = (int)(b / ((long long)bricks_per_axis * bricks_per_axis));
-  const int k = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3), i = bz * 4 + (lane >> 4);
-  const bool valid = (k < N) && (j < N) && (i < N);
-  const double qx = (double)k / (double)N, qy = (double)j / (double)N, qz = (double)i / (double)N;
+  if (b >= (long long)bpa * bpa * bpz) return;
+  const int bx = (int)(b % bpa);
+  const int by = (int)((b / bpa) % bpa);
+  const int bz = (int)(b / ((long long)bpa * bpa));
+  const int k = bx * 4 + (lane & 3), j = by * 4 + ((lane >> 2) & 3);
+  int ip[kZPts];
+  bool valid[kZPts];
+  double qz[kZPts];
+  const double qx = (double)k / (double)N, qy = (double)j / (double)N;
+#pragma unroll
+  for (int s = 0; s < kZPts; ++s) {
+    ip[s] = bz * 4 * kZPts + (lane >> 4) + 4 * s;
+    valid[s] = (k < N) && (j < N) && (ip[s] < N);
+    qz[s] = (double)ip[s] / (double)N;
+  }
   // brick AABB over valid points
-  const int k1 = min(bx * 4 + 3, N - 1), j1 = min(by * 4 + 3, N - 1), i1 = min(bz * 4 + 3, N - 1);
-  const double lx = (double)(bx * 4) / N, ly = (double)(by * 4) / N, lz = (double)(bz * 4) / N;
+  const int k1 = min(bx * 4 + 3, N - 1), j1 = min(by * 4 + 3, N - 1), i1 = min(bz * 4 * kZPts + 4 * kZPts - 1, N - 1);
+  const double lx = (double)(bx * 4) / N, ly = (double)(by * 4) / N, lz = (double)(bz * 4 * kZPts) / N;
   const double hx = (double)k1 / N, hy = (double)j1 / N, hz = (double)i1 / N;
 
   // points farther than the band only need to be shown farther (GridSpec)
-  double best = JJ.band2;
-  double bcut = best * (1.0 + 1e-9);   // the culling threshold, kept with best
-  int bidx = INT_MAX;
-  // work counters (wave-uniform): exact evaluations, lanes that needed
-  // one, face bound checks, staged tiles
+  double best[kZPts], bcut[kZPts];   // bcut: the culling threshold, kept with best
+  int bidx[kZPts];
+#pragma unroll
+  for (int s = 0; s < kZPts; ++s) {
+    best[s] = JJ.band2;
+    bcut[s] = best[s] * (1.0 + 1e-9);
+    bidx[s] = INT_MAX;
+  }
+  // the wave's worst best over its valid points
+  auto worst = [&]() {
+    double m = -INFINITY;
+#pragma unroll
+    for (int s = 0; s < kZPts; ++s) m = valid[s] ? vmax(m, best[s]) : m;
+    return wave_max(m);
+  };
+  // work counters (wave-uniform): exact evaluations (one point per lane),
+  // lanes that needed one, face bound checks (per point), staged tiles
   unsigned long long n_exact = 0, n_needed = 0, n_bound = 0, n_tiles = 0;
 
-  // bound: an upper bound of every valid lane's best (the wave's worst best
-  // when it was taken; bests only shrink, so an older value stays valid)
+  // bound: an upper bound of every valid point's best (the wave's worst
+  // best when it was taken; bests only shrink, so an older value stays valid)
   auto process_tile = [&](int t, double bound) {
     const long long f0 = (long long)t * kTile;
     const int cnt = (int)((nf - f0) < kTile ? (nf - f0) : kTile);
@@ -434,7 +459,7 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
     const int ns = __popcll(m);
     if (ns == 0) return;
     ++n_tiles;
-    n_bound += (unsigned long long)ns;
+    n_bound += (unsigned long long)ns * kZPts;
     // compact the survivors (in face order) and stage their records in LDS
     const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
     if (cand) s_list[rank] = lane;
@@ -449,32 +474,46 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
       dst[c] = src[s_list[r] * kChunks + w];
     }
     __syncthreads();
-    // stage 2, per surviving face: every lane's own AABB bound, then (when
-    // it leaves some lane) the plane bound, then the exact test when some
-    // lane can still improve or tie its best
+    // stage 2, per surviving face: every point's own AABB bound, then (when
+    // it leaves some point) the plane bound, then the exact test when some
+    // lane's point can still improve or tie its best.  The bounds are summed
+    // in the order x, y, z as for a single point.
     for (int r = 0; r < ns; ++r) {
       const GFace& g = s_face[r];
-      double gx = vmax(0.0, vmax(g.lox - qx, qx - g.hix));
-      double gy = vmax(0.0, vmax(g.loy - qy, qy - g.hiy));
-      double gz = vmax(0.0, vmax(g.loz - qz, qz - g.hiz));
-      bool mine = valid && !(gx * gx + gy * gy + gz * gz > bcut);
-      if (!__ballot(mine)) continue;
+      const double gx = vmax(0.0, vmax(g.lox - qx, qx - g.hix));
+      const double gy = vmax(0.0, vmax(g.loy - qy, qy - g.hiy));
+      const double gxy = gx * gx + gy * gy;
+      bool mine[kZPts];
+      bool any = false;
+#pragma unroll
+      for (int s = 0; s < kZPts; ++s) {
+        const double gz = vmax(0.0, vmax(g.loz - qz[s], qz[s] - g.hiz));
+        mine[s] = valid[s] && !(gxy + gz * gz > bcut[s]);
+        any = any || mine[s];
+      }
+      if (!__ballot(any)) continue;
       // the face plane is a second lower bound (0 for degenerate faces)
-      const double dp = g.nx * (qx - g.ax) + g.ny * (qy - g.ay) + g.nz * (qz - g.az);
-      mine = mine && !((dp * dp) * g.inv_nn > bcut);
-      const unsigned long long need = __ballot(mine);
-      if (need) {
+      const double dxy = g.nx * (qx - g.ax) + g.ny * (qy - g.ay);
+#pragma unroll
+      for (int s = 0; s < kZPts; ++s) {
+        const double dp = dxy + g.nz * (qz[s] - g.az);
+        mine[s] = mine[s] && !((dp * dp) * g.inv_nn > bcut[s]);
+      }
+#pragma unroll
+      for (int s = 0; s < kZPts; ++s) {
+        const unsigned long long need = __ballot(mine[s]);
+        if (!need) continue;
         ++n_exact;
         n_needed += (unsigned long long)__popcll(need);
-        // only the lanes that need it evaluate (a lane whose bound exceeds
+        // only the lanes that need it evaluate (a point whose bound exceeds
         // its best cannot improve or tie it), so the face-region / edge
         // branches diverge over those lanes alone
-        if (mine) {
-          const double d = face_d2(g, qx, qy, qz);
-          if (d < best || (d == best && g.face < bidx)) {
-            best = d;
-            bcut = d * (1.0 + 1e-9);
-            bidx = g.face;
+        if (mine[s]) {
+          const double d = face_d2(g, qx, qy, qz[s]);
+          if (d < best[s] || (d == best[s] && g.face < bidx[s])) {
+            best[s] = d;
+            bcut[s] = d * (1.0 + 1e-9);
+            bidx[s] = g.face;
           }
         }
       }
@@ -484,9 +523,9 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
 
   // 1. seeds: the kSupPass super-tiles nearest to the brick's centre (one
   //    scan of the super-tile AABBs; each lane keeps its nearest), then the
-  //    kSeedTiles nearest of their 64 tiles, are evaluated first.  They give every
-  //    lane a near-exact distance, which makes the culling below tight (the
-  //    choice of seeds affects speed only).
+  //    kSeedTiles nearest of their 64 tiles, are evaluated first.  They give
+  //    every point a near-exact distance, which makes the culling below
+  //    tight (the choice of seeds affects speed only).
   int seeds[kSeedTiles];
   int nseed = 0;
   {
@@ -533,18 +572,18 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
       seeds[nseed++] = ot;
     }
   }
-  for (int r = 0; r < nseed; ++r) process_tile(seeds[r], wave_max(valid ? best : -INFINITY));
+  for (int r = 0; r < nseed; ++r) process_tile(seeds[r], worst());
 
-  // 2. every other tile that can still improve (or tie) some lane's best:
+  // 2. every other tile that can still improve (or tie) some point's best:
   //    super-tiles whose AABB is within the wave's worst best of the brick
   //    AABB, then (kSupPass kept super-tiles per pass, lane kSuperTiles q + j
-  //    on tile j of the q-th) their tiles by the same test, and each such tile is
-  //    kept only if some lane's own point-to-AABB bound reaches its own
-  //    best.
+  //    on tile j of the q-th) their tiles by the same test, and each such
+  //    tile is kept only if some point's own point-to-AABB bound reaches its
+  //    own best.
   for (long long c = 0; c < nsuper; c += 64) {
     const long long sidx = c + lane;
     const double slb = sidx < nsuper ? box_gap2(sb, sidx, lx, ly, lz, hx, hy, hz) : INFINITY;
-    const double sbound = wave_max(valid ? best : -INFINITY);
+    const double sbound = worst();
     unsigned long long smask = __ballot(sidx < nsuper && !(slb > sbound * (1.0 + 1e-9)));
     while (smask) {
       int sp[kSupPass];
@@ -557,7 +596,7 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
 #pragma unroll
       for (int u = 0; u < kSupPass; ++u) sq = q == u ? sp[u] : sq;
       const long long t = sq < 0 ? -1 : (long long)sq * kSuperTiles + (lane & (kSuperTiles - 1));
-      const double bound = wave_max(valid ? best : -INFINITY);
+      const double bound = worst();
       bool need = false;
       if (t >= 0 && t < ntiles) {
         bool taken = false;
@@ -569,8 +608,11 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
         const int l = __ffsll((long long)tmask) - 1;
         tmask &= tmask - 1;
         const int tt = __builtin_amdgcn_readlane((int)t, l);
-        const double pl = box_gap2(tb, tt, qx, qy, qz, qx, qy, qz);
-        if (!__ballot(valid && !(pl > bcut))) continue;
+        bool reach = false;
+#pragma unroll
+        for (int s = 0; s < kZPts; ++s)
+          reach = reach || (valid[s] && !(box_gap2(tb, tt, qx, qy, qz[s], qx, qy, qz[s]) > bcut[s]));
+        if (!__ballot(reach)) continue;
         process_tile(tt, bound);
       }
     }
@@ -581,12 +623,14 @@ __global__ void __launch_bounds__(64) k_grid_build(const GridJob* __restrict__ j
     atomicAdd(JJ.stats + kGridStatBound, n_bound * 64ull);
     atomicAdd(JJ.stats + kGridStatTiles, n_tiles);
   }
-  if (valid) {
-    long long cell = ((long long)i * N + j) * N + k;
-    double s = sqrt(best);
-    if (dist64) dist64[cell] = s;
-    if (dist32) dist32[cell] = (float)s;
-    if (nearest) nearest[cell] = bidx == INT_MAX ? -1 : bidx;
+#pragma unroll
+  for (int s = 0; s < kZPts; ++s) {
+    if (!valid[s]) continue;
+    const long long cell = ((long long)ip[s] * N + j) * N + k;
+    const double sq = sqrt(best[s]);
+    if (dist64) dist64[cell] = sq;
+    if (dist32) dist32[cell] = (float)sq;
+    if (nearest) nearest[cell] = bidx[s] == INT_MAX ? -1 : bidx[s];
   }
 }
 
@@ -683,11 +727,11 @@ int grid_build_batch_launch(const GridSpec* specs, int n, void* workspace, hipSt
     MO_CHECK(sp.band2 > 0.0, MO_ERR_ARG, "grid band must be positive");
     J.band2 = sp.band2;
     J.stats = sp.stats;
-    const long long bpa = (sp.N + 3) / 4;
+    const long long bpa = (sp.N + 3) / 4, bpz = (sp.N + 4 * kZPts - 1) / (4 * kZPts);
     max_nf = std::max(max_nf, sp.nf);
     max_tiles = std::max(max_tiles, J.ntiles);
     max_super = std::max(max_super, J.nsuper);
-    max_bricks = std::max(max_bricks, bpa * bpa * bpa);
+    max_bricks = std::max(max_bricks, bpa * bpa * bpz);
   }
   int rc = chain_h2d(d_jobs, jobs.data(), (size_t)n * sizeof(GridJob), stream);
   if (rc) return rc;
