@@ -75,14 +75,14 @@ def adjacency_entries(F):
 # the default workload; keyed by (kernel, pairs, distinct pairs, source
 # verts, grid); other workloads report null.
 MEASURED_TRAFFIC = {("k_lm_solve_lds", 512, 128, 5000, 64): 28.03e9,
-                    ("k_grid_build", 512, 128, 5000, 64): 9.87e9}
+                    ("k_grid_build", 512, 128, 5000, 64): 14.56e9}
 # Wavefront VALU instructions per launch (rocprofv3 SQ_INSTS_VALU, same
 # workload and key): with the live launch time they give the fraction of the
 # SIMDs' issue cycles the kernel's VALU work fills (a wave64 VALU
 # instruction occupies a SIMD 4 cycles; 1024 SIMDs at 2.4 GHz), which is
 # what bounds both kernels.
 MEASURED_VALU_INSTS = {("k_lm_solve_lds", 512, 128, 5000, 64): 1.252e10,
-                       ("k_grid_build", 512, 128, 5000, 64): 1.699e10}
+                       ("k_grid_build", 512, 128, 5000, 64): 1.565e10}
 SIMDS, ENGINE_HZ = 1024, 2.4e9
 
 
